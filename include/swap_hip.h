@@ -1,0 +1,128 @@
+/*
+ * swap_hip.h -- C ABI of the MI355X (gfx950) k-mer graph construction and
+ * k-molecule merging path of SWAP-Assembler 2.
+ *
+ * Plain C types only: callers bind it from C, C++ (the swap driver), Python
+ * (ctypes) or any FFI.  Errors are negative return codes (never exit()); the
+ * text of the last error of a context is available from swap_last_error().
+ * A context owns its device memory; host buffers belong to the caller.  One
+ * context per host thread: the API is thread-compatible, not thread-safe.
+ *
+ * Reference interfaces replaced (file:line in SWAP-Assembler 2):
+ *   swap_create / swap_destroy   parameter::getParameters (sequence.cpp:121-183)
+ *                                for k, c and the flags; new distNodeGraph /
+ *                                new kmerGraph in main (graph.cpp:3474-3477)
+ *   swap_build                   distNodeGraph::constructDistNodeGraph
+ *                                (graph.cpp:1049-1090) with the
+ *                                kmerGraph::constructKmerGraph /
+ *                                distributeKmerGraph loop (kmerGraph.cpp:95-310,
+ *                                337-432), then distNodeGraph::cutoffGraph
+ *                                (graph.cpp:1092-1114)
+ *   swap_merge                   buildKmoleculeGraph (graph.cpp:1117-1137) +
+ *                                simplifyDistNodeGraph (graph.cpp:1392-1566) +
+ *                                masterSimplifyNodeGraph (graph.cpp:2089-2192)
+ *   swap_export_kmer_graph       the kmolecules table after cutoffGraph
+ *                                (graph.cpp:353, 1112)
+ *   swap_export_compacted        the nodes table after the merge (graph.cpp:354)
+ *   swap_format_file /           printKmerGraph (graph.cpp:3352-3440),
+ *   swap_write_file              gatherDistNodeGraph + the contigGraph.txt write
+ *                                (graph.cpp:576-696, 3617-3622), printContigs
+ *                                (graph.cpp:1139-1202)
+ */
+#ifndef SWAP_HIP_H
+#define SWAP_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* return codes */
+#define SWAP_OK 0
+#define SWAP_E_INVAL (-1)     /* bad argument (k not odd in [9,31], NULL pointer, wrong state) */
+#define SWAP_E_EMPTY (-2)     /* empty input buffer */
+#define SWAP_E_FORMAT (-3)    /* no '>' in the input (reference: assert at kmerGraph.cpp:114) */
+#define SWAP_E_TOOBIG (-4)    /* more than 2^32-16 k-mers in one build */
+#define SWAP_E_DEVICE (-5)    /* HIP runtime error (see swap_last_error) */
+#define SWAP_E_NOGPU (-6)     /* no usable gfx950 device */
+#define SWAP_E_IO (-7)        /* file could not be written */
+#define SWAP_E_STATE (-8)     /* call out of order (e.g. merge before build) */
+#define SWAP_E_RANGE (-9)     /* caller buffer too small */
+
+/* swap_cfg.flags */
+#define SWAP_FLAG_KMER_GRAPH 1u  /* -s: stop after the k-mer graph, no merge */
+#define SWAP_FLAG_HASH_ORDER 2u  /* merge in a GPU-native (hash) visit order instead of
+                                    replaying the reference's table order; topology,
+                                    edge strings and contigs are the same, merged
+                                    multiplicities follow the different order */
+
+typedef struct swap_cfg {
+    int k;          /* odd k-mer length, 9..31 (the reference CLI accepts 19..31) */
+    int c;          /* cutoff: edge multiplicities below c are removed */
+    int device;     /* HIP device ordinal */
+    unsigned flags; /* SWAP_FLAG_* */
+} swap_cfg;
+
+typedef struct swap_stats {
+    uint64_t reads;        /* records with at least k bases */
+    uint64_t kmers;        /* input k-mer occurrences (N) */
+    uint64_t distinct;     /* canonical k-mers before cutoff (D) */
+    uint64_t survivors;    /* k-molecules after cutoff (S) */
+    uint64_t kept;         /* nodes after the merge */
+    uint64_t overflow;     /* partitions that fell back to the global-memory table */
+    uint64_t special;      /* (1,1) nodes merged on the host (cycles, fold-backs) */
+    double build_ms;       /* wall time of the last swap_build (device + syncs) */
+    double merge_ms;       /* wall time of the last swap_merge */
+} swap_stats;
+
+typedef struct swap_ctx swap_ctx;
+
+int swap_create(const swap_cfg *cfg, swap_ctx **out);
+void swap_destroy(swap_ctx *ctx);
+
+/* Build from FASTA bytes in host memory (copied to the device). */
+int swap_build(swap_ctx *ctx, const char *fasta, size_t n);
+/* Build from FASTA bytes already in device memory on the context's device. */
+int swap_build_device(swap_ctx *ctx, const void *d_fasta, size_t n);
+
+/* Merge the k-mer graph of the last build into unitig edges. */
+int swap_merge(swap_ctx *ctx);
+
+int swap_get_stats(const swap_ctx *ctx, swap_stats *out);
+
+/* Survivors of the cutoff sorted by key: keys[i] and 8 multiplicities
+ * mult[8*i .. 8*i+7] (slot t = "X followed by base t" for t<4, "rc(X) followed
+ * by base t-4" for t>=4).  Returns the number of records (pass NULL/0 to query),
+ * or a negative code. */
+int64_t swap_export_kmer_graph(swap_ctx *ctx, uint64_t *keys, uint8_t *mult, size_t cap);
+
+/* Output files (byte formats of the reference):
+ *   SWAP_FILE_KMER_GRAPH   kmerGraph.txt      (after a SWAP_FLAG_KMER_GRAPH build)
+ *   SWAP_FILE_CONTIG_GRAPH contigGraph.txt    (after swap_merge)
+ *   SWAP_FILE_NOCE         noCEcontigs.fasta  (after swap_merge) */
+#define SWAP_FILE_KMER_GRAPH 0
+#define SWAP_FILE_CONTIG_GRAPH 1
+#define SWAP_FILE_NOCE 2
+/* Writes the file bytes into buf; returns the byte count (buf NULL: size query). */
+int64_t swap_format_file(swap_ctx *ctx, int which, char *buf, size_t cap);
+/* Writes the file to path; returns bytes written or a negative code. */
+int64_t swap_write_file(swap_ctx *ctx, int which, const char *path);
+
+/* Host utility: iteration order of the reference's __gnu_cxx::hash_map (hash of
+ * graph.cpp:29-32, libstdc++ bucket growth) after n_total inserts, restricted to
+ * the n keys given; ranks[i] is the insertion index of keys[i].  extra_resize
+ * adds the growth check of one more operator[] lookup after the last insert.
+ * Writes the indices of the keys in iteration order to order_out; returns n. */
+int64_t swap_table_order(const uint64_t *keys, const uint64_t *ranks, size_t n, uint64_t n_total,
+                         int extra_resize, uint32_t *order_out);
+
+const char *swap_strerror(int code);
+const char *swap_last_error(const swap_ctx *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SWAP_HIP_H */

@@ -129,6 +129,7 @@ struct oracle_graph {
     std::vector<unsigned long long> surv_keys;    // sorted survivor keys
     std::vector<Arc> surv_arcs;
     std::vector<uint64_t> surv_first;
+    std::vector<uint64_t> surv_rank;              // index of the key among first occurrences
     std::vector<unsigned long long> visit;        // survivors in merge visit order
     std::string kmer_graph_txt, contig_graph_txt, noce_txt;
 };
@@ -433,13 +434,14 @@ oracle_graph *oracle_run(const char *fasta, uint64_t n, int k, int c, int flags)
         std::vector<std::pair<unsigned long long, Arc>> v(kmol.begin(), kmol.end());
         std::sort(v.begin(), v.end(), [](const std::pair<unsigned long long, Arc> &a, const std::pair<unsigned long long, Arc> &b) { return a.first < b.first; });
         std::vector<std::pair<unsigned long long, uint64_t>> fk(first_key.size());
-        for (size_t i = 0; i < first_key.size(); i++) fk[i] = {first_key[i], first_of[i]};
+        for (size_t i = 0; i < first_key.size(); i++) fk[i] = {first_key[i], i};
         std::sort(fk.begin(), fk.end());
         for (auto &e : v) {
             g->surv_keys.push_back(e.first);
             g->surv_arcs.push_back(e.second);
             auto f = std::lower_bound(fk.begin(), fk.end(), std::make_pair(e.first, (uint64_t)0));
-            g->surv_first.push_back(f->second);
+            g->surv_first.push_back(first_of[f->second]);
+            g->surv_rank.push_back(f->second);
         }
     }
     NodeTable nodes;
@@ -487,6 +489,12 @@ uint64_t oracle_survivors(const oracle_graph *g, uint64_t *keys, uint8_t *mult, 
 uint64_t oracle_survivor_first(const oracle_graph *g, uint64_t *first, uint64_t cap) {
     const uint64_t n = g->surv_first.size();
     for (uint64_t i = 0; i < n && i < cap; i++) first[i] = g->surv_first[i];
+    return n;
+}
+
+uint64_t oracle_survivor_rank(const oracle_graph *g, uint64_t *rank, uint64_t cap) {
+    const uint64_t n = g->surv_rank.size();
+    for (uint64_t i = 0; i < n && i < cap; i++) rank[i] = g->surv_rank[i];
     return n;
 }
 

@@ -48,6 +48,9 @@ uint64_t oracle_survivors(const oracle_graph *g, uint64_t *keys, uint8_t *mult, 
 // First-occurrence index (0-based input k-mer index) of each survivor, in the
 // order of oracle_survivors.
 uint64_t oracle_survivor_first(const oracle_graph *g, uint64_t *first, uint64_t cap);
+// Index of each survivor's key among all distinct keys in first-occurrence
+// order (its insertion index into the reference's k-mer table).
+uint64_t oracle_survivor_rank(const oracle_graph *g, uint64_t *rank, uint64_t cap);
 // Survivor keys in the merge visit order (the reference's hash-table order).
 uint64_t oracle_visit_order(const oracle_graph *g, uint64_t *keys, uint64_t cap);
 

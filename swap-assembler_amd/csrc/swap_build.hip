@@ -1,0 +1,866 @@
+// k-mer graph construction on gfx950: FASTA record scan, 2-bit packing,
+// canonical k-mer + edge-mask extraction, device-local hash partitioning and
+// per-partition LDS hash tables with saturating edge counters, cutoff.
+//
+// Replaces (single rank):
+//   kmerGraph::constructKmerGraph   kmerGraph.cpp:95-310  (records, encode, edges)
+//   kmerGraph::distributeKmerGraph  kmerGraph.cpp:337-432 (owner-hash exchange)
+//   distNodeGraph::constructDistNodeGraph graph.cpp:1049-1090 (table, counts)
+//   distNodeGraph::cutoffGraph      graph.cpp:1092-1114
+#include "swap_device.h"
+
+#include <algorithm>
+#include <cstdio>
+
+namespace swapk {
+
+// ---------------------------------------------------------------------------
+// small utilities
+
+__device__ __forceinline__ uint32_t zero_bytes(uint32_t x) {
+    // exact per-byte "is zero" mask: 0x80 in each zero byte
+    return ~(((x & 0x7F7F7F7Fu) + 0x7F7F7F7Fu) | x | 0x7F7F7F7Fu);
+}
+
+__device__ __forceinline__ uint32_t base_code(uint8_t c) {
+    // sanitising map of kmerGraph.cpp:228-233 followed by sequence.cpp:164-168:
+    // ACGT/acgt keep their base, every other byte reads as 'A'.
+    switch (c) {
+        case 'C': case 'c': return 1;
+        case 'G': case 'g': return 2;
+        case 'T': case 't': return 3;
+        default: return 0;
+    }
+}
+
+template <typename T>
+__device__ __forceinline__ T wave_incl_sum(T v) {
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        T o = __shfl_up(v, d, 64);
+        if (lane >= d) v += o;
+    }
+    return v;
+}
+
+// ---------------------------------------------------------------------------
+// exclusive scan (u64 output) over u32 or u64 input: block reduce, recursive
+// scan of the block sums, add back.
+
+constexpr int kScanThreads = 256;
+constexpr int kScanItems = 8;
+constexpr int kScanTile = kScanThreads * kScanItems;
+
+template <typename TIn>
+__global__ void k_scan_local(const TIn *__restrict__ in, uint64_t *__restrict__ out, uint64_t n,
+                             uint64_t *__restrict__ block_sums) {
+    __shared__ uint64_t wsum[kScanThreads / 64];
+    const uint64_t base = (uint64_t)blockIdx.x * kScanTile + (uint64_t)threadIdx.x * kScanItems;
+    uint64_t v[kScanItems];
+    uint64_t s = 0;
+#pragma unroll
+    for (int i = 0; i < kScanItems; i++) {
+        v[i] = (base + i < n) ? (uint64_t)in[base + i] : 0;
+        s += v[i];
+    }
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    uint64_t inc = wave_incl_sum(s);
+    if (lane == 63) wsum[w] = inc;
+    __syncthreads();
+    uint64_t woff = 0;
+    for (int i = 0; i < w; i++) woff += wsum[i];
+    uint64_t run = inc - s + woff;
+#pragma unroll
+    for (int i = 0; i < kScanItems; i++) {
+        if (base + i < n) out[base + i] = run;
+        run += v[i];
+    }
+    if (threadIdx.x == kScanThreads - 1) block_sums[blockIdx.x] = run;
+}
+
+__global__ void k_scan_add(uint64_t *__restrict__ out, uint64_t n, const uint64_t *__restrict__ offs) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] += offs[i / kScanTile];
+}
+
+__global__ void k_u64_to_u32(const uint64_t *__restrict__ in, uint64_t n, uint32_t *__restrict__ out) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = (uint32_t)in[i];
+}
+
+// total <- sum, out <- exclusive prefix.  scratch must hold scan_scratch_words(n).
+uint64_t scan_scratch_words(uint64_t n) {
+    uint64_t w = 0;
+    while (n > 1) {
+        n = (n + kScanTile - 1) / kScanTile;
+        w += n + 1;
+    }
+    return w + 2;
+}
+
+template <typename TIn>
+static void scan_excl(const TIn *in, uint64_t *out, uint64_t n, uint64_t *scratch, uint64_t *d_total,
+                      hipStream_t st) {
+    if (n == 0) {
+        hipMemsetAsync(d_total, 0, 8, st);
+        return;
+    }
+    const uint64_t nb = (n + kScanTile - 1) / kScanTile;
+    uint64_t *sums = scratch;
+    hipLaunchKernelGGL(k_scan_local<TIn>, dim3((unsigned)nb), dim3(kScanThreads), 0, st, in, out, n, sums);
+    if (nb == 1) {
+        hipMemcpyAsync(d_total, sums, 8, hipMemcpyDeviceToDevice, st);
+        return;
+    }
+    uint64_t *sum_scan = scratch + nb;
+    scan_excl<uint64_t>(sums, sum_scan, nb, scratch + 2 * nb, d_total, st);
+    hipLaunchKernelGGL(k_scan_add, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, out, n, sum_scan);
+}
+
+// ---------------------------------------------------------------------------
+// FASTA record scan
+
+constexpr int kTileThreads = 256;
+constexpr int kTileBytes = kTileThreads * 16;
+
+// Per tile: number of '\n' bytes; global: position of the first '>'.
+__global__ void k_tile_count(const uint8_t *__restrict__ buf, uint64_t n, uint32_t *__restrict__ tile_nl,
+                             unsigned long long *__restrict__ first_gt) {
+    const uint64_t base = (uint64_t)blockIdx.x * kTileBytes + (uint64_t)threadIdx.x * 16;
+    uint32_t cnt = 0;
+    unsigned long long gt = ~0ull;
+    if (base + 16 <= n) {
+        const uint4 v = *reinterpret_cast<const uint4 *>(buf + base);
+        const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            cnt += __popc(zero_bytes(w[i] ^ 0x0A0A0A0Au));
+            const uint32_t g = zero_bytes(w[i] ^ 0x3E3E3E3Eu);
+            if (g && gt == ~0ull) gt = base + 4 * i + (__ffs(g) - 1) / 8;
+        }
+    } else {
+        for (uint64_t i = base; i < n && i < base + 16; i++) {
+            const uint8_t c = buf[i];
+            cnt += (c == '\n');
+            if (c == '>' && gt == ~0ull) gt = i;
+        }
+    }
+    // block reduce
+    __shared__ uint32_t scnt[kTileThreads / 64];
+    __shared__ unsigned long long sgt[kTileThreads / 64];
+    for (int d = 32; d > 0; d >>= 1) {
+        cnt += __shfl_xor(cnt, d, 64);
+        unsigned long long o = __shfl_xor(gt, d, 64);
+        gt = o < gt ? o : gt;
+    }
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0) { scnt[w] = cnt; sgt[w] = gt; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t c = 0;
+        unsigned long long g = ~0ull;
+        for (int i = 0; i < kTileThreads / 64; i++) { c += scnt[i]; g = sgt[i] < g ? sgt[i] : g; }
+        tile_nl[blockIdx.x] = c;
+        if (g != ~0ull) atomicMin(first_gt, g);
+    }
+}
+
+// Writes the position of every '\n' in file order.
+__global__ void k_tile_emit(const uint8_t *__restrict__ buf, uint64_t n, const uint64_t *__restrict__ tile_off,
+                            uint64_t *__restrict__ nl_pos) {
+    __shared__ uint32_t wsum[kTileThreads / 64];
+    const uint64_t base = (uint64_t)blockIdx.x * kTileBytes + (uint64_t)threadIdx.x * 16;
+    uint8_t b[16];
+    if (base + 16 <= n) {
+        const uint4 v = *reinterpret_cast<const uint4 *>(buf + base);
+        const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+        for (int i = 0; i < 4; i++)
+#pragma unroll
+            for (int j = 0; j < 4; j++) b[4 * i + j] = (uint8_t)(w[i] >> (8 * j));
+    } else {
+#pragma unroll
+        for (int i = 0; i < 16; i++) b[i] = (base + i < n) ? buf[base + i] : 0;
+    }
+    uint32_t cnt = 0;
+#pragma unroll
+    for (int i = 0; i < 16; i++) cnt += (b[i] == '\n');
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    uint32_t inc = wave_incl_sum(cnt);
+    if (lane == 63) wsum[w] = inc;
+    __syncthreads();
+    uint32_t off = inc - cnt;
+    for (int i = 0; i < w; i++) off += wsum[i];
+    uint64_t o = tile_off[blockIdx.x] + off;
+#pragma unroll
+    for (int i = 0; i < 16; i++)
+        if (b[i] == '\n') nl_pos[o++] = base + i;
+}
+
+// Records are line pairs starting at the first '>' (kmerGraph.cpp:173-181): the
+// header is skipped, the next line is the sequence.
+__global__ void k_records(const uint64_t *__restrict__ nl_pos, const uint64_t *__restrict__ d_j0, uint64_t R, int k,
+                          uint64_t *__restrict__ rec_start, uint32_t *__restrict__ rec_len,
+                          uint32_t *__restrict__ occ_count, uint32_t *__restrict__ word_count,
+                          unsigned long long *__restrict__ n_valid) {
+    const uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= R) return;
+    const uint64_t j0 = *d_j0;
+    const uint64_t s0 = nl_pos[j0 + 2 * r] + 1;
+    const uint64_t s1 = nl_pos[j0 + 2 * r + 1];
+    const uint64_t len64 = s1 - s0;
+    const uint32_t len = len64 > 0xFFFFFFF0ull ? 0xFFFFFFF0u : (uint32_t)len64;
+    rec_start[r] = s0;
+    rec_len[r] = len;
+    const bool ok = len >= (uint32_t)k;
+    const uint64_t bal = __ballot(ok);
+    if ((threadIdx.x & 63) == __ffsll((long long)__ballot(1)) - 1 && bal) atomicAdd(n_valid, (unsigned long long)__popcll(bal));
+    occ_count[r] = ok ? len - k + 1 : 0;
+    word_count[r] = ok ? (len + 31) / 32 : 0;
+}
+
+// j0 = number of newlines before the first '>'; R = complete records after it.
+__global__ void k_locate(const uint64_t *__restrict__ nl_pos, const uint64_t *__restrict__ d_nnl,
+                         const unsigned long long *__restrict__ first_gt, uint64_t *__restrict__ out /*j0,R*/) {
+    const uint64_t nnl = *d_nnl;
+    const unsigned long long gt = *first_gt;
+    uint64_t lo = 0, hi = nnl;
+    while (lo < hi) {
+        const uint64_t mid = (lo + hi) / 2;
+        if (nl_pos[mid] < gt) lo = mid + 1; else hi = mid;
+    }
+    out[0] = lo;
+    out[1] = (gt == ~0ull) ? 0 : (nnl - lo) / 2;
+}
+
+// ---------------------------------------------------------------------------
+// 2-bit packing: one thread per 32-base word, first base in the top bits.
+
+__device__ __forceinline__ uint64_t upper_index(const uint64_t *__restrict__ a, uint64_t n, uint64_t v) {
+    // last i with a[i] <= v (a non-decreasing, a[0] = 0 <= v)
+    uint64_t lo = 0, hi = n;
+    while (hi - lo > 1) {
+        const uint64_t mid = (lo + hi) / 2;
+        if (a[mid] <= v) lo = mid; else hi = mid;
+    }
+    return lo;
+}
+
+__global__ void k_pack(const uint8_t *__restrict__ buf, const uint64_t *__restrict__ rec_start,
+                       const uint32_t *__restrict__ rec_len, const uint64_t *__restrict__ word_base, uint64_t R,
+                       const uint64_t *__restrict__ d_words, uint64_t *__restrict__ packed) {
+    const uint64_t W = *d_words;
+    for (uint64_t w = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; w < W; w += (uint64_t)gridDim.x * blockDim.x) {
+        const uint64_t r = upper_index(word_base, R, w);
+        const uint64_t j = w - word_base[r];
+        const uint32_t len = rec_len[r];
+        const uint8_t *s = buf + rec_start[r] + 32 * j;
+        const int nb = (int)min<uint64_t>(32, len - 32 * j);
+        uint64_t word = 0;
+        for (int i = 0; i < nb; i++) word |= (uint64_t)base_code(s[i]) << (62 - 2 * i);
+        packed[w] = word;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// k-mer occurrences.  One thread per packed word rolls over the (up to) 32
+// k-mers that start in it.  For each occurrence it forms the canonical k-mer
+// X = max(x, rc x) and the 8-bit edge mask of kmerGraph.cpp:249-299 in closed
+// form:  "x followed by b" sets slot b if x is X else 4+b, and "rc x followed by
+// comp(prev)" sets slot 4+comp if x is X else comp.
+
+struct OccCtx {
+    const uint64_t *packed;
+    const uint64_t *word_base;
+    const uint64_t *occ_base;
+    const uint32_t *rec_len;
+    uint64_t R;
+    int k;
+    Mixer mx;
+};
+
+// Calls f(part, rec, occ) for each occurrence starting in word w.
+template <typename F>
+__device__ __forceinline__ void for_each_occ(const OccCtx &c, uint64_t w, F &&f) {
+    const uint64_t r = upper_index(c.word_base, c.R, w);
+    const uint64_t j = w - c.word_base[r];
+    const uint32_t len = c.rec_len[r];
+    const int k = c.k;
+    const uint32_t nk = len - k + 1;
+    const uint32_t p0 = (uint32_t)(32 * j);
+    if (p0 >= nk) return;
+    const uint32_t pend = min<uint32_t>(nk, p0 + 32);
+    const uint64_t wlast = c.word_base[r] + (len + 31) / 32;  // one past the read's last word
+    const uint64_t w0 = c.packed[w];
+    const uint64_t w1 = (w + 1 < wlast) ? c.packed[w + 1] : 0;
+    const uint32_t prev_base = (j > 0) ? (uint32_t)(c.packed[w - 1] & 3) : 0;
+    const uint64_t km = kmask(k);
+    // base at offset i (0..63) from p0
+    auto base_at = [&](int i) -> uint32_t {
+        return (uint32_t)((i < 32 ? (w0 >> (62 - 2 * i)) : (w1 >> (62 - 2 * (i - 32)))) & 3);
+    };
+    // first k-mer: bases [0, k)
+    uint64_t x;
+    {
+        // top 2k bits of the 128-bit w0:w1
+        x = (w0 >> (64 - 2 * k));
+    }
+    uint64_t rcx = revcomp(x, k);
+    const uint64_t obase = c.occ_base[r];
+    const int hb = 2 * k - kPartBits;
+    const uint64_t lowm = (1ull << hb) - 1;
+    for (uint32_t p = p0; p < pend; p++) {
+        const int i = (int)(p - p0);
+        if (i > 0) {
+            const uint32_t b = base_at(i + k - 1);
+            x = ((x << 2) | b) & km;
+            rcx = (rcx >> 2) | ((uint64_t)(3 - b) << (2 * k - 2));
+        }
+        const bool fwd = x > rcx;
+        const uint64_t X = fwd ? x : rcx;
+        uint32_t m = 0;
+        if (p + 1 < nk) {
+            const uint32_t b = base_at(i + k);
+            m |= 1u << (fwd ? b : 4 + b);
+        }
+        if (p > 0) {
+            const uint32_t a = 3 - (i > 0 ? base_at(i - 1) : prev_base);
+            m |= 1u << (fwd ? 4 + a : a);
+        }
+        const uint64_t h = mix_fwd(c.mx, X);
+        f((uint32_t)(h >> hb), ((h & lowm) << 8) | m, obase + p);
+    }
+}
+
+constexpr int kOccThreads = 512;
+
+// Pass 1: bucket sizes per (partition, per-XCD sub-bucket).
+__global__ void __launch_bounds__(kOccThreads) k_occ_hist(OccCtx c, const uint64_t *__restrict__ d_words,
+                                                           uint32_t *__restrict__ hist) {
+    __shared__ uint32_t lh[kParts];
+    for (uint32_t i = threadIdx.x; i < kParts; i += blockDim.x) lh[i] = 0;
+    __syncthreads();
+    const uint64_t W = *d_words;
+    const uint64_t per = (W + gridDim.x - 1) / gridDim.x;
+    const uint64_t wb = per * blockIdx.x, we = min<uint64_t>(W, wb + per);
+    for (uint64_t w = wb + threadIdx.x; w < we; w += blockDim.x)
+        for_each_occ(c, w, [&](uint32_t part, uint64_t, uint64_t) { atomicAdd(&lh[part], 1u); });
+    __syncthreads();
+    const uint32_t vx = blockIdx.x % kVx;
+    for (uint32_t i = threadIdx.x; i < kParts; i += blockDim.x)
+        if (lh[i]) atomicAdd(&hist[i * kVx + vx], lh[i]);
+}
+
+// Pass 2: same word range per block; reserve space once per bucket with one
+// global atomic, then write the records (and first-occurrence indices) through
+// LDS cursors.  Positions are 32-bit: one build holds fewer than 2^32 k-mers.
+template <bool WITH_OCC>
+__global__ void __launch_bounds__(kOccThreads) k_occ_scatter(OccCtx c, const uint64_t *__restrict__ d_words,
+                                                              uint32_t *__restrict__ cursor,
+                                                              uint64_t *__restrict__ rec_out,
+                                                              uint32_t *__restrict__ occ_out) {
+    __shared__ uint32_t lh[kParts];
+    for (uint32_t i = threadIdx.x; i < kParts; i += blockDim.x) lh[i] = 0;
+    __syncthreads();
+    const uint64_t W = *d_words;
+    const uint64_t per = (W + gridDim.x - 1) / gridDim.x;
+    const uint64_t wb = per * blockIdx.x, we = min<uint64_t>(W, wb + per);
+    for (uint64_t w = wb + threadIdx.x; w < we; w += blockDim.x)
+        for_each_occ(c, w, [&](uint32_t part, uint64_t, uint64_t) { atomicAdd(&lh[part], 1u); });
+    __syncthreads();
+    const uint32_t vx = blockIdx.x % kVx;
+    for (uint32_t i = threadIdx.x; i < kParts; i += blockDim.x) {
+        const uint32_t cnt = lh[i];
+        if (cnt) lh[i] = atomicAdd(&cursor[i * kVx + vx], cnt);
+    }
+    __syncthreads();
+    for (uint64_t w = wb + threadIdx.x; w < we; w += blockDim.x)
+        for_each_occ(c, w, [&](uint32_t part, uint64_t rec, uint64_t occ) {
+            const uint32_t pos = atomicAdd(&lh[part], 1u);
+            rec_out[pos] = rec;
+            if (WITH_OCC) occ_out[pos] = (uint32_t)occ;
+        });
+}
+
+// ---------------------------------------------------------------------------
+// Per-partition counting.  One workgroup owns one partition: its records go
+// into an open-addressing table in LDS (linear probing, key inserted by CAS),
+// the 8 edge counters of a slot live in one 64-bit word as saturating bytes
+// (min(count, 255) of graph.cpp:1074-1078), and the first-occurrence index is an
+// atomicMin.  A partition whose distinct keys overflow the LDS table is redone
+// by the global-memory variant of the same kernel.
+
+constexpr int kTabBits = 12;
+constexpr uint32_t kTab = 1u << kTabBits;
+constexpr int kCountThreads = 512;
+constexpr uint64_t kEmpty = ~0ull;
+
+struct CountArgs {
+    const uint64_t *rec;
+    const uint32_t *occ;
+    const uint32_t *bucket;   // exclusive prefix over (partition, sub-bucket), kParts*kVx+1 entries
+    int k;
+    int c;
+    Mixer mx;
+    uint64_t *surv_key;
+    uint64_t *surv_mult;
+    uint32_t *surv_first;
+    unsigned long long *counters;  // [0]=survivors [1]=distinct [2]=overflow partitions
+    uint32_t *overflow_list;
+    uint32_t *bitmap;              // first-occurrence bitmap (reference order), or null
+};
+
+__device__ __forceinline__ uint32_t slot_hash(uint64_t key, uint32_t mask) {
+    return (uint32_t)((key * 0x9E3779B97F4A7C15ull) >> 40) & mask;
+}
+
+// Inserts / updates one record; returns false if the table is full.
+template <bool WITH_OCC>
+__device__ __forceinline__ bool table_add(uint64_t *tkey, uint64_t *tcnt, uint32_t *tfo, uint32_t mask,
+                                          uint64_t rec, uint32_t occ) {
+    const uint64_t key = rec >> 8;
+    const uint32_t m = (uint32_t)(rec & 0xFF);
+    uint32_t s = slot_hash(key, mask);
+    for (uint32_t probe = 0; probe <= mask; probe++) {
+        uint64_t cur = tkey[s];
+        if (cur == kEmpty) {
+            const uint64_t prev = atomicCAS((unsigned long long *)&tkey[s], (unsigned long long)kEmpty,
+                                            (unsigned long long)key);
+            cur = (prev == kEmpty) ? key : prev;
+        }
+        if (cur == key) {
+            if (m) {
+                const uint64_t inc = mask_to_bytes(m);
+                uint64_t old = tcnt[s];
+                while (true) {
+                    const uint64_t nv = sat_add_bytes(old, inc);
+                    if (nv == old) break;
+                    const uint64_t got = atomicCAS((unsigned long long *)&tcnt[s], (unsigned long long)old,
+                                                   (unsigned long long)nv);
+                    if (got == old) break;
+                    old = got;
+                }
+            }
+            if (WITH_OCC) atomicMin(&tfo[s], occ);
+            return true;
+        }
+        s = (s + 1) & mask;
+    }
+    return false;
+}
+
+// Cutoff of graph.cpp:1100-1112 on one slot: multiplicities below c become 0.
+__device__ __forceinline__ uint64_t apply_cutoff(uint64_t cnt, int c) {
+    uint64_t out = 0;
+#pragma unroll
+    for (int t = 0; t < 8; t++) {
+        const uint32_t v = (uint32_t)((cnt >> (8 * t)) & 0xFF);
+        if ((int)v >= c) out |= (uint64_t)v << (8 * t);
+    }
+    return out;
+}
+
+template <bool WITH_OCC>
+__device__ void table_emit(const uint64_t *tkey, const uint64_t *tcnt, const uint32_t *tfo, uint32_t n_slots,
+                           uint32_t part, const CountArgs &a) {
+    const int hb = 2 * a.k - kPartBits;
+    const int lane = threadIdx.x & 63;
+    uint32_t ndist = 0;
+    for (uint32_t s0 = 0; s0 < n_slots; s0 += blockDim.x) {
+        const uint32_t s = s0 + threadIdx.x;
+        bool occupied = false, surv = false;
+        uint64_t key = 0, mult = 0;
+        uint32_t fo = 0;
+        if (s < n_slots) {
+            key = tkey[s];
+            occupied = key != kEmpty;
+            if (occupied) {
+                mult = apply_cutoff(tcnt[s], a.c);
+                surv = mult != 0;
+                if (WITH_OCC) fo = tfo[s];
+            }
+        }
+        ndist += occupied;
+        if (WITH_OCC && occupied && a.bitmap) atomicOr(&a.bitmap[fo >> 5], 1u << (fo & 31));
+        const uint64_t bal = __ballot(surv);
+        if (bal) {
+            unsigned long long base = 0;
+            if (lane == 0) base = atomicAdd(&a.counters[0], (unsigned long long)__popcll(bal));
+            base = __shfl(base, 0, 64);
+            if (surv) {
+                const uint64_t below = bal & ((lane == 0) ? 0ull : (~0ull >> (64 - lane)));
+                const uint64_t i = base + __popcll(below);
+                a.surv_key[i] = mix_inv(a.mx, ((uint64_t)part << hb) | key);
+                a.surv_mult[i] = mult;
+                if (WITH_OCC) a.surv_first[i] = fo;
+            }
+        }
+    }
+    // distinct count
+    for (int d = 32; d > 0; d >>= 1) ndist += __shfl_xor(ndist, d, 64);
+    if (lane == 0 && ndist) atomicAdd(&a.counters[1], (unsigned long long)ndist);
+}
+
+template <bool WITH_OCC>
+__global__ void __launch_bounds__(kCountThreads) k_count_lds(CountArgs a) {
+    __shared__ uint64_t tkey[kTab];
+    __shared__ uint64_t tcnt[kTab];
+    __shared__ uint32_t tfo[WITH_OCC ? kTab : 1];
+    __shared__ int overflow;
+    const uint32_t part = blockIdx.x;
+    for (uint32_t i = threadIdx.x; i < kTab; i += blockDim.x) {
+        tkey[i] = kEmpty;
+        tcnt[i] = 0;
+        if (WITH_OCC) tfo[i] = ~0u;
+    }
+    if (threadIdx.x == 0) overflow = 0;
+    __syncthreads();
+    const uint32_t b = a.bucket[part * kVx], e = a.bucket[part * kVx + kVx];
+    bool ok = true;
+    for (uint32_t i = b + threadIdx.x; i < e && ok; i += blockDim.x)
+        ok = table_add<WITH_OCC>(tkey, tcnt, tfo, kTab - 1, a.rec[i], WITH_OCC ? a.occ[i] : 0);
+    if (!ok) overflow = 1;
+    __syncthreads();
+    if (overflow) {
+        if (threadIdx.x == 0) {
+            const unsigned long long j = atomicAdd(&a.counters[2], 1ull);
+            a.overflow_list[j] = part;
+        }
+        return;
+    }
+    table_emit<WITH_OCC>(tkey, tcnt, tfo, kTab, part, a);
+}
+
+// Global-memory fallback for partitions with more distinct keys than the LDS
+// table holds.  Workgroup j takes overflow partition overflow_list[j] and a table
+// of tab_size[j] slots at tab_off[j].
+template <bool WITH_OCC>
+__global__ void __launch_bounds__(kCountThreads) k_count_global(CountArgs a, const uint64_t *__restrict__ tab_off,
+                                                                 const uint32_t *__restrict__ tab_bits,
+                                                                 uint64_t *gkey, uint64_t *gcnt, uint32_t *gfo) {
+    const uint32_t part = a.overflow_list[blockIdx.x];
+    const uint64_t off = tab_off[blockIdx.x];
+    const uint32_t n_slots = 1u << tab_bits[blockIdx.x];
+    uint64_t *tkey = gkey + off;
+    uint64_t *tcnt = gcnt + off;
+    uint32_t *tfo = WITH_OCC ? gfo + off : nullptr;
+    for (uint32_t i = threadIdx.x; i < n_slots; i += blockDim.x) {
+        tkey[i] = kEmpty;
+        tcnt[i] = 0;
+        if (WITH_OCC) tfo[i] = ~0u;
+    }
+    __syncthreads();
+    const uint32_t b = a.bucket[part * kVx], e = a.bucket[part * kVx + kVx];
+    for (uint32_t i = b + threadIdx.x; i < e; i += blockDim.x)
+        table_add<WITH_OCC>(tkey, tcnt, tfo, n_slots - 1, a.rec[i], WITH_OCC ? a.occ[i] : 0);
+    __threadfence_block();
+    __syncthreads();
+    table_emit<WITH_OCC>(tkey, tcnt, tfo, n_slots, part, a);
+}
+
+// ---------------------------------------------------------------------------
+// First-occurrence ranks (reference order): bit f of the bitmap is set for the
+// first occurrence f of every distinct k-mer, so the number of distinct k-mers
+// first seen before a survivor is a prefix popcount.
+
+__global__ void k_popc_words(const uint32_t *__restrict__ bm, uint64_t nw, uint32_t *__restrict__ cnt,
+                             unsigned long long *__restrict__ last_bit) {
+    const uint64_t w = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (w >= nw) return;
+    const uint32_t v = bm[w];
+    cnt[w] = __popc(v);
+    if (v) atomicMax(last_bit, (unsigned long long)(w * 32 + 31 - __clz(v)));
+}
+
+__global__ void k_surv_rank(const uint32_t *__restrict__ first, const uint32_t *__restrict__ bm,
+                            const uint64_t *__restrict__ prefix, const unsigned long long *__restrict__ d_n,
+                            uint64_t *__restrict__ rank) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= *d_n) return;
+    const uint32_t f = first[i];
+    rank[i] = prefix[f >> 5] + __popc(bm[f >> 5] & ((1u << (f & 31)) - 1u));
+}
+
+// ---------------------------------------------------------------------------
+// Pipeline::build
+
+DevBuf::~DevBuf() {
+    if (p) hipFree(p);
+}
+
+hipError_t DevBuf::reserve(size_t bytes) {
+    if (bytes <= cap && p) return hipSuccess;
+    if (p) hipFree(p);
+    p = nullptr;
+    cap = 0;
+    size_t want = bytes + bytes / 8 + 256;
+    hipError_t e = hipMalloc(&p, want);
+    if (e != hipSuccess) { p = nullptr; return e; }
+    cap = want;
+    return hipSuccess;
+}
+
+static uint64_t inv_odd(uint64_t a) {
+    uint64_t x = a;  // Newton iteration for the inverse mod 2^64
+    for (int i = 0; i < 6; i++) x *= 2 - a * x;
+    return x;
+}
+
+Pipeline::Pipeline(int device, int k, int c, bool reference_order) : dev_(device), k_(k), c_(c), ref_order_(reference_order) {
+    hipSetDevice(dev_);
+    hipStreamCreateWithFlags(&st_, hipStreamNonBlocking);
+    hipHostMalloc((void **)&h_small_, 64 * sizeof(uint64_t), hipHostMallocDefault);
+    mx_.bits = 2 * k;
+    mx_.m = kmask(k);
+    mx_.a1 = 0x9E3779B97F4A7C15ull | 1;
+    mx_.a2 = 0xC2B2AE3D27D4EB4Full | 1;
+    mx_.i1 = inv_odd(mx_.a1);
+    mx_.i2 = inv_odd(mx_.a2);
+    mx_.s = k;  // half of the 2k bits
+}
+
+Pipeline::~Pipeline() {
+    hipSetDevice(dev_);
+    if (h_small_) hipHostFree(h_small_);
+    if (st_) hipStreamDestroy(st_);
+}
+
+int Pipeline::fail(int code, const char *msg) {
+    err_ = msg;
+    return code;
+}
+
+#define SWAP_CHECK(x)                                                        \
+    do {                                                                     \
+        hipError_t e_ = (x);                                                 \
+        if (e_ != hipSuccess) return fail(-5, hipGetErrorString(e_));        \
+    } while (0)
+
+int Pipeline::build(const uint8_t *d_in, uint64_t n) {
+    SWAP_CHECK(hipSetDevice(dev_));
+    n_reads = n_occ = n_distinct = n_surv = n_kept = n_overflow = n_special = 0;
+    if (n == 0) return fail(-2, "empty input");
+    const uint8_t *buf = d_in;
+    if ((reinterpret_cast<uintptr_t>(d_in) & 15) != 0) {
+        SWAP_CHECK(fasta_.reserve(n + 16));
+        SWAP_CHECK(hipMemcpyAsync(fasta_.p, d_in, n, hipMemcpyDeviceToDevice, st_));
+        buf = fasta_.as<uint8_t>();
+    }
+    uint64_t *hs = h_small_;
+    // --- newline scan
+    const uint64_t tiles = (n + kTileBytes - 1) / kTileBytes;
+    SWAP_CHECK(tile_nl_.reserve(tiles * 4));
+    SWAP_CHECK(tile_off_.reserve(tiles * 8));
+    const uint64_t scan_words = scan_scratch_words(std::max<uint64_t>(tiles, 1)) + 16;
+    SWAP_CHECK(small_.reserve(64 * 8));
+    uint64_t *d_small = small_.as<uint64_t>();  // [0]=first_gt [1]=nnl [2]=j0 [3]=R [4]=N [5]=W [8..]=counters
+    SWAP_CHECK(hipMemsetAsync(d_small, 0xFF, 8, st_));
+    SWAP_CHECK(hipMemsetAsync(d_small + 1, 0, 63 * 8, st_));
+    hipLaunchKernelGGL(k_tile_count, dim3((unsigned)tiles), dim3(kTileThreads), 0, st_, buf, n,
+                       tile_nl_.as<uint32_t>(), (unsigned long long *)d_small);
+    SWAP_CHECK(scan_scratch_.reserve(scan_words * 8));
+    scan_excl<uint32_t>(tile_nl_.as<uint32_t>(), tile_off_.as<uint64_t>(), tiles, scan_scratch_.as<uint64_t>(),
+                        d_small + 1, st_);
+    SWAP_CHECK(hipMemcpyAsync(hs, d_small, 2 * 8, hipMemcpyDeviceToHost, st_));
+    SWAP_CHECK(hipStreamSynchronize(st_));
+    if (hs[0] == ~0ull) return fail(-3, "no '>' in the input");
+    const uint64_t nnl = hs[1];
+    SWAP_CHECK(nl_pos_.reserve(std::max<uint64_t>(nnl, 1) * 8));
+    hipLaunchKernelGGL(k_tile_emit, dim3((unsigned)tiles), dim3(kTileThreads), 0, st_, buf, n,
+                       tile_off_.as<uint64_t>(), nl_pos_.as<uint64_t>());
+    hipLaunchKernelGGL(k_locate, dim3(1), dim3(1), 0, st_, nl_pos_.as<uint64_t>(), d_small + 1,
+                       (const unsigned long long *)d_small, d_small + 2);
+    SWAP_CHECK(hipMemcpyAsync(hs + 2, d_small + 2, 2 * 8, hipMemcpyDeviceToHost, st_));
+    SWAP_CHECK(hipStreamSynchronize(st_));
+    const uint64_t R = hs[3];
+    if (R == 0) {
+        return 0;  // no complete record: empty graph
+    }
+    // --- records
+    SWAP_CHECK(rec_start_.reserve(R * 8));
+    SWAP_CHECK(rec_len_.reserve(R * 4));
+    SWAP_CHECK(occ_count_.reserve(R * 4));
+    SWAP_CHECK(word_count_.reserve(R * 4));
+    SWAP_CHECK(occ_base_.reserve((R + 1) * 8));
+    SWAP_CHECK(word_base_.reserve((R + 1) * 8));
+    SWAP_CHECK(scan_scratch_.reserve((scan_scratch_words(R) + 16) * 8));
+    hipLaunchKernelGGL(k_records, dim3((unsigned)((R + 255) / 256)), dim3(256), 0, st_, nl_pos_.as<uint64_t>(),
+                       d_small + 2, R, k_, rec_start_.as<uint64_t>(), rec_len_.as<uint32_t>(),
+                       occ_count_.as<uint32_t>(), word_count_.as<uint32_t>(), (unsigned long long *)(d_small + 7));
+    scan_excl<uint32_t>(occ_count_.as<uint32_t>(), occ_base_.as<uint64_t>(), R, scan_scratch_.as<uint64_t>(),
+                        d_small + 4, st_);
+    scan_excl<uint32_t>(word_count_.as<uint32_t>(), word_base_.as<uint64_t>(), R, scan_scratch_.as<uint64_t>(),
+                        d_small + 5, st_);
+    SWAP_CHECK(hipMemcpyAsync(hs + 4, d_small + 4, 4 * 8, hipMemcpyDeviceToHost, st_));
+    SWAP_CHECK(hipStreamSynchronize(st_));
+    const uint64_t N = hs[4], W = hs[5];
+    n_reads = hs[7];
+    n_occ = N;
+    if (N >= 0xFFFFFFF0ull) return fail(-4, "more than 2^32 k-mers in one build");
+    if (N == 0) return 0;
+    // --- pack
+    SWAP_CHECK(packed_.reserve((W + 1) * 8));
+    {
+        const unsigned grid = (unsigned)std::min<uint64_t>((W + 255) / 256, 65535);
+        hipLaunchKernelGGL(k_pack, dim3(grid), dim3(256), 0, st_, buf, rec_start_.as<uint64_t>(),
+                           rec_len_.as<uint32_t>(), word_base_.as<uint64_t>(), R, d_small + 5,
+                           packed_.as<uint64_t>());
+    }
+    // --- partition
+    OccCtx oc;
+    oc.packed = packed_.as<uint64_t>();
+    oc.word_base = word_base_.as<uint64_t>();
+    oc.occ_base = occ_base_.as<uint64_t>();
+    oc.rec_len = rec_len_.as<uint32_t>();
+    oc.R = R;
+    oc.k = k_;
+    oc.mx = mx_;
+    const uint32_t nb = kParts * kVx;
+    SWAP_CHECK(hist_.reserve((nb + 1) * 4));
+    SWAP_CHECK(bucket_.reserve((nb + 1) * 8));
+    SWAP_CHECK(cursor_.reserve((nb + 1) * 4));
+    SWAP_CHECK(hipMemsetAsync(hist_.p, 0, (nb + 1) * 4, st_));
+    int ncu = 0;
+    hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev_);
+    const unsigned occ_grid = (unsigned)std::max(8, ncu * 2);
+    hipLaunchKernelGGL(k_occ_hist, dim3(occ_grid), dim3(kOccThreads), 0, st_, oc, d_small + 5,
+                       hist_.as<uint32_t>());
+    SWAP_CHECK(scan_scratch_.reserve((scan_scratch_words(nb + 1) + 16) * 8));
+    scan_excl<uint32_t>(hist_.as<uint32_t>(), bucket_.as<uint64_t>(), nb + 1, scan_scratch_.as<uint64_t>(),
+                        d_small + 6, st_);
+    // 32-bit copies: bucket starts and the running cursors
+    {
+        // bucket_ holds u64 prefix; convert into cursor_ (u32) and a u32 copy for the count kernel
+        // (N < 2^32 so the values fit).
+    }
+    SWAP_CHECK(rec_.reserve(N * 8));
+    if (ref_order_) SWAP_CHECK(occ_.reserve(N * 4));
+    hipLaunchKernelGGL(k_u64_to_u32, dim3((nb + 1 + 255) / 256), dim3(256), 0, st_, bucket_.as<uint64_t>(), nb + 1,
+                       cursor_.as<uint32_t>());
+    SWAP_CHECK(hist_.reserve((nb + 1) * 4));
+    SWAP_CHECK(hipMemcpyAsync(hist_.p, cursor_.p, (nb + 1) * 4, hipMemcpyDeviceToDevice, st_));  // hist_ := bucket starts (u32)
+    if (ref_order_)
+        hipLaunchKernelGGL(k_occ_scatter<true>, dim3(occ_grid), dim3(kOccThreads), 0, st_, oc, d_small + 5,
+                           cursor_.as<uint32_t>(), rec_.as<uint64_t>(), occ_.as<uint32_t>());
+    else
+        hipLaunchKernelGGL(k_occ_scatter<false>, dim3(occ_grid), dim3(kOccThreads), 0, st_, oc, d_small + 5,
+                           cursor_.as<uint32_t>(), rec_.as<uint64_t>(), (uint32_t *)nullptr);
+    // --- count + cutoff
+    // survivors <= distinct <= N; reserve N to avoid a sync (grow-only buffers)
+    SWAP_CHECK(surv_key_.reserve(N * 8));
+    SWAP_CHECK(surv_mult_.reserve(N * 8));
+    if (ref_order_) {
+        SWAP_CHECK(surv_first_.reserve(N * 4));
+        SWAP_CHECK(bitmap_.reserve(((N + 31) / 32 + 1) * 4));
+        SWAP_CHECK(hipMemsetAsync(bitmap_.p, 0, ((N + 31) / 32 + 1) * 4, st_));
+    }
+    SWAP_CHECK(overflow_.reserve(kParts * 4));
+    CountArgs ca;
+    ca.rec = rec_.as<uint64_t>();
+    ca.occ = ref_order_ ? occ_.as<uint32_t>() : nullptr;
+    ca.bucket = hist_.as<uint32_t>();
+    ca.k = k_;
+    ca.c = c_;
+    ca.mx = mx_;
+    ca.surv_key = surv_key_.as<uint64_t>();
+    ca.surv_mult = surv_mult_.as<uint64_t>();
+    ca.surv_first = ref_order_ ? surv_first_.as<uint32_t>() : nullptr;
+    ca.counters = (unsigned long long *)(d_small + 8);
+    ca.overflow_list = overflow_.as<uint32_t>();
+    ca.bitmap = ref_order_ ? bitmap_.as<uint32_t>() : nullptr;
+    if (ref_order_)
+        hipLaunchKernelGGL(k_count_lds<true>, dim3(kParts), dim3(kCountThreads), 0, st_, ca);
+    else
+        hipLaunchKernelGGL(k_count_lds<false>, dim3(kParts), dim3(kCountThreads), 0, st_, ca);
+    SWAP_CHECK(hipGetLastError());
+    SWAP_CHECK(hipMemcpyAsync(hs + 8, d_small + 8, 3 * 8, hipMemcpyDeviceToHost, st_));
+    SWAP_CHECK(hipStreamSynchronize(st_));
+    n_overflow = hs[10];
+    if (n_overflow) {
+        // size a global table per overflowing partition from its record count
+        std::vector<uint32_t> list(n_overflow), starts(nb + 1);
+        SWAP_CHECK(hipMemcpy(list.data(), overflow_.p, n_overflow * 4, hipMemcpyDeviceToHost));
+        SWAP_CHECK(hipMemcpy(starts.data(), hist_.p, (nb + 1) * 4, hipMemcpyDeviceToHost));
+        std::vector<uint64_t> off(n_overflow);
+        std::vector<uint32_t> bits(n_overflow);
+        uint64_t tot = 0;
+        for (uint64_t j = 0; j < n_overflow; j++) {
+            const uint32_t p = list[j];
+            const uint64_t cnt = (uint64_t)starts[p * kVx + kVx] - starts[p * kVx];
+            uint32_t b = kTabBits + 1;
+            while ((1ull << b) < 2 * cnt) b++;
+            bits[j] = b;
+            off[j] = tot;
+            tot += 1ull << b;
+        }
+        SWAP_CHECK(gtab_key_.reserve(tot * 8));
+        SWAP_CHECK(gtab_cnt_.reserve(tot * 8));
+        if (ref_order_) SWAP_CHECK(gtab_fo_.reserve(tot * 4));
+        DevBuf d_off, d_bits;
+        SWAP_CHECK(d_off.reserve(n_overflow * 8));
+        SWAP_CHECK(d_bits.reserve(n_overflow * 4));
+        SWAP_CHECK(hipMemcpy(d_off.p, off.data(), n_overflow * 8, hipMemcpyHostToDevice));
+        SWAP_CHECK(hipMemcpy(d_bits.p, bits.data(), n_overflow * 4, hipMemcpyHostToDevice));
+        if (ref_order_)
+            hipLaunchKernelGGL(k_count_global<true>, dim3((unsigned)n_overflow), dim3(kCountThreads), 0, st_, ca,
+                               d_off.as<uint64_t>(), d_bits.as<uint32_t>(), gtab_key_.as<uint64_t>(),
+                               gtab_cnt_.as<uint64_t>(), gtab_fo_.as<uint32_t>());
+        else
+            hipLaunchKernelGGL(k_count_global<false>, dim3((unsigned)n_overflow), dim3(kCountThreads), 0, st_, ca,
+                               d_off.as<uint64_t>(), d_bits.as<uint32_t>(), gtab_key_.as<uint64_t>(),
+                               gtab_cnt_.as<uint64_t>(), (uint32_t *)nullptr);
+        SWAP_CHECK(hipGetLastError());
+        SWAP_CHECK(hipMemcpyAsync(hs + 8, d_small + 8, 3 * 8, hipMemcpyDeviceToHost, st_));
+        SWAP_CHECK(hipStreamSynchronize(st_));
+    }
+    n_surv = hs[8];
+    n_distinct = hs[9];
+    if (ref_order_ && n_surv) {
+        const uint64_t nw = (N + 31) / 32;
+        SWAP_CHECK(popc_.reserve(nw * 4));
+        SWAP_CHECK(surv_rank_.reserve(n_surv * 8));
+        SWAP_CHECK(hipMemsetAsync(d_small + 12, 0, 8, st_));
+        hipLaunchKernelGGL(k_popc_words, dim3((unsigned)((nw + 255) / 256)), dim3(256), 0, st_,
+                           bitmap_.as<uint32_t>(), nw, popc_.as<uint32_t>(), (unsigned long long *)(d_small + 12));
+        // the bitmap prefix reuses word_base_ (the pack and scatter are done)
+        SWAP_CHECK(word_base_.reserve(nw * 8 + 8));
+        SWAP_CHECK(scan_scratch_.reserve((scan_scratch_words(nw) + 16) * 8));
+        scan_excl<uint32_t>(popc_.as<uint32_t>(), word_base_.as<uint64_t>(), nw, scan_scratch_.as<uint64_t>(),
+                            d_small + 13, st_);
+        hipLaunchKernelGGL(k_surv_rank, dim3((unsigned)((n_surv + 255) / 256)), dim3(256), 0, st_,
+                           surv_first_.as<uint32_t>(), bitmap_.as<uint32_t>(), word_base_.as<uint64_t>(),
+                           (const unsigned long long *)(d_small + 8), surv_rank_.as<uint64_t>());
+        SWAP_CHECK(hipMemcpyAsync(hs + 12, d_small + 12, 8, hipMemcpyDeviceToHost, st_));
+        SWAP_CHECK(hipStreamSynchronize(st_));
+        last_first_ = hs[12];
+    }
+    n_reads = hs[7];
+    return 0;
+}
+
+int Pipeline::export_kmer_graph(HostKmerGraph &out) {
+    SWAP_CHECK(hipSetDevice(dev_));
+    out.key.resize(n_surv);
+    out.mult.resize(n_surv);
+    out.first.clear();
+    out.rank.clear();
+    out.distinct = n_distinct;
+    // the table sees one more operator[] after the last new key when a later
+    // input k-mer has an edge; taken as "any k-mer after the last first occurrence"
+    out.tail_lookup = ref_order_ && n_occ > last_first_ + 1;
+    if (n_surv) {
+        SWAP_CHECK(hipMemcpyAsync(out.key.data(), surv_key_.p, n_surv * 8, hipMemcpyDeviceToHost, st_));
+        SWAP_CHECK(hipMemcpyAsync(out.mult.data(), surv_mult_.p, n_surv * 8, hipMemcpyDeviceToHost, st_));
+        if (ref_order_) {
+            out.first.resize(n_surv);
+            out.rank.resize(n_surv);
+            SWAP_CHECK(hipMemcpyAsync(out.first.data(), surv_first_.p, n_surv * 4, hipMemcpyDeviceToHost, st_));
+            SWAP_CHECK(hipMemcpyAsync(out.rank.data(), surv_rank_.p, n_surv * 8, hipMemcpyDeviceToHost, st_));
+        }
+    }
+    SWAP_CHECK(hipStreamSynchronize(st_));
+    return 0;
+}
+
+}  // namespace swapk

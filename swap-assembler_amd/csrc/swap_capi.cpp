@@ -1,0 +1,236 @@
+// C ABI of the k-mer graph path (declared in include/swap_hip.h) and the host
+// writers of the reference's output formats.
+#include "swap_hip.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "swap_device.h"
+#include "swap_host.h"
+
+struct swap_ctx {
+    swap_cfg cfg;
+    std::unique_ptr<swapk::Pipeline> pipe;
+    swapk::DevBuf input;
+    bool built = false, merged = false;
+    double build_ms = 0, merge_ms = 0;
+    std::string err;
+    // host copies, filled on demand
+    bool have_graph = false;
+    swapk::HostKmerGraph graph;     // sorted by key
+    std::vector<uint32_t> visit;    // survivor index -> merge visit rank
+    bool have_compacted = false;
+    swapk::HostCompacted comp;
+};
+
+namespace {
+
+int set_err(swap_ctx *c, int code, const std::string &msg) {
+    if (c) c->err = msg;
+    return code;
+}
+
+using clk = std::chrono::steady_clock;
+
+int ensure_graph(swap_ctx *c) {
+    if (c->have_graph) return 0;
+    if (!c->built) return set_err(c, SWAP_E_STATE, "no build");
+    swapk::HostKmerGraph g;
+    int rc = c->pipe->export_kmer_graph(g);
+    if (rc) return set_err(c, rc, c->pipe->last_error());
+    // sort by key (the device emits survivors in partition order)
+    std::vector<uint32_t> idx(g.key.size());
+    std::iota(idx.begin(), idx.end(), 0u);
+    std::sort(idx.begin(), idx.end(), [&](uint32_t a, uint32_t b) { return g.key[a] < g.key[b]; });
+    c->graph.key.resize(idx.size());
+    c->graph.mult.resize(idx.size());
+    c->graph.first.resize(g.first.empty() ? 0 : idx.size());
+    for (size_t i = 0; i < idx.size(); i++) {
+        c->graph.key[i] = g.key[idx[i]];
+        c->graph.mult[i] = g.mult[idx[i]];
+        if (!g.first.empty()) c->graph.first[i] = g.first[idx[i]];
+    }
+    if (!g.rank.empty()) {
+        c->graph.rank.resize(idx.size());
+        for (size_t i = 0; i < idx.size(); i++) c->graph.rank[i] = g.rank[idx[i]];
+    }
+    c->graph.distinct = g.distinct;
+    c->graph.tail_lookup = g.tail_lookup;
+    c->have_graph = true;
+    return 0;
+}
+
+// Output order of kmerGraph.txt: the reference's table order after the gather
+// (reference order), else ascending key.
+void kmer_graph_order(swap_ctx *c, std::vector<uint32_t> &order) {
+    order.clear();
+    if (c->graph.rank.empty()) return;
+    uint64_t buckets = 0;
+    std::vector<uint32_t> visit = swapk::visit_order(c->graph.key, c->graph.rank, c->graph.distinct,
+                                                     c->graph.tail_lookup, &buckets);
+    std::vector<uint32_t> pos(visit.size());
+    for (size_t i = 0; i < visit.size(); i++) pos[visit[i]] = (uint32_t)i;
+    order = swapk::gather_order(c->graph.key, pos, buckets);
+}
+
+}  // namespace
+
+extern "C" {
+
+int swap_create(const swap_cfg *cfg, swap_ctx **out) {
+    if (!cfg || !out) return SWAP_E_INVAL;
+    *out = nullptr;
+    if (cfg->k < 9 || cfg->k > 31 || (cfg->k % 2) == 0) return SWAP_E_INVAL;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return SWAP_E_NOGPU;
+    if (cfg->device < 0 || cfg->device >= ndev) return SWAP_E_INVAL;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, cfg->device) != hipSuccess) return SWAP_E_NOGPU;
+    if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0) return SWAP_E_NOGPU;
+    swap_ctx *c = new swap_ctx();
+    c->cfg = *cfg;
+    c->pipe.reset(new swapk::Pipeline(cfg->device, cfg->k, cfg->c, !(cfg->flags & SWAP_FLAG_HASH_ORDER)));
+    *out = c;
+    return SWAP_OK;
+}
+
+void swap_destroy(swap_ctx *c) { delete c; }
+
+int swap_build_device(swap_ctx *c, const void *d_fasta, size_t n) {
+    if (!c || (!d_fasta && n)) return SWAP_E_INVAL;
+    c->built = c->merged = c->have_graph = c->have_compacted = false;
+    if (n == 0) return set_err(c, SWAP_E_EMPTY, "empty input");
+    auto t0 = clk::now();
+    int rc = c->pipe->build(static_cast<const uint8_t *>(d_fasta), n);
+    c->build_ms = std::chrono::duration<double, std::milli>(clk::now() - t0).count();
+    if (rc) return set_err(c, rc, c->pipe->last_error());
+    c->built = true;
+    return SWAP_OK;
+}
+
+int swap_build(swap_ctx *c, const char *fasta, size_t n) {
+    if (!c || (!fasta && n)) return SWAP_E_INVAL;
+    if (n == 0) return set_err(c, SWAP_E_EMPTY, "empty input");
+    if (hipSetDevice(c->cfg.device) != hipSuccess) return set_err(c, SWAP_E_DEVICE, "hipSetDevice");
+    if (c->input.reserve(n + 16) != hipSuccess) return set_err(c, SWAP_E_DEVICE, "device allocation of the input");
+    if (hipMemcpy(c->input.p, fasta, n, hipMemcpyHostToDevice) != hipSuccess)
+        return set_err(c, SWAP_E_DEVICE, "copy of the input to the device");
+    return swap_build_device(c, c->input.p, n);
+}
+
+int swap_merge(swap_ctx *c) {
+    if (!c) return SWAP_E_INVAL;
+    if (!c->built) return set_err(c, SWAP_E_STATE, "merge before build");
+    if (c->cfg.flags & SWAP_FLAG_KMER_GRAPH) return set_err(c, SWAP_E_STATE, "context was created for -s (no merge)");
+    auto t0 = clk::now();
+    int rc = swapk::host_merge(*c->pipe, c->cfg, c->visit, c->comp);
+    c->merge_ms = std::chrono::duration<double, std::milli>(clk::now() - t0).count();
+    if (rc) return set_err(c, rc, c->pipe->last_error());
+    c->merged = true;
+    c->have_compacted = true;
+    return SWAP_OK;
+}
+
+int swap_get_stats(const swap_ctx *c, swap_stats *s) {
+    if (!c || !s) return SWAP_E_INVAL;
+    s->reads = c->pipe->n_reads;
+    s->kmers = c->pipe->n_occ;
+    s->distinct = c->pipe->n_distinct;
+    s->survivors = c->pipe->n_surv;
+    s->kept = c->pipe->n_kept;
+    s->overflow = c->pipe->n_overflow;
+    s->special = c->pipe->n_special;
+    s->build_ms = c->build_ms;
+    s->merge_ms = c->merge_ms;
+    return SWAP_OK;
+}
+
+int64_t swap_export_kmer_graph(swap_ctx *c, uint64_t *keys, uint8_t *mult, size_t cap) {
+    if (!c) return SWAP_E_INVAL;
+    int rc = ensure_graph(c);
+    if (rc) return rc;
+    const size_t n = c->graph.key.size();
+    if ((keys || mult) && cap < n) return set_err(c, SWAP_E_RANGE, "buffer too small");
+    for (size_t i = 0; i < n && (keys || mult); i++) {
+        if (keys) keys[i] = c->graph.key[i];
+        if (mult) for (int t = 0; t < 8; t++) mult[8 * i + t] = (uint8_t)(c->graph.mult[i] >> (8 * t));
+    }
+    return (int64_t)n;
+}
+
+int64_t swap_format_file(swap_ctx *c, int which, char *buf, size_t cap) {
+    if (!c) return SWAP_E_INVAL;
+    std::string out;
+    int rc = 0;
+    if (which == SWAP_FILE_KMER_GRAPH) {
+        rc = ensure_graph(c);
+        if (rc) return rc;
+        std::vector<uint32_t> order;
+        kmer_graph_order(c, order);
+        swapk::format_kmer_graph(c->cfg.k, c->graph, order, out);
+    } else if (which == SWAP_FILE_CONTIG_GRAPH || which == SWAP_FILE_NOCE) {
+        if (!c->merged) return set_err(c, SWAP_E_STATE, "no merge");
+        if (which == SWAP_FILE_CONTIG_GRAPH) swapk::format_contig_graph(c->comp, out);
+        else swapk::format_noce(c->cfg.k, c->comp, out);
+    } else {
+        return set_err(c, SWAP_E_INVAL, "unknown file");
+    }
+    if (buf) {
+        if (cap < out.size()) return set_err(c, SWAP_E_RANGE, "buffer too small");
+        std::memcpy(buf, out.data(), out.size());
+    }
+    return (int64_t)out.size();
+}
+
+int64_t swap_write_file(swap_ctx *c, int which, const char *path) {
+    if (!c || !path) return SWAP_E_INVAL;
+    int64_t n = swap_format_file(c, which, nullptr, 0);
+    if (n < 0) return n;
+    std::vector<char> b((size_t)n);
+    n = swap_format_file(c, which, b.data(), b.size());
+    if (n < 0) return n;
+    FILE *f = std::fopen(path, "wb");
+    if (!f) return set_err(c, SWAP_E_IO, std::string("cannot open ") + path);
+    const bool ok = std::fwrite(b.data(), 1, b.size(), f) == b.size();
+    std::fclose(f);
+    return ok ? n : set_err(c, SWAP_E_IO, std::string("short write to ") + path);
+}
+
+int64_t swap_table_order(const uint64_t *keys, const uint64_t *ranks, size_t n, uint64_t n_total, int extra_resize,
+                         uint32_t *order_out) {
+    if ((!keys || !ranks || !order_out) && n) return SWAP_E_INVAL;
+    std::vector<uint64_t> k(keys, keys + n), r(ranks, ranks + n);
+    for (size_t i = 0; i < n; i++)
+        if (r[i] >= n_total) return SWAP_E_INVAL;
+    std::vector<uint32_t> o = swapk::table_order(k, r, n_total, extra_resize != 0, nullptr);
+    std::copy(o.begin(), o.end(), order_out);
+    return (int64_t)n;
+}
+
+const char *swap_strerror(int code) {
+    switch (code) {
+        case SWAP_OK: return "ok";
+        case SWAP_E_INVAL: return "invalid argument";
+        case SWAP_E_EMPTY: return "empty input";
+        case SWAP_E_FORMAT: return "input has no FASTA record ('>')";
+        case SWAP_E_TOOBIG: return "too many k-mers for one build";
+        case SWAP_E_DEVICE: return "HIP runtime error";
+        case SWAP_E_NOGPU: return "no gfx950 device";
+        case SWAP_E_IO: return "I/O error";
+        case SWAP_E_STATE: return "call out of order";
+        case SWAP_E_RANGE: return "buffer too small";
+        default: return "unknown error";
+    }
+}
+
+const char *swap_last_error(const swap_ctx *c) { return c ? c->err.c_str() : ""; }
+
+}  // extern "C"

@@ -1,0 +1,74 @@
+// Shared device/host helpers for the k-mer graph path (gfx950).
+//
+// K-mer encoding follows kmerGraph.cpp:16-24 (A=0 C=1 G=2 T=3, first base in the
+// highest bits) and the canonical form is max(x, rc x) (kmerGraph.cpp:246-247).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+#define SWAP_HD __host__ __device__ __forceinline__
+
+namespace swapk {
+
+// Partition count for device-local hash partitioning: 2^kPartBits buckets keyed
+// by the top bits of a bijective mix of the canonical k-mer.
+constexpr int kPartBits = 14;
+constexpr uint32_t kParts = 1u << kPartBits;
+constexpr int kVx = 8;  // per-XCD sub-buckets (one L2 per XCD)
+
+SWAP_HD uint64_t kmask(int k) { return (k >= 32) ? ~0ull : ((1ull << (2 * k)) - 1); }
+
+// Reverse complement of a 2k-bit k-mer in O(1): complement every 2-bit group,
+// reverse the group order, shift down to the low 2k bits.
+SWAP_HD uint64_t revcomp(uint64_t x, int k) {
+    x = ~x;
+    x = ((x >> 2) & 0x3333333333333333ull) | ((x & 0x3333333333333333ull) << 2);
+    x = ((x >> 4) & 0x0F0F0F0F0F0F0F0Full) | ((x & 0x0F0F0F0F0F0F0F0Full) << 4);
+    x = ((x >> 8) & 0x00FF00FF00FF00FFull) | ((x & 0x00FF00FF00FF00FFull) << 8);
+    x = ((x >> 16) & 0x0000FFFF0000FFFFull) | ((x & 0x0000FFFF0000FFFFull) << 16);
+    x = (x >> 32) | (x << 32);
+    return x >> (64 - 2 * k);
+}
+
+// Bijective mixer on the 2k-bit k-mer space used for partitioning: odd
+// multiply mod 2^2k, xor-shift, odd multiply.  The inverse is applied to the
+// bits kept in the partition records to recover the k-mer.
+struct Mixer {
+    int bits;          // 2k
+    uint64_t m;        // 2k-bit mask
+    uint64_t a1, a2;   // odd multipliers
+    uint64_t i1, i2;   // their inverses mod 2^64 (truncated to 2k bits on use)
+    int s;             // xor-shift amount
+};
+
+SWAP_HD uint64_t mix_fwd(const Mixer &mx, uint64_t x) {
+    x = (x * mx.a1) & mx.m;
+    x ^= x >> mx.s;
+    x = (x * mx.a2) & mx.m;
+    return x;
+}
+
+SWAP_HD uint64_t mix_inv(const Mixer &mx, uint64_t y) {
+    y = (y * mx.i2) & mx.m;
+    uint64_t x = y;
+    for (int sh = mx.s; sh < mx.bits; sh += mx.s) x = y ^ (x >> mx.s);
+    x &= mx.m;
+    x = (x * mx.i1) & mx.m;
+    return x;
+}
+
+// Saturating add of per-byte increments (each byte of inc is 0 or 1).
+SWAP_HD uint64_t sat_add_bytes(uint64_t v, uint64_t inc) {
+    // bytes already at 255 drop their increment
+    const uint64_t full = v & (v >> 1) & (v >> 2) & (v >> 3) & (v >> 4) & (v >> 5) & (v >> 6) & (v >> 7) & 0x0101010101010101ull;
+    return v + (inc & ~full);
+}
+
+// Expand an 8-bit edge mask to one byte per bit.
+SWAP_HD uint64_t mask_to_bytes(uint32_t m) {
+    uint64_t r = 0;
+    for (int t = 0; t < 8; t++) r |= (uint64_t)((m >> t) & 1u) << (8 * t);
+    return r;
+}
+
+}  // namespace swapk

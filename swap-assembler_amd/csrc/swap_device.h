@@ -1,0 +1,99 @@
+// Device-side pipeline of the k-mer graph path (one sample on one GPU).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "swap_common.h"
+
+namespace swapk {
+
+// Grow-only device buffer.
+struct DevBuf {
+    void *p = nullptr;
+    size_t cap = 0;
+    ~DevBuf();
+    hipError_t reserve(size_t bytes);
+    template <typename T> T *as() const { return static_cast<T *>(p); }
+};
+
+struct Timing {
+    float ms_build = 0, ms_merge = 0;
+};
+
+// Survivors of the cutoff: one record per k-molecule (canonical k-mer), with the
+// 8 saturated edge multiplicities packed one byte per slot (slot t in byte t).
+struct HostKmerGraph {
+    std::vector<uint64_t> key;
+    std::vector<uint64_t> mult;
+    std::vector<uint32_t> first;  // first input k-mer index of the key (reference order mode)
+    std::vector<uint64_t> rank;   // number of distinct k-mers first seen before this one
+    uint64_t distinct = 0;        // D
+    bool tail_lookup = false;     // an input k-mer with edges follows the last first occurrence
+};
+
+// Compacted graph after the merge: the kept nodes, each with 8 edge slots
+// (offset/length into a base pool, multiplicity).
+struct HostCompacted {
+    std::vector<uint64_t> key;       // kept node ids
+    std::vector<uint64_t> off;       // 8 per node: offset of slot string in pool
+    std::vector<uint32_t> len;       // 8 per node
+    std::vector<uint8_t> mult;       // 8 per node
+    std::vector<char> pool;          // edge strings, bases as 'A','C','G','T'
+};
+
+class Pipeline {
+public:
+    Pipeline(int device, int k, int c, bool reference_order);
+    ~Pipeline();
+    int device() const { return dev_; }
+    hipStream_t stream() const { return st_; }
+
+    // Builds the k-mer graph from FASTA bytes already in device memory.
+    // Returns 0 or a negative SWAP_E* code.
+    int build(const uint8_t *d_fasta, uint64_t n);
+    // Compacts unitigs (k-molecule merge).  prio: per-survivor visit rank in
+    // survivor order (device or host array, may be null for the built-in
+    // hash order).
+    int merge(const uint32_t *host_prio);
+
+    // Counters of the last build / merge.
+    uint64_t n_reads = 0, n_occ = 0, n_distinct = 0, n_surv = 0, n_kept = 0, n_overflow = 0;
+    uint64_t n_special = 0;  // (1,1) nodes in cycles / fold-backs, merged on the host
+
+    int export_kmer_graph(HostKmerGraph &out);
+    int export_compacted(HostCompacted &out);
+
+    // Device pointers of the survivors (valid after build()).
+    const uint64_t *d_surv_key() const { return surv_key_.as<uint64_t>(); }
+    const uint64_t *d_surv_mult() const { return surv_mult_.as<uint64_t>(); }
+
+    const std::string &last_error() const { return err_; }
+
+    int k() const { return k_; }
+    int c() const { return c_; }
+    float ms_build_kernels = 0;
+
+private:
+    int fail(int code, const char *msg);
+    int dev_;
+    int k_, c_;
+    bool ref_order_;
+    hipStream_t st_ = nullptr;
+    Mixer mx_;
+    std::string err_;
+    // build buffers
+    DevBuf fasta_, tile_nl_, tile_off_, nl_pos_, scan_scratch_, small_, rec_start_, rec_len_, occ_count_,
+        word_count_, occ_base_, word_base_, packed_, hist_, bucket_, cursor_, rec_, occ_, surv_key_, surv_mult_,
+        surv_first_, surv_rank_, popc_, overflow_, gtab_key_, gtab_cnt_, gtab_fo_, bitmap_;
+    // merge buffers
+    DevBuf idx_key_, idx_val_, node_, walk_, walk2_, kept_, edge_, pool_, chain_, tree_, prio_, misc_;
+    // host mirrors of small results
+    uint64_t *h_small_ = nullptr;
+    uint64_t last_first_ = 0;
+};
+
+const char *error_string(int code);
+
+}  // namespace swapk

@@ -1,0 +1,186 @@
+"""ctypes binding of libswaphip.so (include/swap_hip.h) for Python callers.
+
+The library is the product: every call goes through its C ABI.  There is no
+Python fallback -- if the library or a gfx950 device is missing, the calls
+raise.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libswaphip.so")
+GEN_PATH = os.path.join(HERE, "lib", "libswapgen.so")
+
+SWAP_FLAG_KMER_GRAPH = 1
+SWAP_FLAG_HASH_ORDER = 2
+SWAP_FILE_KMER_GRAPH = 0
+SWAP_FILE_CONTIG_GRAPH = 1
+SWAP_FILE_NOCE = 2
+
+# exported entry points and their signatures (mirrors include/swap_hip.h)
+EXPORTS = [
+    "swap_create", "swap_destroy", "swap_build", "swap_build_device", "swap_merge", "swap_get_stats",
+    "swap_export_kmer_graph", "swap_format_file", "swap_write_file", "swap_strerror", "swap_last_error",
+    "swap_table_order",
+]
+
+
+class SwapCfg(ctypes.Structure):
+    _fields_ = [("k", ctypes.c_int), ("c", ctypes.c_int), ("device", ctypes.c_int), ("flags", ctypes.c_uint)]
+
+
+class SwapStats(ctypes.Structure):
+    _fields_ = [("reads", ctypes.c_uint64), ("kmers", ctypes.c_uint64), ("distinct", ctypes.c_uint64),
+                ("survivors", ctypes.c_uint64), ("kept", ctypes.c_uint64), ("overflow", ctypes.c_uint64),
+                ("special", ctypes.c_uint64), ("build_ms", ctypes.c_double), ("merge_ms", ctypes.c_double)]
+
+
+_lib = None
+
+
+def lib():
+    """Loads libswaphip.so (raises OSError if it was not built)."""
+    global _lib
+    if _lib is None:
+        L = ctypes.CDLL(LIB_PATH)
+        L.swap_create.argtypes = [ctypes.POINTER(SwapCfg), ctypes.POINTER(ctypes.c_void_p)]
+        L.swap_create.restype = ctypes.c_int
+        L.swap_destroy.argtypes = [ctypes.c_void_p]
+        L.swap_destroy.restype = None
+        L.swap_build.argtypes = [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_size_t]
+        L.swap_build.restype = ctypes.c_int
+        L.swap_build_device.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t]
+        L.swap_build_device.restype = ctypes.c_int
+        L.swap_merge.argtypes = [ctypes.c_void_p]
+        L.swap_merge.restype = ctypes.c_int
+        L.swap_get_stats.argtypes = [ctypes.c_void_p, ctypes.POINTER(SwapStats)]
+        L.swap_get_stats.restype = ctypes.c_int
+        L.swap_export_kmer_graph.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t]
+        L.swap_export_kmer_graph.restype = ctypes.c_int64
+        L.swap_format_file.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_size_t]
+        L.swap_format_file.restype = ctypes.c_int64
+        L.swap_write_file.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_char_p]
+        L.swap_write_file.restype = ctypes.c_int64
+        L.swap_strerror.argtypes = [ctypes.c_int]
+        L.swap_strerror.restype = ctypes.c_char_p
+        L.swap_last_error.argtypes = [ctypes.c_void_p]
+        L.swap_last_error.restype = ctypes.c_char_p
+        L.swap_table_order.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_uint64,
+                                       ctypes.c_int, ctypes.c_void_p]
+        L.swap_table_order.restype = ctypes.c_int64
+        _lib = L
+    return _lib
+
+
+class SwapError(RuntimeError):
+    def __init__(self, code, msg=""):
+        self.code = code
+        text = lib().swap_strerror(code).decode()
+        super().__init__("%s (%d)%s" % (text, code, (": " + msg) if msg else ""))
+
+
+class Assembler:
+    """One assembly context on one device (a thin wrapper of swap_ctx)."""
+
+    def __init__(self, k, c, device=0, kmer_graph_only=False, hash_order=False):
+        self.k, self.c = k, c
+        cfg = SwapCfg(k, c, device, (SWAP_FLAG_KMER_GRAPH if kmer_graph_only else 0)
+                      | (SWAP_FLAG_HASH_ORDER if hash_order else 0))
+        h = ctypes.c_void_p()
+        rc = lib().swap_create(ctypes.byref(cfg), ctypes.byref(h))
+        if rc:
+            raise SwapError(rc)
+        self._h = h
+
+    def close(self):
+        if self._h:
+            lib().swap_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc):
+        if rc < 0:
+            raise SwapError(int(rc), lib().swap_last_error(self._h).decode())
+        return rc
+
+    def build(self, fasta: bytes):
+        self._check(lib().swap_build(self._h, fasta, len(fasta)))
+
+    def build_device(self, ptr: int, n: int):
+        self._check(lib().swap_build_device(self._h, ctypes.c_void_p(ptr), n))
+
+    def merge(self):
+        self._check(lib().swap_merge(self._h))
+
+    def stats(self):
+        s = SwapStats()
+        self._check(lib().swap_get_stats(self._h, ctypes.byref(s)))
+        return {f: getattr(s, f) for f, _ in SwapStats._fields_}
+
+    def kmer_graph(self):
+        """(keys u64[S], mult u8[S,8]) sorted by key."""
+        n = self._check(lib().swap_export_kmer_graph(self._h, None, None, 0))
+        keys = np.zeros(n, dtype=np.uint64)
+        mult = np.zeros((n, 8), dtype=np.uint8)
+        if n:
+            self._check(lib().swap_export_kmer_graph(self._h, keys.ctypes.data, mult.ctypes.data, n))
+        return keys, mult
+
+    def file_bytes(self, which):
+        n = self._check(lib().swap_format_file(self._h, which, None, 0))
+        buf = ctypes.create_string_buffer(max(n, 1))
+        self._check(lib().swap_format_file(self._h, which, buf, n))
+        return buf.raw[:n]
+
+    def write_file(self, which, path):
+        return self._check(lib().swap_write_file(self._h, which, path.encode()))
+
+
+def table_order(keys, ranks, n_total, extra_resize=False):
+    """Iteration order of the reference's hash table for a subset of keys."""
+    keys = np.ascontiguousarray(keys, dtype=np.uint64)
+    ranks = np.ascontiguousarray(ranks, dtype=np.uint64)
+    out = np.zeros(len(keys), dtype=np.uint32)
+    rc = lib().swap_table_order(keys.ctypes.data, ranks.ctypes.data, len(keys), n_total, int(extra_resize),
+                                out.ctypes.data)
+    if rc < 0:
+        raise SwapError(int(rc))
+    return out
+
+
+def generate(workload, index=0):
+    """FASTA bytes of a named workload (small, bacterial, repeat, batch, request)
+    from the seeded generator; returns (bytes, k, c)."""
+    g = ctypes.CDLL(GEN_PATH)
+    g.swapgen_workload_size.argtypes = [ctypes.c_char_p, ctypes.c_uint64, ctypes.POINTER(ctypes.c_int),
+                                        ctypes.POINTER(ctypes.c_int)]
+    g.swapgen_workload_size.restype = ctypes.c_uint64
+    g.swapgen_workload.argtypes = [ctypes.c_char_p, ctypes.c_uint64, ctypes.c_void_p, ctypes.c_uint64]
+    g.swapgen_workload.restype = ctypes.c_uint64
+    k, c = ctypes.c_int(), ctypes.c_int()
+    n = g.swapgen_workload_size(workload.encode(), index, ctypes.byref(k), ctypes.byref(c))
+    if n == 0:
+        raise ValueError("unknown workload " + workload)
+    buf = ctypes.create_string_buffer(n)
+    m = g.swapgen_workload(workload.encode(), index, buf, n)
+    assert m == n
+    return buf.raw[:n], k.value, c.value
+
+
+def generate_params(genome_len, read_len, coverage, error_rate, seed, repeat_frac=0.0, families=0):
+    g = ctypes.CDLL(GEN_PATH)
+    g.swapgen_fasta.argtypes = [ctypes.c_uint64, ctypes.c_uint32, ctypes.c_double, ctypes.c_double, ctypes.c_uint64,
+                                ctypes.c_double, ctypes.c_uint32, ctypes.c_void_p, ctypes.c_uint64]
+    g.swapgen_fasta.restype = ctypes.c_uint64
+    n = g.swapgen_fasta(genome_len, read_len, coverage, error_rate, seed, repeat_frac, families, None, 0)
+    buf = ctypes.create_string_buffer(n)
+    m = g.swapgen_fasta(genome_len, read_len, coverage, error_rate, seed, repeat_frac, families, buf, n)
+    assert m == n
+    return buf.raw[:n]

@@ -1,0 +1,95 @@
+"""GPU k-mer graph build (libswaphip.so through its C ABI) against the oracle:
+bit-exact survivors, multiplicities and kmerGraph.txt bytes (order included)."""
+import gzip
+import json
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import swapamd
+from oracle_binding import Oracle
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+CASES = json.load(open(os.path.join(GOLD, "cases.json")))
+
+pytestmark = pytest.mark.gpu
+
+
+def gold_fa(name):
+    with gzip.open(os.path.join(GOLD, name + ".fa.gz"), "rb") as f:
+        return f.read()
+
+
+def check_build(fa, k, c, hash_order=False, check_file=True):
+    print("case k=%d c=%d bytes=%d hash_order=%s" % (k, c, len(fa), hash_order), flush=True)
+    o = Oracle(fa, k, c, kmer_graph_only=True)
+    a = swapamd.Assembler(k, c, kmer_graph_only=True, hash_order=hash_order)
+    a.build(fa)
+    st = a.stats()
+    ost = o.stats()
+    assert st["kmers"] == ost["kmers"]
+    assert st["reads"] == ost["reads"]
+    assert st["distinct"] == ost["distinct"]
+    assert st["survivors"] == ost["survivors"]
+    keys, mult = a.kmer_graph()
+    okeys, omult = o.survivors()
+    assert np.array_equal(keys, okeys)
+    assert np.array_equal(mult, omult)
+    if check_file:
+        got = a.file_bytes(swapamd.SWAP_FILE_KMER_GRAPH)
+        want = o.file(0)
+        if hash_order:
+            assert sorted(got.splitlines()) == sorted(want.splitlines())
+        else:
+            assert got == want
+    a.close()
+    return st
+
+
+@pytest.mark.parametrize("name", sorted(CASES))
+def test_golden_inputs(name):
+    case = CASES[name]
+    check_build(gold_fa(name), case["k"], case["c"])
+
+
+@pytest.mark.parametrize("k", [9, 15, 19, 23, 27, 31])
+def test_k_values(k):
+    fa = swapamd.generate_params(30000, 90, 20.0, 0.01, 100 + k)
+    check_build(fa, k, 2)
+
+
+def test_hash_order_same_graph():
+    fa = gold_fa("uniform_k31_c3")
+    check_build(fa, 31, 3, hash_order=True)
+
+
+def test_small_workload_full_size():
+    # the 1 Mbp / 30x configuration (21M input k-mers)
+    fa, k, c = swapamd.generate("small")
+    st = check_build(fa, k, c, check_file=False)
+    assert st["kmers"] == 21_000_000
+
+
+def test_bad_input_errors():
+    a = swapamd.Assembler(21, 2, kmer_graph_only=True)
+    with pytest.raises(swapamd.SwapError) as e:
+        a.build(b"no records here\n")
+    assert e.value.code == -3
+    with pytest.raises(swapamd.SwapError):
+        a.build(b"")
+    # no complete record: empty graph, not an error
+    a.build(b">only header\n")
+    assert a.stats()["survivors"] == 0
+
+
+def test_repeat_builds_reuse_context():
+    fa1 = gold_fa("uniform_k21_c2")
+    fa2 = gold_fa("dirty_k21_c2")
+    a = swapamd.Assembler(21, 2, kmer_graph_only=True)
+    for fa in (fa1, fa2, fa1):
+        a.build(fa)
+        keys, mult = a.kmer_graph()
+        okeys, omult = Oracle(fa, 21, 2).survivors()
+        assert np.array_equal(keys, okeys) and np.array_equal(mult, omult)

@@ -1,0 +1,140 @@
+"""The CPU oracle against the reference's own outputs (golden fixtures made by
+tests/golden/make_golden.py from the unmodified reference) and against the
+textbook definition of the k-mer graph."""
+import gzip
+import json
+import os
+import random
+
+import numpy as np
+import pytest
+
+from oracle_binding import Oracle
+import kmer_textbook
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+CASES = json.load(open(os.path.join(GOLD, "cases.json")))
+
+
+def gold(name, what):
+    with gzip.open(os.path.join(GOLD, "%s.%s.gz" % (name, what)), "rb") as f:
+        return f.read()
+
+
+@pytest.mark.parametrize("name", sorted(CASES))
+def test_kmer_graph_matches_reference(name):
+    case = CASES[name]
+    fa = gold(name, "fa")
+    o = Oracle(fa, case["k"], case["c"], kmer_graph_only=True)
+    got = sorted(o.file(0).decode().splitlines())
+    want = gold(name, "kmerGraph").decode().splitlines()
+    assert len(got) == case["kmer_graph_lines"]
+    assert got == want
+
+
+@pytest.mark.parametrize("name", sorted(CASES))
+def test_noce_contigs_match_reference(name):
+    case = CASES[name]
+    fa = gold(name, "fa")
+    o = Oracle(fa, case["k"], case["c"])
+    got = sorted(o.file(2).decode().splitlines())
+    want = gold(name, "noce").decode().splitlines()
+    assert got == want
+
+
+def _random_fasta(seed, G, L, n, err, extra=False):
+    rng = random.Random(seed)
+    g = "".join(rng.choice("ACGT") for _ in range(G))
+    out = []
+    for i in range(n):
+        st = rng.randrange(G - L + 1)
+        s = g[st:st + L]
+        if rng.random() < 0.5:
+            s = kmer_textbook.rc(s)
+        s = "".join(ch if rng.random() >= err else rng.choice("ACGT") for ch in s)
+        if extra and rng.random() < 0.1:
+            s = s.lower()
+        if extra and rng.random() < 0.05:
+            s = s[: rng.randrange(len(s))]
+        out.append(">r%d\n%s\n" % (i, s))
+    return "".join(out).encode()
+
+
+@pytest.mark.parametrize("seed,k,c", [(1, 21, 0), (2, 19, 2), (3, 31, 1), (4, 25, 3)])
+def test_survivors_match_textbook(seed, k, c):
+    fa = _random_fasta(seed, 600, 60, 400, 0.01, extra=True)
+    o = Oracle(fa, k, c)
+    keys, mult = o.survivors()
+    want = kmer_textbook.graph(fa, k, c)
+    assert len(keys) == len(want)
+    for X, m in zip(keys.tolist(), mult.tolist()):
+        assert want[X] == m
+
+
+def test_saturation_at_255():
+    # one 40-base read repeated 300 times: every interior edge is seen 300 times
+    rng = random.Random(7)
+    s = "".join(rng.choice("ACGT") for _ in range(40))
+    fa = "".join(">r%d\n%s\n" % (i, s) for i in range(300)).encode()
+    o = Oracle(fa, 21, 1)
+    keys, mult = o.survivors()
+    assert mult.max() == 255
+    want = kmer_textbook.graph(fa, 21, 1)
+    for X, m in zip(keys.tolist(), mult.tolist()):
+        assert want[X] == m
+
+
+def test_palindromic_edge_counts_twice():
+    # k=19: a 20-mer that is its own reverse complement
+    half = "ACGTTGCAAGCTTAGC"[:10]
+    w = half + kmer_textbook.rc(half)
+    assert w == kmer_textbook.rc(w)
+    fa = (">r\n%s\n" % w).encode()
+    o = Oracle(fa, 19, 0)
+    keys, mult = o.survivors()
+    # one node (prefix and suffix of w are reverse complements), one slot counted twice
+    assert len(keys) == 1
+    assert sorted(mult[0].tolist()) == [0] * 7 + [2]
+
+
+def test_edge_symmetry_invariant():
+    # every edge X -b-> Y is seen from Y with the same multiplicity
+    fa = _random_fasta(11, 2000, 80, 900, 0.01)
+    k = 21
+    o = Oracle(fa, k, 2)
+    keys, mult = o.survivors()
+    table = {int(x): m for x, m in zip(keys.tolist(), mult.tolist())}
+    mask = (1 << (2 * k)) - 1
+
+    def rcv(x):
+        r = 0
+        for _ in range(k):
+            r = (r << 2) | (3 - (x & 3))
+            x >>= 2
+        return r
+
+    for X, m in table.items():
+        for s in range(8):
+            if not m[s]:
+                continue
+            o_ = X if s < 4 else rcv(X)
+            t = ((o_ << 2) | (s & 3)) & mask
+            Y = max(t, rcv(t))
+            assert Y in table
+            # back edge: from rc(t) followed by complement of o_'s first base
+            first = (o_ >> (2 * (k - 1))) & 3
+            back = rcv(t)
+            bslot = (3 - first) if back == Y else 4 + (3 - first)
+            assert table[Y][bslot] == m[s]
+
+
+def test_unterminated_last_record_is_ignored():
+    fa = b">a\nACGTACGTACGTACGTACGTACGTAC\n>b\nTTTTGGGGCCCCAAAATTTTGGGG"
+    o = Oracle(fa, 21, 0)
+    assert o.stats()["reads"] == 1
+
+
+def test_records_before_first_gt_are_skipped():
+    fa = b"junk\nmore junk\n>a\nACGTACGTACGTACGTACGTACGTAC\n"
+    o = Oracle(fa, 21, 0)
+    assert o.stats()["reads"] == 1 and o.stats()["kmers"] == 6
