@@ -1,0 +1,170 @@
+"""Benchmark: k-mer graph build + k-molecule merge on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload bacterial]
+                    [--hash-order] [--no-cpu-baseline]
+
+A step is one pass of the path over one batch: swap_build_device on FASTA bytes
+already resident in HBM, then swap_merge.  Default workload: the bacterial-size
+configuration (seeded random 4.6 Mbp genome at 50% GC, 100 bp reads from both
+strands to 50x = 2.3M reads, 1% substitutions, k=31, c=5), synthetic data from
+the project's generator.  Prints one JSON line.
+
+roofline: the dominant device kernel of the step, its algorithmic bytes per
+launch (the words it must read and write once) over its device time measured
+with HIP events recorded on the library's own stream around the launch.
+cpu_baseline: the single-threaded CPU oracle (a restatement of the
+reference's np=1 path) on a bounded prefix of the same reads.
+"""
+import argparse
+import json
+import os
+import subprocess
+import sys
+import tempfile
+import time
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(REPO, "swap-assembler_amd"))
+
+HBM_PEAK = 8.0e12  # MI355X HBM3E, bytes/s
+
+# phase index -> kernel name (see swap_stats.phase_ms)
+PHASES = {0: "record_scan", 1: "pack_2bit", 2: "partition_hist", 3: "partition_scatter", 4: "count_lds_cutoff",
+          5: "count_global_overflow", 6: "first_occurrence_rank"}
+
+
+def algorithmic_bytes(phase, st, words, ref_order):
+    """Words a phase must move once (reads + writes), per launch."""
+    N, S, D, R = st["kmers"], st["survivors"], st["distinct"], st["reads"]
+    occ = 4 if ref_order else 0
+    if phase == 3:   # partition scatter: packed reads twice (count + write passes), records out
+        return 2 * 8 * words + 2 * 16 * R + N * (8 + occ)
+    if phase == 4:   # LDS count: records in, survivors out, first-occurrence bits
+        return N * (8 + occ) + S * (8 + 8 + occ) + (4 * D if ref_order else 0)
+    if phase == 2:   # partition histogram: packed reads once
+        return 8 * words + 16 * R
+    if phase == 1:   # 2-bit pack: sequence bytes in, packed words out
+        return 100 * R + 8 * words
+    return 0
+
+
+def cpu_baseline(fa: bytes, k: int, c: int, target_s=15.0):
+    """Times the oracle CLI on growing prefixes of the reads until ~target_s."""
+    exe = os.path.join(REPO, "oracle", "swap_oracle")
+    if not os.path.exists(exe):
+        return None
+    tmpdir = tempfile.mkdtemp(prefix="cpu_base_")
+    frac = 0.02
+    res = None
+    while True:
+        cut = int(len(fa) * frac)
+        # end the sample on a record boundary
+        pos = fa.rfind(b"\n>", 0, cut)
+        sample = fa[:pos + 1] if pos > 0 else fa
+        path = os.path.join(tmpdir, "s.fa")
+        with open(path, "wb") as f:
+            f.write(sample)
+        out = os.path.join(tmpdir, "o%d" % int(frac * 1000))
+        t0 = time.time()
+        p = subprocess.run([exe, "-k", str(k), "-c", str(c), "-i", path, "-o", out], capture_output=True, text=True,
+                           timeout=600)
+        wall = time.time() - t0
+        info = json.loads(p.stdout.strip().splitlines()[-1])
+        res = {"value": info["kmers"] / info["seconds"], "unit": "input k-mers/s", "cores": 1, "kind": "port",
+               "sample": "first %.1f%% of the reads (%d input k-mers), oracle build+cutoff+merge, %.1f s" % (
+                   100 * len(sample) / len(fa), info["kmers"], info["seconds"])}
+        if wall >= target_s / 3 or frac >= 1.0:
+            break
+        frac = min(1.0, frac * max(2.0, target_s / max(wall, 0.05) * 0.8))
+    return res
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default="bacterial")
+    ap.add_argument("--hash-order", action="store_true")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (sharding is described in DESIGN.md)",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import swapamd
+
+    t0 = time.time()
+    fa, k, c = swapamd.generate(args.workload)
+    print("generated %s: %d bytes in %.1f s" % (args.workload, len(fa), time.time() - t0), file=sys.stderr,
+          flush=True)
+    dev = swapamd.DeviceBuffer(fa)
+    ref_order = not args.hash_order
+    a = swapamd.Assembler(k, c, hash_order=args.hash_order)
+    has_merge = True
+
+    def step():
+        nonlocal has_merge
+        a.build_device(dev.ptr.value, dev.n)
+        if has_merge:
+            try:
+                a.merge()
+            except swapamd.SwapError as e:
+                if e.code != -8:
+                    raise
+                has_merge = False
+
+    for i in range(args.warmup):
+        step()
+        print("warmup %d done" % i, file=sys.stderr, flush=True)
+    swapamd.hip().hipDeviceSynchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    swapamd.hip().hipDeviceSynchronize()
+    dt = time.perf_counter() - t0
+    st = a.stats()
+    ms = 1000.0 * dt / args.steps
+    N = st["kmers"]
+    value = N / (dt / args.steps)
+
+    # dominant kernel of the last step
+    reads = st["reads"]
+    words = reads * ((len(fa.split(b"\n", 2)[1]) + 31) // 32)
+    phases = {i: st["phase_ms"][i] for i in PHASES if st["phase_ms"][i] > 0}
+    dom = max(phases, key=lambda i: phases[i])
+    alg = algorithmic_bytes(dom, st, words, ref_order)
+    achieved = alg / (phases[dom] * 1e-3) if alg else None
+    roof = {"bound": "hbm", "kernel": PHASES[dom], "achieved": achieved / 1e9 if achieved else None,
+            "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": (achieved / HBM_PEAK) if achieved else None,
+            "traffic": None, "algorithmic_bytes": alg, "kernel_ms": phases[dom]}
+    cpu = None if args.no_cpu_baseline else cpu_baseline(fa, k, c)
+    line = {
+        "metric": "input k-mers/s (build+merge) at 4.6Mbp/50x",
+        "value": value,
+        "unit": "input k-mers/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u64",
+        "data": "synthetic",
+        "config": {"workload": args.workload, "genome_bp": 4600000, "read_len": 100, "coverage": 50,
+                   "reads": reads, "k": k, "c": c, "order": "hash" if args.hash_order else "reference",
+                   "merge": has_merge, "parallelism": "1 GPU"},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+        "stats": {kk: st[kk] for kk in ("kmers", "distinct", "survivors", "kept", "overflow", "special")},
+        "phase_ms": {PHASES[i]: round(v, 4) for i, v in phases.items()},
+    }
+    print(json.dumps(line))
+    a.close()
+    dev.close()
+
+
+if __name__ == "__main__":
+    main()

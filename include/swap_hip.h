@@ -75,6 +75,11 @@ typedef struct swap_stats {
     uint64_t special;      /* (1,1) nodes merged on the host (cycles, fold-backs) */
     double build_ms;       /* wall time of the last swap_build (device + syncs) */
     double merge_ms;       /* wall time of the last swap_merge */
+    /* device time of each phase (events on the context's stream): 0 record scan,
+     * 1 2-bit pack, 2 partition histogram, 3 partition scatter, 4 LDS count +
+     * cutoff, 5 global-table count (overflow), 6 first-occurrence ranks,
+     * 8.. merge phases */
+    double phase_ms[16];
 } swap_stats;
 
 typedef struct swap_ctx swap_ctx;

@@ -91,12 +91,13 @@ __global__ void k_u64_to_u32(const uint64_t *__restrict__ in, uint64_t n, uint32
 
 // total <- sum, out <- exclusive prefix.  scratch must hold scan_scratch_words(n).
 uint64_t scan_scratch_words(uint64_t n) {
+    // each level holds its block sums and their exclusive scan
     uint64_t w = 0;
     while (n > 1) {
         n = (n + kScanTile - 1) / kScanTile;
-        w += n + 1;
+        w += 2 * n + 2;
     }
-    return w + 2;
+    return w + 4;
 }
 
 template <typename TIn>
@@ -618,10 +619,25 @@ Pipeline::Pipeline(int device, int k, int c, bool reference_order) : dev_(device
     mx_.i1 = inv_odd(mx_.a1);
     mx_.i2 = inv_odd(mx_.a2);
     mx_.s = k;  // half of the 2k bits
+    for (auto &e : ev_) hipEventCreate(&e);
+}
+
+void Pipeline::phase_begin(int i) {
+    hipEventRecord(ev_[2 * i], st_);
+    ev_used_[i] = true;
+}
+void Pipeline::phase_end(int i) { hipEventRecord(ev_[2 * i + 1], st_); }
+void Pipeline::collect_phases() {
+    hipStreamSynchronize(st_);
+    for (int i = 0; i < kPhases; i++) {
+        phase_ms[i] = 0;
+        if (ev_used_[i]) hipEventElapsedTime(&phase_ms[i], ev_[2 * i], ev_[2 * i + 1]);
+    }
 }
 
 Pipeline::~Pipeline() {
     hipSetDevice(dev_);
+    for (auto &e : ev_) if (e) hipEventDestroy(e);
     if (h_small_) hipHostFree(h_small_);
     if (st_) hipStreamDestroy(st_);
 }
@@ -657,6 +673,8 @@ int Pipeline::build(const uint8_t *d_in, uint64_t n) {
     uint64_t *d_small = small_.as<uint64_t>();  // [0]=first_gt [1]=nnl [2]=j0 [3]=R [4]=N [5]=W [8..]=counters
     SWAP_CHECK(hipMemsetAsync(d_small, 0xFF, 8, st_));
     SWAP_CHECK(hipMemsetAsync(d_small + 1, 0, 63 * 8, st_));
+    for (int i = 0; i < kPhases; i++) ev_used_[i] = false;
+    phase_begin(0);
     hipLaunchKernelGGL(k_tile_count, dim3((unsigned)tiles), dim3(kTileThreads), 0, st_, buf, n,
                        tile_nl_.as<uint32_t>(), (unsigned long long *)d_small);
     SWAP_CHECK(scan_scratch_.reserve(scan_words * 8));
@@ -692,6 +710,7 @@ int Pipeline::build(const uint8_t *d_in, uint64_t n) {
                         d_small + 4, st_);
     scan_excl<uint32_t>(word_count_.as<uint32_t>(), word_base_.as<uint64_t>(), R, scan_scratch_.as<uint64_t>(),
                         d_small + 5, st_);
+    phase_end(0);
     SWAP_CHECK(hipMemcpyAsync(hs + 4, d_small + 4, 4 * 8, hipMemcpyDeviceToHost, st_));
     SWAP_CHECK(hipStreamSynchronize(st_));
     const uint64_t N = hs[4], W = hs[5];
@@ -703,9 +722,11 @@ int Pipeline::build(const uint8_t *d_in, uint64_t n) {
     SWAP_CHECK(packed_.reserve((W + 1) * 8));
     {
         const unsigned grid = (unsigned)std::min<uint64_t>((W + 255) / 256, 65535);
+        phase_begin(1);
         hipLaunchKernelGGL(k_pack, dim3(grid), dim3(256), 0, st_, buf, rec_start_.as<uint64_t>(),
                            rec_len_.as<uint32_t>(), word_base_.as<uint64_t>(), R, d_small + 5,
                            packed_.as<uint64_t>());
+        phase_end(1);
     }
     // --- partition
     OccCtx oc;
@@ -724,8 +745,10 @@ int Pipeline::build(const uint8_t *d_in, uint64_t n) {
     int ncu = 0;
     hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev_);
     const unsigned occ_grid = (unsigned)std::max(8, ncu * 2);
+    phase_begin(2);
     hipLaunchKernelGGL(k_occ_hist, dim3(occ_grid), dim3(kOccThreads), 0, st_, oc, d_small + 5,
                        hist_.as<uint32_t>());
+    phase_end(2);
     SWAP_CHECK(scan_scratch_.reserve((scan_scratch_words(nb + 1) + 16) * 8));
     scan_excl<uint32_t>(hist_.as<uint32_t>(), bucket_.as<uint64_t>(), nb + 1, scan_scratch_.as<uint64_t>(),
                         d_small + 6, st_);
@@ -740,12 +763,14 @@ int Pipeline::build(const uint8_t *d_in, uint64_t n) {
                        cursor_.as<uint32_t>());
     SWAP_CHECK(hist_.reserve((nb + 1) * 4));
     SWAP_CHECK(hipMemcpyAsync(hist_.p, cursor_.p, (nb + 1) * 4, hipMemcpyDeviceToDevice, st_));  // hist_ := bucket starts (u32)
+    phase_begin(3);
     if (ref_order_)
         hipLaunchKernelGGL(k_occ_scatter<true>, dim3(occ_grid), dim3(kOccThreads), 0, st_, oc, d_small + 5,
                            cursor_.as<uint32_t>(), rec_.as<uint64_t>(), occ_.as<uint32_t>());
     else
         hipLaunchKernelGGL(k_occ_scatter<false>, dim3(occ_grid), dim3(kOccThreads), 0, st_, oc, d_small + 5,
                            cursor_.as<uint32_t>(), rec_.as<uint64_t>(), (uint32_t *)nullptr);
+    phase_end(3);
     // --- count + cutoff
     // survivors <= distinct <= N; reserve N to avoid a sync (grow-only buffers)
     SWAP_CHECK(surv_key_.reserve(N * 8));
@@ -769,10 +794,12 @@ int Pipeline::build(const uint8_t *d_in, uint64_t n) {
     ca.counters = (unsigned long long *)(d_small + 8);
     ca.overflow_list = overflow_.as<uint32_t>();
     ca.bitmap = ref_order_ ? bitmap_.as<uint32_t>() : nullptr;
+    phase_begin(4);
     if (ref_order_)
         hipLaunchKernelGGL(k_count_lds<true>, dim3(kParts), dim3(kCountThreads), 0, st_, ca);
     else
         hipLaunchKernelGGL(k_count_lds<false>, dim3(kParts), dim3(kCountThreads), 0, st_, ca);
+    phase_end(4);
     SWAP_CHECK(hipGetLastError());
     SWAP_CHECK(hipMemcpyAsync(hs + 8, d_small + 8, 3 * 8, hipMemcpyDeviceToHost, st_));
     SWAP_CHECK(hipStreamSynchronize(st_));
@@ -802,6 +829,7 @@ int Pipeline::build(const uint8_t *d_in, uint64_t n) {
         SWAP_CHECK(d_bits.reserve(n_overflow * 4));
         SWAP_CHECK(hipMemcpy(d_off.p, off.data(), n_overflow * 8, hipMemcpyHostToDevice));
         SWAP_CHECK(hipMemcpy(d_bits.p, bits.data(), n_overflow * 4, hipMemcpyHostToDevice));
+        phase_begin(5);
         if (ref_order_)
             hipLaunchKernelGGL(k_count_global<true>, dim3((unsigned)n_overflow), dim3(kCountThreads), 0, st_, ca,
                                d_off.as<uint64_t>(), d_bits.as<uint32_t>(), gtab_key_.as<uint64_t>(),
@@ -810,6 +838,7 @@ int Pipeline::build(const uint8_t *d_in, uint64_t n) {
             hipLaunchKernelGGL(k_count_global<false>, dim3((unsigned)n_overflow), dim3(kCountThreads), 0, st_, ca,
                                d_off.as<uint64_t>(), d_bits.as<uint32_t>(), gtab_key_.as<uint64_t>(),
                                gtab_cnt_.as<uint64_t>(), (uint32_t *)nullptr);
+        phase_end(5);
         SWAP_CHECK(hipGetLastError());
         SWAP_CHECK(hipMemcpyAsync(hs + 8, d_small + 8, 3 * 8, hipMemcpyDeviceToHost, st_));
         SWAP_CHECK(hipStreamSynchronize(st_));
@@ -821,6 +850,7 @@ int Pipeline::build(const uint8_t *d_in, uint64_t n) {
         SWAP_CHECK(popc_.reserve(nw * 4));
         SWAP_CHECK(surv_rank_.reserve(n_surv * 8));
         SWAP_CHECK(hipMemsetAsync(d_small + 12, 0, 8, st_));
+        phase_begin(6);
         hipLaunchKernelGGL(k_popc_words, dim3((unsigned)((nw + 255) / 256)), dim3(256), 0, st_,
                            bitmap_.as<uint32_t>(), nw, popc_.as<uint32_t>(), (unsigned long long *)(d_small + 12));
         // the bitmap prefix reuses word_base_ (the pack and scatter are done)
@@ -831,11 +861,13 @@ int Pipeline::build(const uint8_t *d_in, uint64_t n) {
         hipLaunchKernelGGL(k_surv_rank, dim3((unsigned)((n_surv + 255) / 256)), dim3(256), 0, st_,
                            surv_first_.as<uint32_t>(), bitmap_.as<uint32_t>(), word_base_.as<uint64_t>(),
                            (const unsigned long long *)(d_small + 8), surv_rank_.as<uint64_t>());
+        phase_end(6);
         SWAP_CHECK(hipMemcpyAsync(hs + 12, d_small + 12, 8, hipMemcpyDeviceToHost, st_));
         SWAP_CHECK(hipStreamSynchronize(st_));
         last_first_ = hs[12];
     }
     n_reads = hs[7];
+    collect_phases();
     return 0;
 }
 

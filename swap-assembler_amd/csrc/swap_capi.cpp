@@ -150,6 +150,7 @@ int swap_get_stats(const swap_ctx *c, swap_stats *s) {
     s->special = c->pipe->n_special;
     s->build_ms = c->build_ms;
     s->merge_ms = c->merge_ms;
+    for (int i = 0; i < 16; i++) s->phase_ms[i] = c->pipe->phase_ms[i];
     return SWAP_OK;
 }
 

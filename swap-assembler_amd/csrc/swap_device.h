@@ -73,7 +73,15 @@ public:
 
     int k() const { return k_; }
     int c() const { return c_; }
-    float ms_build_kernels = 0;
+    // Device time of the build phases of the last build, from events recorded
+    // on the pipeline stream around the launches: 0 record scan, 1 pack,
+    // 2 partition histogram, 3 partition scatter, 4 LDS count + cutoff,
+    // 5 overflow count, 6 first-occurrence ranks; merge phases from 8.
+    static constexpr int kPhases = 16;
+    float phase_ms[kPhases] = {0};
+    void phase_begin(int i);
+    void phase_end(int i);
+    void collect_phases();
 
 private:
     int fail(int code, const char *msg);
@@ -92,6 +100,8 @@ private:
     // host mirrors of small results
     uint64_t *h_small_ = nullptr;
     uint64_t last_first_ = 0;
+    hipEvent_t ev_[2 * kPhases] = {};
+    bool ev_used_[kPhases] = {};
 };
 
 const char *error_string(int code);

@@ -34,7 +34,8 @@ class SwapCfg(ctypes.Structure):
 class SwapStats(ctypes.Structure):
     _fields_ = [("reads", ctypes.c_uint64), ("kmers", ctypes.c_uint64), ("distinct", ctypes.c_uint64),
                 ("survivors", ctypes.c_uint64), ("kept", ctypes.c_uint64), ("overflow", ctypes.c_uint64),
-                ("special", ctypes.c_uint64), ("build_ms", ctypes.c_double), ("merge_ms", ctypes.c_double)]
+                ("special", ctypes.c_uint64), ("build_ms", ctypes.c_double), ("merge_ms", ctypes.c_double),
+                ("phase_ms", ctypes.c_double * 16)]
 
 
 _lib = None
@@ -122,7 +123,9 @@ class Assembler:
     def stats(self):
         s = SwapStats()
         self._check(lib().swap_get_stats(self._h, ctypes.byref(s)))
-        return {f: getattr(s, f) for f, _ in SwapStats._fields_}
+        d = {f: getattr(s, f) for f, _ in SwapStats._fields_}
+        d["phase_ms"] = list(s.phase_ms)
+        return d
 
     def kmer_graph(self):
         """(keys u64[S], mult u8[S,8]) sorted by key."""
@@ -184,3 +187,55 @@ def generate_params(genome_len, read_len, coverage, error_rate, seed, repeat_fra
     m = g.swapgen_fasta(genome_len, read_len, coverage, error_rate, seed, repeat_frac, families, buf, n)
     assert m == n
     return buf.raw[:n]
+
+
+# ---------------------------------------------------------------------------
+# minimal HIP runtime access (device-resident inputs for benchmarks), no torch
+
+_hip = None
+
+
+def hip():
+    global _hip
+    if _hip is None:
+        H = ctypes.CDLL("libamdhip64.so")
+        H.hipMalloc.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t]
+        H.hipFree.argtypes = [ctypes.c_void_p]
+        H.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
+        H.hipDeviceSynchronize.argtypes = []
+        H.hipSetDevice.argtypes = [ctypes.c_int]
+        H.hipGetDeviceCount.argtypes = [ctypes.POINTER(ctypes.c_int)]
+        for f in ("hipMalloc", "hipFree", "hipMemcpy", "hipDeviceSynchronize", "hipSetDevice", "hipGetDeviceCount"):
+            getattr(H, f).restype = ctypes.c_int
+        _hip = H
+    return _hip
+
+
+class DeviceBuffer:
+    """Bytes copied once into device memory (hipMalloc), freed on close()."""
+
+    def __init__(self, data: bytes, device=0):
+        H = hip()
+        if H.hipSetDevice(device):
+            raise RuntimeError("hipSetDevice failed")
+        self.ptr = ctypes.c_void_p()
+        if H.hipMalloc(ctypes.byref(self.ptr), max(len(data), 1)):
+            raise RuntimeError("hipMalloc failed")
+        if H.hipMemcpy(self.ptr, data, len(data), 1):  # hipMemcpyHostToDevice
+            raise RuntimeError("hipMemcpy failed")
+        self.n = len(data)
+
+    def close(self):
+        if self.ptr:
+            hip().hipFree(self.ptr)
+            self.ptr = None
+
+
+def gpu_count():
+    n = ctypes.c_int(0)
+    try:
+        if hip().hipGetDeviceCount(ctypes.byref(n)):
+            return 0
+    except OSError:
+        return 0
+    return n.value
