@@ -8,6 +8,8 @@
 //   distNodeGraph::constructDistNodeGraph graph.cpp:1049-1090 (table, counts)
 //   distNodeGraph::cutoffGraph      graph.cpp:1092-1114
 #include "swap_device.h"
+#include "swap_host.h"
+#include "swap_scan.h"
 
 #include <algorithm>
 #include <cstdio>
@@ -31,92 +33,6 @@ __device__ __forceinline__ uint32_t base_code(uint8_t c) {
         case 'T': case 't': return 3;
         default: return 0;
     }
-}
-
-template <typename T>
-__device__ __forceinline__ T wave_incl_sum(T v) {
-    const int lane = threadIdx.x & 63;
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-        T o = __shfl_up(v, d, 64);
-        if (lane >= d) v += o;
-    }
-    return v;
-}
-
-// ---------------------------------------------------------------------------
-// exclusive scan (u64 output) over u32 or u64 input: block reduce, recursive
-// scan of the block sums, add back.
-
-constexpr int kScanThreads = 256;
-constexpr int kScanItems = 8;
-constexpr int kScanTile = kScanThreads * kScanItems;
-
-template <typename TIn>
-__global__ void k_scan_local(const TIn *__restrict__ in, uint64_t *__restrict__ out, uint64_t n,
-                             uint64_t *__restrict__ block_sums) {
-    __shared__ uint64_t wsum[kScanThreads / 64];
-    const uint64_t base = (uint64_t)blockIdx.x * kScanTile + (uint64_t)threadIdx.x * kScanItems;
-    uint64_t v[kScanItems];
-    uint64_t s = 0;
-#pragma unroll
-    for (int i = 0; i < kScanItems; i++) {
-        v[i] = (base + i < n) ? (uint64_t)in[base + i] : 0;
-        s += v[i];
-    }
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    uint64_t inc = wave_incl_sum(s);
-    if (lane == 63) wsum[w] = inc;
-    __syncthreads();
-    uint64_t woff = 0;
-    for (int i = 0; i < w; i++) woff += wsum[i];
-    uint64_t run = inc - s + woff;
-#pragma unroll
-    for (int i = 0; i < kScanItems; i++) {
-        if (base + i < n) out[base + i] = run;
-        run += v[i];
-    }
-    if (threadIdx.x == kScanThreads - 1) block_sums[blockIdx.x] = run;
-}
-
-__global__ void k_scan_add(uint64_t *__restrict__ out, uint64_t n, const uint64_t *__restrict__ offs) {
-    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) out[i] += offs[i / kScanTile];
-}
-
-__global__ void k_u64_to_u32(const uint64_t *__restrict__ in, uint64_t n, uint32_t *__restrict__ out) {
-    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) out[i] = (uint32_t)in[i];
-}
-
-// total <- sum, out <- exclusive prefix.  scratch must hold scan_scratch_words(n).
-uint64_t scan_scratch_words(uint64_t n) {
-    // each level holds its block sums and their exclusive scan
-    uint64_t w = 0;
-    while (n > 1) {
-        n = (n + kScanTile - 1) / kScanTile;
-        w += 2 * n + 2;
-    }
-    return w + 4;
-}
-
-template <typename TIn>
-static void scan_excl(const TIn *in, uint64_t *out, uint64_t n, uint64_t *scratch, uint64_t *d_total,
-                      hipStream_t st) {
-    if (n == 0) {
-        hipMemsetAsync(d_total, 0, 8, st);
-        return;
-    }
-    const uint64_t nb = (n + kScanTile - 1) / kScanTile;
-    uint64_t *sums = scratch;
-    hipLaunchKernelGGL(k_scan_local<TIn>, dim3((unsigned)nb), dim3(kScanThreads), 0, st, in, out, n, sums);
-    if (nb == 1) {
-        hipMemcpyAsync(d_total, sums, 8, hipMemcpyDeviceToDevice, st);
-        return;
-    }
-    uint64_t *sum_scan = scratch + nb;
-    scan_excl<uint64_t>(sums, sum_scan, nb, scratch + 2 * nb, d_total, st);
-    hipLaunchKernelGGL(k_scan_add, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, out, n, sum_scan);
 }
 
 // ---------------------------------------------------------------------------
@@ -163,7 +79,7 @@ __global__ void k_tile_count(const uint8_t *__restrict__ buf, uint64_t n, uint32
         unsigned long long g = ~0ull;
         for (int i = 0; i < kTileThreads / 64; i++) { c += scnt[i]; g = sgt[i] < g ? sgt[i] : g; }
         tile_nl[blockIdx.x] = c;
-        if (g != ~0ull) atomicMin(first_gt, g);
+        if (g != ~0ull && g < *(volatile unsigned long long *)first_gt) atomicMin(first_gt, g);
     }
 }
 
@@ -205,18 +121,26 @@ __global__ void k_records(const uint64_t *__restrict__ nl_pos, const uint64_t *_
                           uint64_t *__restrict__ rec_start, uint32_t *__restrict__ rec_len,
                           uint32_t *__restrict__ occ_count, uint32_t *__restrict__ word_count,
                           unsigned long long *__restrict__ n_valid) {
-    const uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= R) return;
+    const uint64_t r0 = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const bool live = r0 < R;
+    const uint64_t r = live ? r0 : R - 1;
     const uint64_t j0 = *d_j0;
     const uint64_t s0 = nl_pos[j0 + 2 * r] + 1;
     const uint64_t s1 = nl_pos[j0 + 2 * r + 1];
     const uint64_t len64 = s1 - s0;
     const uint32_t len = len64 > 0xFFFFFFF0ull ? 0xFFFFFFF0u : (uint32_t)len64;
+    const bool ok = live && len >= (uint32_t)k;
+    // valid-record count: one atomic per workgroup, spread over 32 counters
+    __shared__ uint32_t wcnt;
+    if (threadIdx.x == 0) wcnt = 0;
+    __syncthreads();
+    const uint64_t bal = __ballot(ok);
+    if ((threadIdx.x & 63) == 0 && bal) atomicAdd(&wcnt, (uint32_t)__popcll(bal));
+    __syncthreads();
+    if (threadIdx.x == 0 && wcnt) atomicAdd(&n_valid[(blockIdx.x & 31) * 16], (unsigned long long)wcnt);
+    if (!live) return;
     rec_start[r] = s0;
     rec_len[r] = len;
-    const bool ok = len >= (uint32_t)k;
-    const uint64_t bal = __ballot(ok);
-    if ((threadIdx.x & 63) == __ffsll((long long)__ballot(1)) - 1 && bal) atomicAdd(n_valid, (unsigned long long)__popcll(bal));
     occ_count[r] = ok ? len - k + 1 : 0;
     word_count[r] = ok ? (len + 31) / 32 : 0;
 }
@@ -409,7 +333,7 @@ struct CountArgs {
     uint32_t *surv_first;
     unsigned long long *counters;  // [0]=survivors [1]=distinct [2]=overflow partitions
     uint32_t *overflow_list;
-    uint32_t *bitmap;              // first-occurrence bitmap (reference order), or null
+    uint32_t *dfo;                 // first-occurrence index of every distinct key (reference order), or null
 };
 
 __device__ __forceinline__ uint32_t slot_hash(uint64_t key, uint32_t mask) {
@@ -462,45 +386,65 @@ __device__ __forceinline__ uint64_t apply_cutoff(uint64_t cnt, int c) {
     return out;
 }
 
+// Emits the survivors of one partition table and (reference order) the
+// first-occurrence index of every distinct key.  Slots are taken 8 per thread
+// per round; one atomic per workgroup and round reserves the output ranges.
 template <bool WITH_OCC>
 __device__ void table_emit(const uint64_t *tkey, const uint64_t *tcnt, const uint32_t *tfo, uint32_t n_slots,
                            uint32_t part, const CountArgs &a) {
+    __shared__ uint64_t wsum[kCountThreads / 64];
+    __shared__ unsigned long long base_s, base_d;
     const int hb = 2 * a.k - kPartBits;
-    const int lane = threadIdx.x & 63;
-    uint32_t ndist = 0;
-    for (uint32_t s0 = 0; s0 < n_slots; s0 += blockDim.x) {
-        const uint32_t s = s0 + threadIdx.x;
-        bool occupied = false, surv = false;
-        uint64_t key = 0, mult = 0;
-        uint32_t fo = 0;
-        if (s < n_slots) {
-            key = tkey[s];
-            occupied = key != kEmpty;
-            if (occupied) {
-                mult = apply_cutoff(tcnt[s], a.c);
-                surv = mult != 0;
-                if (WITH_OCC) fo = tfo[s];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    constexpr int kPer = 8;
+    for (uint32_t r0 = 0; r0 < n_slots; r0 += blockDim.x * kPer) {
+        uint64_t keys[kPer], mults[kPer];
+        uint32_t fos[kPer];
+        uint32_t ns = 0, nd = 0;
+#pragma unroll
+        for (int j = 0; j < kPer; j++) {
+            const uint32_t s = r0 + threadIdx.x * kPer + j;
+            keys[j] = kEmpty;
+            mults[j] = 0;
+            fos[j] = 0;
+            if (s < n_slots) {
+                keys[j] = tkey[s];
+                if (keys[j] != kEmpty) {
+                    mults[j] = apply_cutoff(tcnt[s], a.c);
+                    if (WITH_OCC) fos[j] = tfo[s];
+                    nd++;
+                    ns += mults[j] != 0;
+                }
             }
         }
-        ndist += occupied;
-        if (WITH_OCC && occupied && a.bitmap) atomicOr(&a.bitmap[fo >> 5], 1u << (fo & 31));
-        const uint64_t bal = __ballot(surv);
-        if (bal) {
-            unsigned long long base = 0;
-            if (lane == 0) base = atomicAdd(&a.counters[0], (unsigned long long)__popcll(bal));
-            base = __shfl(base, 0, 64);
-            if (surv) {
-                const uint64_t below = bal & ((lane == 0) ? 0ull : (~0ull >> (64 - lane)));
-                const uint64_t i = base + __popcll(below);
-                a.surv_key[i] = mix_inv(a.mx, ((uint64_t)part << hb) | key);
-                a.surv_mult[i] = mult;
-                if (WITH_OCC) a.surv_first[i] = fo;
+        // block exclusive scan of (survivors, distinct) packed in one u64
+        const uint64_t v = ((uint64_t)nd << 32) | ns;
+        const uint64_t inc = wave_incl_sum(v);
+        if (lane == 63) wsum[w] = inc;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            uint64_t run = 0;
+            for (int i = 0; i < nw; i++) { const uint64_t t = wsum[i]; wsum[i] = run; run += t; }
+            const uint32_t ts = (uint32_t)run, td = (uint32_t)(run >> 32);
+            base_s = ts ? atomicAdd(&a.counters[0], (unsigned long long)ts) : 0;
+            base_d = td ? atomicAdd(&a.counters[1], (unsigned long long)td) : 0;
+        }
+        __syncthreads();
+        const uint64_t ex = inc - v + wsum[w];
+        uint64_t os = base_s + (uint32_t)ex, od = base_d + (ex >> 32);
+#pragma unroll
+        for (int j = 0; j < kPer; j++) {
+            if (keys[j] == kEmpty) continue;
+            if (WITH_OCC && a.dfo) a.dfo[od++] = fos[j];
+            if (mults[j]) {
+                a.surv_key[os] = mix_inv(a.mx, ((uint64_t)part << hb) | keys[j]);
+                a.surv_mult[os] = mults[j];
+                if (WITH_OCC) a.surv_first[os] = fos[j];
+                os++;
             }
         }
+        __syncthreads();
     }
-    // distinct count
-    for (int d = 32; d > 0; d >>= 1) ndist += __shfl_xor(ndist, d, 64);
-    if (lane == 0 && ndist) atomicAdd(&a.counters[1], (unsigned long long)ndist);
 }
 
 template <bool WITH_OCC>
@@ -561,26 +505,46 @@ __global__ void __launch_bounds__(kCountThreads) k_count_global(CountArgs a, con
 }
 
 // ---------------------------------------------------------------------------
-// First-occurrence ranks (reference order): bit f of the bitmap is set for the
-// first occurrence f of every distinct k-mer, so the number of distinct k-mers
-// first seen before a survivor is a prefix popcount.
+// Reference order support: the reference's k-mer table grows (rehashes) when
+// its (P+1)-th distinct key arrives, P running through a fixed prime list.
+// Which survivors were inserted before each growth follows from the first-
+// occurrence index of the distinct key of each growth rank, selected from the
+// first-occurrence indices of all distinct keys by a histogram pass and a
+// collection pass over the few buckets that hold the wanted ranks.
 
-__global__ void k_popc_words(const uint32_t *__restrict__ bm, uint64_t nw, uint32_t *__restrict__ cnt,
-                             unsigned long long *__restrict__ last_bit) {
-    const uint64_t w = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (w >= nw) return;
-    const uint32_t v = bm[w];
-    cnt[w] = __popc(v);
-    if (v) atomicMax(last_bit, (unsigned long long)(w * 32 + 31 - __clz(v)));
+constexpr int kSelBits = 14;
+constexpr uint32_t kSelBins = 1u << kSelBits;
+
+__global__ void k_sel_hist(const uint32_t *__restrict__ dfo, const unsigned long long *__restrict__ d_n, int shift,
+                           uint32_t *__restrict__ hist, unsigned long long *__restrict__ maxfo) {
+    __shared__ uint32_t lh[kSelBins];
+    __shared__ uint32_t lmax;
+    for (uint32_t i = threadIdx.x; i < kSelBins; i += blockDim.x) lh[i] = 0;
+    if (threadIdx.x == 0) lmax = 0;
+    __syncthreads();
+    const uint64_t n = *d_n;
+    uint32_t mx = 0;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t f = dfo[i];
+        atomicAdd(&lh[f >> shift], 1u);
+        mx = f > mx ? f : mx;
+    }
+    atomicMax(&lmax, mx);
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < kSelBins; i += blockDim.x)
+        if (lh[i]) atomicAdd(&hist[i], lh[i]);
+    if (threadIdx.x == 0) atomicMax(maxfo, (unsigned long long)lmax);
 }
 
-__global__ void k_surv_rank(const uint32_t *__restrict__ first, const uint32_t *__restrict__ bm,
-                            const uint64_t *__restrict__ prefix, const unsigned long long *__restrict__ d_n,
-                            uint64_t *__restrict__ rank) {
-    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= *d_n) return;
-    const uint32_t f = first[i];
-    rank[i] = prefix[f >> 5] + __popc(bm[f >> 5] & ((1u << (f & 31)) - 1u));
+__global__ void k_sel_collect(const uint32_t *__restrict__ dfo, const unsigned long long *__restrict__ d_n, int shift,
+                              const int *__restrict__ target, const uint32_t *__restrict__ off,
+                              uint32_t *__restrict__ fill, uint32_t *__restrict__ out) {
+    const uint64_t n = *d_n;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t f = dfo[i];
+        const int t = target[f >> shift];
+        if (t >= 0) out[off[t] + atomicAdd(&fill[t], 1u)] = f;
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -611,7 +575,7 @@ static uint64_t inv_odd(uint64_t a) {
 Pipeline::Pipeline(int device, int k, int c, bool reference_order) : dev_(device), k_(k), c_(c), ref_order_(reference_order) {
     hipSetDevice(dev_);
     hipStreamCreateWithFlags(&st_, hipStreamNonBlocking);
-    hipHostMalloc((void **)&h_small_, 64 * sizeof(uint64_t), hipHostMallocDefault);
+    hipHostMalloc((void **)&h_small_, 1024 * sizeof(uint64_t), hipHostMallocDefault);
     mx_.bits = 2 * k;
     mx_.m = kmask(k);
     mx_.a1 = 0x9E3779B97F4A7C15ull | 1;
@@ -669,10 +633,10 @@ int Pipeline::build(const uint8_t *d_in, uint64_t n) {
     SWAP_CHECK(tile_nl_.reserve(tiles * 4));
     SWAP_CHECK(tile_off_.reserve(tiles * 8));
     const uint64_t scan_words = scan_scratch_words(std::max<uint64_t>(tiles, 1)) + 16;
-    SWAP_CHECK(small_.reserve(64 * 8));
+    SWAP_CHECK(small_.reserve(1024 * 8));
     uint64_t *d_small = small_.as<uint64_t>();  // [0]=first_gt [1]=nnl [2]=j0 [3]=R [4]=N [5]=W [8..]=counters
     SWAP_CHECK(hipMemsetAsync(d_small, 0xFF, 8, st_));
-    SWAP_CHECK(hipMemsetAsync(d_small + 1, 0, 63 * 8, st_));
+    SWAP_CHECK(hipMemsetAsync(d_small + 1, 0, 1023 * 8, st_));
     for (int i = 0; i < kPhases; i++) ev_used_[i] = false;
     phase_begin(0);
     hipLaunchKernelGGL(k_tile_count, dim3((unsigned)tiles), dim3(kTileThreads), 0, st_, buf, n,
@@ -705,16 +669,18 @@ int Pipeline::build(const uint8_t *d_in, uint64_t n) {
     SWAP_CHECK(scan_scratch_.reserve((scan_scratch_words(R) + 16) * 8));
     hipLaunchKernelGGL(k_records, dim3((unsigned)((R + 255) / 256)), dim3(256), 0, st_, nl_pos_.as<uint64_t>(),
                        d_small + 2, R, k_, rec_start_.as<uint64_t>(), rec_len_.as<uint32_t>(),
-                       occ_count_.as<uint32_t>(), word_count_.as<uint32_t>(), (unsigned long long *)(d_small + 7));
+                       occ_count_.as<uint32_t>(), word_count_.as<uint32_t>(), (unsigned long long *)(d_small + 512));
     scan_excl<uint32_t>(occ_count_.as<uint32_t>(), occ_base_.as<uint64_t>(), R, scan_scratch_.as<uint64_t>(),
                         d_small + 4, st_);
     scan_excl<uint32_t>(word_count_.as<uint32_t>(), word_base_.as<uint64_t>(), R, scan_scratch_.as<uint64_t>(),
                         d_small + 5, st_);
     phase_end(0);
-    SWAP_CHECK(hipMemcpyAsync(hs + 4, d_small + 4, 4 * 8, hipMemcpyDeviceToHost, st_));
+    SWAP_CHECK(hipMemcpyAsync(hs + 4, d_small + 4, 2 * 8, hipMemcpyDeviceToHost, st_));
+    SWAP_CHECK(hipMemcpyAsync(hs + 512, d_small + 512, 512 * 8, hipMemcpyDeviceToHost, st_));
     SWAP_CHECK(hipStreamSynchronize(st_));
     const uint64_t N = hs[4], W = hs[5];
-    n_reads = hs[7];
+    n_reads = 0;
+    for (int i = 0; i < 32; i++) n_reads += hs[512 + 16 * i];
     n_occ = N;
     if (N >= 0xFFFFFFF0ull) return fail(-4, "more than 2^32 k-mers in one build");
     if (N == 0) return 0;
@@ -777,8 +743,7 @@ int Pipeline::build(const uint8_t *d_in, uint64_t n) {
     SWAP_CHECK(surv_mult_.reserve(N * 8));
     if (ref_order_) {
         SWAP_CHECK(surv_first_.reserve(N * 4));
-        SWAP_CHECK(bitmap_.reserve(((N + 31) / 32 + 1) * 4));
-        SWAP_CHECK(hipMemsetAsync(bitmap_.p, 0, ((N + 31) / 32 + 1) * 4, st_));
+        SWAP_CHECK(dfo_.reserve(N * 4));
     }
     SWAP_CHECK(overflow_.reserve(kParts * 4));
     CountArgs ca;
@@ -793,7 +758,7 @@ int Pipeline::build(const uint8_t *d_in, uint64_t n) {
     ca.surv_first = ref_order_ ? surv_first_.as<uint32_t>() : nullptr;
     ca.counters = (unsigned long long *)(d_small + 8);
     ca.overflow_list = overflow_.as<uint32_t>();
-    ca.bitmap = ref_order_ ? bitmap_.as<uint32_t>() : nullptr;
+    ca.dfo = ref_order_ ? dfo_.as<uint32_t>() : nullptr;
     phase_begin(4);
     if (ref_order_)
         hipLaunchKernelGGL(k_count_lds<true>, dim3(kParts), dim3(kCountThreads), 0, st_, ca);
@@ -845,28 +810,60 @@ int Pipeline::build(const uint8_t *d_in, uint64_t n) {
     }
     n_surv = hs[8];
     n_distinct = hs[9];
-    if (ref_order_ && n_surv) {
-        const uint64_t nw = (N + 31) / 32;
-        SWAP_CHECK(popc_.reserve(nw * 4));
-        SWAP_CHECK(surv_rank_.reserve(n_surv * 8));
+    if (ref_order_ && n_distinct) {
+        // growth ranks of the reference's k-mer table
+        std::vector<uint64_t> starts = table_epoch_starts(n_distinct, false);
+        int shift = 0;
+        while ((N >> shift) >= kSelBins) shift++;
+        SWAP_CHECK(sel_.reserve((2 * kSelBins + 64) * 4));
+        uint32_t *d_hist = sel_.as<uint32_t>();
+        SWAP_CHECK(hipMemsetAsync(d_hist, 0, kSelBins * 4, st_));
         SWAP_CHECK(hipMemsetAsync(d_small + 12, 0, 8, st_));
         phase_begin(6);
-        hipLaunchKernelGGL(k_popc_words, dim3((unsigned)((nw + 255) / 256)), dim3(256), 0, st_,
-                           bitmap_.as<uint32_t>(), nw, popc_.as<uint32_t>(), (unsigned long long *)(d_small + 12));
-        // the bitmap prefix reuses word_base_ (the pack and scatter are done)
-        SWAP_CHECK(word_base_.reserve(nw * 8 + 8));
-        SWAP_CHECK(scan_scratch_.reserve((scan_scratch_words(nw) + 16) * 8));
-        scan_excl<uint32_t>(popc_.as<uint32_t>(), word_base_.as<uint64_t>(), nw, scan_scratch_.as<uint64_t>(),
-                            d_small + 13, st_);
-        hipLaunchKernelGGL(k_surv_rank, dim3((unsigned)((n_surv + 255) / 256)), dim3(256), 0, st_,
-                           surv_first_.as<uint32_t>(), bitmap_.as<uint32_t>(), word_base_.as<uint64_t>(),
-                           (const unsigned long long *)(d_small + 8), surv_rank_.as<uint64_t>());
-        phase_end(6);
+        hipLaunchKernelGGL(k_sel_hist, dim3(ncu), dim3(1024), 0, st_, dfo_.as<uint32_t>(),
+                           (const unsigned long long *)(d_small + 9), shift, d_hist,
+                           (unsigned long long *)(d_small + 12));
+        std::vector<uint32_t> hist(kSelBins);
+        SWAP_CHECK(hipMemcpyAsync(hist.data(), d_hist, kSelBins * 4, hipMemcpyDeviceToHost, st_));
         SWAP_CHECK(hipMemcpyAsync(hs + 12, d_small + 12, 8, hipMemcpyDeviceToHost, st_));
         SWAP_CHECK(hipStreamSynchronize(st_));
         last_first_ = hs[12];
+        // bucket and in-bucket position of each wanted rank (starts[0] = 0 is the first insert)
+        std::vector<uint64_t> cum(kSelBins + 1, 0);
+        for (uint32_t i = 0; i < kSelBins; i++) cum[i + 1] = cum[i] + hist[i];
+        std::vector<int> target(kSelBins, -1);
+        std::vector<uint32_t> toff, tbin;
+        std::vector<std::pair<int, uint64_t>> want;  // (target, position in bucket) per start
+        for (uint64_t T : starts) {
+            const uint32_t b = (uint32_t)(std::upper_bound(cum.begin(), cum.end(), T) - cum.begin()) - 1;
+            if (target[b] < 0) {
+                target[b] = (int)tbin.size();
+                tbin.push_back(b);
+            }
+            want.push_back({target[b], T - cum[b]});
+        }
+        uint32_t tot = 0;
+        for (uint32_t b : tbin) { toff.push_back(tot); tot += hist[b]; }
+        SWAP_CHECK(sel_target_.reserve(kSelBins * 4 + toff.size() * 8 + 64));
+        int *d_target = sel_target_.as<int>();
+        uint32_t *d_off = reinterpret_cast<uint32_t *>(d_target + kSelBins);
+        uint32_t *d_fill = d_off + toff.size() + 1;
+        SWAP_CHECK(sel_out_.reserve((uint64_t)tot * 4 + 16));
+        SWAP_CHECK(hipMemcpyAsync(d_target, target.data(), kSelBins * 4, hipMemcpyHostToDevice, st_));
+        SWAP_CHECK(hipMemcpyAsync(d_off, toff.data(), toff.size() * 4, hipMemcpyHostToDevice, st_));
+        SWAP_CHECK(hipMemsetAsync(d_fill, 0, toff.size() * 4, st_));
+        hipLaunchKernelGGL(k_sel_collect, dim3(ncu * 2), dim3(1024), 0, st_, dfo_.as<uint32_t>(),
+                           (const unsigned long long *)(d_small + 9), shift, d_target, d_off, d_fill,
+                           sel_out_.as<uint32_t>());
+        phase_end(6);
+        std::vector<uint32_t> vals(tot);
+        SWAP_CHECK(hipMemcpyAsync(vals.data(), sel_out_.p, (uint64_t)tot * 4, hipMemcpyDeviceToHost, st_));
+        SWAP_CHECK(hipStreamSynchronize(st_));
+        for (size_t t = 0; t < tbin.size(); t++)
+            std::sort(vals.begin() + toff[t], vals.begin() + toff[t] + hist[tbin[t]]);
+        epoch_first_.clear();
+        for (auto &wv : want) epoch_first_.push_back(vals[toff[wv.first] + wv.second]);
     }
-    n_reads = hs[7];
     collect_phases();
     return 0;
 }
@@ -876,7 +873,7 @@ int Pipeline::export_kmer_graph(HostKmerGraph &out) {
     out.key.resize(n_surv);
     out.mult.resize(n_surv);
     out.first.clear();
-    out.rank.clear();
+    out.epoch_first = epoch_first_;
     out.distinct = n_distinct;
     // the table sees one more operator[] after the last new key when a later
     // input k-mer has an edge; taken as "any k-mer after the last first occurrence"
@@ -886,9 +883,7 @@ int Pipeline::export_kmer_graph(HostKmerGraph &out) {
         SWAP_CHECK(hipMemcpyAsync(out.mult.data(), surv_mult_.p, n_surv * 8, hipMemcpyDeviceToHost, st_));
         if (ref_order_) {
             out.first.resize(n_surv);
-            out.rank.resize(n_surv);
             SWAP_CHECK(hipMemcpyAsync(out.first.data(), surv_first_.p, n_surv * 4, hipMemcpyDeviceToHost, st_));
-            SWAP_CHECK(hipMemcpyAsync(out.rank.data(), surv_rank_.p, n_surv * 8, hipMemcpyDeviceToHost, st_));
         }
     }
     SWAP_CHECK(hipStreamSynchronize(st_));

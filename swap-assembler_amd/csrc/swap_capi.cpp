@@ -58,10 +58,7 @@ int ensure_graph(swap_ctx *c) {
         c->graph.mult[i] = g.mult[idx[i]];
         if (!g.first.empty()) c->graph.first[i] = g.first[idx[i]];
     }
-    if (!g.rank.empty()) {
-        c->graph.rank.resize(idx.size());
-        for (size_t i = 0; i < idx.size(); i++) c->graph.rank[i] = g.rank[idx[i]];
-    }
+    c->graph.epoch_first = g.epoch_first;
     c->graph.distinct = g.distinct;
     c->graph.tail_lookup = g.tail_lookup;
     c->have_graph = true;
@@ -72,9 +69,10 @@ int ensure_graph(swap_ctx *c) {
 // (reference order), else ascending key.
 void kmer_graph_order(swap_ctx *c, std::vector<uint32_t> &order) {
     order.clear();
-    if (c->graph.rank.empty()) return;
+    if (c->graph.first.empty()) return;
     uint64_t buckets = 0;
-    std::vector<uint32_t> visit = swapk::visit_order(c->graph.key, c->graph.rank, c->graph.distinct,
+    std::vector<uint64_t> first(c->graph.first.begin(), c->graph.first.end());
+    std::vector<uint32_t> visit = swapk::visit_order(c->graph.key, first, c->graph.epoch_first, c->graph.distinct,
                                                      c->graph.tail_lookup, &buckets);
     std::vector<uint32_t> pos(visit.size());
     for (size_t i = 0; i < visit.size(); i++) pos[visit[i]] = (uint32_t)i;

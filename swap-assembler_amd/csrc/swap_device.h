@@ -28,7 +28,7 @@ struct HostKmerGraph {
     std::vector<uint64_t> key;
     std::vector<uint64_t> mult;
     std::vector<uint32_t> first;  // first input k-mer index of the key (reference order mode)
-    std::vector<uint64_t> rank;   // number of distinct k-mers first seen before this one
+    std::vector<uint64_t> epoch_first;  // first-occurrence index of the key of each table growth
     uint64_t distinct = 0;        // D
     bool tail_lookup = false;     // an input k-mer with edges follows the last first occurrence
 };
@@ -41,6 +41,19 @@ struct HostCompacted {
     std::vector<uint32_t> len;       // 8 per node
     std::vector<uint8_t> mult;       // 8 per node
     std::vector<char> pool;          // edge strings, bases as 'A','C','G','T'
+};
+
+// Device merge results copied to the host.
+struct MergeDev {
+    uint64_t n_kept = 0;
+    std::vector<uint32_t> kept_src;   // survivor index of each kept node
+    std::vector<uint32_t> elen;       // 8 per kept node: edge string length (0 = no edge / replayed on host)
+    std::vector<uint64_t> eoff;       // 8 per kept node: offset into pool
+    std::vector<uint8_t> emult;       // 8 per kept node
+    std::vector<uint8_t> especial;    // 8 per kept node: edge enters a cycle / fold-back component
+    std::vector<char> pool;
+    std::vector<uint32_t> special;    // (1,1) survivors of cycle / fold-back components
+    uint64_t n11 = 0;
 };
 
 class Pipeline {
@@ -56,7 +69,7 @@ public:
     // Compacts unitigs (k-molecule merge).  prio: per-survivor visit rank in
     // survivor order (device or host array, may be null for the built-in
     // hash order).
-    int merge(const uint32_t *host_prio);
+    int merge_device(const uint64_t *host_prio, MergeDev &out);
 
     // Counters of the last build / merge.
     uint64_t n_reads = 0, n_occ = 0, n_distinct = 0, n_surv = 0, n_kept = 0, n_overflow = 0;
@@ -94,12 +107,18 @@ private:
     // build buffers
     DevBuf fasta_, tile_nl_, tile_off_, nl_pos_, scan_scratch_, small_, rec_start_, rec_len_, occ_count_,
         word_count_, occ_base_, word_base_, packed_, hist_, bucket_, cursor_, rec_, occ_, surv_key_, surv_mult_,
-        surv_first_, surv_rank_, popc_, overflow_, gtab_key_, gtab_cnt_, gtab_fo_, bitmap_;
+        surv_first_, dfo_, sel_, sel_target_, sel_out_, overflow_, gtab_key_, gtab_cnt_, gtab_fo_;
     // merge buffers
-    DevBuf idx_key_, idx_val_, node_, walk_, walk2_, kept_, edge_, pool_, chain_, tree_, prio_, misc_;
+    enum {
+        M_IKEY, M_IVAL, M_PRIO, M_IS11, M_NXT, M_EX, M_SELF, M_WALKA, M_WALKB, M_SPA, M_SPB, M_FLAG, M_POS,
+        M_KEPTOF, M_KEPTSRC, M_ELEN, M_EMULT, M_EOTHER, M_CSIZE, M_ESPEC, M_EOFF, M_COFF, M_POOL, M_CPRI,
+        M_CMUL, M_CEDGE, M_LEV, M_LEVOFF, M_PAR, M_NEED, M_NL, M_NR, M_INL, M_INR, M_CNT, M_SPEC, M_COUNT
+    };
+    DevBuf mb_[M_COUNT];
     // host mirrors of small results
     uint64_t *h_small_ = nullptr;
     uint64_t last_first_ = 0;
+    std::vector<uint64_t> epoch_first_;
     hipEvent_t ev_[2 * kPhases] = {};
     bool ev_used_[kPhases] = {};
 };

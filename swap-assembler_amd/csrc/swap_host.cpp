@@ -78,9 +78,14 @@ void sort_by_bucket(std::vector<uint32_t> &bucket, std::vector<uint32_t> &idx) {
 
 uint64_t table_buckets(uint64_t n_total, bool extra_resize) { return make_epochs(n_total, extra_resize).P.back(); }
 
-std::vector<uint32_t> table_order(const std::vector<uint64_t> &key, const std::vector<uint64_t> &rank,
-                                  uint64_t n_total, bool extra_resize, uint64_t *bucket_count) {
-    const Epochs ep = make_epochs(n_total, extra_resize);
+std::vector<uint64_t> table_epoch_starts(uint64_t n_total, bool extra_resize) {
+    return make_epochs(n_total, extra_resize).start;
+}
+
+namespace {
+
+std::vector<uint32_t> order_with_epochs(const std::vector<uint64_t> &key, const std::vector<uint64_t> &rank,
+                                        const Epochs &ep) {
     const int f = (int)ep.P.size() - 1;
     const size_t n = key.size();
     std::vector<uint64_t> h(n);
@@ -106,16 +111,31 @@ std::vector<uint32_t> table_order(const std::vector<uint64_t> &key, const std::v
         if (j - i > 1) std::sort(idx.begin() + i, idx.begin() + j, [&](uint32_t a, uint32_t b) { return before(a, b, f); });
         i = j;
     }
-    if (bucket_count) *bucket_count = ep.P[f];
     return idx;
 }
 
-std::vector<uint32_t> visit_order(const std::vector<uint64_t> &key, const std::vector<uint64_t> &rank,
-                                  uint64_t distinct, bool tail_lookup, uint64_t *node_buckets) {
+}  // namespace
+
+std::vector<uint32_t> table_order(const std::vector<uint64_t> &key, const std::vector<uint64_t> &rank,
+                                  uint64_t n_total, bool extra_resize, uint64_t *bucket_count) {
+    const Epochs ep = make_epochs(n_total, extra_resize);
+    if (bucket_count) *bucket_count = ep.P.back();
+    return order_with_epochs(key, rank, ep);
+}
+
+std::vector<uint32_t> visit_order(const std::vector<uint64_t> &key, const std::vector<uint64_t> &first,
+                                  const std::vector<uint64_t> &epoch_first, uint64_t distinct, bool tail_lookup,
+                                  uint64_t *node_buckets) {
     const size_t S = key.size();
     // kmolecules: every distinct k-mer inserted at its first occurrence
-    // (graph.cpp:1058-1070)
-    std::vector<uint32_t> o1 = table_order(key, rank, distinct, tail_lookup, nullptr);
+    // (graph.cpp:1058-1070).  Insertion order among survivors is the order of
+    // their first occurrences; epoch e starts at the first occurrence of the
+    // distinct key whose insert grew the table.
+    Epochs e1 = make_epochs(distinct, tail_lookup);
+    for (size_t e = 0; e < e1.start.size() && e < epoch_first.size(); e++) e1.start[e] = epoch_first[e];
+    if (e1.start.size() > epoch_first.size())  // growth on a lookup after the last insert
+        for (size_t e = epoch_first.size(); e < e1.start.size(); e++) e1.start[e] = ~0ull;
+    std::vector<uint32_t> o1 = order_with_epochs(key, first, e1);
     // cutoffGraph: survivors inserted into a fresh table in kmolecules order,
     // then copy-assigned back with the same buckets (graph.cpp:1096-1112)
     std::vector<uint64_t> r2(S);
@@ -143,6 +163,31 @@ std::vector<uint32_t> gather_order(const std::vector<uint64_t> &key, const std::
     return idx;
 }
 
+namespace {
+inline uint32_t code_of_base(char ch) {
+    switch (ch) { case 'C': return 1; case 'G': return 2; case 'T': return 3; default: return 0; }
+}
+}  // namespace
+
+// canonical terminal of orient(X, slot) + s: last k bases, max with its rc.
+// Returns +1 if the walk ends on the forward strand of the result.
+int terminal(uint64_t X, int slot, const char *s, size_t L, int k, uint64_t *id) {
+    const uint64_t km = kmask(k);
+    uint64_t t;
+    if (L >= (size_t)k) {
+        t = 0;
+        for (size_t j = L - k; j < L; j++) t = (t << 2) | code_of_base(s[j]);
+    } else {
+        t = slot >= 4 ? revcomp(X, k) : X;
+        for (size_t j = 0; j < L; j++) t = ((t << 2) | code_of_base(s[j])) & km;
+    }
+    const uint64_t r = revcomp(t, k);
+    if (t < r) { *id = r; return -1; }
+    *id = t;
+    return 1;
+}
+
+
 // ---------------------------------------------------------------------------
 // writers
 
@@ -163,24 +208,6 @@ inline void put_kmer(std::string &out, uint64_t x, int k) {
 
 inline uint32_t code_of(char ch) {
     switch (ch) { case 'C': return 1; case 'G': return 2; case 'T': return 3; default: return 0; }
-}
-
-// canonical terminal of orient(X, slot) + s: last k bases, max with its rc.
-// Returns +1 if the walk ends on the forward strand of the result.
-inline int terminal(uint64_t X, int slot, const char *s, size_t L, int k, uint64_t *id) {
-    const uint64_t km = kmask(k);
-    uint64_t t;
-    if (L >= (size_t)k) {
-        t = 0;
-        for (size_t j = L - k; j < L; j++) t = (t << 2) | code_of(s[j]);
-    } else {
-        t = slot >= 4 ? revcomp(X, k) : X;
-        for (size_t j = 0; j < L; j++) t = ((t << 2) | code_of(s[j])) & km;
-    }
-    const uint64_t r = revcomp(t, k);
-    if (t < r) { *id = r; return -1; }
-    *id = t;
-    return 1;
 }
 
 }  // namespace
@@ -264,6 +291,318 @@ void format_noce(int k, const HostCompacted &c, std::string &out) {
             out.push_back('\n');
         }
     }
+}
+
+}  // namespace swapk
+
+// ---------------------------------------------------------------------------
+// merge driver
+
+namespace swapk {
+
+namespace {
+
+// A node of a cycle / fold-back component (or a kept node next to one), with
+// the reference's string edges (graph.cpp:42-49).
+struct HNode {
+    uint64_t id = 0;
+    std::string arcs[8];
+    uint8_t m[8] = {0};
+    bool live = true;
+    bool one = false;  // exactly one left and one right edge
+    uint64_t prio = 0;
+    int64_t kept = -1;  // kept index, or -1 for a (1,1) node of the component
+};
+
+struct Replay {
+    int k;
+    uint64_t km;
+    std::unordered_map<uint64_t, HNode> nodes;
+
+    uint64_t orient(uint64_t X, int s) const { return s >= 4 ? revcomp(X, k) : X; }
+    // node::getNodeID (graph.cpp:164-196)
+    int neighbour(const HNode &n, int s, uint64_t *id) const {
+        return terminal(n.id, s, n.arcs[s].data(), n.arcs[s].size(), k, id);
+    }
+    // node::getSrcNodeEdge, direction 2 (graph.cpp:263-324)
+    int src_slot(const HNode &d, uint64_t src, int side) const {
+        for (int i = 0; i < 8; i++) {
+            uint64_t id;
+            const int r = neighbour(d, i, &id);
+            if (id == src && (r == 1 ? 1 : 0) == side) return i;
+        }
+        return -1;
+    }
+};
+
+inline unsigned char splice_service(unsigned char ma, size_t la, unsigned char mb, size_t lb) {
+    const size_t s = (size_t)ma * la + (size_t)mb * lb;
+    const int pre = (int)(unsigned)s;
+    return (unsigned char)((size_t)(long)pre / (la + lb));
+}
+
+inline unsigned char splice_master(unsigned char ma, size_t la, unsigned char mb, size_t lb) {
+    return (unsigned char)(((size_t)ma * la + (size_t)mb * lb) / (la + lb));
+}
+
+}  // namespace
+
+int host_merge(Pipeline &p, const swap_cfg &cfg, std::vector<uint32_t> &visit, HostCompacted &out) {
+    const bool ref = !(cfg.flags & SWAP_FLAG_HASH_ORDER);
+    const int k = cfg.k;
+    out = HostCompacted();
+    HostKmerGraph g;
+    int rc = p.export_kmer_graph(g);  // device order
+    if (rc) return rc;
+    const size_t S = g.key.size();
+    std::vector<uint64_t> prio;
+    uint64_t P3 = 0;
+    if (ref && S) {
+        std::vector<uint64_t> first(g.first.begin(), g.first.end());
+        std::vector<uint32_t> order = visit_order(g.key, first, g.epoch_first, g.distinct, g.tail_lookup, &P3);
+        prio.resize(S);
+        for (size_t i = 0; i < S; i++) prio[order[i]] = i;
+    }
+    MergeDev md;
+    rc = p.merge_device(ref && S ? prio.data() : nullptr, md);
+    if (rc) return rc;
+    if (!ref) {
+        // the device's built-in priority: the k-mer mixer, recomputed here
+        prio.resize(S);
+        Mixer mx;
+        mx.bits = 2 * k;
+        mx.m = kmask(k);
+        mx.a1 = 0x9E3779B97F4A7C15ull | 1;
+        mx.a2 = 0xC2B2AE3D27D4EB4Full | 1;
+        mx.s = k;
+        for (size_t i = 0; i < S; i++) prio[i] = mix_fwd(mx, g.key[i]);
+    }
+    const size_t nk = md.n_kept;
+    // per kept node: 8 (string, multiplicity) taken from the device results
+    struct Out {
+        uint64_t id;
+        uint64_t prio;
+        std::string arcs[8];
+        uint8_t m[8];
+    };
+    std::vector<Out> live;
+    live.reserve(nk + md.special.size());
+    std::unordered_map<uint64_t, size_t> live_of;  // key -> index in live (only kept nodes next to specials)
+    for (size_t i = 0; i < nk; i++) {
+        Out o;
+        const uint32_t sidx = md.kept_src[i];
+        o.id = g.key[sidx];
+        o.prio = prio[sidx];
+        bool any = false;
+        for (int s = 0; s < 8; s++) {
+            const uint32_t L = md.elen[8 * i + s];
+            o.m[s] = L ? md.emult[8 * i + s] : 0;
+            if (L) { o.arcs[s].assign(md.pool.data() + md.eoff[8 * i + s], L); any = true; }
+        }
+        (void)any;
+        live.push_back(std::move(o));
+    }
+    // ---- cycles and fold-backs: the reference's sequential semantics, replayed
+    if (!md.special.empty()) {
+        Replay R;
+        R.k = k;
+        R.km = kmask(k);
+        std::unordered_map<uint64_t, uint32_t> sidx_of;
+        sidx_of.reserve(S * 2);
+        for (size_t i = 0; i < S; i++) sidx_of[g.key[i]] = (uint32_t)i;
+        auto make = [&](uint32_t sidx, int64_t keptIdx) {
+            HNode n;
+            n.id = g.key[sidx];
+            n.prio = prio[sidx];
+            n.kept = keptIdx;
+            int nr = 0, nl = 0;
+            for (int s = 0; s < 8; s++) {
+                n.m[s] = (uint8_t)(g.mult[sidx] >> (8 * s));
+                if (n.m[s]) {
+                    n.arcs[s] = std::string(1, "ACGT"[s & 3]);
+                    (s < 4 ? nr : nl)++;
+                }
+            }
+            n.one = nr == 1 && nl == 1;
+            return n;
+        };
+        std::unordered_map<uint64_t, int64_t> kept_index;
+        for (size_t i = 0; i < nk; i++) kept_index[g.key[md.kept_src[i]]] = (int64_t)i;
+        for (uint32_t sidx : md.special) R.nodes[g.key[sidx]] = make(sidx, -1);
+        // kept neighbours of the component
+        std::vector<uint64_t> spec_keys;
+        for (auto &kv : R.nodes) spec_keys.push_back(kv.first);
+        for (uint64_t key : spec_keys) {
+            const HNode &n = R.nodes[key];
+            for (int s = 0; s < 8; s++) {
+                if (n.arcs[s].empty()) continue;
+                uint64_t id;
+                R.neighbour(n, s, &id);
+                if (R.nodes.count(id)) continue;
+                auto it = kept_index.find(id);
+                if (it == kept_index.end()) continue;  // cannot happen: neighbours are survivors
+                R.nodes[id] = make(sidx_of[id], it->second);
+            }
+        }
+        // all (1,1) priorities, for the deferred unlock of a half-locked neighbour
+        std::vector<uint64_t> one_prio;
+        auto next_one_after = [&](uint64_t after, uint64_t excluding_prio) -> uint64_t {
+            if (one_prio.empty()) {
+                for (size_t i = 0; i < S; i++) {
+                    const uint64_t m = g.mult[i];
+                    int nr = 0, nl = 0;
+                    for (int s = 0; s < 4; s++) nr += ((m >> (8 * s)) & 0xFF) != 0;
+                    for (int s = 4; s < 8; s++) nl += ((m >> (8 * s)) & 0xFF) != 0;
+                    if (nr == 1 && nl == 1) one_prio.push_back(prio[i]);
+                }
+                std::sort(one_prio.begin(), one_prio.end());
+            }
+            auto it = std::upper_bound(one_prio.begin(), one_prio.end(), after);
+            while (it != one_prio.end() && *it == excluding_prio) ++it;
+            return it == one_prio.end() ? ~0ull : *it;
+        };
+        // SWAP pass (graph.cpp:1392-1566) over the component's (1,1) nodes in visit order
+        std::vector<HNode *> order;
+        for (auto &kv : R.nodes)
+            if (kv.second.kept < 0) order.push_back(&kv.second);
+        std::sort(order.begin(), order.end(), [](const HNode *a, const HNode *b) { return a->prio < b->prio; });
+        bool pending = false;
+        uint64_t pendingId = 0, pendingAt = 0, pendingPrio = 0;
+        for (HNode *Mp : order) {
+            HNode &M = *Mp;
+            if (!M.live) continue;
+            if (pending && next_one_after(pendingAt, pendingPrio) < M.prio) pending = false;
+            if (pending && pendingId == M.id) continue;  // still locked: its own trylock fails
+            int li = -1, ri = -1, nl = 0, nr = 0;
+            for (int s = 4; s < 8; s++) if (!M.arcs[s].empty()) { nl++; li = s; }
+            for (int s = 0; s < 4; s++) if (!M.arcs[s].empty()) { nr++; ri = s; }
+            if (nl != 1 || nr != 1) continue;
+            uint64_t L, Rr;
+            R.neighbour(M, li, &L);
+            R.neighbour(M, ri, &Rr);
+            pending = false;
+            const bool lockL = L != M.id;
+            const bool lockR = (Rr != L) ? (Rr != M.id) : lockL;
+            if (!(lockL && lockR)) {
+                if (lockL || lockR) {
+                    pending = true;
+                    pendingId = lockL ? L : Rr;
+                    pendingAt = M.prio;
+                    auto it = R.nodes.find(pendingId);
+                    pendingPrio = (it != R.nodes.end() && it->second.kept < 0) ? it->second.prio : ~0ull;
+                }
+                continue;
+            }
+            const std::string leftEdge = M.arcs[li], rightEdge = M.arcs[ri];
+            const uint8_t leftM = M.m[li], rightM = M.m[ri];
+            const uint64_t dsts[2] = {Rr, L};
+            const int sides[2] = {0, 1};
+            const std::string *strs[2] = {&leftEdge, &rightEdge};
+            const uint8_t ms[2] = {leftM, rightM};
+            for (int q = 0; q < 2; q++) {
+                HNode &D = R.nodes[dsts[q]];
+                const int idx = R.src_slot(D, M.id, sides[q]);
+                if (idx < 0) return SWAP_E_DEVICE;
+                D.m[idx] = splice_service(D.m[idx], D.arcs[idx].size(), ms[q], strs[q]->size());
+                D.arcs[idx] += *strs[q];
+            }
+            M.live = false;
+        }
+        // masterSimplifyNodeGraph (graph.cpp:2089-2192) over the live (1,1) nodes left
+        std::vector<HNode *> rest;
+        for (auto &kv : R.nodes)
+            if (kv.second.kept < 0 && kv.second.live) rest.push_back(&kv.second);
+        if (ref && P3) {
+            std::sort(rest.begin(), rest.end(), [&](const HNode *a, const HNode *b) {
+                const uint64_t ba = ref_hash(a->id) % P3, bb = ref_hash(b->id) % P3;
+                if (ba != bb) return ba < bb;
+                return a->prio > b->prio;
+            });
+        } else {
+            std::sort(rest.begin(), rest.end(), [](const HNode *a, const HNode *b) { return a->prio < b->prio; });
+        }
+        for (HNode *Mp : rest) {
+            HNode &M = *Mp;
+            if (!M.live) continue;
+            int left = -1, right = -1, nl = 0, nr = 0;
+            for (int s = 0; s < 4; s++) if (!M.arcs[s].empty()) { right = s; nr++; }
+            for (int s = 4; s < 8; s++) if (!M.arcs[s].empty()) { left = s; nl++; }
+            if (nr != 1 || nl != 1) continue;
+            uint64_t rightNode, leftNode;
+            R.neighbour(M, right, &rightNode);
+            R.neighbour(M, left, &leftNode);
+            auto itR = R.nodes.find(rightNode), itL = R.nodes.find(leftNode);
+            if (itR == R.nodes.end() || itL == R.nodes.end()) continue;
+            const int rr = R.src_slot(itR->second, M.id, 0);
+            const int lr = R.src_slot(itL->second, M.id, 1);
+            if (rr == -1 || lr == -1) continue;
+            if (leftNode == M.id || rightNode == M.id) continue;
+            HNode &Rn = itR->second;
+            Rn.m[rr] = splice_master(Rn.m[rr], Rn.arcs[rr].size(), M.m[left], M.arcs[left].size());
+            Rn.arcs[rr] += M.arcs[left];
+            HNode &Ln = itL->second;
+            Ln.m[lr] = splice_master(Ln.m[lr], Ln.arcs[lr].size(), M.m[right], M.arcs[right].size());
+            Ln.arcs[lr] += M.arcs[right];
+            M.live = false;
+        }
+        // results: kept neighbours take the replayed strings on their component slots
+        for (auto &kv : R.nodes) {
+            HNode &n = kv.second;
+            if (n.kept >= 0) {
+                Out &o = live[(size_t)n.kept];
+                for (int s = 0; s < 8; s++)
+                    if (md.especial[8 * (size_t)n.kept + s]) { o.arcs[s] = n.arcs[s]; o.m[s] = n.m[s]; }
+            } else if (n.live) {
+                Out o;
+                o.id = n.id;
+                o.prio = n.prio;
+                for (int s = 0; s < 8; s++) { o.arcs[s] = n.arcs[s]; o.m[s] = n.arcs[s].empty() ? 0 : n.m[s]; }
+                live.push_back(std::move(o));
+            }
+        }
+    }
+    // drop nodes without edges (graph.cpp:587-590)
+    {
+        size_t w = 0;
+        for (size_t i = 0; i < live.size(); i++) {
+            bool any = false;
+            for (int s = 0; s < 8; s++) if (!live[i].arcs[s].empty()) any = true;
+            if (any) { if (w != i) live[w] = std::move(live[i]); w++; }
+        }
+        live.resize(w);
+    }
+    // output order: the table order after the gather (reference order), else by key
+    std::vector<uint32_t> ord(live.size());
+    std::iota(ord.begin(), ord.end(), 0u);
+    if (ref && P3) {
+        std::vector<uint64_t> keys(live.size());
+        std::vector<uint32_t> pos(live.size());
+        for (size_t i = 0; i < live.size(); i++) { keys[i] = live[i].id; pos[i] = (uint32_t)live[i].prio; }
+        ord = gather_order(keys, pos, P3);
+    } else {
+        std::sort(ord.begin(), ord.end(), [&](uint32_t a, uint32_t b) { return live[a].id < live[b].id; });
+    }
+    size_t pool_n = 0;
+    for (auto &o : live) for (int s = 0; s < 8; s++) pool_n += o.arcs[s].size();
+    out.key.resize(live.size());
+    out.off.resize(8 * live.size());
+    out.len.resize(8 * live.size());
+    out.mult.resize(8 * live.size());
+    out.pool.resize(pool_n);
+    size_t at = 0;
+    for (size_t r = 0; r < ord.size(); r++) {
+        const Out &o = live[ord[r]];
+        out.key[r] = o.id;
+        for (int s = 0; s < 8; s++) {
+            out.off[8 * r + s] = at;
+            out.len[8 * r + s] = (uint32_t)o.arcs[s].size();
+            out.mult[8 * r + s] = o.arcs[s].empty() ? 0 : o.m[s];
+            if (!o.arcs[s].empty()) { memcpy(out.pool.data() + at, o.arcs[s].data(), o.arcs[s].size()); at += o.arcs[s].size(); }
+        }
+    }
+    p.n_kept = live.size();
+    visit.clear();
+    return 0;
 }
 
 }  // namespace swapk

@@ -27,13 +27,18 @@ std::vector<uint32_t> table_order(const std::vector<uint64_t> &key, const std::v
 
 // Bucket count of the table after n inserts.
 uint64_t table_buckets(uint64_t n_total, bool extra_resize);
+// Insert ranks at which the table starts a new bucket count (first entry 0).
+std::vector<uint64_t> table_epoch_starts(uint64_t n_total, bool extra_resize);
 
 // The reference's visit order of the merge (the nodes table, graph.cpp:1123-1133)
-// from the survivors' first-occurrence ranks (rank among all distinct k-mers,
-// D of them) -- three table builds chained (kmolecules, cutoff copy, nodes).
-// Returns survivor indices in visit order and the nodes table bucket count.
-std::vector<uint32_t> visit_order(const std::vector<uint64_t> &key, const std::vector<uint64_t> &rank,
-                                  uint64_t distinct, bool tail_lookup, uint64_t *node_buckets);
+// -- three table builds chained (kmolecules, cutoff copy, nodes).  first[i] is
+// the first-occurrence index of survivor i; epoch_first[e] the first-occurrence
+// index of the distinct key whose insert started growth epoch e of the k-mer
+// table (D distinct keys in all).  Returns survivor indices in visit order and
+// the nodes table bucket count.
+std::vector<uint32_t> visit_order(const std::vector<uint64_t> &key, const std::vector<uint64_t> &first,
+                                  const std::vector<uint64_t> &epoch_first, uint64_t distinct, bool tail_lookup,
+                                  uint64_t *node_buckets);
 
 // Order after the gather (graph.cpp:3614-3615, 891): live nodes re-inserted, in
 // visit order, into the cleared nodes table (bucket count kept).

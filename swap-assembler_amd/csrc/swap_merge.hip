@@ -1,11 +1,760 @@
-// k-molecule merge (placeholder until the device merge lands).
+// k-molecule merge on gfx950: unitig compaction by list ranking, edge strings
+// by scatter, merged multiplicities by Cartesian-tree evaluation.
+//
+// Replaces (single rank):
+//   distNodeGraph::buildKmoleculeGraph    graph.cpp:1117-1137
+//   distNodeGraph::simplifyDistNodeGraph  graph.cpp:1392-1566 (SWAP)
+//   distNodeGraph::masterSimplifyNodeGraph graph.cpp:2089-2192
+//
+// The reference absorbs, in its table's visit order, every node with exactly
+// one left and one right edge (graph.cpp:1429-1438): the right neighbour's
+// back edge is extended by the node's left edge and vice versa, and each
+// extended edge takes m = floor((m_a*len_a + m_b*len_b) / (len_a + len_b))
+// (graph.cpp:1339-1348).  Absorbing along a maximal chain of such nodes between
+// two kept nodes therefore yields one edge string per end -- the unitig read
+// from that end -- and a multiplicity equal to evaluating the chain's
+// Cartesian tree keyed by visit time (the last-visited node is the root,
+// combining the merged left and right parts).  Here:
+//   * nodes are looked up through a device hash index of the survivors;
+//   * chains are ranked by pointer jumping over oriented nodes (2 per node);
+//   * each oriented node writes its base into the string of the end it is
+//     reached from; kept-to-kept edges keep their one base;
+//   * nearest-larger-priority neighbours come from a max hierarchy, and the
+//     tree is evaluated bottom-up with one thread climbing from each leaf, the
+//     last-arriving child finishing its parent.
+// Chains that close on themselves (cycles) or fold back onto a node's other
+// strand (palindromic paths) depend on the exact sequence of lock outcomes in
+// the reference; they are rare and are replayed on the host.
 #include "swap_host.h"
+#include "swap_scan.h"
+
+#include <algorithm>
+#include <chrono>
+#include <cstdio>
+#include <map>
+#include <numeric>
+#include <unordered_map>
 
 namespace swapk {
 
-int host_merge(Pipeline &p, const swap_cfg &cfg, std::vector<uint32_t> &visit, HostCompacted &out) {
-    (void)p; (void)cfg; (void)visit; (void)out;
-    return SWAP_E_STATE;
+constexpr uint32_t kNil = 0xFFFFFFFFu;
+constexpr uint64_t kInf = ~0ull;
+constexpr uint64_t kIdxEmpty = ~0ull;
+
+// ---------------------------------------------------------------------------
+// survivor index
+
+__device__ __forceinline__ uint32_t idx_hash(uint64_t key, uint32_t mask) {
+    return (uint32_t)((key * 0xD6E8FEB86659FD93ull) >> 32) & mask;
+}
+
+__global__ void k_index_build(const uint64_t *__restrict__ key, uint64_t n, uint64_t *ikey, uint32_t *ival,
+                              uint32_t mask) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint64_t k = key[i];
+    uint32_t s = idx_hash(k, mask);
+    while (true) {
+        const uint64_t prev = atomicCAS((unsigned long long *)&ikey[s], (unsigned long long)kIdxEmpty,
+                                        (unsigned long long)k);
+        if (prev == kIdxEmpty || prev == k) {
+            ival[s] = (uint32_t)i;
+            return;
+        }
+        s = (s + 1) & mask;
+    }
+}
+
+__device__ __forceinline__ uint32_t index_find(const uint64_t *__restrict__ ikey, const uint32_t *__restrict__ ival,
+                                               uint32_t mask, uint64_t k) {
+    uint32_t s = idx_hash(k, mask);
+    while (true) {
+        const uint64_t cur = ikey[s];
+        if (cur == k) return ival[s];
+        if (cur == kIdxEmpty) return kNil;
+        s = (s + 1) & mask;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// classification
+
+struct MergeArgs {
+    const uint64_t *key;
+    const uint64_t *mult;
+    uint64_t S;
+    int k;
+    const uint64_t *ikey;
+    const uint32_t *ival;
+    uint32_t imask;
+};
+
+__device__ __forceinline__ uint64_t orient_kmer(uint64_t X, int slot, int k) { return slot >= 4 ? revcomp(X, k) : X; }
+
+// one step along slot s of X: returns target oriented id (2j+o), or kNil if missing
+__device__ __forceinline__ uint32_t hop(const MergeArgs &a, uint64_t X, int s) {
+    const uint64_t km = kmask(a.k);
+    const uint64_t t = ((orient_kmer(X, s, a.k) << 2) | (uint64_t)(s & 3)) & km;
+    const uint64_t r = revcomp(t, a.k);
+    const uint64_t Y = t > r ? t : r;
+    const uint32_t j = index_find(a.ikey, a.ival, a.imask, Y);
+    if (j == kNil) return kNil;
+    return (j << 1) | (t == Y ? 0u : 1u);
+}
+
+__device__ __forceinline__ int nz_count(uint64_t m, int lo) {
+    int c = 0;
+#pragma unroll
+    for (int t = 0; t < 4; t++) c += ((m >> (8 * (lo + t))) & 0xFF) != 0;
+    return c;
+}
+
+__device__ __forceinline__ int nz_slot(uint64_t m, int lo) {
+    int s = -1;
+#pragma unroll
+    for (int t = 0; t < 4; t++)
+        if ((m >> (8 * (lo + t))) & 0xFF) s = lo + t;
+    return s;
+}
+
+// is11[i]; for (1,1) nodes: nxt[2i] (through the right slot), nxt[2i+1] (left
+// slot), exit multiplicities ex[2i], ex[2i+1], self-adjacency flag.
+__global__ void k_classify(MergeArgs a, uint8_t *__restrict__ is11, uint32_t *__restrict__ nxt,
+                           uint8_t *__restrict__ ex, uint8_t *__restrict__ selfadj,
+                           unsigned long long *__restrict__ counters /*[0]=n11 [1]=missing*/) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.S) return;
+    const uint64_t m = a.mult[i];
+    const bool one = nz_count(m, 0) == 1 && nz_count(m, 4) == 1;
+    is11[i] = one;
+    if (!one) return;
+    const uint64_t X = a.key[i];
+    const int r = nz_slot(m, 0), l = nz_slot(m, 4);
+    const uint32_t nr = hop(a, X, r), nl = hop(a, X, l);
+    nxt[2 * i] = nr;
+    nxt[2 * i + 1] = nl;
+    ex[2 * i] = (uint8_t)(m >> (8 * r));
+    ex[2 * i + 1] = (uint8_t)(m >> (8 * l));
+    selfadj[i] = ((nr >> 1) == (uint32_t)i) || ((nl >> 1) == (uint32_t)i);
+    if (nr == kNil || nl == kNil) atomicAdd(&counters[1], 1ull);
+}
+
+// Pointer-jumping state per oriented (1,1) node v = 2i+o.
+struct Walk {
+    uint32_t succ;   // next oriented (1,1) node still to jump over, or kNil
+    uint32_t dist;   // nodes from v to the terminal (v included)
+    uint32_t term;   // arrival at the kept node: 2*K + orientation
+    uint32_t last;   // last oriented (1,1) node before the terminal
+};
+
+__global__ void k_walk_init(const uint8_t *__restrict__ is11, const uint32_t *__restrict__ nxt,
+                            const uint8_t *__restrict__ selfadj, uint64_t S, Walk *__restrict__ w,
+                            uint8_t *__restrict__ sp) {
+    const uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= 2 * S) return;
+    const uint64_t i = v >> 1;
+    Walk x;
+    if (!is11[i]) {
+        x.succ = kNil; x.dist = 0; x.term = kNil; x.last = kNil;
+        w[v] = x;
+        sp[v] = 0;
+        return;
+    }
+    const uint32_t t = nxt[v];
+    x.dist = 1;
+    x.last = (uint32_t)v;
+    if (t == kNil || !is11[t >> 1]) {
+        x.succ = kNil;
+        x.term = t;
+    } else {
+        x.succ = t;
+        x.term = kNil;
+    }
+    w[v] = x;
+    sp[v] = selfadj[i];
+}
+
+__global__ void k_walk_jump(const Walk *__restrict__ win, const uint8_t *__restrict__ spin, uint64_t n2,
+                            Walk *__restrict__ wout, uint8_t *__restrict__ spout, unsigned int *__restrict__ changed) {
+    const uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= n2) return;
+    Walk x = win[v];
+    uint8_t f = spin[v];
+    if (x.succ != kNil) {
+        const Walk y = win[x.succ];
+        f |= spin[x.succ];
+        x.dist += y.dist;
+        x.succ = y.succ;
+        x.term = y.term;
+        x.last = y.last;
+    }
+    wout[v] = x;
+    spout[v] = f;
+}
+
+// kept-edge id of the end a walk arrives at: slot of the kept node K whose own
+// walk enters the chain at `last` (reversed).
+__device__ __forceinline__ uint32_t end_slot(const uint64_t *__restrict__ key, uint32_t term, uint32_t last, int k) {
+    const uint32_t a = term & 1u;
+    const uint64_t L = key[last >> 1];
+    const uint32_t first = (last & 1u) ? (uint32_t)(3 - (L & 3)) : (uint32_t)((L >> (2 * (k - 1))) & 3);
+    return (a ? 0u : 4u) + (3u - first);
+}
+
+// kept index of survivors (exclusive scan of !is11 gives it); per kept edge:
+// length, initial multiplicity, owner flag and chain size.
+struct EdgeArgs {
+    const uint64_t *key;
+    const uint64_t *mult;
+    const uint8_t *is11;
+    const Walk *walk;
+    const uint8_t *sp;
+    const uint32_t *kept_of;   // survivor -> kept index (valid for kept survivors)
+    const uint32_t *kept_src;  // kept index -> survivor
+    uint64_t n_kept;
+    MergeArgs ma;
+    uint32_t *elen;     // 8 per kept node (0: empty or special)
+    uint8_t *emult;     // 8 per kept node
+    uint32_t *eother;   // other end kept-edge id (for chains), else kNil
+    uint32_t *csize;    // owner: chain nodes + 2 sentinels, else 0
+    uint8_t *especial;  // 1 if the edge enters a cycle / fold-back component
+    unsigned long long *counters;  // [2] special edges
+};
+
+__global__ void k_kept_flags(const uint8_t *__restrict__ is11, uint64_t S, uint32_t *__restrict__ flag) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < S) flag[i] = is11[i] ? 0u : 1u;
+}
+
+__global__ void k_kept_list(const uint8_t *__restrict__ is11, const uint64_t *__restrict__ pos, uint64_t S,
+                            uint32_t *__restrict__ kept_of, uint32_t *__restrict__ kept_src) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= S) return;
+    if (!is11[i]) {
+        kept_of[i] = (uint32_t)pos[i];
+        kept_src[pos[i]] = (uint32_t)i;
+    }
+}
+
+__global__ void k_kept_edges(EdgeArgs e) {
+    const uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= 8 * e.n_kept) return;
+    const uint32_t kap = (uint32_t)(g >> 3);
+    const int s = (int)(g & 7);
+    const uint32_t i = e.kept_src[kap];
+    const uint64_t m = e.mult[i];
+    const uint8_t mm = (uint8_t)(m >> (8 * s));
+    e.emult[g] = mm;
+    e.elen[g] = 0;
+    e.eother[g] = kNil;
+    e.csize[g] = 0;
+    e.especial[g] = 0;
+    if (!mm) return;
+    const uint32_t t = hop(e.ma, e.key[i], s);
+    if (t == kNil) return;
+    const uint32_t j = t >> 1;
+    if (!e.is11[j]) {
+        e.elen[g] = 1;
+        return;
+    }
+    const Walk w = e.walk[t];
+    if (w.succ != kNil || e.sp[t] || w.term == kNil) {
+        e.especial[g] = 1;
+        atomicAdd(&e.counters[2], 1ull);
+        return;
+    }
+    e.elen[g] = w.dist + 1;
+    const uint32_t K2 = w.term >> 1;
+    const uint32_t other = 8 * e.kept_of[K2] + end_slot(e.key, w.term, w.last, e.ma.k);
+    e.eother[g] = other;
+    if ((uint32_t)g < other) e.csize[g] = w.dist + 2;
+}
+
+// ---------------------------------------------------------------------------
+// strings and chain arrays
+
+struct FillArgs {
+    const uint64_t *key;
+    const uint8_t *is11;
+    const Walk *walk;
+    const uint8_t *sp;
+    const uint32_t *kept_of;
+    const uint64_t *eoff;     // pool offset per kept edge
+    const uint32_t *elen;
+    const uint64_t *coff;     // chain segment offset per owner edge
+    const uint32_t *eother;
+    const uint8_t *ex;        // exit multiplicities per oriented node
+    const uint64_t *prio;     // visit priority per survivor
+    uint64_t S;
+    int k;
+    char *pool;
+    uint64_t *cpri;           // chain priorities (sentinels = kInf)
+    uint8_t *cmul;            // unit-edge multiplicity leaving each chain position
+    uint32_t *cedge;          // at sentinel positions: the kept edge id
+};
+
+__device__ __forceinline__ char base_char(uint32_t b) { return "ACGT"[b & 3]; }
+
+__device__ __forceinline__ uint32_t last_base(uint64_t X, uint32_t o, int k) {
+    return o ? (uint32_t)(3 - ((X >> (2 * (k - 1))) & 3)) : (uint32_t)(X & 3);
+}
+
+// For every normal (1,1) node and both orientations: its base in the string of
+// the edge it is reached from; and its slot in the owner's chain array.
+__global__ void k_fill_nodes(FillArgs f) {
+    const uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= 2 * f.S) return;
+    const uint64_t j = v >> 1;
+    if (!f.is11[j]) return;
+    const uint32_t o = (uint32_t)(v & 1);
+    const uint64_t vr = v ^ 1;  // the walk back to the end this orientation starts from
+    const Walk b = f.walk[vr];
+    if (b.succ != kNil || b.term == kNil || f.sp[v] || f.sp[vr]) return;
+    const Walk a = f.walk[v];
+    const uint32_t startEdge = 8 * f.kept_of[b.term >> 1] + end_slot(f.key, b.term, b.last, f.k);
+    const uint64_t X = f.key[j];
+    f.pool[f.eoff[startEdge] + b.dist - 1] = base_char(last_base(X, o, f.k));
+    // chain array (written once, from the owner's direction)
+    const uint32_t endEdge = 8 * f.kept_of[a.term >> 1] + end_slot(f.key, a.term, a.last, f.k);
+    if (startEdge < endEdge) {
+        const uint64_t base = f.coff[startEdge];
+        const uint64_t pos = base + b.dist;  // sentinel at base, nodes at 1..n
+        f.cpri[pos] = f.prio[j];
+        f.cmul[pos] = f.ex[v];
+    }
+}
+
+// kept edges: final base; chain sentinels and the first unit edge
+__global__ void k_fill_edges(FillArgs f, const uint8_t *__restrict__ emult, uint64_t n_edges) {
+    const uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= n_edges) return;
+    const uint32_t L = f.elen[g];
+    if (!L) return;
+    // the last base of the edge is the last base of the oriented k-mer reached
+    // one hop (L == 1) or a chain (L > 1) away; computed by the caller-side walk
+    // through the survivor arrays would need the terminal, which is recorded in
+    // eother for chains -- for L == 1 it is the slot base itself.
+    if (L == 1) {
+        f.pool[f.eoff[g]] = base_char((uint32_t)(g & 7));
+        return;
+    }
+    const uint32_t other = f.eother[g];
+    if ((uint32_t)g < other) {
+        const uint64_t base = f.coff[g];
+        const uint64_t n = L - 1;
+        f.cpri[base] = kInf;
+        f.cmul[base] = emult[g];
+        f.cedge[base] = (uint32_t)g;
+        f.cpri[base + n + 1] = kInf;
+        f.cmul[base + n + 1] = 0;
+        f.cedge[base + n + 1] = other;
+    }
+}
+
+// Last base of every chain edge: the other end's kept node, as reached.
+__global__ void k_fill_last(const uint32_t *__restrict__ elen, const uint32_t *__restrict__ eother,
+                            const uint32_t *__restrict__ kept_src, const uint64_t *__restrict__ key,
+                            const uint64_t *__restrict__ eoff, uint64_t n_edges, int k, char *__restrict__ pool) {
+    const uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= n_edges) return;
+    const uint32_t L = elen[g];
+    if (L < 2) return;
+    const uint32_t other = eother[g];
+    // arrival orientation at the other kept node: its slot half says from which
+    // strand its own walk leaves, the arrival is on the opposite strand
+    const uint64_t K2 = key[kept_src[other >> 3]];
+    const uint32_t oslot = other & 7;
+    const uint32_t arr = oslot < 4 ? 1u : 0u;  // other walks from K2 (slots 0-3) => we arrive at rc(K2)
+    pool[eoff[g] + L - 1] = base_char(last_base(K2, arr, k));
+}
+
+// ---------------------------------------------------------------------------
+// nearest larger priority (max hierarchy) and Cartesian-tree evaluation
+
+__global__ void k_max_level(const uint64_t *__restrict__ in, uint64_t n_in, uint64_t *__restrict__ out,
+                            uint64_t n_out) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n_out) return;
+    const uint64_t a = 2 * i < n_in ? in[2 * i] : 0;
+    const uint64_t b = 2 * i + 1 < n_in ? in[2 * i + 1] : 0;
+    out[i] = a > b ? a : b;
+}
+
+struct TreeArgs {
+    const uint64_t *lev;      // concatenated levels
+    const uint64_t *lev_off;  // offset of each level
+    int nlev;
+    uint64_t M;               // chain array length
+    const uint8_t *cmul;
+    const uint32_t *cedge;
+    uint32_t *par;            // parent position, kNil for roots / sentinels
+    uint8_t *need;            // children to wait for (0..2); 255 for sentinels
+    uint32_t *nl;
+    uint32_t *nr;
+};
+
+__device__ __forceinline__ uint64_t levv(const TreeArgs &t, int l, uint64_t i) { return t.lev[t.lev_off[l] + i]; }
+
+__device__ uint64_t nearest_left(const TreeArgs &t, uint64_t x, uint64_t v) {
+    int64_t p = (int64_t)x - 1;
+    int l = 0;
+    while (true) {
+        while (l + 1 < t.nlev && (((uint64_t)p + 1) & ((2ull << l) - 1)) == 0) l++;
+        if (levv(t, l, (((uint64_t)p + 1) >> l) - 1) > v) break;
+        p -= (int64_t)1 << l;
+    }
+    while (l > 0) {
+        l--;
+        if (levv(t, l, (((uint64_t)p + 1) >> l) - 1) <= v) p -= (int64_t)1 << l;
+    }
+    return (uint64_t)p;
+}
+
+__device__ uint64_t nearest_right(const TreeArgs &t, uint64_t x, uint64_t v) {
+    uint64_t p = x + 1;
+    int l = 0;
+    while (true) {
+        while (l + 1 < t.nlev && (p & ((2ull << l) - 1)) == 0) l++;
+        if (levv(t, l, p >> l) > v) break;
+        p += 1ull << l;
+    }
+    while (l > 0) {
+        l--;
+        if (levv(t, l, p >> l) <= v) p += 1ull << l;
+    }
+    return p;
+}
+
+__global__ void k_tree_links(TreeArgs t) {
+    const uint64_t q = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= t.M) return;
+    const uint64_t v = t.lev[q];
+    if (v == kInf) {
+        t.par[q] = kNil;
+        t.need[q] = 255;
+        return;
+    }
+    const uint64_t L = nearest_left(t, q, v), R = nearest_right(t, q, v);
+    const uint64_t vl = t.lev[L], vr = t.lev[R];
+    t.nl[q] = (uint32_t)L;
+    t.nr[q] = (uint32_t)R;
+    if (vl == kInf && vr == kInf) t.par[q] = kNil;
+    else t.par[q] = (uint32_t)(vl < vr ? L : R);
+    t.need[q] = (uint8_t)((L != q - 1) + (R != q + 1));
+}
+
+// merged (multiplicity, length) of two spliced edges, with the reference's
+// arithmetic: the sum is formed in size_t, stored into an int, divided by the
+// size_t length and stored into an unsigned char (graph.cpp:1343-1345).
+__device__ __forceinline__ uint64_t splice(uint64_t a, uint64_t b) {
+    const uint64_t ma = a & 0xFF, la = a >> 8, mb = b & 0xFF, lb = b >> 8;
+    const uint64_t s = ma * la + mb * lb;
+    const int32_t pre = (int32_t)(uint32_t)s;
+    const uint64_t q = (uint64_t)(int64_t)pre / (la + lb);
+    return ((la + lb) << 8) | (q & 0xFF);
+}
+
+__global__ void k_tree_eval(TreeArgs t, uint64_t *inL, uint64_t *inR, uint32_t *cnt, uint8_t *emult_out) {
+    const uint64_t q0 = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q0 >= t.M) return;
+    if (t.need[q0] != 0) return;
+    uint64_t q = q0;
+    auto unit = [&](uint64_t pos) -> uint64_t { return (1ull << 8) | t.cmul[pos]; };
+    uint64_t val = splice(unit(q - 1), unit(q));
+    while (true) {
+        const uint32_t p = t.par[q];
+        if (p == kNil) {
+            // root: both ends of the chain get the value
+            const uint8_t m = (uint8_t)(val & 0xFF);
+            emult_out[t.cedge[t.nl[q]]] = m;
+            emult_out[t.cedge[t.nr[q]]] = m;
+            return;
+        }
+        uint64_t *slot = (q < p) ? &inL[p] : &inR[p];
+        __hip_atomic_store(slot, val, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const uint32_t old = __hip_atomic_fetch_add(&cnt[p], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        if (old + 1 < t.need[p]) return;  // the sibling finishes p
+        const uint64_t left = (t.nl[p] == p - 1) ? unit(p - 1)
+                                                 : __hip_atomic_load(&inL[p], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const uint64_t right = (t.nr[p] == p + 1) ? unit(p)
+                                                  : __hip_atomic_load(&inR[p], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        val = splice(left, right);
+        q = p;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// helpers
+
+__global__ void k_prio_hash(const uint64_t *__restrict__ key, uint64_t S, Mixer mx, uint64_t *__restrict__ prio) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < S) prio[i] = mix_fwd(mx, key[i]);
+}
+
+__global__ void k_u32_to_u64(const uint32_t *__restrict__ in, uint64_t n, uint64_t *__restrict__ out) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = in[i];
+}
+
+template <typename T>
+__global__ void k_fill(T *__restrict__ p, uint64_t n, T v) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+static inline unsigned nblk(uint64_t n, unsigned b = 256) { return (unsigned)((n + b - 1) / b); }
+
+}  // namespace swapk
+
+namespace swapk {
+
+__global__ void k_special_flags(const uint8_t *__restrict__ is11, const Walk *__restrict__ w, const uint8_t *__restrict__ sp,
+                                uint64_t S, uint32_t *__restrict__ list, unsigned long long *__restrict__ n) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    bool f = false;
+    if (i < S && is11[i]) {
+        const Walk a = w[2 * i], b = w[2 * i + 1];
+        f = a.succ != kNil || b.succ != kNil || a.term == kNil || b.term == kNil || sp[2 * i] || sp[2 * i + 1];
+    }
+    const uint64_t bal = __ballot(f);
+    if (!bal) return;
+    const int lane = threadIdx.x & 63;
+    unsigned long long base = 0;
+    if (lane == __ffsll((long long)bal) - 1) base = atomicAdd(n, (unsigned long long)__popcll(bal));
+    base = __shfl(base, __ffsll((long long)bal) - 1, 64);
+    if (f) list[base + __popcll(bal & ((1ull << lane) - 1))] = (uint32_t)i;
+}
+
+#define MCHECK(x)                                                            \
+    do {                                                                     \
+        hipError_t e_ = (x);                                                 \
+        if (e_ != hipSuccess) return fail(-5, hipGetErrorString(e_));        \
+    } while (0)
+
+int Pipeline::merge_device(const uint64_t *host_prio, MergeDev &out) {
+    MCHECK(hipSetDevice(dev_));
+    out = MergeDev();
+    const uint64_t S = n_surv;
+    if (S == 0) return 0;
+    if (S >= 0x7FFFFFF0ull) return fail(-4, "too many survivors for the merge");
+    uint64_t *d_small = small_.as<uint64_t>();
+    uint64_t *hs = h_small_;
+    MCHECK(hipMemsetAsync(d_small + 16, 0, 16 * 8, st_));
+    unsigned long long *cnt = (unsigned long long *)(d_small + 16);
+    // --- survivor index
+    uint32_t cap = 1024;
+    while (cap < 2 * S) cap <<= 1;
+    MCHECK(mb_[M_IKEY].reserve((uint64_t)cap * 8));
+    MCHECK(mb_[M_IVAL].reserve((uint64_t)cap * 4));
+    MCHECK(hipMemsetAsync(mb_[M_IKEY].p, 0xFF, (uint64_t)cap * 8, st_));
+    phase_begin(8);
+    hipLaunchKernelGGL(k_index_build, dim3(nblk(S)), dim3(256), 0, st_, surv_key_.as<uint64_t>(), S,
+                       mb_[M_IKEY].as<uint64_t>(), mb_[M_IVAL].as<uint32_t>(), cap - 1);
+    // --- priorities (visit time)
+    MCHECK(mb_[M_PRIO].reserve(S * 8));
+    if (host_prio)
+        MCHECK(hipMemcpyAsync(mb_[M_PRIO].p, host_prio, S * 8, hipMemcpyHostToDevice, st_));
+    else
+        hipLaunchKernelGGL(k_prio_hash, dim3(nblk(S)), dim3(256), 0, st_, surv_key_.as<uint64_t>(), S, mx_,
+                           mb_[M_PRIO].as<uint64_t>());
+    // --- classification
+    MergeArgs ma;
+    ma.key = surv_key_.as<uint64_t>();
+    ma.mult = surv_mult_.as<uint64_t>();
+    ma.S = S;
+    ma.k = k_;
+    ma.ikey = mb_[M_IKEY].as<uint64_t>();
+    ma.ival = mb_[M_IVAL].as<uint32_t>();
+    ma.imask = cap - 1;
+    MCHECK(mb_[M_IS11].reserve(S));
+    MCHECK(mb_[M_NXT].reserve(2 * S * 4));
+    MCHECK(mb_[M_EX].reserve(2 * S));
+    MCHECK(mb_[M_SELF].reserve(S));
+    hipLaunchKernelGGL(k_classify, dim3(nblk(S)), dim3(256), 0, st_, ma, mb_[M_IS11].as<uint8_t>(),
+                       mb_[M_NXT].as<uint32_t>(), mb_[M_EX].as<uint8_t>(), mb_[M_SELF].as<uint8_t>(), cnt);
+    phase_end(8);
+    // --- list ranking by pointer jumping over oriented (1,1) nodes
+    MCHECK(mb_[M_WALKA].reserve(2 * S * sizeof(Walk)));
+    MCHECK(mb_[M_WALKB].reserve(2 * S * sizeof(Walk)));
+    MCHECK(mb_[M_SPA].reserve(2 * S));
+    MCHECK(mb_[M_SPB].reserve(2 * S));
+    Walk *wa = mb_[M_WALKA].as<Walk>(), *wb = mb_[M_WALKB].as<Walk>();
+    uint8_t *spa = mb_[M_SPA].as<uint8_t>(), *spb = mb_[M_SPB].as<uint8_t>();
+    phase_begin(9);
+    hipLaunchKernelGGL(k_walk_init, dim3(nblk(2 * S)), dim3(256), 0, st_, mb_[M_IS11].as<uint8_t>(),
+                       mb_[M_NXT].as<uint32_t>(), mb_[M_SELF].as<uint8_t>(), S, wa, spa);
+    int rounds = 1;
+    while ((1ull << rounds) < 2 * S + 2) rounds++;
+    rounds++;
+    for (int r = 0; r < rounds; r++) {
+        hipLaunchKernelGGL(k_walk_jump, dim3(nblk(2 * S)), dim3(256), 0, st_, wa, spa, 2 * S, wb, spb,
+                           (unsigned int *)(d_small + 24));
+        std::swap(wa, wb);
+        std::swap(spa, spb);
+    }
+    phase_end(9);
+    // --- kept nodes
+    MCHECK(mb_[M_FLAG].reserve(S * 4));
+    MCHECK(mb_[M_POS].reserve((S + 1) * 8));
+    MCHECK(scan_scratch_.reserve((scan_scratch_words(S + 1) + 16) * 8));
+    phase_begin(10);
+    hipLaunchKernelGGL(k_kept_flags, dim3(nblk(S)), dim3(256), 0, st_, mb_[M_IS11].as<uint8_t>(), S,
+                       mb_[M_FLAG].as<uint32_t>());
+    scan_excl<uint32_t>(mb_[M_FLAG].as<uint32_t>(), mb_[M_POS].as<uint64_t>(), S, scan_scratch_.as<uint64_t>(),
+                        d_small + 20, st_);
+    MCHECK(mb_[M_SPEC].reserve(S * 4));
+    hipLaunchKernelGGL(k_special_flags, dim3(nblk(S)), dim3(256), 0, st_, mb_[M_IS11].as<uint8_t>(), wa, spa, S,
+                       mb_[M_SPEC].as<uint32_t>(), (unsigned long long *)(d_small + 21));
+    MCHECK(hipMemcpyAsync(hs + 16, d_small + 16, 8 * 8, hipMemcpyDeviceToHost, st_));
+    MCHECK(hipStreamSynchronize(st_));
+    const uint64_t n11 = hs[16], missing = hs[17], nk = hs[20], nspec = hs[21];
+    if (missing) return fail(-5, "edge to a k-mer missing from the graph (edge symmetry broken)");
+    out.n11 = n11;
+    out.n_kept = nk;
+    MCHECK(mb_[M_KEPTOF].reserve(S * 4));
+    MCHECK(mb_[M_KEPTSRC].reserve(nk * 4 + 4));
+    hipLaunchKernelGGL(k_kept_list, dim3(nblk(S)), dim3(256), 0, st_, mb_[M_IS11].as<uint8_t>(),
+                       mb_[M_POS].as<uint64_t>(), S, mb_[M_KEPTOF].as<uint32_t>(), mb_[M_KEPTSRC].as<uint32_t>());
+    // --- kept edges
+    const uint64_t ne = 8 * nk;
+    MCHECK(mb_[M_ELEN].reserve(ne * 4 + 4));
+    MCHECK(mb_[M_EMULT].reserve(ne + 4));
+    MCHECK(mb_[M_EOTHER].reserve(ne * 4 + 4));
+    MCHECK(mb_[M_CSIZE].reserve(ne * 4 + 4));
+    MCHECK(mb_[M_ESPEC].reserve(ne + 4));
+    EdgeArgs ea;
+    ea.key = ma.key;
+    ea.mult = ma.mult;
+    ea.is11 = mb_[M_IS11].as<uint8_t>();
+    ea.walk = wa;
+    ea.sp = spa;
+    ea.kept_of = mb_[M_KEPTOF].as<uint32_t>();
+    ea.kept_src = mb_[M_KEPTSRC].as<uint32_t>();
+    ea.n_kept = nk;
+    ea.ma = ma;
+    ea.elen = mb_[M_ELEN].as<uint32_t>();
+    ea.emult = mb_[M_EMULT].as<uint8_t>();
+    ea.eother = mb_[M_EOTHER].as<uint32_t>();
+    ea.csize = mb_[M_CSIZE].as<uint32_t>();
+    ea.especial = mb_[M_ESPEC].as<uint8_t>();
+    ea.counters = cnt;
+    if (ne) hipLaunchKernelGGL(k_kept_edges, dim3(nblk(ne)), dim3(256), 0, st_, ea);
+    MCHECK(mb_[M_EOFF].reserve((ne + 1) * 8));
+    MCHECK(mb_[M_COFF].reserve((ne + 1) * 8));
+    MCHECK(scan_scratch_.reserve((scan_scratch_words(ne + 1) + 16) * 8));
+    scan_excl<uint32_t>(mb_[M_ELEN].as<uint32_t>(), mb_[M_EOFF].as<uint64_t>(), ne, scan_scratch_.as<uint64_t>(),
+                        d_small + 22, st_);
+    scan_excl<uint32_t>(mb_[M_CSIZE].as<uint32_t>(), mb_[M_COFF].as<uint64_t>(), ne, scan_scratch_.as<uint64_t>(),
+                        d_small + 23, st_);
+    MCHECK(hipMemcpyAsync(hs + 22, d_small + 22, 2 * 8, hipMemcpyDeviceToHost, st_));
+    MCHECK(hipStreamSynchronize(st_));
+    phase_end(10);
+    const uint64_t pool_n = ne ? hs[22] : 0, M = ne ? hs[23] : 0;
+    // --- strings and chain arrays
+    MCHECK(mb_[M_POOL].reserve(pool_n + 16));
+    MCHECK(mb_[M_CPRI].reserve(M * 8 + 16));
+    MCHECK(mb_[M_CMUL].reserve(M + 16));
+    MCHECK(mb_[M_CEDGE].reserve(M * 4 + 16));
+    FillArgs fa;
+    fa.key = ma.key;
+    fa.is11 = mb_[M_IS11].as<uint8_t>();
+    fa.walk = wa;
+    fa.sp = spa;
+    fa.kept_of = mb_[M_KEPTOF].as<uint32_t>();
+    fa.eoff = mb_[M_EOFF].as<uint64_t>();
+    fa.elen = mb_[M_ELEN].as<uint32_t>();
+    fa.coff = mb_[M_COFF].as<uint64_t>();
+    fa.eother = mb_[M_EOTHER].as<uint32_t>();
+    fa.ex = mb_[M_EX].as<uint8_t>();
+    fa.prio = mb_[M_PRIO].as<uint64_t>();
+    fa.S = S;
+    fa.k = k_;
+    fa.pool = mb_[M_POOL].as<char>();
+    fa.cpri = mb_[M_CPRI].as<uint64_t>();
+    fa.cmul = mb_[M_CMUL].as<uint8_t>();
+    fa.cedge = mb_[M_CEDGE].as<uint32_t>();
+    phase_begin(11);
+    if (ne) {
+        hipLaunchKernelGGL(k_fill_edges, dim3(nblk(ne)), dim3(256), 0, st_, fa, mb_[M_EMULT].as<uint8_t>(), ne);
+        hipLaunchKernelGGL(k_fill_nodes, dim3(nblk(2 * S)), dim3(256), 0, st_, fa);
+        hipLaunchKernelGGL(k_fill_last, dim3(nblk(ne)), dim3(256), 0, st_, mb_[M_ELEN].as<uint32_t>(),
+                           mb_[M_EOTHER].as<uint32_t>(), mb_[M_KEPTSRC].as<uint32_t>(), ma.key,
+                           mb_[M_EOFF].as<uint64_t>(), ne, k_, mb_[M_POOL].as<char>());
+    }
+    phase_end(11);
+    // --- merged multiplicities: Cartesian trees over the chain arrays
+    phase_begin(12);
+    if (M) {
+        std::vector<uint64_t> loff;
+        uint64_t tot = 0, len = M;
+        int nlev = 0;
+        while (true) {
+            loff.push_back(tot);
+            tot += len;
+            nlev++;
+            if (len == 1) break;
+            len = (len + 1) / 2;
+        }
+        MCHECK(mb_[M_LEV].reserve(tot * 8 + 16));
+        MCHECK(mb_[M_LEVOFF].reserve(loff.size() * 8 + 16));
+        uint64_t *lev = mb_[M_LEV].as<uint64_t>();
+        MCHECK(hipMemcpyAsync(lev, fa.cpri, M * 8, hipMemcpyDeviceToDevice, st_));
+        MCHECK(hipMemcpyAsync(mb_[M_LEVOFF].p, loff.data(), loff.size() * 8, hipMemcpyHostToDevice, st_));
+        len = M;
+        for (int l = 1; l < nlev; l++) {
+            const uint64_t nl = (len + 1) / 2;
+            hipLaunchKernelGGL(k_max_level, dim3(nblk(nl)), dim3(256), 0, st_, lev + loff[l - 1], len, lev + loff[l], nl);
+            len = nl;
+        }
+        MCHECK(mb_[M_PAR].reserve(M * 4));
+        MCHECK(mb_[M_NEED].reserve(M));
+        MCHECK(mb_[M_NL].reserve(M * 4));
+        MCHECK(mb_[M_NR].reserve(M * 4));
+        MCHECK(mb_[M_INL].reserve(M * 8));
+        MCHECK(mb_[M_INR].reserve(M * 8));
+        MCHECK(mb_[M_CNT].reserve(M * 4));
+        MCHECK(hipMemsetAsync(mb_[M_CNT].p, 0, M * 4, st_));
+        TreeArgs ta;
+        ta.lev = lev;
+        ta.lev_off = mb_[M_LEVOFF].as<uint64_t>();
+        ta.nlev = nlev;
+        ta.M = M;
+        ta.cmul = fa.cmul;
+        ta.cedge = fa.cedge;
+        ta.par = mb_[M_PAR].as<uint32_t>();
+        ta.need = mb_[M_NEED].as<uint8_t>();
+        ta.nl = mb_[M_NL].as<uint32_t>();
+        ta.nr = mb_[M_NR].as<uint32_t>();
+        hipLaunchKernelGGL(k_tree_links, dim3(nblk(M)), dim3(256), 0, st_, ta);
+        hipLaunchKernelGGL(k_tree_eval, dim3(nblk(M)), dim3(256), 0, st_, ta, mb_[M_INL].as<uint64_t>(),
+                           mb_[M_INR].as<uint64_t>(), mb_[M_CNT].as<uint32_t>(), mb_[M_EMULT].as<uint8_t>());
+    }
+    phase_end(12);
+    MCHECK(hipGetLastError());
+    // --- results to the host
+    out.kept_src.resize(nk);
+    out.elen.resize(ne);
+    out.eoff.resize(ne);
+    out.emult.resize(ne);
+    out.especial.resize(ne);
+    out.pool.resize(pool_n);
+    if (nk) MCHECK(hipMemcpyAsync(out.kept_src.data(), mb_[M_KEPTSRC].p, nk * 4, hipMemcpyDeviceToHost, st_));
+    if (ne) {
+        MCHECK(hipMemcpyAsync(out.elen.data(), mb_[M_ELEN].p, ne * 4, hipMemcpyDeviceToHost, st_));
+        MCHECK(hipMemcpyAsync(out.eoff.data(), mb_[M_EOFF].p, ne * 8, hipMemcpyDeviceToHost, st_));
+        MCHECK(hipMemcpyAsync(out.emult.data(), mb_[M_EMULT].p, ne, hipMemcpyDeviceToHost, st_));
+        MCHECK(hipMemcpyAsync(out.especial.data(), mb_[M_ESPEC].p, ne, hipMemcpyDeviceToHost, st_));
+    }
+    if (pool_n) MCHECK(hipMemcpyAsync(out.pool.data(), mb_[M_POOL].p, pool_n, hipMemcpyDeviceToHost, st_));
+    if (nspec) {
+        out.special.resize(nspec);
+        MCHECK(hipMemcpyAsync(out.special.data(), mb_[M_SPEC].p, nspec * 4, hipMemcpyDeviceToHost, st_));
+    }
+    MCHECK(hipStreamSynchronize(st_));
+    collect_phases();
+    n_kept = nk;
+    n_special = nspec;
+    return 0;
 }
 
 }  // namespace swapk

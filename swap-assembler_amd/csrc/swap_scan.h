@@ -1,0 +1,94 @@
+// Device-wide exclusive scan and wave helpers shared by the build and merge.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+namespace swapk {
+
+template <typename T>
+__device__ __forceinline__ T wave_incl_sum(T v) {
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        T o = __shfl_up(v, d, 64);
+        if (lane >= d) v += o;
+    }
+    return v;
+}
+
+// ---------------------------------------------------------------------------
+// exclusive scan (u64 output) over u32 or u64 input: block reduce, recursive
+// scan of the block sums, add back.
+
+constexpr int kScanThreads = 256;
+constexpr int kScanItems = 8;
+constexpr int kScanTile = kScanThreads * kScanItems;
+
+template <typename TIn>
+static __global__ void k_scan_local(const TIn *__restrict__ in, uint64_t *__restrict__ out, uint64_t n,
+                             uint64_t *__restrict__ block_sums) {
+    __shared__ uint64_t wsum[kScanThreads / 64];
+    const uint64_t base = (uint64_t)blockIdx.x * kScanTile + (uint64_t)threadIdx.x * kScanItems;
+    uint64_t v[kScanItems];
+    uint64_t s = 0;
+#pragma unroll
+    for (int i = 0; i < kScanItems; i++) {
+        v[i] = (base + i < n) ? (uint64_t)in[base + i] : 0;
+        s += v[i];
+    }
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    uint64_t inc = wave_incl_sum(s);
+    if (lane == 63) wsum[w] = inc;
+    __syncthreads();
+    uint64_t woff = 0;
+    for (int i = 0; i < w; i++) woff += wsum[i];
+    uint64_t run = inc - s + woff;
+#pragma unroll
+    for (int i = 0; i < kScanItems; i++) {
+        if (base + i < n) out[base + i] = run;
+        run += v[i];
+    }
+    if (threadIdx.x == kScanThreads - 1) block_sums[blockIdx.x] = run;
+}
+
+static __global__ void k_scan_add(uint64_t *__restrict__ out, uint64_t n, const uint64_t *__restrict__ offs) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] += offs[i / kScanTile];
+}
+
+static __global__ void k_u64_to_u32(const uint64_t *__restrict__ in, uint64_t n, uint32_t *__restrict__ out) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = (uint32_t)in[i];
+}
+
+// total <- sum, out <- exclusive prefix.  scratch must hold scan_scratch_words(n).
+static inline uint64_t scan_scratch_words(uint64_t n) {
+    // each level holds its block sums and their exclusive scan
+    uint64_t w = 0;
+    while (n > 1) {
+        n = (n + kScanTile - 1) / kScanTile;
+        w += 2 * n + 2;
+    }
+    return w + 4;
+}
+
+template <typename TIn>
+static void scan_excl(const TIn *in, uint64_t *out, uint64_t n, uint64_t *scratch, uint64_t *d_total,
+                      hipStream_t st) {
+    if (n == 0) {
+        hipMemsetAsync(d_total, 0, 8, st);
+        return;
+    }
+    const uint64_t nb = (n + kScanTile - 1) / kScanTile;
+    uint64_t *sums = scratch;
+    hipLaunchKernelGGL(k_scan_local<TIn>, dim3((unsigned)nb), dim3(kScanThreads), 0, st, in, out, n, sums);
+    if (nb == 1) {
+        hipMemcpyAsync(d_total, sums, 8, hipMemcpyDeviceToDevice, st);
+        return;
+    }
+    uint64_t *sum_scan = scratch + nb;
+    scan_excl<uint64_t>(sums, sum_scan, nb, scratch + 2 * nb, d_total, st);
+    hipLaunchKernelGGL(k_scan_add, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, out, n, sum_scan);
+}
+
+}  // namespace swapk

@@ -1,0 +1,117 @@
+"""GPU k-molecule merge (through the C ABI) against the oracle: contigGraph.txt
+and noCEcontigs.fasta byte for byte in the reference order, and the same
+compacted topology / contig set in the GPU-native hash order."""
+import gzip
+import json
+import os
+import random
+
+import numpy as np
+import pytest
+
+import swapamd
+from oracle_binding import Oracle
+import kmer_textbook
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+CASES = json.load(open(os.path.join(GOLD, "cases.json")))
+
+pytestmark = pytest.mark.gpu
+
+
+def gold_fa(name):
+    with gzip.open(os.path.join(GOLD, name + ".fa.gz"), "rb") as f:
+        return f.read()
+
+
+def strip_mult(text):
+    """contigGraph lines without the 8 multiplicities and padding."""
+    out = []
+    for line in text.decode().splitlines():
+        if not line.strip():
+            continue
+        parts = line.split("#")
+        out.append("#".join(parts[:8]))
+    return sorted(out)
+
+
+def check_merge(fa, k, c, hash_order=False):
+    print("merge case k=%d c=%d bytes=%d hash_order=%s" % (k, c, len(fa), hash_order), flush=True)
+    o = Oracle(fa, k, c)
+    a = swapamd.Assembler(k, c, hash_order=hash_order)
+    a.build(fa)
+    a.merge()
+    cg = a.file_bytes(swapamd.SWAP_FILE_CONTIG_GRAPH)
+    noce = a.file_bytes(swapamd.SWAP_FILE_NOCE)
+    ocg, onoce = o.file(1), o.file(2)
+    st = a.stats()
+    assert st["kept"] == o.stats()["kept"]
+    if hash_order:
+        assert strip_mult(cg) == strip_mult(ocg)
+        assert sorted(noce.splitlines()) == sorted(onoce.splitlines())
+    else:
+        assert cg == ocg
+        assert noce == onoce
+    a.close()
+    return st
+
+
+@pytest.mark.parametrize("name", sorted(CASES))
+def test_golden_inputs(name):
+    case = CASES[name]
+    check_merge(gold_fa(name), case["k"], case["c"])
+
+
+@pytest.mark.parametrize("name", sorted(CASES))
+def test_golden_inputs_hash_order(name):
+    case = CASES[name]
+    check_merge(gold_fa(name), case["k"], case["c"], hash_order=True)
+
+
+def circular_reads(seed, G, L, n, err):
+    rng = random.Random(seed)
+    g = "".join(rng.choice("ACGT") for _ in range(G))
+    out = []
+    for i in range(n):
+        st = rng.randrange(G)
+        s = (g + g[:L])[st:st + L]
+        if rng.random() < 0.5:
+            s = kmer_textbook.rc(s)
+        s = "".join(ch if rng.random() >= err else rng.choice("ACGT") for ch in s)
+        out.append(">r%d\n%s\n" % (i, s))
+    return "".join(out).encode()
+
+
+def test_isolated_cycle():
+    # a circular genome without errors: one isolated cycle of (1,1) nodes
+    fa = circular_reads(21, 3000, 100, 900, 0.0)
+    st = check_merge(fa, 21, 1)
+    assert st["special"] > 0
+
+
+def test_fold_back_palindrome():
+    rng = random.Random(5)
+    left = "".join(rng.choice("ACGT") for _ in range(400))
+    arm = "".join(rng.choice("ACGT") for _ in range(120))
+    g = left + arm + kmer_textbook.rc(arm)
+    reads = []
+    for i in range(1500):
+        st = rng.randrange(len(g) - 100 + 1)
+        s = g[st:st + 100]
+        if rng.random() < 0.5:
+            s = kmer_textbook.rc(s)
+        reads.append(">r%d\n%s\n" % (i, s))
+    fa = "".join(reads).encode()
+    check_merge(fa, 21, 2)
+    check_merge(fa, 21, 2, hash_order=True)
+
+
+@pytest.mark.parametrize("k", [9, 19, 25, 31])
+def test_random_genomes(k):
+    fa = swapamd.generate_params(40000, 100, 25.0, 0.01, 200 + k)
+    check_merge(fa, k, 2)
+
+
+def test_small_workload_full_size():
+    fa, k, c = swapamd.generate("small")
+    check_merge(fa, k, c)
