@@ -123,6 +123,18 @@ int64_t swap_write_file(swap_ctx *ctx, int which, const char *path);
 int64_t swap_table_order(const uint64_t *keys, const uint64_t *ranks, size_t n, uint64_t n_total,
                          int extra_resize, uint32_t *order_out);
 
+/* Host utility: the reference's merge visit order (its nodes table order,
+ * graph.cpp:1123-1133) for n survivors with keys[i] and first-occurrence
+ * indices first[i]; epoch_first[e] is the first-occurrence index of the
+ * distinct k-mer whose insert started growth epoch e of the k-mer table
+ * (n_epochs entries, the first 0), distinct the number of distinct k-mers.
+ * Writes survivor indices in visit order; returns n. */
+int64_t swap_visit_order(const uint64_t *keys, const uint64_t *first, size_t n, const uint64_t *epoch_first,
+                         size_t n_epochs, uint64_t distinct, int tail_lookup, uint32_t *order_out);
+/* Insert ranks at which a table of n_total inserts grows (first entry 0);
+ * returns the count written (at most cap). */
+int64_t swap_table_epochs(uint64_t n_total, int extra_resize, uint64_t *starts, size_t cap);
+
 const char *swap_strerror(int code);
 const char *swap_last_error(const swap_ctx *ctx);
 

@@ -130,6 +130,7 @@ struct oracle_graph {
     std::vector<Arc> surv_arcs;
     std::vector<uint64_t> surv_first;
     std::vector<uint64_t> surv_rank;              // index of the key among first occurrences
+    std::vector<uint64_t> first_in_order;         // first occurrence of the i-th distinct key
     std::vector<unsigned long long> visit;        // survivors in merge visit order
     std::string kmer_graph_txt, contig_graph_txt, noce_txt;
 };
@@ -421,6 +422,7 @@ oracle_graph *oracle_run(const char *fasta, uint64_t n, int k, int c, int flags)
     std::vector<unsigned long long> first_key;
     if (!build_table(g, fasta, n, kmol, &first_of, &first_key)) { delete g; return nullptr; }
     g->distinct = kmol.size();
+    g->first_in_order = first_of;
     {
         std::vector<std::pair<unsigned long long, Arc>> v(kmol.begin(), kmol.end());
         std::sort(v.begin(), v.end(), [](const std::pair<unsigned long long, Arc> &a, const std::pair<unsigned long long, Arc> &b) { return a.first < b.first; });
@@ -496,6 +498,10 @@ uint64_t oracle_survivor_rank(const oracle_graph *g, uint64_t *rank, uint64_t ca
     const uint64_t n = g->surv_rank.size();
     for (uint64_t i = 0; i < n && i < cap; i++) rank[i] = g->surv_rank[i];
     return n;
+}
+
+uint64_t oracle_first_of_rank(const oracle_graph *g, uint64_t rank) {
+    return rank < g->first_in_order.size() ? g->first_in_order[rank] : ~0ull;
 }
 
 uint64_t oracle_visit_order(const oracle_graph *g, uint64_t *keys, uint64_t cap) {

@@ -51,6 +51,8 @@ uint64_t oracle_survivor_first(const oracle_graph *g, uint64_t *first, uint64_t 
 // Index of each survivor's key among all distinct keys in first-occurrence
 // order (its insertion index into the reference's k-mer table).
 uint64_t oracle_survivor_rank(const oracle_graph *g, uint64_t *rank, uint64_t cap);
+// First-occurrence index of the distinct key inserted rank-th into the table.
+uint64_t oracle_first_of_rank(const oracle_graph *g, uint64_t rank);
 // Survivor keys in the merge visit order (the reference's hash-table order).
 uint64_t oracle_visit_order(const oracle_graph *g, uint64_t *keys, uint64_t cap);
 

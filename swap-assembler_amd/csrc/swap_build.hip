@@ -205,7 +205,8 @@ struct OccCtx {
     Mixer mx;
 };
 
-// Calls f(part, rec, occ) for each occurrence starting in word w.
+// Calls f(h, mask, occ) for each occurrence starting in word w: h is the mixed
+// canonical k-mer (2k bits), mask its 8 edge bits, occ its input k-mer index.
 template <typename F>
 __device__ __forceinline__ void for_each_occ(const OccCtx &c, uint64_t w, F &&f) {
     const uint64_t r = upper_index(c.word_base, c.R, w);
@@ -233,8 +234,6 @@ __device__ __forceinline__ void for_each_occ(const OccCtx &c, uint64_t w, F &&f)
     }
     uint64_t rcx = revcomp(x, k);
     const uint64_t obase = c.occ_base[r];
-    const int hb = 2 * k - kPartBits;
-    const uint64_t lowm = (1ull << hb) - 1;
     for (uint32_t p = p0; p < pend; p++) {
         const int i = (int)(p - p0);
         if (i > 0) {
@@ -253,59 +252,190 @@ __device__ __forceinline__ void for_each_occ(const OccCtx &c, uint64_t w, F &&f)
             const uint32_t a = 3 - (i > 0 ? base_at(i - 1) : prev_base);
             m |= 1u << (fwd ? 4 + a : a);
         }
-        const uint64_t h = mix_fwd(c.mx, X);
-        f((uint32_t)(h >> hb), ((h & lowm) << 8) | m, obase + p);
+        f(mix_fwd(c.mx, X), m, obase + p);
     }
 }
 
-constexpr int kOccThreads = 512;
+// Two-level radix partitioning of the occurrence records by the top bits of the
+// mixed key: 2^kL1Bits buckets first, then 2^kL2Bits sub-buckets of each, so
+// that every write goes out as a run of records staged and sorted in LDS.
+// Level-1 records keep the key bits below the level-1 bits:
+//   rec = (h mod 2^(2k-8)) << 8 | mask        (2k <= 62, so 2k bits in all)
+// and the level-2 bucket is read back from the record.
+constexpr int kL1Bits = 8;
+constexpr uint32_t kL1 = 1u << kL1Bits;
+constexpr int kL2Bits = kPartBits - kL1Bits;
+constexpr uint32_t kL2 = 1u << kL2Bits;
+constexpr int kOccThreads = 256;
+constexpr int kChunkRecs = kOccThreads * 32;  // a word starts at most 32 k-mers
 
-// Pass 1: bucket sizes per (partition, per-XCD sub-bucket).
-__global__ void __launch_bounds__(kOccThreads) k_occ_hist(OccCtx c, const uint64_t *__restrict__ d_words,
-                                                           uint32_t *__restrict__ hist) {
+// Per workgroup word range (same split in the histogram and level-1 passes).
+__device__ __forceinline__ void word_range(uint64_t W, uint64_t &wb, uint64_t &we) {
+    const uint64_t per = (W + gridDim.x - 1) / gridDim.x;
+    wb = per * blockIdx.x;
+    we = min<uint64_t>(W, wb + per);
+    if (wb > W) wb = W;
+}
+
+// Final partition sizes (global) and per-workgroup level-1 counts.
+__global__ void __launch_bounds__(kOccThreads) k_part_hist(OccCtx c, const uint64_t *__restrict__ d_words,
+                                                            uint32_t *__restrict__ hist, uint32_t *__restrict__ hist1) {
     __shared__ uint32_t lh[kParts];
     for (uint32_t i = threadIdx.x; i < kParts; i += blockDim.x) lh[i] = 0;
     __syncthreads();
-    const uint64_t W = *d_words;
-    const uint64_t per = (W + gridDim.x - 1) / gridDim.x;
-    const uint64_t wb = per * blockIdx.x, we = min<uint64_t>(W, wb + per);
+    uint64_t wb, we;
+    word_range(*d_words, wb, we);
+    const int sh = 2 * c.k - kPartBits;
     for (uint64_t w = wb + threadIdx.x; w < we; w += blockDim.x)
-        for_each_occ(c, w, [&](uint32_t part, uint64_t, uint64_t) { atomicAdd(&lh[part], 1u); });
+        for_each_occ(c, w, [&](uint64_t h, uint32_t, uint64_t) { atomicAdd(&lh[h >> sh], 1u); });
     __syncthreads();
-    const uint32_t vx = blockIdx.x % kVx;
     for (uint32_t i = threadIdx.x; i < kParts; i += blockDim.x)
-        if (lh[i]) atomicAdd(&hist[i * kVx + vx], lh[i]);
+        if (lh[i]) atomicAdd(&hist[i], lh[i]);
+    for (uint32_t b = threadIdx.x; b < kL1; b += blockDim.x) {
+        uint32_t s1 = 0;
+        for (uint32_t j = 0; j < kL2; j++) s1 += lh[b * kL2 + j];
+        hist1[(uint64_t)b * gridDim.x + blockIdx.x] = s1;
+    }
 }
 
-// Pass 2: same word range per block; reserve space once per bucket with one
-// global atomic, then write the records (and first-occurrence indices) through
-// LDS cursors.  Positions are 32-bit: one build holds fewer than 2^32 k-mers.
+// Exclusive scan of n small counts held in LDS by one wave (n <= 1024).
+__device__ __forceinline__ void lds_excl_scan(const uint32_t *cnt, uint32_t *off, uint32_t n) {
+    if (threadIdx.x < 64) {
+        const int lane = threadIdx.x;
+        uint32_t run = 0;
+        for (uint32_t b0 = 0; b0 < n; b0 += 64) {
+            const uint32_t v = (b0 + lane < n) ? cnt[b0 + lane] : 0;
+            const uint32_t inc = wave_incl_sum(v);
+            if (b0 + lane < n) off[b0 + lane] = run + inc - v;
+            run += __shfl(inc, 63, 64);
+        }
+    }
+}
+
+// Level 1: records of the workgroup's words, chunk by chunk, sorted by level-1
+// bucket in LDS and written as runs at offsets fixed by the histogram pass.
 template <bool WITH_OCC>
-__global__ void __launch_bounds__(kOccThreads) k_occ_scatter(OccCtx c, const uint64_t *__restrict__ d_words,
-                                                              uint32_t *__restrict__ cursor,
-                                                              uint64_t *__restrict__ rec_out,
-                                                              uint32_t *__restrict__ occ_out) {
-    __shared__ uint32_t lh[kParts];
-    for (uint32_t i = threadIdx.x; i < kParts; i += blockDim.x) lh[i] = 0;
+__global__ void __launch_bounds__(kOccThreads) k_part1(OccCtx c, const uint64_t *__restrict__ d_words,
+                                                        const uint64_t *__restrict__ base1,
+                                                        uint64_t *__restrict__ rec_out, uint32_t *__restrict__ occ_out) {
+    __shared__ uint64_t srec[kChunkRecs];
+    __shared__ uint32_t socc[WITH_OCC ? kChunkRecs : 1];
+    __shared__ uint8_t sbin[kChunkRecs];
+    __shared__ uint32_t gbase[kL1], lcnt[kL1], loff[kL1], lfill[kL1];
+    for (uint32_t b = threadIdx.x; b < kL1; b += blockDim.x) gbase[b] = (uint32_t)base1[(uint64_t)b * gridDim.x + blockIdx.x];
+    uint64_t wb, we;
+    word_range(*d_words, wb, we);
+    const int sh1 = 2 * c.k - kL1Bits;
+    const uint64_t low1 = (1ull << sh1) - 1;
+    for (uint64_t w0 = wb; w0 < we; w0 += blockDim.x) {
+        const uint64_t w = w0 + threadIdx.x;
+        for (uint32_t b = threadIdx.x; b < kL1; b += blockDim.x) { lcnt[b] = 0; lfill[b] = 0; }
+        __syncthreads();
+        if (w < we) for_each_occ(c, w, [&](uint64_t h, uint32_t, uint64_t) { atomicAdd(&lcnt[h >> sh1], 1u); });
+        __syncthreads();
+        lds_excl_scan(lcnt, loff, kL1);
+        __syncthreads();
+        if (w < we)
+            for_each_occ(c, w, [&](uint64_t h, uint32_t m, uint64_t occ) {
+                const uint32_t b = (uint32_t)(h >> sh1);
+                const uint32_t slot = loff[b] + atomicAdd(&lfill[b], 1u);
+                srec[slot] = ((h & low1) << 8) | m;
+                if (WITH_OCC) socc[slot] = (uint32_t)occ;
+                sbin[slot] = (uint8_t)b;
+            });
+        __syncthreads();
+        const uint32_t total = loff[kL1 - 1] + lcnt[kL1 - 1];
+        for (uint32_t i = threadIdx.x; i < total; i += blockDim.x) {
+            const uint32_t b = sbin[i];
+            const uint32_t g = gbase[b] + (i - loff[b]);
+            rec_out[g] = srec[i];
+            if (WITH_OCC) occ_out[g] = socc[i];
+        }
+        __syncthreads();
+        for (uint32_t b = threadIdx.x; b < kL1; b += blockDim.x) gbase[b] += lcnt[b];
+        __syncthreads();
+    }
+}
+
+// Level 2: each workgroup takes one chunk of one level-1 bucket, sorts it by
+// level-2 bucket in LDS and appends the runs to the final partitions.
+// chunk_start[b] = first chunk index of level-1 bucket b (kL1 + 1 entries).
+template <bool WITH_OCC>
+__global__ void __launch_bounds__(kOccThreads) k_part2(const uint64_t *__restrict__ rec_in,
+                                                        const uint32_t *__restrict__ occ_in,
+                                                        const uint32_t *__restrict__ l1_start,
+                                                        const uint32_t *__restrict__ chunk_start, int k,
+                                                        uint32_t *__restrict__ cursor, uint64_t *__restrict__ rec_out,
+                                                        uint32_t *__restrict__ occ_out) {
+    __shared__ uint64_t srec[kChunkRecs];
+    __shared__ uint32_t socc[WITH_OCC ? kChunkRecs : 1];
+    __shared__ uint8_t sbin[kChunkRecs];
+    __shared__ uint32_t gbase[kL2], lcnt[kL2], loff[kL2], lfill[kL2];
+    __shared__ uint32_t sb, sfirst;
+    const uint32_t chunk = blockIdx.x;
+    if (threadIdx.x == 0) {
+        uint32_t lo = 0, hi = kL1;  // last b with chunk_start[b] <= chunk
+        while (hi - lo > 1) {
+            const uint32_t mid = (lo + hi) / 2;
+            if (chunk_start[mid] <= chunk) lo = mid; else hi = mid;
+        }
+        sb = lo;
+        sfirst = chunk_start[lo];
+    }
+    for (uint32_t b = threadIdx.x; b < kL2; b += blockDim.x) { lcnt[b] = 0; lfill[b] = 0; }
     __syncthreads();
-    const uint64_t W = *d_words;
-    const uint64_t per = (W + gridDim.x - 1) / gridDim.x;
-    const uint64_t wb = per * blockIdx.x, we = min<uint64_t>(W, wb + per);
-    for (uint64_t w = wb + threadIdx.x; w < we; w += blockDim.x)
-        for_each_occ(c, w, [&](uint32_t part, uint64_t, uint64_t) { atomicAdd(&lh[part], 1u); });
-    __syncthreads();
-    const uint32_t vx = blockIdx.x % kVx;
-    for (uint32_t i = threadIdx.x; i < kParts; i += blockDim.x) {
-        const uint32_t cnt = lh[i];
-        if (cnt) lh[i] = atomicAdd(&cursor[i * kVx + vx], cnt);
+    if (chunk >= chunk_start[kL1]) return;
+    const uint32_t b1 = sb;
+    const uint64_t a = (uint64_t)l1_start[b1] + (uint64_t)(chunk - sfirst) * kChunkRecs;
+    const uint64_t e = min<uint64_t>((uint64_t)l1_start[b1 + 1], a + kChunkRecs);
+    const uint32_t n = (uint32_t)(e - a);
+    const int sh2 = 8 + 2 * k - kPartBits;
+    uint64_t r[kChunkRecs / kOccThreads];
+    uint32_t o[kChunkRecs / kOccThreads];
+#pragma unroll
+    for (int j = 0; j < kChunkRecs / kOccThreads; j++) {
+        const uint32_t i = j * kOccThreads + threadIdx.x;
+        if (i < n) {
+            r[j] = rec_in[a + i];
+            if (WITH_OCC) o[j] = occ_in[a + i];
+            atomicAdd(&lcnt[(r[j] >> sh2) & (kL2 - 1)], 1u);
+        }
     }
     __syncthreads();
-    for (uint64_t w = wb + threadIdx.x; w < we; w += blockDim.x)
-        for_each_occ(c, w, [&](uint32_t part, uint64_t rec, uint64_t occ) {
-            const uint32_t pos = atomicAdd(&lh[part], 1u);
-            rec_out[pos] = rec;
-            if (WITH_OCC) occ_out[pos] = (uint32_t)occ;
-        });
+    lds_excl_scan(lcnt, loff, kL2);
+    __syncthreads();
+    for (uint32_t b = threadIdx.x; b < kL2; b += blockDim.x)
+        if (lcnt[b]) gbase[b] = atomicAdd(&cursor[b1 * kL2 + b], lcnt[b]);
+#pragma unroll
+    for (int j = 0; j < kChunkRecs / kOccThreads; j++) {
+        const uint32_t i = j * kOccThreads + threadIdx.x;
+        if (i < n) {
+            const uint32_t b = (uint32_t)((r[j] >> sh2) & (kL2 - 1));
+            const uint32_t slot = loff[b] + atomicAdd(&lfill[b], 1u);
+            srec[slot] = r[j];
+            if (WITH_OCC) socc[slot] = o[j];
+            sbin[slot] = (uint8_t)b;
+        }
+    }
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
+        const uint32_t b = sbin[i];
+        const uint32_t g = gbase[b] + (i - loff[b]);
+        rec_out[g] = srec[i];
+        if (WITH_OCC) occ_out[g] = socc[i];
+    }
+}
+
+// chunk_start[b] for the level-2 pass from the final partition starts.
+__global__ void k_l1_chunks(const uint32_t *__restrict__ pstart, uint32_t *__restrict__ l1_start,
+                            uint32_t *__restrict__ chunk_start) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    uint32_t ch = 0;
+    for (uint32_t b = 0; b <= kL1; b++) {
+        l1_start[b] = pstart[b * kL2];
+        chunk_start[b] = ch;
+        if (b < kL1) ch += (pstart[(b + 1) * kL2] - pstart[b * kL2] + kChunkRecs - 1) / kChunkRecs;
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -324,7 +454,7 @@ constexpr uint64_t kEmpty = ~0ull;
 struct CountArgs {
     const uint64_t *rec;
     const uint32_t *occ;
-    const uint32_t *bucket;   // exclusive prefix over (partition, sub-bucket), kParts*kVx+1 entries
+    const uint32_t *bucket;   // start of each final partition, kParts+1 entries
     int k;
     int c;
     Mixer mx;
@@ -394,7 +524,7 @@ __device__ void table_emit(const uint64_t *tkey, const uint64_t *tcnt, const uin
                            uint32_t part, const CountArgs &a) {
     __shared__ uint64_t wsum[kCountThreads / 64];
     __shared__ unsigned long long base_s, base_d;
-    const int hb = 2 * a.k - kPartBits;
+    const int hb = 2 * a.k - kL1Bits;
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
     constexpr int kPer = 8;
     for (uint32_t r0 = 0; r0 < n_slots; r0 += blockDim.x * kPer) {
@@ -437,7 +567,7 @@ __device__ void table_emit(const uint64_t *tkey, const uint64_t *tcnt, const uin
             if (keys[j] == kEmpty) continue;
             if (WITH_OCC && a.dfo) a.dfo[od++] = fos[j];
             if (mults[j]) {
-                a.surv_key[os] = mix_inv(a.mx, ((uint64_t)part << hb) | keys[j]);
+                a.surv_key[os] = mix_inv(a.mx, ((uint64_t)(part >> kL2Bits) << hb) | keys[j]);
                 a.surv_mult[os] = mults[j];
                 if (WITH_OCC) a.surv_first[os] = fos[j];
                 os++;
@@ -461,7 +591,7 @@ __global__ void __launch_bounds__(kCountThreads) k_count_lds(CountArgs a) {
     }
     if (threadIdx.x == 0) overflow = 0;
     __syncthreads();
-    const uint32_t b = a.bucket[part * kVx], e = a.bucket[part * kVx + kVx];
+    const uint32_t b = a.bucket[part], e = a.bucket[part + 1];
     bool ok = true;
     for (uint32_t i = b + threadIdx.x; i < e && ok; i += blockDim.x)
         ok = table_add<WITH_OCC>(tkey, tcnt, tfo, kTab - 1, a.rec[i], WITH_OCC ? a.occ[i] : 0);
@@ -496,7 +626,7 @@ __global__ void __launch_bounds__(kCountThreads) k_count_global(CountArgs a, con
         if (WITH_OCC) tfo[i] = ~0u;
     }
     __syncthreads();
-    const uint32_t b = a.bucket[part * kVx], e = a.bucket[part * kVx + kVx];
+    const uint32_t b = a.bucket[part], e = a.bucket[part + 1];
     for (uint32_t i = b + threadIdx.x; i < e; i += blockDim.x)
         table_add<WITH_OCC>(tkey, tcnt, tfo, n_slots - 1, a.rec[i], WITH_OCC ? a.occ[i] : 0);
     __threadfence_block();
@@ -703,40 +833,58 @@ int Pipeline::build(const uint8_t *d_in, uint64_t n) {
     oc.R = R;
     oc.k = k_;
     oc.mx = mx_;
-    const uint32_t nb = kParts * kVx;
-    SWAP_CHECK(hist_.reserve((nb + 1) * 4));
-    SWAP_CHECK(bucket_.reserve((nb + 1) * 8));
-    SWAP_CHECK(cursor_.reserve((nb + 1) * 4));
-    SWAP_CHECK(hipMemsetAsync(hist_.p, 0, (nb + 1) * 4, st_));
     int ncu = 0;
     hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev_);
     const unsigned occ_grid = (unsigned)std::max(8, ncu * 2);
+    const uint64_t n1 = (uint64_t)kL1 * occ_grid;
+    SWAP_CHECK(hist_.reserve((kParts + 1) * 4));   // final partition sizes, then starts (u32)
+    SWAP_CHECK(bucket_.reserve((kParts + 1) * 8));
+    SWAP_CHECK(cursor_.reserve((kParts + 1) * 4));
+    SWAP_CHECK(hist1_.reserve(n1 * 4));
+    SWAP_CHECK(base1_.reserve((n1 + 1) * 8));
+    SWAP_CHECK(l1info_.reserve((2 * kL1 + 4) * 4));
+    SWAP_CHECK(hipMemsetAsync(hist_.p, 0, (kParts + 1) * 4, st_));
     phase_begin(2);
-    hipLaunchKernelGGL(k_occ_hist, dim3(occ_grid), dim3(kOccThreads), 0, st_, oc, d_small + 5,
-                       hist_.as<uint32_t>());
+    hipLaunchKernelGGL(k_part_hist, dim3(occ_grid), dim3(kOccThreads), 0, st_, oc, d_small + 5,
+                       hist_.as<uint32_t>(), hist1_.as<uint32_t>());
     phase_end(2);
-    SWAP_CHECK(scan_scratch_.reserve((scan_scratch_words(nb + 1) + 16) * 8));
-    scan_excl<uint32_t>(hist_.as<uint32_t>(), bucket_.as<uint64_t>(), nb + 1, scan_scratch_.as<uint64_t>(),
+    SWAP_CHECK(scan_scratch_.reserve((scan_scratch_words(n1 + kParts + 1) + 16) * 8));
+    scan_excl<uint32_t>(hist_.as<uint32_t>(), bucket_.as<uint64_t>(), kParts + 1, scan_scratch_.as<uint64_t>(),
                         d_small + 6, st_);
-    // 32-bit copies: bucket starts and the running cursors
-    {
-        // bucket_ holds u64 prefix; convert into cursor_ (u32) and a u32 copy for the count kernel
-        // (N < 2^32 so the values fit).
-    }
+    scan_excl<uint32_t>(hist1_.as<uint32_t>(), base1_.as<uint64_t>(), n1, scan_scratch_.as<uint64_t>(),
+                        d_small + 6, st_);
+    // u32 partition starts for the count pass, and the level-2 cursors
+    hipLaunchKernelGGL(k_u64_to_u32, dim3((kParts + 1 + 255) / 256), dim3(256), 0, st_, bucket_.as<uint64_t>(),
+                       (uint64_t)kParts + 1, hist_.as<uint32_t>());
+    SWAP_CHECK(hipMemcpyAsync(cursor_.p, hist_.p, (kParts + 1) * 4, hipMemcpyDeviceToDevice, st_));
+    uint32_t *l1_start = l1info_.as<uint32_t>();
+    uint32_t *chunk_start = l1_start + kL1 + 2;
+    hipLaunchKernelGGL(k_l1_chunks, dim3(1), dim3(64), 0, st_, hist_.as<uint32_t>(), l1_start, chunk_start);
+    SWAP_CHECK(rec1_.reserve(N * 8));
     SWAP_CHECK(rec_.reserve(N * 8));
-    if (ref_order_) SWAP_CHECK(occ_.reserve(N * 4));
-    hipLaunchKernelGGL(k_u64_to_u32, dim3((nb + 1 + 255) / 256), dim3(256), 0, st_, bucket_.as<uint64_t>(), nb + 1,
-                       cursor_.as<uint32_t>());
-    SWAP_CHECK(hist_.reserve((nb + 1) * 4));
-    SWAP_CHECK(hipMemcpyAsync(hist_.p, cursor_.p, (nb + 1) * 4, hipMemcpyDeviceToDevice, st_));  // hist_ := bucket starts (u32)
+    if (ref_order_) {
+        SWAP_CHECK(occ1_.reserve(N * 4));
+        SWAP_CHECK(occ_.reserve(N * 4));
+    }
     phase_begin(3);
     if (ref_order_)
-        hipLaunchKernelGGL(k_occ_scatter<true>, dim3(occ_grid), dim3(kOccThreads), 0, st_, oc, d_small + 5,
-                           cursor_.as<uint32_t>(), rec_.as<uint64_t>(), occ_.as<uint32_t>());
+        hipLaunchKernelGGL(k_part1<true>, dim3(occ_grid), dim3(kOccThreads), 0, st_, oc, d_small + 5,
+                           base1_.as<uint64_t>(), rec1_.as<uint64_t>(), occ1_.as<uint32_t>());
     else
-        hipLaunchKernelGGL(k_occ_scatter<false>, dim3(occ_grid), dim3(kOccThreads), 0, st_, oc, d_small + 5,
-                           cursor_.as<uint32_t>(), rec_.as<uint64_t>(), (uint32_t *)nullptr);
+        hipLaunchKernelGGL(k_part1<false>, dim3(occ_grid), dim3(kOccThreads), 0, st_, oc, d_small + 5,
+                           base1_.as<uint64_t>(), rec1_.as<uint64_t>(), (uint32_t *)nullptr);
     phase_end(3);
+    const unsigned grid2 = (unsigned)(N / kChunkRecs + kL1 + 1);
+    phase_begin(7);
+    if (ref_order_)
+        hipLaunchKernelGGL(k_part2<true>, dim3(grid2), dim3(kOccThreads), 0, st_, rec1_.as<uint64_t>(),
+                           occ1_.as<uint32_t>(), l1_start, chunk_start, k_, cursor_.as<uint32_t>(),
+                           rec_.as<uint64_t>(), occ_.as<uint32_t>());
+    else
+        hipLaunchKernelGGL(k_part2<false>, dim3(grid2), dim3(kOccThreads), 0, st_, rec1_.as<uint64_t>(),
+                           (const uint32_t *)nullptr, l1_start, chunk_start, k_, cursor_.as<uint32_t>(),
+                           rec_.as<uint64_t>(), (uint32_t *)nullptr);
+    phase_end(7);
     // --- count + cutoff
     // survivors <= distinct <= N; reserve N to avoid a sync (grow-only buffers)
     SWAP_CHECK(surv_key_.reserve(N * 8));
@@ -771,15 +919,15 @@ int Pipeline::build(const uint8_t *d_in, uint64_t n) {
     n_overflow = hs[10];
     if (n_overflow) {
         // size a global table per overflowing partition from its record count
-        std::vector<uint32_t> list(n_overflow), starts(nb + 1);
+        std::vector<uint32_t> list(n_overflow), starts(kParts + 1);
         SWAP_CHECK(hipMemcpy(list.data(), overflow_.p, n_overflow * 4, hipMemcpyDeviceToHost));
-        SWAP_CHECK(hipMemcpy(starts.data(), hist_.p, (nb + 1) * 4, hipMemcpyDeviceToHost));
+        SWAP_CHECK(hipMemcpy(starts.data(), hist_.p, (kParts + 1) * 4, hipMemcpyDeviceToHost));
         std::vector<uint64_t> off(n_overflow);
         std::vector<uint32_t> bits(n_overflow);
         uint64_t tot = 0;
         for (uint64_t j = 0; j < n_overflow; j++) {
             const uint32_t p = list[j];
-            const uint64_t cnt = (uint64_t)starts[p * kVx + kVx] - starts[p * kVx];
+            const uint64_t cnt = (uint64_t)starts[p + 1] - starts[p];
             uint32_t b = kTabBits + 1;
             while ((1ull << b) < 2 * cnt) b++;
             bits[j] = b;

@@ -214,6 +214,21 @@ int64_t swap_table_order(const uint64_t *keys, const uint64_t *ranks, size_t n, 
     return (int64_t)n;
 }
 
+int64_t swap_visit_order(const uint64_t *keys, const uint64_t *first, size_t n, const uint64_t *epoch_first,
+                         size_t n_epochs, uint64_t distinct, int tail_lookup, uint32_t *order_out) {
+    if ((!keys || !first || !order_out) && n) return SWAP_E_INVAL;
+    std::vector<uint64_t> k(keys, keys + n), f(first, first + n), e(epoch_first, epoch_first + n_epochs);
+    std::vector<uint32_t> o = swapk::visit_order(k, f, e, distinct, tail_lookup != 0, nullptr);
+    std::copy(o.begin(), o.end(), order_out);
+    return (int64_t)n;
+}
+
+int64_t swap_table_epochs(uint64_t n_total, int extra_resize, uint64_t *starts, size_t cap) {
+    std::vector<uint64_t> s = swapk::table_epoch_starts(n_total, extra_resize != 0);
+    for (size_t i = 0; i < s.size() && i < cap; i++) starts[i] = s[i];
+    return (int64_t)s.size();
+}
+
 const char *swap_strerror(int code) {
     switch (code) {
         case SWAP_OK: return "ok";

@@ -14,7 +14,6 @@ namespace swapk {
 // by the top bits of a bijective mix of the canonical k-mer.
 constexpr int kPartBits = 14;
 constexpr uint32_t kParts = 1u << kPartBits;
-constexpr int kVx = 8;  // per-XCD sub-buckets (one L2 per XCD)
 
 SWAP_HD uint64_t kmask(int k) { return (k >= 32) ? ~0ull : ((1ull << (2 * k)) - 1); }
 

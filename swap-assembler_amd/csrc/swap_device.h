@@ -88,8 +88,9 @@ public:
     int c() const { return c_; }
     // Device time of the build phases of the last build, from events recorded
     // on the pipeline stream around the launches: 0 record scan, 1 pack,
-    // 2 partition histogram, 3 partition scatter, 4 LDS count + cutoff,
-    // 5 overflow count, 6 first-occurrence ranks; merge phases from 8.
+    // 2 partition histogram, 3 level-1 partition, 4 LDS count + cutoff,
+    // 5 overflow count, 6 growth-rank selection, 7 level-2 partition; merge
+    // phases from 8.
     static constexpr int kPhases = 16;
     float phase_ms[kPhases] = {0};
     void phase_begin(int i);
@@ -106,7 +107,7 @@ private:
     std::string err_;
     // build buffers
     DevBuf fasta_, tile_nl_, tile_off_, nl_pos_, scan_scratch_, small_, rec_start_, rec_len_, occ_count_,
-        word_count_, occ_base_, word_base_, packed_, hist_, bucket_, cursor_, rec_, occ_, surv_key_, surv_mult_,
+        word_count_, occ_base_, word_base_, packed_, hist_, bucket_, cursor_, hist1_, base1_, l1info_, rec1_, occ1_, rec_, occ_, surv_key_, surv_mult_,
         surv_first_, dfo_, sel_, sel_target_, sel_out_, overflow_, gtab_key_, gtab_cnt_, gtab_fo_;
     // merge buffers
     enum {

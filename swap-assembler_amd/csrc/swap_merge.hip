@@ -54,7 +54,7 @@ __global__ void k_index_build(const uint64_t *__restrict__ key, uint64_t n, uint
     if (i >= n) return;
     const uint64_t k = key[i];
     uint32_t s = idx_hash(k, mask);
-    while (true) {
+    for (uint32_t probe = 0; probe <= mask; probe++) {  // the table is at most half full
         const uint64_t prev = atomicCAS((unsigned long long *)&ikey[s], (unsigned long long)kIdxEmpty,
                                         (unsigned long long)k);
         if (prev == kIdxEmpty || prev == k) {
@@ -68,12 +68,13 @@ __global__ void k_index_build(const uint64_t *__restrict__ key, uint64_t n, uint
 __device__ __forceinline__ uint32_t index_find(const uint64_t *__restrict__ ikey, const uint32_t *__restrict__ ival,
                                                uint32_t mask, uint64_t k) {
     uint32_t s = idx_hash(k, mask);
-    while (true) {
+    for (uint32_t probe = 0; probe <= mask; probe++) {
         const uint64_t cur = ikey[s];
         if (cur == k) return ival[s];
         if (cur == kIdxEmpty) return kNil;
         s = (s + 1) & mask;
     }
+    return kNil;
 }
 
 // ---------------------------------------------------------------------------
@@ -391,6 +392,7 @@ struct TreeArgs {
     uint8_t *need;            // children to wait for (0..2); 255 for sentinels
     uint32_t *nl;
     uint32_t *nr;
+    unsigned int *err;        // set when a search leaves the array (a chain without its sentinel)
 };
 
 __device__ __forceinline__ uint64_t levv(const TreeArgs &t, int l, uint64_t i) { return t.lev[t.lev_off[l] + i]; }
@@ -399,6 +401,7 @@ __device__ uint64_t nearest_left(const TreeArgs &t, uint64_t x, uint64_t v) {
     int64_t p = (int64_t)x - 1;
     int l = 0;
     while (true) {
+        if (p < 0) { atomicOr(t.err, 1u); return x - 1; }
         while (l + 1 < t.nlev && (((uint64_t)p + 1) & ((2ull << l) - 1)) == 0) l++;
         if (levv(t, l, (((uint64_t)p + 1) >> l) - 1) > v) break;
         p -= (int64_t)1 << l;
@@ -414,7 +417,8 @@ __device__ uint64_t nearest_right(const TreeArgs &t, uint64_t x, uint64_t v) {
     uint64_t p = x + 1;
     int l = 0;
     while (true) {
-        while (l + 1 < t.nlev && (p & ((2ull << l) - 1)) == 0) l++;
+        if (p >= t.M) { atomicOr(t.err, 1u); return x + 1; }
+        while (l + 1 < t.nlev && (p & ((2ull << l) - 1)) == 0 && (p >> (l + 1)) < ((t.M + (2ull << l) - 1) >> (l + 1))) l++;
         if (levv(t, l, p >> l) > v) break;
         p += 1ull << l;
     }
@@ -435,6 +439,11 @@ __global__ void k_tree_links(TreeArgs t) {
         return;
     }
     const uint64_t L = nearest_left(t, q, v), R = nearest_right(t, q, v);
+    if (L >= t.M || R >= t.M) {  // reported through err; keep the position out of the evaluation
+        t.par[q] = kNil;
+        t.need[q] = 255;
+        return;
+    }
     const uint64_t vl = t.lev[L], vr = t.lev[R];
     t.nl[q] = (uint32_t)L;
     t.nr[q] = (uint32_t)R;
@@ -461,7 +470,7 @@ __global__ void k_tree_eval(TreeArgs t, uint64_t *inL, uint64_t *inR, uint32_t *
     uint64_t q = q0;
     auto unit = [&](uint64_t pos) -> uint64_t { return (1ull << 8) | t.cmul[pos]; };
     uint64_t val = splice(unit(q - 1), unit(q));
-    while (true) {
+    for (uint64_t steps = 0; steps <= t.M; steps++) {  // parents have strictly larger priority
         const uint32_t p = t.par[q];
         if (p == kNil) {
             // root: both ends of the chain get the value
@@ -725,6 +734,7 @@ int Pipeline::merge_device(const uint64_t *host_prio, MergeDev &out) {
         ta.need = mb_[M_NEED].as<uint8_t>();
         ta.nl = mb_[M_NL].as<uint32_t>();
         ta.nr = mb_[M_NR].as<uint32_t>();
+        ta.err = (unsigned int *)(d_small + 25);
         hipLaunchKernelGGL(k_tree_links, dim3(nblk(M)), dim3(256), 0, st_, ta);
         hipLaunchKernelGGL(k_tree_eval, dim3(nblk(M)), dim3(256), 0, st_, ta, mb_[M_INL].as<uint64_t>(),
                            mb_[M_INR].as<uint64_t>(), mb_[M_CNT].as<uint32_t>(), mb_[M_EMULT].as<uint8_t>());
@@ -750,7 +760,9 @@ int Pipeline::merge_device(const uint64_t *host_prio, MergeDev &out) {
         out.special.resize(nspec);
         MCHECK(hipMemcpyAsync(out.special.data(), mb_[M_SPEC].p, nspec * 4, hipMemcpyDeviceToHost, st_));
     }
+    MCHECK(hipMemcpyAsync(hs + 25, d_small + 25, 8, hipMemcpyDeviceToHost, st_));
     MCHECK(hipStreamSynchronize(st_));
+    if ((uint32_t)hs[25]) return fail(-5, "merge: a chain array lost its end marker");
     collect_phases();
     n_kept = nk;
     n_special = nspec;

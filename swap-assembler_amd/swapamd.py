@@ -23,7 +23,7 @@ SWAP_FILE_NOCE = 2
 EXPORTS = [
     "swap_create", "swap_destroy", "swap_build", "swap_build_device", "swap_merge", "swap_get_stats",
     "swap_export_kmer_graph", "swap_format_file", "swap_write_file", "swap_strerror", "swap_last_error",
-    "swap_table_order",
+    "swap_table_order", "swap_visit_order", "swap_table_epochs",
 ]
 
 
@@ -71,6 +71,11 @@ def lib():
         L.swap_table_order.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_uint64,
                                        ctypes.c_int, ctypes.c_void_p]
         L.swap_table_order.restype = ctypes.c_int64
+        L.swap_visit_order.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p,
+                                       ctypes.c_size_t, ctypes.c_uint64, ctypes.c_int, ctypes.c_void_p]
+        L.swap_visit_order.restype = ctypes.c_int64
+        L.swap_table_epochs.argtypes = [ctypes.c_uint64, ctypes.c_int, ctypes.c_void_p, ctypes.c_size_t]
+        L.swap_table_epochs.restype = ctypes.c_int64
         _lib = L
     return _lib
 
@@ -153,6 +158,24 @@ def table_order(keys, ranks, n_total, extra_resize=False):
     out = np.zeros(len(keys), dtype=np.uint32)
     rc = lib().swap_table_order(keys.ctypes.data, ranks.ctypes.data, len(keys), n_total, int(extra_resize),
                                 out.ctypes.data)
+    if rc < 0:
+        raise SwapError(int(rc))
+    return out
+
+
+def table_epochs(n_total, extra_resize=False):
+    out = np.zeros(64, dtype=np.uint64)
+    n = lib().swap_table_epochs(n_total, int(extra_resize), out.ctypes.data, 64)
+    return out[:n]
+
+
+def visit_order(keys, first, epoch_first, distinct, tail_lookup=False):
+    keys = np.ascontiguousarray(keys, dtype=np.uint64)
+    first = np.ascontiguousarray(first, dtype=np.uint64)
+    ef = np.ascontiguousarray(epoch_first, dtype=np.uint64)
+    out = np.zeros(len(keys), dtype=np.uint32)
+    rc = lib().swap_visit_order(keys.ctypes.data, first.ctypes.data, len(keys), ef.ctypes.data, len(ef), distinct,
+                                int(tail_lookup), out.ctypes.data)
     if rc < 0:
         raise SwapError(int(rc))
     return out

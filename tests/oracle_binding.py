@@ -28,6 +28,8 @@ def _lib():
         for f in ("oracle_survivor_first", "oracle_survivor_rank", "oracle_visit_order"):
             getattr(L, f).argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64]
             getattr(L, f).restype = ctypes.c_uint64
+        L.oracle_first_of_rank.argtypes = [ctypes.c_void_p, ctypes.c_uint64]
+        L.oracle_first_of_rank.restype = ctypes.c_uint64
         L.oracle_file.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_uint64]
         L.oracle_file.restype = ctypes.c_uint64
         _L = L
@@ -77,6 +79,9 @@ class Oracle:
         if n:
             _lib().oracle_survivor_rank(self._h, out.ctypes.data, n)
         return out
+
+    def first_of_rank(self, r):
+        return _lib().oracle_first_of_rank(self._h, r)
 
     def visit_order(self):
         n = _lib().oracle_visit_order(self._h, None, 0)

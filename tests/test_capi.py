@@ -82,3 +82,20 @@ def test_table_order_small_cases_match_container():
     hv = (h >> np.uint64(32)) ^ (h & np.uint64(0xFFFFFFFF)).astype(np.uint32).astype(np.int32).astype(np.int64).astype(np.uint64)
     b = hv % np.uint64(6151)
     assert np.all(np.diff(b.astype(np.int64)) >= 0)
+
+
+@pytest.mark.parametrize("name", sorted(json.load(open(os.path.join(GOLD, "cases.json")))))
+def test_visit_order_from_first_occurrences(name):
+    """The replay used by the library (first-occurrence indices + the first
+    occurrence of each growth rank) reproduces the reference container's order."""
+    case = json.load(open(os.path.join(GOLD, "cases.json")))[name]
+    with gzip.open(os.path.join(GOLD, name + ".fa.gz"), "rb") as f:
+        fa = f.read()
+    o = Oracle(fa, case["k"], case["c"])
+    keys, _ = o.survivors()
+    first = o.survivor_first()
+    D = o.stats()["distinct"]
+    starts = swapamd.table_epochs(D, False)
+    ef = [o.first_of_rank(int(t)) for t in starts]
+    order = swapamd.visit_order(keys, first, ef, D, False)
+    assert np.array_equal(keys[order], o.visit_order())
