@@ -29,16 +29,20 @@ sys.path.insert(0, os.path.join(REPO, "swap-assembler_amd"))
 HBM_PEAK = 8.0e12  # MI355X HBM3E, bytes/s
 
 # phase index -> kernel name (see swap_stats.phase_ms)
-PHASES = {0: "record_scan", 1: "pack_2bit", 2: "partition_hist", 3: "partition_scatter", 4: "count_lds_cutoff",
-          5: "count_global_overflow", 6: "first_occurrence_rank"}
+PHASES = {0: "record_scan", 1: "pack_2bit", 2: "partition_hist", 3: "partition_level1", 4: "count_lds_cutoff",
+          5: "count_global_overflow", 6: "growth_rank_select", 7: "partition_level2", 8: "survivor_index",
+          9: "list_ranking", 10: "kept_edges", 11: "edge_strings", 12: "cartesian_trees", 13: "visit_order",
+          14: "classify"}
 
 
 def algorithmic_bytes(phase, st, words, ref_order):
     """Words a phase must move once (reads + writes), per launch."""
     N, S, D, R = st["kmers"], st["survivors"], st["distinct"], st["reads"]
     occ = 4 if ref_order else 0
-    if phase == 3:   # partition scatter: packed reads twice (count + write passes), records out
+    if phase == 3:   # level-1 partition: packed reads twice (count + place passes), records out
         return 2 * 8 * words + 2 * 16 * R + N * (8 + occ)
+    if phase == 7:   # level-2 partition: records in, records out
+        return 2 * N * (8 + occ)
     if phase == 4:   # LDS count: records in, survivors out, first-occurrence bits
         return N * (8 + occ) + S * (8 + 8 + occ) + (4 * D if ref_order else 0)
     if phase == 2:   # partition histogram: packed reads once

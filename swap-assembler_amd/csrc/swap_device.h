@@ -41,7 +41,20 @@ struct HostCompacted {
     std::vector<uint32_t> len;       // 8 per node
     std::vector<uint8_t> mult;       // 8 per node
     std::vector<char> pool;          // edge strings, bases as 'A','C','G','T'
+    // nodes are stored in the order of contigGraph.txt (the gather walks the
+    // nodes table in visit order); noce_order lists them in the order of the
+    // table the gather re-fills, which printContigs walks
+    std::vector<uint32_t> noce_order;
 };
+
+// Growth epochs of a reference hash table: epoch e starts at insert rank
+// start[e] (or at the first occurrence of that rank's key) with P[e] buckets.
+struct EpochTabHost {
+    int n = 0;
+    uint64_t start[32] = {0};
+    uint64_t P[32] = {0};
+};
+EpochTabHost epoch_table(uint64_t n_total, bool extra, const std::vector<uint64_t> *starts_override);
 
 // Device merge results copied to the host.
 struct MergeDev {
@@ -53,6 +66,10 @@ struct MergeDev {
     std::vector<uint8_t> especial;    // 8 per kept node: edge enters a cycle / fold-back component
     std::vector<char> pool;
     std::vector<uint32_t> special;    // (1,1) survivors of cycle / fold-back components
+    std::vector<uint64_t> kept_prio;  // visit priority of each kept node
+    std::vector<uint64_t> kept_key;   // key of each kept node
+    // all survivors (device order), copied only when special components exist
+    std::vector<uint64_t> all_key, all_mult, all_prio;
     uint64_t n11 = 0;
 };
 
@@ -69,7 +86,9 @@ public:
     // Compacts unitigs (k-molecule merge).  prio: per-survivor visit rank in
     // survivor order (device or host array, may be null for the built-in
     // hash order).
-    int merge_device(const uint64_t *host_prio, MergeDev &out);
+    int merge_device(bool reference_order, MergeDev &out);
+    int visit_priorities(uint64_t *d_prio);
+    int table_order_device(const uint64_t *d_rank, const EpochTabHost &tab, uint32_t *d_order);
 
     // Counters of the last build / merge.
     uint64_t n_reads = 0, n_occ = 0, n_distinct = 0, n_surv = 0, n_kept = 0, n_overflow = 0;
@@ -116,6 +135,8 @@ private:
         M_CMUL, M_CEDGE, M_LEV, M_LEVOFF, M_PAR, M_NEED, M_NL, M_NR, M_INL, M_INR, M_CNT, M_SPEC, M_COUNT
     };
     DevBuf mb_[M_COUNT];
+    enum { O_H, O_INS, O_BKT, O_BKT2, O_IDX, O_TMP, O_RANK, O_ORD, O_COUNT };
+    DevBuf ob_[O_COUNT];
     // host mirrors of small results
     uint64_t *h_small_ = nullptr;
     uint64_t last_first_ = 0;

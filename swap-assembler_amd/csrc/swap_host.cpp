@@ -3,6 +3,8 @@
 #include "swap_host.h"
 
 #include <algorithm>
+#include <chrono>
+#include <cstdio>
 #include <cstring>
 #include <functional>
 #include <numeric>
@@ -55,22 +57,23 @@ Epochs make_epochs(uint64_t n_total, bool extra_resize) {
     return e;
 }
 
-// LSD radix sort of (bucket, index) by bucket, stable.
+// LSD radix sort of (bucket, index) by bucket, stable: four 8-bit passes over
+// packed (bucket << 32 | index) words.
 void sort_by_bucket(std::vector<uint32_t> &bucket, std::vector<uint32_t> &idx) {
     const size_t n = idx.size();
-    std::vector<uint32_t> b2(n), i2(n);
-    for (int pass = 0; pass < 2; pass++) {
-        const int sh = 16 * pass;
-        std::vector<size_t> cnt(65537, 0);
-        for (size_t i = 0; i < n; i++) cnt[((bucket[i] >> sh) & 0xFFFF) + 1]++;
-        for (int d = 0; d < 65536; d++) cnt[d + 1] += cnt[d];
-        for (size_t i = 0; i < n; i++) {
-            const size_t pos = cnt[(bucket[i] >> sh) & 0xFFFF]++;
-            b2[pos] = bucket[i];
-            i2[pos] = idx[i];
-        }
-        bucket.swap(b2);
-        idx.swap(i2);
+    std::vector<uint64_t> a(n), b(n);
+    for (size_t i = 0; i < n; i++) a[i] = ((uint64_t)bucket[i] << 32) | idx[i];
+    for (int pass = 0; pass < 4; pass++) {
+        const int sh = 32 + 8 * pass;
+        size_t cnt[257] = {0};
+        for (size_t i = 0; i < n; i++) cnt[((a[i] >> sh) & 0xFF) + 1]++;
+        for (int d = 0; d < 256; d++) cnt[d + 1] += cnt[d];
+        for (size_t i = 0; i < n; i++) b[cnt[(a[i] >> sh) & 0xFF]++] = a[i];
+        a.swap(b);
+    }
+    for (size_t i = 0; i < n; i++) {
+        bucket[i] = (uint32_t)(a[i] >> 32);
+        idx[i] = (uint32_t)a[i];
     }
 }
 
@@ -89,28 +92,54 @@ std::vector<uint32_t> order_with_epochs(const std::vector<uint64_t> &key, const 
     const int f = (int)ep.P.size() - 1;
     const size_t n = key.size();
     std::vector<uint64_t> h(n);
-    for (size_t i = 0; i < n; i++) h[i] = ref_hash(key[i]);
-    // a precedes b in the iteration order at the end of epoch e
-    std::function<bool(uint32_t, uint32_t, int)> before = [&](uint32_t a, uint32_t b, int e) -> bool {
-        const uint64_t ba = h[a] % ep.P[e], bb = h[b] % ep.P[e];
-        if (ba != bb) return ba < bb;
-        const bool ia = ep.of(rank[a]) == e, ib = ep.of(rank[b]) == e;
-        if (ia && ib) return rank[a] > rank[b];  // head insertion: newest first
-        if (ia != ib) return ia;                 // inserted after the rehash: in front
-        return before(b, a, e - 1);              // rehash reverses each chain
+    std::vector<int16_t> ins(n);  // insertion epoch of each element
+    for (size_t i = 0; i < n; i++) {
+        h[i] = ref_hash(key[i]);
+        ins[i] = (int16_t)ep.of(rank[i]);
+    }
+    // a precedes b in the iteration order at the end of epoch e.  Within a
+    // bucket: elements inserted during e (newest first, head insertion), then
+    // the ones carried over by the rehash that opened e, in reverse of their
+    // order at the end of e-1.
+    auto before = [&](uint32_t a, uint32_t b, int e) -> bool {
+        bool flip = false;
+        for (;; e--) {
+            const uint64_t ba = h[a] % ep.P[e], bb = h[b] % ep.P[e];
+            if (ba != bb) return (ba < bb) != flip;
+            const bool ia = ins[a] == e, ib = ins[b] == e;
+            if (ia && ib) return (rank[a] > rank[b]) != flip;
+            if (ia != ib) return ia != flip;
+            flip = !flip;
+        }
     };
     std::vector<uint32_t> bucket(n), idx(n);
+#ifdef SWAP_TIME_ORDER
+    auto T0 = std::chrono::steady_clock::now();
+#endif
     for (size_t i = 0; i < n; i++) {
         bucket[i] = (uint32_t)(h[i] % ep.P[f]);
         idx[i] = (uint32_t)i;
     }
     sort_by_bucket(bucket, idx);
+#ifdef SWAP_TIME_ORDER
+    auto T1 = std::chrono::steady_clock::now();
+    fprintf(stderr, "sort %.3f\n", std::chrono::duration<double>(T1 - T0).count());
+#endif
     for (size_t i = 0; i < n;) {
         size_t j = i + 1;
         while (j < n && bucket[j] == bucket[i]) j++;
-        if (j - i > 1) std::sort(idx.begin() + i, idx.begin() + j, [&](uint32_t a, uint32_t b) { return before(a, b, f); });
+        // insertion sort: buckets hold a handful of elements
+        for (size_t x = i + 1; x < j; x++) {
+            const uint32_t v = idx[x];
+            size_t y = x;
+            while (y > i && before(v, idx[y - 1], f)) { idx[y] = idx[y - 1]; y--; }
+            idx[y] = v;
+        }
         i = j;
     }
+#ifdef SWAP_TIME_ORDER
+    fprintf(stderr, "ties %.3f\n", std::chrono::duration<double>(std::chrono::steady_clock::now() - T1).count());
+#endif
     return idx;
 }
 
@@ -275,7 +304,8 @@ void format_contig_graph(const HostCompacted &c, std::string &out) {
 // Contig_Length - k bases whose start node is not smaller than its end.
 void format_noce(int k, const HostCompacted &c, std::string &out) {
     const size_t n = c.key.size();
-    for (size_t i = 0; i < n; i++) {
+    for (size_t oi = 0; oi < n; oi++) {
+        const size_t i = c.noce_order.empty() ? oi : c.noce_order[oi];
         const uint64_t X = c.key[i];
         for (int s = 0; s < 8; s++) {
             const uint32_t L = c.len[8 * i + s];
@@ -351,32 +381,11 @@ int host_merge(Pipeline &p, const swap_cfg &cfg, std::vector<uint32_t> &visit, H
     const bool ref = !(cfg.flags & SWAP_FLAG_HASH_ORDER);
     const int k = cfg.k;
     out = HostCompacted();
-    HostKmerGraph g;
-    int rc = p.export_kmer_graph(g);  // device order
-    if (rc) return rc;
-    const size_t S = g.key.size();
-    std::vector<uint64_t> prio;
-    uint64_t P3 = 0;
-    if (ref && S) {
-        std::vector<uint64_t> first(g.first.begin(), g.first.end());
-        std::vector<uint32_t> order = visit_order(g.key, first, g.epoch_first, g.distinct, g.tail_lookup, &P3);
-        prio.resize(S);
-        for (size_t i = 0; i < S; i++) prio[order[i]] = i;
-    }
     MergeDev md;
-    rc = p.merge_device(ref && S ? prio.data() : nullptr, md);
+    int rc = p.merge_device(ref, md);
     if (rc) return rc;
-    if (!ref) {
-        // the device's built-in priority: the k-mer mixer, recomputed here
-        prio.resize(S);
-        Mixer mx;
-        mx.bits = 2 * k;
-        mx.m = kmask(k);
-        mx.a1 = 0x9E3779B97F4A7C15ull | 1;
-        mx.a2 = 0xC2B2AE3D27D4EB4Full | 1;
-        mx.s = k;
-        for (size_t i = 0; i < S; i++) prio[i] = mix_fwd(mx, g.key[i]);
-    }
+    const size_t S = p.n_surv;
+    const uint64_t P3 = ref ? table_buckets(S, false) : 0;
     const size_t nk = md.n_kept;
     // per kept node: 8 (string, multiplicity) taken from the device results
     struct Out {
@@ -390,9 +399,8 @@ int host_merge(Pipeline &p, const swap_cfg &cfg, std::vector<uint32_t> &visit, H
     std::unordered_map<uint64_t, size_t> live_of;  // key -> index in live (only kept nodes next to specials)
     for (size_t i = 0; i < nk; i++) {
         Out o;
-        const uint32_t sidx = md.kept_src[i];
-        o.id = g.key[sidx];
-        o.prio = prio[sidx];
+        o.id = md.kept_key[i];
+        o.prio = md.kept_prio[i];
         bool any = false;
         for (int s = 0; s < 8; s++) {
             const uint32_t L = md.elen[8 * i + s];
@@ -407,17 +415,18 @@ int host_merge(Pipeline &p, const swap_cfg &cfg, std::vector<uint32_t> &visit, H
         Replay R;
         R.k = k;
         R.km = kmask(k);
+        const std::vector<uint64_t> &gkey = md.all_key, &gmult = md.all_mult, &prio = md.all_prio;
         std::unordered_map<uint64_t, uint32_t> sidx_of;
         sidx_of.reserve(S * 2);
-        for (size_t i = 0; i < S; i++) sidx_of[g.key[i]] = (uint32_t)i;
+        for (size_t i = 0; i < S; i++) sidx_of[gkey[i]] = (uint32_t)i;
         auto make = [&](uint32_t sidx, int64_t keptIdx) {
             HNode n;
-            n.id = g.key[sidx];
+            n.id = gkey[sidx];
             n.prio = prio[sidx];
             n.kept = keptIdx;
             int nr = 0, nl = 0;
             for (int s = 0; s < 8; s++) {
-                n.m[s] = (uint8_t)(g.mult[sidx] >> (8 * s));
+                n.m[s] = (uint8_t)(gmult[sidx] >> (8 * s));
                 if (n.m[s]) {
                     n.arcs[s] = std::string(1, "ACGT"[s & 3]);
                     (s < 4 ? nr : nl)++;
@@ -427,8 +436,8 @@ int host_merge(Pipeline &p, const swap_cfg &cfg, std::vector<uint32_t> &visit, H
             return n;
         };
         std::unordered_map<uint64_t, int64_t> kept_index;
-        for (size_t i = 0; i < nk; i++) kept_index[g.key[md.kept_src[i]]] = (int64_t)i;
-        for (uint32_t sidx : md.special) R.nodes[g.key[sidx]] = make(sidx, -1);
+        for (size_t i = 0; i < nk; i++) kept_index[md.kept_key[i]] = (int64_t)i;
+        for (uint32_t sidx : md.special) R.nodes[gkey[sidx]] = make(sidx, -1);
         // kept neighbours of the component
         std::vector<uint64_t> spec_keys;
         for (auto &kv : R.nodes) spec_keys.push_back(kv.first);
@@ -449,7 +458,7 @@ int host_merge(Pipeline &p, const swap_cfg &cfg, std::vector<uint32_t> &visit, H
         auto next_one_after = [&](uint64_t after, uint64_t excluding_prio) -> uint64_t {
             if (one_prio.empty()) {
                 for (size_t i = 0; i < S; i++) {
-                    const uint64_t m = g.mult[i];
+                    const uint64_t m = gmult[i];
                     int nr = 0, nl = 0;
                     for (int s = 0; s < 4; s++) nr += ((m >> (8 * s)) & 0xFF) != 0;
                     for (int s = 4; s < 8; s++) nl += ((m >> (8 * s)) & 0xFF) != 0;
@@ -571,16 +580,23 @@ int host_merge(Pipeline &p, const swap_cfg &cfg, std::vector<uint32_t> &visit, H
         }
         live.resize(w);
     }
-    // output order: the table order after the gather (reference order), else by key
+    // contigGraph.txt order: the gather walks the nodes table, i.e. visit order
+    // (reference order); by key otherwise
     std::vector<uint32_t> ord(live.size());
     std::iota(ord.begin(), ord.end(), 0u);
+    if (ref && P3)
+        std::sort(ord.begin(), ord.end(), [&](uint32_t a, uint32_t b) { return live[a].prio < live[b].prio; });
+    else
+        std::sort(ord.begin(), ord.end(), [&](uint32_t a, uint32_t b) { return live[a].id < live[b].id; });
+    // noCEcontigs.fasta order: the re-filled table (graph.cpp:3615, 891)
+    out.noce_order.resize(live.size());
     if (ref && P3) {
         std::vector<uint64_t> keys(live.size());
         std::vector<uint32_t> pos(live.size());
-        for (size_t i = 0; i < live.size(); i++) { keys[i] = live[i].id; pos[i] = (uint32_t)live[i].prio; }
-        ord = gather_order(keys, pos, P3);
+        for (size_t r = 0; r < ord.size(); r++) { keys[r] = live[ord[r]].id; pos[r] = (uint32_t)r; }
+        out.noce_order = gather_order(keys, pos, P3);
     } else {
-        std::sort(ord.begin(), ord.end(), [&](uint32_t a, uint32_t b) { return live[a].id < live[b].id; });
+        std::iota(out.noce_order.begin(), out.noce_order.end(), 0u);
     }
     size_t pool_n = 0;
     for (auto &o : live) for (int s = 0; s < 8; s++) pool_n += o.arcs[s].size();

@@ -505,6 +505,12 @@ __global__ void k_u32_to_u64(const uint32_t *__restrict__ in, uint64_t n, uint64
     if (i < n) out[i] = in[i];
 }
 
+__global__ void k_gather_u64(const uint64_t *__restrict__ src, const uint32_t *__restrict__ idx, uint64_t n,
+                             uint64_t *__restrict__ out) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = src[idx[i]];
+}
+
 template <typename T>
 __global__ void k_fill(T *__restrict__ p, uint64_t n, T v) {
     const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -540,7 +546,7 @@ __global__ void k_special_flags(const uint8_t *__restrict__ is11, const Walk *__
         if (e_ != hipSuccess) return fail(-5, hipGetErrorString(e_));        \
     } while (0)
 
-int Pipeline::merge_device(const uint64_t *host_prio, MergeDev &out) {
+int Pipeline::merge_device(bool reference_order, MergeDev &out) {
     MCHECK(hipSetDevice(dev_));
     out = MergeDev();
     const uint64_t S = n_surv;
@@ -561,11 +567,17 @@ int Pipeline::merge_device(const uint64_t *host_prio, MergeDev &out) {
                        mb_[M_IKEY].as<uint64_t>(), mb_[M_IVAL].as<uint32_t>(), cap - 1);
     // --- priorities (visit time)
     MCHECK(mb_[M_PRIO].reserve(S * 8));
-    if (host_prio)
-        MCHECK(hipMemcpyAsync(mb_[M_PRIO].p, host_prio, S * 8, hipMemcpyHostToDevice, st_));
-    else
+    phase_end(8);
+    phase_begin(13);
+    if (reference_order) {
+        const int rc = visit_priorities(mb_[M_PRIO].as<uint64_t>());
+        if (rc) return rc;
+    } else {
         hipLaunchKernelGGL(k_prio_hash, dim3(nblk(S)), dim3(256), 0, st_, surv_key_.as<uint64_t>(), S, mx_,
                            mb_[M_PRIO].as<uint64_t>());
+    }
+    phase_end(13);
+    phase_begin(14);
     // --- classification
     MergeArgs ma;
     ma.key = surv_key_.as<uint64_t>();
@@ -581,7 +593,7 @@ int Pipeline::merge_device(const uint64_t *host_prio, MergeDev &out) {
     MCHECK(mb_[M_SELF].reserve(S));
     hipLaunchKernelGGL(k_classify, dim3(nblk(S)), dim3(256), 0, st_, ma, mb_[M_IS11].as<uint8_t>(),
                        mb_[M_NXT].as<uint32_t>(), mb_[M_EX].as<uint8_t>(), mb_[M_SELF].as<uint8_t>(), cnt);
-    phase_end(8);
+    phase_end(14);
     // --- list ranking by pointer jumping over oriented (1,1) nodes
     MCHECK(mb_[M_WALKA].reserve(2 * S * sizeof(Walk)));
     MCHECK(mb_[M_WALKB].reserve(2 * S * sizeof(Walk)));
@@ -756,9 +768,27 @@ int Pipeline::merge_device(const uint64_t *host_prio, MergeDev &out) {
         MCHECK(hipMemcpyAsync(out.especial.data(), mb_[M_ESPEC].p, ne, hipMemcpyDeviceToHost, st_));
     }
     if (pool_n) MCHECK(hipMemcpyAsync(out.pool.data(), mb_[M_POOL].p, pool_n, hipMemcpyDeviceToHost, st_));
+    out.kept_prio.resize(nk);
+    if (nk) {
+        MCHECK(mb_[M_CPRI].reserve(std::max<uint64_t>(M, nk) * 8 + 16));
+        hipLaunchKernelGGL(k_gather_u64, dim3(nblk(nk)), dim3(256), 0, st_, mb_[M_PRIO].as<uint64_t>(),
+                           mb_[M_KEPTSRC].as<uint32_t>(), nk, mb_[M_CPRI].as<uint64_t>());
+        MCHECK(hipMemcpyAsync(out.kept_prio.data(), mb_[M_CPRI].p, nk * 8, hipMemcpyDeviceToHost, st_));
+        MCHECK(mb_[M_INL].reserve(nk * 8 + 16));
+        hipLaunchKernelGGL(k_gather_u64, dim3(nblk(nk)), dim3(256), 0, st_, surv_key_.as<uint64_t>(),
+                           mb_[M_KEPTSRC].as<uint32_t>(), nk, mb_[M_INL].as<uint64_t>());
+        out.kept_key.resize(nk);
+        MCHECK(hipMemcpyAsync(out.kept_key.data(), mb_[M_INL].p, nk * 8, hipMemcpyDeviceToHost, st_));
+    }
     if (nspec) {
         out.special.resize(nspec);
         MCHECK(hipMemcpyAsync(out.special.data(), mb_[M_SPEC].p, nspec * 4, hipMemcpyDeviceToHost, st_));
+        out.all_key.resize(S);
+        out.all_mult.resize(S);
+        out.all_prio.resize(S);
+        MCHECK(hipMemcpyAsync(out.all_key.data(), surv_key_.p, S * 8, hipMemcpyDeviceToHost, st_));
+        MCHECK(hipMemcpyAsync(out.all_mult.data(), surv_mult_.p, S * 8, hipMemcpyDeviceToHost, st_));
+        MCHECK(hipMemcpyAsync(out.all_prio.data(), mb_[M_PRIO].p, S * 8, hipMemcpyDeviceToHost, st_));
     }
     MCHECK(hipMemcpyAsync(hs + 25, d_small + 25, 8, hipMemcpyDeviceToHost, st_));
     MCHECK(hipStreamSynchronize(st_));

@@ -1,0 +1,169 @@
+// The reference's merge visit order computed on the device.
+//
+// SWAP-Assembler 2 visits nodes in the iteration order of a libstdc++
+// __gnu_cxx::hash_map (graph.cpp:1409) that was filled from two other such
+// tables (graph.cpp:1058-1070, 1096-1112, 1123-1133).  Iteration order is
+// bucket-major (bucket = hash % bucket_count, hash of graph.cpp:29-32); within a
+// bucket it depends on the history of inserts (head insertion) and rehashes
+// (which reverse each chain), and that history only depends on each key's
+// insertion rank and on the ranks at which the table grew.  So each table
+// order is: sort by final bucket (radix sort), then order each bucket's few
+// keys with a comparator that walks back through the growth epochs.
+#include <cstring>
+
+#include <rocprim/device/device_radix_sort.hpp>
+
+#include "swap_host.h"
+
+namespace swapk {
+
+struct EpochTab {
+    int n;              // number of epochs
+    uint64_t start[32]; // first insert rank of each epoch
+    uint64_t P[32];     // bucket count during each epoch
+};
+
+__device__ __forceinline__ uint64_t dev_ref_hash(uint64_t x) {
+    return (x >> 32) ^ (uint64_t)(int64_t)(int32_t)(uint32_t)(x & 0xFFFFFFFFull);
+}
+
+__global__ void k_ord_prep(const uint64_t *__restrict__ key, const uint64_t *__restrict__ rank, uint64_t n,
+                           EpochTab et, uint64_t *__restrict__ h, uint8_t *__restrict__ ins,
+                           uint32_t *__restrict__ bucket, uint32_t *__restrict__ idx) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint64_t hv = dev_ref_hash(key[i]);
+    const uint64_t r = rank[i];
+    int e = 0;
+    for (int j = 1; j < et.n; j++)
+        if (et.start[j] <= r) e = j;
+    h[i] = hv;
+    ins[i] = (uint8_t)e;
+    bucket[i] = (uint32_t)(hv % et.P[et.n - 1]);
+    idx[i] = (uint32_t)i;
+}
+
+// a precedes b at the end of epoch e (see the host restatement in swap_host.cpp)
+__device__ __forceinline__ bool ord_before(uint32_t a, uint32_t b, int e, const EpochTab &et,
+                                           const uint64_t *__restrict__ h, const uint8_t *__restrict__ ins,
+                                           const uint64_t *__restrict__ rank) {
+    bool flip = false;
+    for (; e >= 0; e--) {
+        const uint64_t ba = h[a] % et.P[e], bb = h[b] % et.P[e];
+        if (ba != bb) return (ba < bb) != flip;
+        const bool ia = ins[a] == e, ib = ins[b] == e;
+        if (ia && ib) return (rank[a] > rank[b]) != flip;
+        if (ia != ib) return ia != flip;
+        flip = !flip;
+    }
+    return false;
+}
+
+// One thread per bucket run of the sorted array: insertion sort of its keys.
+__global__ void k_ord_ties(const uint32_t *__restrict__ bucket, uint32_t *__restrict__ idx, uint64_t n, EpochTab et,
+                           const uint64_t *__restrict__ h, const uint8_t *__restrict__ ins,
+                           const uint64_t *__restrict__ rank) {
+    const uint64_t q = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= n) return;
+    const uint32_t b = bucket[q];
+    if (q > 0 && bucket[q - 1] == b) return;
+    uint64_t e = q + 1;
+    while (e < n && bucket[e] == b) e++;
+    if (e - q < 2) return;
+    const int f = et.n - 1;
+    for (uint64_t x = q + 1; x < e; x++) {
+        const uint32_t v = idx[x];
+        uint64_t y = x;
+        while (y > q && ord_before(v, idx[y - 1], f, et, h, ins, rank)) {
+            idx[y] = idx[y - 1];
+            y--;
+        }
+        idx[y] = v;
+    }
+}
+
+__global__ void k_ord_positions(const uint32_t *__restrict__ order, uint64_t n, uint64_t *__restrict__ pos) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) pos[order[i]] = i;
+}
+
+__global__ void k_u32_to_u64_o(const uint32_t *__restrict__ in, uint64_t n, uint64_t *__restrict__ out) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = in[i];
+}
+
+#define OCHECK(x)                                                            \
+    do {                                                                     \
+        hipError_t e_ = (x);                                                 \
+        if (e_ != hipSuccess) return fail(-5, hipGetErrorString(e_));        \
+    } while (0)
+
+static inline unsigned oblk(uint64_t n) { return (unsigned)((n + 255) / 256); }
+
+// Iteration order of one table: sorted survivor indices into d_order.
+int Pipeline::table_order_device(const uint64_t *d_rank, const EpochTabHost &tab, uint32_t *d_order) {
+    const uint64_t S = n_surv;
+    EpochTab et;
+    memset(&et, 0, sizeof et);
+    et.n = tab.n;
+    for (int e = 0; e < tab.n; e++) { et.start[e] = tab.start[e]; et.P[e] = tab.P[e]; }
+    OCHECK(ob_[O_H].reserve(S * 8));
+    OCHECK(ob_[O_INS].reserve(S));
+    OCHECK(ob_[O_BKT].reserve(S * 4));
+    OCHECK(ob_[O_BKT2].reserve(S * 4));
+    OCHECK(ob_[O_IDX].reserve(S * 4));
+    hipLaunchKernelGGL(k_ord_prep, dim3(oblk(S)), dim3(256), 0, st_, surv_key_.as<uint64_t>(), d_rank, S, et,
+                       ob_[O_H].as<uint64_t>(), ob_[O_INS].as<uint8_t>(), ob_[O_BKT].as<uint32_t>(),
+                       ob_[O_IDX].as<uint32_t>());
+    int bits = 1;
+    while (bits < 32 && (1ull << bits) < et.P[et.n - 1]) bits++;
+    size_t tb = 0;
+    OCHECK(rocprim::radix_sort_pairs(nullptr, tb, ob_[O_BKT].as<uint32_t>(), ob_[O_BKT2].as<uint32_t>(),
+                                     ob_[O_IDX].as<uint32_t>(), d_order, (size_t)S, 0, bits, st_));
+    OCHECK(ob_[O_TMP].reserve(tb + 16));
+    OCHECK(rocprim::radix_sort_pairs(ob_[O_TMP].p, tb, ob_[O_BKT].as<uint32_t>(), ob_[O_BKT2].as<uint32_t>(),
+                                     ob_[O_IDX].as<uint32_t>(), d_order, (size_t)S, 0, bits, st_));
+    hipLaunchKernelGGL(k_ord_ties, dim3(oblk(S)), dim3(256), 0, st_, ob_[O_BKT2].as<uint32_t>(), d_order, S, et,
+                       ob_[O_H].as<uint64_t>(), ob_[O_INS].as<uint8_t>(), d_rank);
+    return 0;
+}
+
+EpochTabHost epoch_table(uint64_t n_total, bool extra, const std::vector<uint64_t> *starts_override) {
+    EpochTabHost t;
+    std::vector<uint64_t> s = table_epoch_starts(n_total, extra);
+    t.n = (int)std::min<size_t>(s.size(), 32);
+    uint64_t P = table_buckets(0, false);
+    for (int e = 0; e < t.n; e++) {
+        if (e > 0) P = table_buckets(s[e] + 1, false);
+        t.P[e] = P;
+        t.start[e] = s[e];
+        if (starts_override) t.start[e] = e < (int)starts_override->size() ? (*starts_override)[e] : ~0ull;
+    }
+    return t;
+}
+
+// Visit priorities (position in the nodes table order) of all survivors.
+int Pipeline::visit_priorities(uint64_t *d_prio) {
+    const uint64_t S = n_surv;
+    OCHECK(ob_[O_RANK].reserve(S * 8));
+    OCHECK(ob_[O_ORD].reserve(S * 4));
+    uint64_t *rank = ob_[O_RANK].as<uint64_t>();
+    uint32_t *ord = ob_[O_ORD].as<uint32_t>();
+    // kmolecules: ranks are first occurrences, epochs start at the first
+    // occurrence of the key of each growth rank
+    hipLaunchKernelGGL(k_u32_to_u64_o, dim3(oblk(S)), dim3(256), 0, st_, surv_first_.as<uint32_t>(), S, rank);
+    const bool tail = n_occ > last_first_ + 1;
+    int rc = table_order_device(rank, epoch_table(n_distinct, tail, &epoch_first_), ord);
+    if (rc) return rc;
+    // cutoff copy and nodes table: ranks are positions in the previous order
+    const EpochTabHost tabS = epoch_table(S, false, nullptr);
+    for (int t = 0; t < 2; t++) {
+        hipLaunchKernelGGL(k_ord_positions, dim3(oblk(S)), dim3(256), 0, st_, ord, S, rank);
+        rc = table_order_device(rank, tabS, ord);
+        if (rc) return rc;
+    }
+    hipLaunchKernelGGL(k_ord_positions, dim3(oblk(S)), dim3(256), 0, st_, ord, S, d_prio);
+    return 0;
+}
+
+}  // namespace swapk
