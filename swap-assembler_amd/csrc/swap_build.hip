@@ -670,10 +670,28 @@ __global__ void k_sel_collect(const uint32_t *__restrict__ dfo, const unsigned l
                               const int *__restrict__ target, const uint32_t *__restrict__ off,
                               uint32_t *__restrict__ fill, uint32_t *__restrict__ out) {
     const uint64_t n = *d_n;
-    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
-        const uint32_t f = dfo[i];
-        const int t = target[f >> shift];
-        if (t >= 0) out[off[t] + atomicAdd(&fill[t], 1u)] = f;
+    const int lane = threadIdx.x & 63;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    const uint64_t n_up = (n + stride - 1) / stride * stride;  // whole waves iterate together
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n_up; i += stride) {
+        const uint32_t f = i < n ? dfo[i] : 0;
+        int t = i < n ? target[f >> shift] : -1;
+        // one atomic per (wave, target): the first values of a build crowd the
+        // few lowest buckets
+        uint64_t pending = __ballot(t >= 0);
+        while (pending) {
+            const int leader = __ffsll((long long)pending) - 1;
+            const int lt = __shfl(t, leader, 64);
+            const uint64_t same = __ballot(t == lt);
+            uint32_t base = 0;
+            if (lane == leader) base = atomicAdd(&fill[lt], (uint32_t)__popcll(same));
+            base = __shfl(base, leader, 64);
+            if (t == lt) {
+                out[off[lt] + base + __popcll(same & ((1ull << lane) - 1))] = f;
+                t = -1;
+            }
+            pending &= ~same;
+        }
     }
 }
 

@@ -16,7 +16,8 @@
 // Cartesian tree keyed by visit time (the last-visited node is the root,
 // combining the merged left and right parts).  Here:
 //   * nodes are looked up through a device hash index of the survivors;
-//   * chains are ranked by pointer jumping over oriented nodes (2 per node);
+//   * chains are ranked over oriented nodes (2 per node) with a ruling set:
+//     sampled rulers walk to the next ruler, then pointer jumping over rulers;
 //   * each oriented node writes its base into the string of the end it is
 //     reached from; kept-to-kept edges keep their one base;
 //   * nearest-larger-priority neighbours come from a max hierarchy, and the
@@ -148,37 +149,79 @@ struct Walk {
     uint32_t last;   // last oriented (1,1) node before the terminal
 };
 
-__global__ void k_walk_init(const uint8_t *__restrict__ is11, const uint32_t *__restrict__ nxt,
-                            const uint8_t *__restrict__ selfadj, uint64_t S, Walk *__restrict__ w,
-                            uint8_t *__restrict__ sp) {
+// Ruling-set list ranking.  Rulers: the first oriented node of every walk
+// that starts at a kept node, plus a hashed 1/kRuleStride sample of the rest
+// (so cycles and long chains are cut into short pieces).  Each ruler walks to
+// the next ruler or to a kept node, labelling the nodes it passes with
+// (ruler, offset); pointer jumping then runs over the rulers only.
+constexpr uint32_t kRuleStride = 16;
+
+__global__ void k_rs_mark(const uint8_t *__restrict__ is11, const uint32_t *__restrict__ nxt, uint64_t S,
+                          uint32_t *__restrict__ rflag, uint32_t stride) {
     const uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= 2 * S) return;
-    const uint64_t i = v >> 1;
-    Walk x;
-    if (!is11[i]) {
-        x.succ = kNil; x.dist = 0; x.term = kNil; x.last = kNil;
-        w[v] = x;
-        sp[v] = 0;
-        return;
+    uint32_t f = 0;
+    if (is11[v >> 1]) {
+        const uint32_t back = nxt[v ^ 1];
+        f = (back == kNil || !is11[back >> 1]) ? 1u : ((uint32_t)((v * 0x9E3779B97F4A7C15ull) >> 32) % stride == 0);
     }
-    const uint32_t t = nxt[v];
-    x.dist = 1;
-    x.last = (uint32_t)v;
-    if (t == kNil || !is11[t >> 1]) {
-        x.succ = kNil;
-        x.term = t;
-    } else {
-        x.succ = t;
-        x.term = kNil;
-    }
-    w[v] = x;
-    sp[v] = selfadj[i];
+    rflag[v] = f;
 }
 
-__global__ void k_walk_jump(const Walk *__restrict__ win, const uint8_t *__restrict__ spin, uint64_t n2,
-                            Walk *__restrict__ wout, uint8_t *__restrict__ spout, unsigned int *__restrict__ changed) {
+__global__ void k_rs_list(const uint32_t *__restrict__ rflag, const uint64_t *__restrict__ rpos, uint64_t n2,
+                          uint32_t *__restrict__ rid, uint32_t *__restrict__ rlist) {
     const uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= n2) return;
+    if (rflag[v]) {
+        rid[v] = (uint32_t)rpos[v];
+        rlist[rpos[v]] = (uint32_t)v;
+    } else {
+        rid[v] = kNil;
+    }
+}
+
+// one thread per ruler: walk to the next ruler / kept node
+__global__ void k_rs_walk(const uint32_t *__restrict__ rlist, uint64_t nr, const uint8_t *__restrict__ is11,
+                          const uint32_t *__restrict__ nxt, const uint8_t *__restrict__ selfadj,
+                          const uint32_t *__restrict__ rid, uint64_t S, uint32_t *__restrict__ label_r,
+                          uint32_t *__restrict__ label_off, Walk *__restrict__ rw, uint8_t *__restrict__ rsp) {
+    const uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= nr) return;
+    uint32_t u = rlist[r];
+    uint8_t sp = selfadj[u >> 1];
+    Walk x;
+    x.succ = kNil;
+    x.term = kNil;
+    x.last = u;
+    uint32_t steps = 1;
+    for (uint64_t guard = 0; guard <= 2 * S; guard++) {
+        const uint32_t t = nxt[u];
+        if (t == kNil || !is11[t >> 1]) {  // arrives at a kept node
+            x.term = t;
+            x.last = u;
+            break;
+        }
+        if (rid[t] != kNil) {  // next ruler
+            x.succ = rid[t];
+            x.last = u;
+            break;
+        }
+        label_r[t] = (uint32_t)r;
+        label_off[t] = steps;
+        sp |= selfadj[t >> 1];
+        u = t;
+        steps++;
+    }
+    x.dist = steps;
+    rw[r] = x;
+    rsp[r] = sp;
+}
+
+// pointer jumping over rulers: succ is a ruler index
+__global__ void k_rs_jump(const Walk *__restrict__ win, const uint8_t *__restrict__ spin, uint64_t nr,
+                          Walk *__restrict__ wout, uint8_t *__restrict__ spout) {
+    const uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= nr) return;
     Walk x = win[v];
     uint8_t f = spin[v];
     if (x.succ != kNil) {
@@ -191,6 +234,36 @@ __global__ void k_walk_jump(const Walk *__restrict__ win, const uint8_t *__restr
     }
     wout[v] = x;
     spout[v] = f;
+}
+
+// per oriented node: walk state from its ruler
+__global__ void k_rs_final(const uint8_t *__restrict__ is11, const uint32_t *__restrict__ rid,
+                           const uint32_t *__restrict__ label_r, const uint32_t *__restrict__ label_off,
+                           const Walk *__restrict__ rw, const uint8_t *__restrict__ rsp, uint64_t S,
+                           Walk *__restrict__ w, uint8_t *__restrict__ sp) {
+    const uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= 2 * S) return;
+    Walk x;
+    x.succ = kNil; x.dist = 0; x.term = kNil; x.last = kNil;
+    uint8_t f = 0;
+    if (is11[v >> 1]) {
+        uint32_t r = rid[v], off = 0;
+        if (r == kNil) {
+            r = label_r[v];
+            off = label_off[v];
+        }
+        if (r == kNil) {  // a cycle no ruler reached: never terminates
+            x.succ = (uint32_t)v;
+            f = 1;
+        } else {
+            x = rw[r];
+            x.dist -= off;
+            f = rsp[r];
+            if (x.succ != kNil) x.succ = (uint32_t)v;  // ruler chain without a kept end
+        }
+    }
+    w[v] = x;
+    sp[v] = f;
 }
 
 // kept-edge id of the end a walk arrives at: slot of the kept node K whose own
@@ -463,31 +536,39 @@ __device__ __forceinline__ uint64_t splice(uint64_t a, uint64_t b) {
     return ((la + lb) << 8) | (q & 0xFF);
 }
 
-__global__ void k_tree_eval(TreeArgs t, uint64_t *inL, uint64_t *inR, uint32_t *cnt, uint8_t *emult_out) {
+// Bottom-up evaluation.  The length of a subtree's merged edge is fixed by
+// positions (a node spans the unit edges between its nearest larger
+// neighbours), so only the 8-bit multiplicity travels up.  Each parent has one
+// 64-bit word: left multiplicity (bits 0-7), right (8-15), arrival flags (16,
+// 17).  A child publishes with a single atomicOr and learns from the returned
+// word whether its sibling already arrived -- exactly one of them sees both
+// and goes on with the parent, with no fences between separate locations.
+__global__ void k_tree_eval(TreeArgs t, unsigned long long *word, uint8_t *emult_out) {
     const uint64_t q0 = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (q0 >= t.M) return;
     if (t.need[q0] != 0) return;
     uint64_t q = q0;
-    auto unit = [&](uint64_t pos) -> uint64_t { return (1ull << 8) | t.cmul[pos]; };
-    uint64_t val = splice(unit(q - 1), unit(q));
+    // multiplicity of q's merged edge from its two inputs
+    auto eval = [&](uint64_t p, uint32_t mL, uint32_t mR) -> uint32_t {
+        const uint64_t lenL = p - t.nl[p], lenR = t.nr[p] - p;
+        return (uint32_t)(splice(((uint64_t)lenL << 8) | mL, ((uint64_t)lenR << 8) | mR) & 0xFF);
+    };
+    uint32_t m = eval(q, t.cmul[q - 1], t.cmul[q]);
     for (uint64_t steps = 0; steps <= t.M; steps++) {  // parents have strictly larger priority
         const uint32_t p = t.par[q];
         if (p == kNil) {
-            // root: both ends of the chain get the value
-            const uint8_t m = (uint8_t)(val & 0xFF);
-            emult_out[t.cedge[t.nl[q]]] = m;
-            emult_out[t.cedge[t.nr[q]]] = m;
+            emult_out[t.cedge[t.nl[q]]] = (uint8_t)m;
+            emult_out[t.cedge[t.nr[q]]] = (uint8_t)m;
             return;
         }
-        uint64_t *slot = (q < p) ? &inL[p] : &inR[p];
-        __hip_atomic_store(slot, val, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        const uint32_t old = __hip_atomic_fetch_add(&cnt[p], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-        if (old + 1 < t.need[p]) return;  // the sibling finishes p
-        const uint64_t left = (t.nl[p] == p - 1) ? unit(p - 1)
-                                                 : __hip_atomic_load(&inL[p], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        const uint64_t right = (t.nr[p] == p + 1) ? unit(p)
-                                                  : __hip_atomic_load(&inR[p], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        val = splice(left, right);
+        const int side = (q < p) ? 0 : 1;
+        const unsigned long long mine = ((unsigned long long)m << (8 * side)) | (1ull << (16 + side));
+        const unsigned long long w = atomicOr(&word[p], mine) | mine;
+        const bool needL = t.nl[p] != p - 1, needR = t.nr[p] != p + 1;
+        if ((needL && !(w & (1ull << 16))) || (needR && !(w & (1ull << 17)))) return;  // sibling finishes p
+        const uint32_t mL = needL ? (uint32_t)(w & 0xFF) : t.cmul[p - 1];
+        const uint32_t mR = needR ? (uint32_t)((w >> 8) & 0xFF) : t.cmul[p];
+        m = eval(p, mL, mR);
         q = p;
     }
 }
@@ -594,25 +675,51 @@ int Pipeline::merge_device(bool reference_order, MergeDev &out) {
     hipLaunchKernelGGL(k_classify, dim3(nblk(S)), dim3(256), 0, st_, ma, mb_[M_IS11].as<uint8_t>(),
                        mb_[M_NXT].as<uint32_t>(), mb_[M_EX].as<uint8_t>(), mb_[M_SELF].as<uint8_t>(), cnt);
     phase_end(14);
-    // --- list ranking by pointer jumping over oriented (1,1) nodes
+    // --- list ranking: ruling set, then pointer jumping over the rulers
     MCHECK(mb_[M_WALKA].reserve(2 * S * sizeof(Walk)));
     MCHECK(mb_[M_WALKB].reserve(2 * S * sizeof(Walk)));
     MCHECK(mb_[M_SPA].reserve(2 * S));
     MCHECK(mb_[M_SPB].reserve(2 * S));
+    MCHECK(mb_[M_FLAG].reserve(2 * S * 4));
+    MCHECK(mb_[M_POS].reserve((2 * S + 1) * 8));
+    MCHECK(mb_[M_NL].reserve(2 * S * 4));      // ruler id per oriented node
+    MCHECK(mb_[M_NR].reserve(2 * S * 4));      // label: ruler
+    MCHECK(mb_[M_PAR].reserve(2 * S * 4));     // label: offset
+    MCHECK(scan_scratch_.reserve((scan_scratch_words(2 * S + 1) + 16) * 8));
     Walk *wa = mb_[M_WALKA].as<Walk>(), *wb = mb_[M_WALKB].as<Walk>();
     uint8_t *spa = mb_[M_SPA].as<uint8_t>(), *spb = mb_[M_SPB].as<uint8_t>();
+    uint32_t *rid = mb_[M_NL].as<uint32_t>(), *lab_r = mb_[M_NR].as<uint32_t>(), *lab_o = mb_[M_PAR].as<uint32_t>();
     phase_begin(9);
-    hipLaunchKernelGGL(k_walk_init, dim3(nblk(2 * S)), dim3(256), 0, st_, mb_[M_IS11].as<uint8_t>(),
-                       mb_[M_NXT].as<uint32_t>(), mb_[M_SELF].as<uint8_t>(), S, wa, spa);
-    int rounds = 1;
-    while ((1ull << rounds) < 2 * S + 2) rounds++;
-    rounds++;
-    for (int r = 0; r < rounds; r++) {
-        hipLaunchKernelGGL(k_walk_jump, dim3(nblk(2 * S)), dim3(256), 0, st_, wa, spa, 2 * S, wb, spb,
-                           (unsigned int *)(d_small + 24));
-        std::swap(wa, wb);
-        std::swap(spa, spb);
+    hipLaunchKernelGGL(k_rs_mark, dim3(nblk(2 * S)), dim3(256), 0, st_, mb_[M_IS11].as<uint8_t>(),
+                       mb_[M_NXT].as<uint32_t>(), S, mb_[M_FLAG].as<uint32_t>(), kRuleStride);
+    scan_excl<uint32_t>(mb_[M_FLAG].as<uint32_t>(), mb_[M_POS].as<uint64_t>(), 2 * S, scan_scratch_.as<uint64_t>(),
+                        d_small + 26, st_);
+    MCHECK(hipMemcpyAsync(hs + 26, d_small + 26, 8, hipMemcpyDeviceToHost, st_));
+    MCHECK(hipStreamSynchronize(st_));
+    const uint64_t nr = hs[26];
+    MCHECK(mb_[M_ELEN].reserve(nr * 4 + 4));   // ruler list (reused below for edge lengths)
+    MCHECK(mb_[M_CPRI].reserve((2 * nr + 2) * sizeof(Walk) + 16));
+    MCHECK(mb_[M_CMUL].reserve(2 * nr + 16));
+    uint32_t *rlist = mb_[M_ELEN].as<uint32_t>();
+    Walk *rwa = mb_[M_CPRI].as<Walk>(), *rwb = rwa + nr + 1;
+    uint8_t *rspa = mb_[M_CMUL].as<uint8_t>(), *rspb = rspa + nr + 1;
+    MCHECK(hipMemsetAsync(lab_r, 0xFF, 2 * S * 4, st_));
+    hipLaunchKernelGGL(k_rs_list, dim3(nblk(2 * S)), dim3(256), 0, st_, mb_[M_FLAG].as<uint32_t>(),
+                       mb_[M_POS].as<uint64_t>(), 2 * S, rid, rlist);
+    if (nr) {
+        hipLaunchKernelGGL(k_rs_walk, dim3(nblk(nr)), dim3(256), 0, st_, rlist, nr, mb_[M_IS11].as<uint8_t>(),
+                           mb_[M_NXT].as<uint32_t>(), mb_[M_SELF].as<uint8_t>(), rid, S, lab_r, lab_o, rwa, rspa);
+        int rounds = 1;
+        while ((1ull << rounds) < nr + 2) rounds++;
+        rounds++;
+        for (int r = 0; r < rounds; r++) {
+            hipLaunchKernelGGL(k_rs_jump, dim3(nblk(nr)), dim3(256), 0, st_, rwa, rspa, nr, rwb, rspb);
+            std::swap(rwa, rwb);
+            std::swap(rspa, rspb);
+        }
     }
+    hipLaunchKernelGGL(k_rs_final, dim3(nblk(2 * S)), dim3(256), 0, st_, mb_[M_IS11].as<uint8_t>(), rid, lab_r,
+                       lab_o, rwa, rspa, S, wa, spa);
     phase_end(9);
     // --- kept nodes
     MCHECK(mb_[M_FLAG].reserve(S * 4));
@@ -731,10 +838,8 @@ int Pipeline::merge_device(bool reference_order, MergeDev &out) {
         MCHECK(mb_[M_NEED].reserve(M));
         MCHECK(mb_[M_NL].reserve(M * 4));
         MCHECK(mb_[M_NR].reserve(M * 4));
-        MCHECK(mb_[M_INL].reserve(M * 8));
         MCHECK(mb_[M_INR].reserve(M * 8));
-        MCHECK(mb_[M_CNT].reserve(M * 4));
-        MCHECK(hipMemsetAsync(mb_[M_CNT].p, 0, M * 4, st_));
+        MCHECK(hipMemsetAsync(mb_[M_INR].p, 0, M * 8, st_));
         TreeArgs ta;
         ta.lev = lev;
         ta.lev_off = mb_[M_LEVOFF].as<uint64_t>();
@@ -748,8 +853,8 @@ int Pipeline::merge_device(bool reference_order, MergeDev &out) {
         ta.nr = mb_[M_NR].as<uint32_t>();
         ta.err = (unsigned int *)(d_small + 25);
         hipLaunchKernelGGL(k_tree_links, dim3(nblk(M)), dim3(256), 0, st_, ta);
-        hipLaunchKernelGGL(k_tree_eval, dim3(nblk(M)), dim3(256), 0, st_, ta, mb_[M_INL].as<uint64_t>(),
-                           mb_[M_INR].as<uint64_t>(), mb_[M_CNT].as<uint32_t>(), mb_[M_EMULT].as<uint8_t>());
+        hipLaunchKernelGGL(k_tree_eval, dim3(nblk(M)), dim3(256), 0, st_, ta,
+                           (unsigned long long *)mb_[M_INR].p, mb_[M_EMULT].as<uint8_t>());
     }
     phase_end(12);
     MCHECK(hipGetLastError());
