@@ -13,6 +13,7 @@
 
 #include <algorithm>
 #include <cstdio>
+#include <cstring>
 
 namespace swapk {
 
@@ -666,32 +667,150 @@ __global__ void k_sel_hist(const uint32_t *__restrict__ dfo, const unsigned long
     if (threadIdx.x == 0) atomicMax(maxfo, (unsigned long long)lmax);
 }
 
-__global__ void k_sel_collect(const uint32_t *__restrict__ dfo, const unsigned long long *__restrict__ d_n, int shift,
-                              const int *__restrict__ target, const uint32_t *__restrict__ off,
-                              uint32_t *__restrict__ fill, uint32_t *__restrict__ out) {
+struct SelInfo;
+// Device-side selection bookkeeping (no host round trip): bin and in-bin
+// position of each wanted rank, and the compact list of bins to collect.
+struct SelInfo {
+    int n_starts;
+    int n_targets;
+    int tgt[32];        // target index of each start
+    uint32_t pos[32];   // position of the start's rank inside its bin
+    uint32_t tbin[32];  // bin of each target
+    uint32_t toff[33];  // offset of each target's values
+    uint32_t fill[32];  // append cursors
+};
+
+__global__ void __launch_bounds__(1024) k_sel_targets(const uint32_t *__restrict__ hist, const uint64_t *__restrict__ starts,
+                                                      int n_starts, int *__restrict__ target, SelInfo *__restrict__ si) {
+    __shared__ uint32_t cum[kSelBins + 1];
+    __shared__ uint32_t wsum[16];
+    constexpr int per = kSelBins / 1024;
+    const int t = threadIdx.x;
+    uint32_t loc[per];
+    uint32_t sum = 0;
+#pragma unroll
+    for (int j = 0; j < per; j++) { loc[j] = hist[t * per + j]; sum += loc[j]; }
+    const uint32_t inc = wave_incl_sum(sum);
+    if ((t & 63) == 63) wsum[t >> 6] = inc;
+    __syncthreads();
+    uint32_t off = inc - sum;
+    for (int w = 0; w < (t >> 6); w++) off += wsum[w];
+#pragma unroll
+    for (int j = 0; j < per; j++) { cum[t * per + j] = off; off += loc[j]; }
+    if (t == 1023) cum[kSelBins] = off;
+    for (uint32_t b = t; b < kSelBins; b += 1024) target[b] = -1;
+    __syncthreads();
+    if (t == 0) {
+        int nt = 0;
+        uint32_t tot = 0;
+        for (int j = 0; j < n_starts && j < 32; j++) {
+            const uint64_t T = starts[j];
+            uint32_t lo = 0, hi = kSelBins;  // last bin with cum <= T
+            while (hi - lo > 1) {
+                const uint32_t mid = (lo + hi) / 2;
+                if (cum[mid] <= T) lo = mid; else hi = mid;
+            }
+            int ti = -1;
+            for (int q = 0; q < nt; q++) if (si->tbin[q] == lo) ti = q;
+            if (ti < 0) {
+                ti = nt++;
+                si->tbin[ti] = lo;
+                si->toff[ti] = tot;
+                si->fill[ti] = 0;
+                tot += cum[lo + 1] - cum[lo];
+                target[lo] = ti;
+            }
+            si->tgt[j] = ti;
+            si->pos[j] = (uint32_t)(T - cum[lo]);
+        }
+        si->toff[nt] = tot;
+        si->n_starts = n_starts;
+        si->n_targets = nt;
+    }
+}
+
+__global__ void k_sel_collect_si(const uint32_t *__restrict__ dfo, const unsigned long long *__restrict__ d_n, int shift,
+                                 const int *__restrict__ target, SelInfo *__restrict__ si, uint32_t *__restrict__ out) {
     const uint64_t n = *d_n;
     const int lane = threadIdx.x & 63;
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-    const uint64_t n_up = (n + stride - 1) / stride * stride;  // whole waves iterate together
+    const uint64_t n_up = (n + stride - 1) / stride * stride;
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n_up; i += stride) {
         const uint32_t f = i < n ? dfo[i] : 0;
         int t = i < n ? target[f >> shift] : -1;
-        // one atomic per (wave, target): the first values of a build crowd the
-        // few lowest buckets
         uint64_t pending = __ballot(t >= 0);
         while (pending) {
             const int leader = __ffsll((long long)pending) - 1;
             const int lt = __shfl(t, leader, 64);
             const uint64_t same = __ballot(t == lt);
             uint32_t base = 0;
-            if (lane == leader) base = atomicAdd(&fill[lt], (uint32_t)__popcll(same));
+            if (lane == leader) base = atomicAdd(&si->fill[lt], (uint32_t)__popcll(same));
             base = __shfl(base, leader, 64);
             if (t == lt) {
-                out[off[lt] + base + __popcll(same & ((1ull << lane) - 1))] = f;
+                out[si->toff[lt] + base + __popcll(same & ((1ull << lane) - 1))] = f;
                 t = -1;
             }
             pending &= ~same;
         }
+    }
+}
+
+// Workgroup per target bin: the bin's distinct first occurrences as a bitmap
+// in LDS, then the wanted ranks by prefix popcount.  Writes the first
+// occurrence of each growth rank into the epoch table of the k-mer table.
+__global__ void __launch_bounds__(1024) k_sel_pick(const SelInfo *__restrict__ si, const uint32_t *__restrict__ vals,
+                                                   int shift, uint64_t *__restrict__ epoch_start) {
+    __shared__ uint32_t bm[1u << (18 - 5)];
+    __shared__ uint32_t part[1024];
+    __shared__ uint32_t found;
+    const int tg = blockIdx.x;
+    if (tg >= si->n_targets) return;
+    const uint32_t nw = (1u << shift) >= 32 ? (1u << shift) / 32 : 1;
+    for (uint32_t i = threadIdx.x; i < nw; i += blockDim.x) bm[i] = 0;
+    __syncthreads();
+    const uint32_t base = si->tbin[tg] << shift;
+    for (uint32_t i = si->toff[tg] + threadIdx.x; i < si->toff[tg + 1]; i += blockDim.x) {
+        const uint32_t bit = vals[i] - base;
+        atomicOr(&bm[bit >> 5], 1u << (bit & 31));
+    }
+    __syncthreads();
+    const uint32_t per = (nw + blockDim.x - 1) / blockDim.x;
+    uint32_t c = 0;
+    for (uint32_t j = 0; j < per; j++) {
+        const uint32_t wi = threadIdx.x * per + j;
+        if (wi < nw) c += __popc(bm[wi]);
+    }
+    part[threadIdx.x] = c;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t run = 0;
+        for (int i = 0; i < (int)blockDim.x; i++) { const uint32_t v = part[i]; part[i] = run; run += v; }
+    }
+    __syncthreads();
+    for (int j = 0; j < si->n_starts; j++) {
+        if (si->tgt[j] != tg) continue;
+        const uint32_t want = si->pos[j];  // 0-based rank inside the bin
+        if (threadIdx.x == 0) found = 0xFFFFFFFFu;
+        __syncthreads();
+        const uint32_t lo = part[threadIdx.x];
+        if (want >= lo && want < lo + c) {
+            uint32_t r = want - lo;
+            for (uint32_t q = 0; q < per; q++) {
+                const uint32_t wi = threadIdx.x * per + q;
+                if (wi >= nw) break;
+                uint32_t w = bm[wi];
+                const uint32_t pc = __popc(w);
+                if (r < pc) {
+                    for (uint32_t z = 0; z < r; z++) w &= w - 1;  // drop the r lowest set bits
+                    found = wi * 32 + (__ffs(w) - 1);
+                    break;
+                }
+                r -= pc;
+            }
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) epoch_start[j] = (uint64_t)base + found;
+        __syncthreads();
     }
 }
 
@@ -724,6 +843,9 @@ Pipeline::Pipeline(int device, int k, int c, bool reference_order) : dev_(device
     hipSetDevice(dev_);
     hipStreamCreateWithFlags(&st_, hipStreamNonBlocking);
     hipHostMalloc((void **)&h_small_, 1024 * sizeof(uint64_t), hipHostMallocDefault);
+    hipHostMalloc((void **)&h_starts_, 64 * sizeof(uint64_t), hipHostMallocDefault);
+    hipHostMalloc((void **)&h_etab_, sizeof(EpochTabHost), hipHostMallocDefault);
+    hipHostMalloc((void **)&h_etabS_, sizeof(EpochTabHost), hipHostMallocDefault);
     mx_.bits = 2 * k;
     mx_.m = kmask(k);
     mx_.a1 = 0x9E3779B97F4A7C15ull | 1;
@@ -751,6 +873,9 @@ Pipeline::~Pipeline() {
     hipSetDevice(dev_);
     for (auto &e : ev_) if (e) hipEventDestroy(e);
     if (h_small_) hipHostFree(h_small_);
+    if (h_starts_) hipHostFree(h_starts_);
+    if (h_etab_) hipHostFree(h_etab_);
+    if (h_etabS_) hipHostFree(h_etabS_);
     if (st_) hipStreamDestroy(st_);
 }
 
@@ -977,58 +1102,42 @@ int Pipeline::build(const uint8_t *d_in, uint64_t n) {
     n_surv = hs[8];
     n_distinct = hs[9];
     if (ref_order_ && n_distinct) {
-        // growth ranks of the reference's k-mer table
+        // growth ranks of the reference's k-mer table: the first occurrence of
+        // the key of each growth rank, selected on the device
         std::vector<uint64_t> starts = table_epoch_starts(n_distinct, false);
+        if (starts.size() > 32) starts.resize(32);
         int shift = 0;
         while ((N >> shift) >= kSelBins) shift++;
-        SWAP_CHECK(sel_.reserve((2 * kSelBins + 64) * 4));
+        SWAP_CHECK(sel_.reserve((kSelBins + 64) * 4 + 32 * 8 + sizeof(SelInfo) + sizeof(EpochTabHost) + 256));
         uint32_t *d_hist = sel_.as<uint32_t>();
+        uint64_t *d_starts = reinterpret_cast<uint64_t *>(d_hist + kSelBins + 64);
+        SelInfo *d_si = reinterpret_cast<SelInfo *>(d_starts + 32);
+        std::memcpy(h_starts_, starts.data(), starts.size() * 8);
+        SWAP_CHECK(hipMemcpyAsync(d_starts, h_starts_, starts.size() * 8, hipMemcpyHostToDevice, st_));
         SWAP_CHECK(hipMemsetAsync(d_hist, 0, kSelBins * 4, st_));
         SWAP_CHECK(hipMemsetAsync(d_small + 12, 0, 8, st_));
+        SWAP_CHECK(sel_target_.reserve(kSelBins * 4 + 64));
+        SWAP_CHECK(sel_out_.reserve(N * 4 + 16));
+        // the k-mer table's epoch table: bucket counts from D, starts filled below
+        epoch1_ = epoch_table(n_distinct, false, nullptr);
+        SWAP_CHECK(etab_.reserve(sizeof(EpochTabHost) + 64));
+        std::memcpy(h_etab_, &epoch1_, sizeof(EpochTabHost));
+        SWAP_CHECK(hipMemcpyAsync(etab_.p, h_etab_, sizeof(EpochTabHost), hipMemcpyHostToDevice, st_));
+        EpochTabHost *d_et = etab_.as<EpochTabHost>();
         phase_begin(6);
         hipLaunchKernelGGL(k_sel_hist, dim3(ncu), dim3(1024), 0, st_, dfo_.as<uint32_t>(),
                            (const unsigned long long *)(d_small + 9), shift, d_hist,
                            (unsigned long long *)(d_small + 12));
-        std::vector<uint32_t> hist(kSelBins);
-        SWAP_CHECK(hipMemcpyAsync(hist.data(), d_hist, kSelBins * 4, hipMemcpyDeviceToHost, st_));
-        SWAP_CHECK(hipMemcpyAsync(hs + 12, d_small + 12, 8, hipMemcpyDeviceToHost, st_));
-        SWAP_CHECK(hipStreamSynchronize(st_));
-        last_first_ = hs[12];
-        // bucket and in-bucket position of each wanted rank (starts[0] = 0 is the first insert)
-        std::vector<uint64_t> cum(kSelBins + 1, 0);
-        for (uint32_t i = 0; i < kSelBins; i++) cum[i + 1] = cum[i] + hist[i];
-        std::vector<int> target(kSelBins, -1);
-        std::vector<uint32_t> toff, tbin;
-        std::vector<std::pair<int, uint64_t>> want;  // (target, position in bucket) per start
-        for (uint64_t T : starts) {
-            const uint32_t b = (uint32_t)(std::upper_bound(cum.begin(), cum.end(), T) - cum.begin()) - 1;
-            if (target[b] < 0) {
-                target[b] = (int)tbin.size();
-                tbin.push_back(b);
-            }
-            want.push_back({target[b], T - cum[b]});
-        }
-        uint32_t tot = 0;
-        for (uint32_t b : tbin) { toff.push_back(tot); tot += hist[b]; }
-        SWAP_CHECK(sel_target_.reserve(kSelBins * 4 + toff.size() * 8 + 64));
-        int *d_target = sel_target_.as<int>();
-        uint32_t *d_off = reinterpret_cast<uint32_t *>(d_target + kSelBins);
-        uint32_t *d_fill = d_off + toff.size() + 1;
-        SWAP_CHECK(sel_out_.reserve((uint64_t)tot * 4 + 16));
-        SWAP_CHECK(hipMemcpyAsync(d_target, target.data(), kSelBins * 4, hipMemcpyHostToDevice, st_));
-        SWAP_CHECK(hipMemcpyAsync(d_off, toff.data(), toff.size() * 4, hipMemcpyHostToDevice, st_));
-        SWAP_CHECK(hipMemsetAsync(d_fill, 0, toff.size() * 4, st_));
-        hipLaunchKernelGGL(k_sel_collect, dim3(ncu * 2), dim3(1024), 0, st_, dfo_.as<uint32_t>(),
-                           (const unsigned long long *)(d_small + 9), shift, d_target, d_off, d_fill,
+        hipLaunchKernelGGL(k_sel_targets, dim3(1), dim3(1024), 0, st_, d_hist, d_starts, (int)starts.size(),
+                           sel_target_.as<int>(), d_si);
+        hipLaunchKernelGGL(k_sel_collect_si, dim3(ncu * 2), dim3(1024), 0, st_, dfo_.as<uint32_t>(),
+                           (const unsigned long long *)(d_small + 9), shift, sel_target_.as<int>(), d_si,
                            sel_out_.as<uint32_t>());
+        hipLaunchKernelGGL(k_sel_pick, dim3(32), dim3(1024), 0, st_, d_si, sel_out_.as<uint32_t>(), shift,
+                           &d_et->start[0]);
         phase_end(6);
-        std::vector<uint32_t> vals(tot);
-        SWAP_CHECK(hipMemcpyAsync(vals.data(), sel_out_.p, (uint64_t)tot * 4, hipMemcpyDeviceToHost, st_));
-        SWAP_CHECK(hipStreamSynchronize(st_));
-        for (size_t t = 0; t < tbin.size(); t++)
-            std::sort(vals.begin() + toff[t], vals.begin() + toff[t] + hist[tbin[t]]);
         epoch_first_.clear();
-        for (auto &wv : want) epoch_first_.push_back(vals[toff[wv.first] + wv.second]);
+        epoch_first_valid_ = false;
     }
     collect_phases();
     return 0;
@@ -1039,6 +1148,15 @@ int Pipeline::export_kmer_graph(HostKmerGraph &out) {
     out.key.resize(n_surv);
     out.mult.resize(n_surv);
     out.first.clear();
+    if (ref_order_ && n_distinct && !epoch_first_valid_) {
+        EpochTabHost et;
+        SWAP_CHECK(hipMemcpyAsync(&et, etab_.p, sizeof et, hipMemcpyDeviceToHost, st_));
+        SWAP_CHECK(hipMemcpyAsync(hs_tail(), d_small_ptr() + 12, 8, hipMemcpyDeviceToHost, st_));
+        SWAP_CHECK(hipStreamSynchronize(st_));
+        epoch_first_.assign(et.start, et.start + et.n);
+        last_first_ = *hs_tail();
+        epoch_first_valid_ = true;
+    }
     out.epoch_first = epoch_first_;
     out.distinct = n_distinct;
     // the table sees one more operator[] after the last new key when a later

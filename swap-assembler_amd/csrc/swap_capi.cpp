@@ -129,11 +129,12 @@ int swap_merge(swap_ctx *c) {
     if (!c->built) return set_err(c, SWAP_E_STATE, "merge before build");
     if (c->cfg.flags & SWAP_FLAG_KMER_GRAPH) return set_err(c, SWAP_E_STATE, "context was created for -s (no merge)");
     auto t0 = clk::now();
-    int rc = swapk::host_merge(*c->pipe, c->cfg, c->visit, c->comp);
+    bool mat = false;
+    int rc = swapk::host_merge(*c->pipe, c->cfg, &mat, c->comp);
     c->merge_ms = std::chrono::duration<double, std::milli>(clk::now() - t0).count();
     if (rc) return set_err(c, rc, c->pipe->last_error());
     c->merged = true;
-    c->have_compacted = true;
+    c->have_compacted = mat;
     return SWAP_OK;
 }
 
@@ -177,6 +178,11 @@ int64_t swap_format_file(swap_ctx *c, int which, char *buf, size_t cap) {
         swapk::format_kmer_graph(c->cfg.k, c->graph, order, out);
     } else if (which == SWAP_FILE_CONTIG_GRAPH || which == SWAP_FILE_NOCE) {
         if (!c->merged) return set_err(c, SWAP_E_STATE, "no merge");
+        if (!c->have_compacted) {
+            rc = swapk::materialize_merge(*c->pipe, c->cfg, c->comp);
+            if (rc) return set_err(c, rc, c->pipe->last_error());
+            c->have_compacted = true;
+        }
         if (which == SWAP_FILE_CONTIG_GRAPH) swapk::format_contig_graph(c->comp, out);
         else swapk::format_noce(c->cfg.k, c->comp, out);
     } else {

@@ -87,8 +87,9 @@ public:
     // survivor order (device or host array, may be null for the built-in
     // hash order).
     int merge_device(bool reference_order, MergeDev &out);
+    int fetch_merge(MergeDev &out);
     int visit_priorities(uint64_t *d_prio);
-    int table_order_device(const uint64_t *d_rank, const EpochTabHost &tab, uint32_t *d_order);
+    int table_order_device(const uint64_t *d_rank, const EpochTabHost &tab, const void *d_et, uint32_t *d_order);
 
     // Counters of the last build / merge.
     uint64_t n_reads = 0, n_occ = 0, n_distinct = 0, n_surv = 0, n_kept = 0, n_overflow = 0;
@@ -140,6 +141,14 @@ private:
     // host mirrors of small results
     uint64_t *h_small_ = nullptr;
     uint64_t last_first_ = 0;
+    uint64_t merged_ne_ = 0, merged_pool_ = 0, merged_M_ = 0;
+    bool epoch_first_valid_ = false;
+    EpochTabHost epoch1_;           // growth epochs of the k-mer table (starts on the device)
+    DevBuf etab_, etabS_;           // device epoch tables: k-mer table, S-sized tables
+    uint64_t *h_starts_ = nullptr;  // pinned staging
+    EpochTabHost *h_etab_ = nullptr, *h_etabS_ = nullptr;
+    uint64_t *hs_tail() { return h_small_ + 40; }
+    uint64_t *d_small_ptr() { return small_.as<uint64_t>(); }
     std::vector<uint64_t> epoch_first_;
     hipEvent_t ev_[2 * kPhases] = {};
     bool ev_used_[kPhases] = {};

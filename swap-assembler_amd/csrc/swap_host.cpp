@@ -377,12 +377,26 @@ inline unsigned char splice_master(unsigned char ma, size_t la, unsigned char mb
 
 }  // namespace
 
-int host_merge(Pipeline &p, const swap_cfg &cfg, std::vector<uint32_t> &visit, HostCompacted &out) {
+int host_merge(Pipeline &p, const swap_cfg &cfg, bool *materialized, HostCompacted &out) {
+    const bool ref = !(cfg.flags & SWAP_FLAG_HASH_ORDER);
+    MergeDev md;
+    int rc = p.merge_device(ref, md);
+    if (rc) return rc;
+    *materialized = false;
+    if (p.n_special == 0) return 0;  // results stay on the device until exported
+    // cycles / fold-backs are replayed on the host now, as part of the merge
+    rc = materialize_merge(p, cfg, out);
+    if (rc) return rc;
+    *materialized = true;
+    return 0;
+}
+
+int materialize_merge(Pipeline &p, const swap_cfg &cfg, HostCompacted &out) {
     const bool ref = !(cfg.flags & SWAP_FLAG_HASH_ORDER);
     const int k = cfg.k;
     out = HostCompacted();
     MergeDev md;
-    int rc = p.merge_device(ref, md);
+    int rc = p.fetch_merge(md);
     if (rc) return rc;
     const size_t S = p.n_surv;
     const uint64_t P3 = ref ? table_buckets(S, false) : 0;
@@ -617,7 +631,6 @@ int host_merge(Pipeline &p, const swap_cfg &cfg, std::vector<uint32_t> &visit, H
         }
     }
     p.n_kept = live.size();
-    visit.clear();
     return 0;
 }
 

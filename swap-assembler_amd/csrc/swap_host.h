@@ -49,6 +49,10 @@ void format_kmer_graph(int k, const HostKmerGraph &g, const std::vector<uint32_t
 void format_contig_graph(const HostCompacted &c, std::string &out);
 void format_noce(int k, const HostCompacted &c, std::string &out);
 
-int host_merge(Pipeline &p, const swap_cfg &cfg, std::vector<uint32_t> &visit, HostCompacted &out);
+// Runs the merge (device kernels; cycles and fold-backs replayed on the host).
+// When nothing needs the host replay the results stay on the device and
+// *materialized is false; materialize_merge() copies and assembles them.
+int host_merge(Pipeline &p, const swap_cfg &cfg, bool *materialized, HostCompacted &out);
+int materialize_merge(Pipeline &p, const swap_cfg &cfg, HostCompacted &out);
 
 }  // namespace swapk

@@ -858,7 +858,24 @@ int Pipeline::merge_device(bool reference_order, MergeDev &out) {
     }
     phase_end(12);
     MCHECK(hipGetLastError());
-    // --- results to the host
+    MCHECK(hipMemcpyAsync(hs + 25, d_small + 25, 8, hipMemcpyDeviceToHost, st_));
+    MCHECK(hipStreamSynchronize(st_));
+    if ((uint32_t)hs[25]) return fail(-5, "merge: a chain array lost its end marker");
+    collect_phases();
+    n_kept = nk;
+    n_special = nspec;
+    merged_ne_ = ne;
+    merged_pool_ = pool_n;
+    merged_M_ = M;
+    return 0;
+}
+
+// Copies the merge results of the last merge_device to the host.
+int Pipeline::fetch_merge(MergeDev &out) {
+    MCHECK(hipSetDevice(dev_));
+    out = MergeDev();
+    const uint64_t S = n_surv, nk = n_kept, ne = merged_ne_, pool_n = merged_pool_, nspec = n_special;
+    out.n_kept = nk;
     out.kept_src.resize(nk);
     out.elen.resize(ne);
     out.eoff.resize(ne);
@@ -874,16 +891,16 @@ int Pipeline::merge_device(bool reference_order, MergeDev &out) {
     }
     if (pool_n) MCHECK(hipMemcpyAsync(out.pool.data(), mb_[M_POOL].p, pool_n, hipMemcpyDeviceToHost, st_));
     out.kept_prio.resize(nk);
+    out.kept_key.resize(nk);
     if (nk) {
-        MCHECK(mb_[M_CPRI].reserve(std::max<uint64_t>(M, nk) * 8 + 16));
+        MCHECK(mb_[M_INL].reserve(2 * nk * 8 + 16));
+        uint64_t *tmp = mb_[M_INL].as<uint64_t>();
         hipLaunchKernelGGL(k_gather_u64, dim3(nblk(nk)), dim3(256), 0, st_, mb_[M_PRIO].as<uint64_t>(),
-                           mb_[M_KEPTSRC].as<uint32_t>(), nk, mb_[M_CPRI].as<uint64_t>());
-        MCHECK(hipMemcpyAsync(out.kept_prio.data(), mb_[M_CPRI].p, nk * 8, hipMemcpyDeviceToHost, st_));
-        MCHECK(mb_[M_INL].reserve(nk * 8 + 16));
+                           mb_[M_KEPTSRC].as<uint32_t>(), nk, tmp);
         hipLaunchKernelGGL(k_gather_u64, dim3(nblk(nk)), dim3(256), 0, st_, surv_key_.as<uint64_t>(),
-                           mb_[M_KEPTSRC].as<uint32_t>(), nk, mb_[M_INL].as<uint64_t>());
-        out.kept_key.resize(nk);
-        MCHECK(hipMemcpyAsync(out.kept_key.data(), mb_[M_INL].p, nk * 8, hipMemcpyDeviceToHost, st_));
+                           mb_[M_KEPTSRC].as<uint32_t>(), nk, tmp + nk);
+        MCHECK(hipMemcpyAsync(out.kept_prio.data(), tmp, nk * 8, hipMemcpyDeviceToHost, st_));
+        MCHECK(hipMemcpyAsync(out.kept_key.data(), tmp + nk, nk * 8, hipMemcpyDeviceToHost, st_));
     }
     if (nspec) {
         out.special.resize(nspec);
@@ -895,12 +912,7 @@ int Pipeline::merge_device(bool reference_order, MergeDev &out) {
         MCHECK(hipMemcpyAsync(out.all_mult.data(), surv_mult_.p, S * 8, hipMemcpyDeviceToHost, st_));
         MCHECK(hipMemcpyAsync(out.all_prio.data(), mb_[M_PRIO].p, S * 8, hipMemcpyDeviceToHost, st_));
     }
-    MCHECK(hipMemcpyAsync(hs + 25, d_small + 25, 8, hipMemcpyDeviceToHost, st_));
     MCHECK(hipStreamSynchronize(st_));
-    if ((uint32_t)hs[25]) return fail(-5, "merge: a chain array lost its end marker");
-    collect_phases();
-    n_kept = nk;
-    n_special = nspec;
     return 0;
 }
 

@@ -28,10 +28,11 @@ __device__ __forceinline__ uint64_t dev_ref_hash(uint64_t x) {
 }
 
 __global__ void k_ord_prep(const uint64_t *__restrict__ key, const uint64_t *__restrict__ rank, uint64_t n,
-                           EpochTab et, uint64_t *__restrict__ h, uint8_t *__restrict__ ins,
+                           const EpochTab *__restrict__ etp, uint64_t *__restrict__ h, uint8_t *__restrict__ ins,
                            uint32_t *__restrict__ bucket, uint32_t *__restrict__ idx) {
     const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
+    const EpochTab &et = *etp;
     const uint64_t hv = dev_ref_hash(key[i]);
     const uint64_t r = rank[i];
     int e = 0;
@@ -60,11 +61,12 @@ __device__ __forceinline__ bool ord_before(uint32_t a, uint32_t b, int e, const 
 }
 
 // One thread per bucket run of the sorted array: insertion sort of its keys.
-__global__ void k_ord_ties(const uint32_t *__restrict__ bucket, uint32_t *__restrict__ idx, uint64_t n, EpochTab et,
-                           const uint64_t *__restrict__ h, const uint8_t *__restrict__ ins,
-                           const uint64_t *__restrict__ rank) {
+__global__ void k_ord_ties(const uint32_t *__restrict__ bucket, uint32_t *__restrict__ idx, uint64_t n,
+                           const EpochTab *__restrict__ etp, const uint64_t *__restrict__ h,
+                           const uint8_t *__restrict__ ins, const uint64_t *__restrict__ rank) {
     const uint64_t q = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= n) return;
+    const EpochTab &et = *etp;
     const uint32_t b = bucket[q];
     if (q > 0 && bucket[q - 1] == b) return;
     uint64_t e = q + 1;
@@ -100,13 +102,12 @@ __global__ void k_u32_to_u64_o(const uint32_t *__restrict__ in, uint64_t n, uint
 
 static inline unsigned oblk(uint64_t n) { return (unsigned)((n + 255) / 256); }
 
-// Iteration order of one table: sorted survivor indices into d_order.
-int Pipeline::table_order_device(const uint64_t *d_rank, const EpochTabHost &tab, uint32_t *d_order) {
+// Iteration order of one table: sorted survivor indices into d_order.  The
+// epoch table lives in device memory (d_et); tab is its host copy (the bucket
+// counts are needed for the sort width; the starts may differ).
+int Pipeline::table_order_device(const uint64_t *d_rank, const EpochTabHost &tab, const void *d_et, uint32_t *d_order) {
     const uint64_t S = n_surv;
-    EpochTab et;
-    memset(&et, 0, sizeof et);
-    et.n = tab.n;
-    for (int e = 0; e < tab.n; e++) { et.start[e] = tab.start[e]; et.P[e] = tab.P[e]; }
+    const EpochTab *et = static_cast<const EpochTab *>(d_et);
     OCHECK(ob_[O_H].reserve(S * 8));
     OCHECK(ob_[O_INS].reserve(S));
     OCHECK(ob_[O_BKT].reserve(S * 4));
@@ -116,7 +117,7 @@ int Pipeline::table_order_device(const uint64_t *d_rank, const EpochTabHost &tab
                        ob_[O_H].as<uint64_t>(), ob_[O_INS].as<uint8_t>(), ob_[O_BKT].as<uint32_t>(),
                        ob_[O_IDX].as<uint32_t>());
     int bits = 1;
-    while (bits < 32 && (1ull << bits) < et.P[et.n - 1]) bits++;
+    while (bits < 32 && (1ull << bits) < tab.P[tab.n - 1]) bits++;
     size_t tb = 0;
     OCHECK(rocprim::radix_sort_pairs(nullptr, tb, ob_[O_BKT].as<uint32_t>(), ob_[O_BKT2].as<uint32_t>(),
                                      ob_[O_IDX].as<uint32_t>(), d_order, (size_t)S, 0, bits, st_));
@@ -149,17 +150,18 @@ int Pipeline::visit_priorities(uint64_t *d_prio) {
     OCHECK(ob_[O_ORD].reserve(S * 4));
     uint64_t *rank = ob_[O_RANK].as<uint64_t>();
     uint32_t *ord = ob_[O_ORD].as<uint32_t>();
-    // kmolecules: ranks are first occurrences, epochs start at the first
-    // occurrence of the key of each growth rank
+    // kmolecules: ranks are first occurrences; the epoch table's starts are the
+    // first occurrences of the growth ranks, selected on the device by the build
     hipLaunchKernelGGL(k_u32_to_u64_o, dim3(oblk(S)), dim3(256), 0, st_, surv_first_.as<uint32_t>(), S, rank);
-    const bool tail = n_occ > last_first_ + 1;
-    int rc = table_order_device(rank, epoch_table(n_distinct, tail, &epoch_first_), ord);
+    int rc = table_order_device(rank, epoch1_, etab_.p, ord);
     if (rc) return rc;
     // cutoff copy and nodes table: ranks are positions in the previous order
-    const EpochTabHost tabS = epoch_table(S, false, nullptr);
+    *h_etabS_ = epoch_table(S, false, nullptr);
+    OCHECK(etabS_.reserve(sizeof(EpochTabHost) + 64));
+    OCHECK(hipMemcpyAsync(etabS_.p, h_etabS_, sizeof(EpochTabHost), hipMemcpyHostToDevice, st_));
     for (int t = 0; t < 2; t++) {
         hipLaunchKernelGGL(k_ord_positions, dim3(oblk(S)), dim3(256), 0, st_, ord, S, rank);
-        rc = table_order_device(rank, tabS, ord);
+        rc = table_order_device(rank, *h_etabS_, etabS_.p, ord);
         if (rc) return rc;
     }
     hipLaunchKernelGGL(k_ord_positions, dim3(oblk(S)), dim3(256), 0, st_, ord, S, d_prio);
