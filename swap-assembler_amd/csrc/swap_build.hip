@@ -161,7 +161,16 @@ __global__ void k_locate(const uint64_t *__restrict__ nl_pos, const uint64_t *__
 }
 
 // ---------------------------------------------------------------------------
-// 2-bit packing: one thread per 32-base word, first base in the top bits.
+// 2-bit packing: one thread per 32-base word, first base in the top bits.  Each
+// workgroup takes 256 consecutive words; the record of each word is found by a
+// binary search in an LDS copy of the word-start table around the workgroup's
+// first word.  Next to the packed word goes its occurrence descriptor:
+//   bits 0-31  input index of the first k-mer starting in the word
+//   bits 32-37 number of k-mers starting in the word (0..32)
+//   bit 40     the word is not the first of its read (a predecessor base exists)
+//   bit 41     the word holds its read's last k-mer (which has no successor)
+
+constexpr int kPackThreads = 256;
 
 __device__ __forceinline__ uint64_t upper_index(const uint64_t *__restrict__ a, uint64_t n, uint64_t v) {
     // last i with a[i] <= v (a non-decreasing, a[0] = 0 <= v)
@@ -173,19 +182,52 @@ __device__ __forceinline__ uint64_t upper_index(const uint64_t *__restrict__ a, 
     return lo;
 }
 
-__global__ void k_pack(const uint8_t *__restrict__ buf, const uint64_t *__restrict__ rec_start,
-                       const uint32_t *__restrict__ rec_len, const uint64_t *__restrict__ word_base, uint64_t R,
-                       const uint64_t *__restrict__ d_words, uint64_t *__restrict__ packed) {
+__global__ void __launch_bounds__(kPackThreads) k_pack(const uint8_t *__restrict__ buf,
+                                                       const uint64_t *__restrict__ rec_start,
+                                                       const uint32_t *__restrict__ rec_len,
+                                                       const uint64_t *__restrict__ word_base,
+                                                       const uint64_t *__restrict__ occ_base, uint64_t R, int k,
+                                                       const uint64_t *__restrict__ d_words,
+                                                       uint64_t *__restrict__ packed, uint64_t *__restrict__ wmeta) {
+    __shared__ uint64_t swb[kPackThreads + 1];
+    __shared__ uint64_t sr0;
     const uint64_t W = *d_words;
-    for (uint64_t w = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; w < W; w += (uint64_t)gridDim.x * blockDim.x) {
-        const uint64_t r = upper_index(word_base, R, w);
-        const uint64_t j = w - word_base[r];
-        const uint32_t len = rec_len[r];
-        const uint8_t *s = buf + rec_start[r] + 32 * j;
-        const int nb = (int)min<uint64_t>(32, len - 32 * j);
-        uint64_t word = 0;
-        for (int i = 0; i < nb; i++) word |= (uint64_t)base_code(s[i]) << (62 - 2 * i);
-        packed[w] = word;
+    for (uint64_t w0 = (uint64_t)blockIdx.x * kPackThreads; w0 < W; w0 += (uint64_t)gridDim.x * kPackThreads) {
+        if (threadIdx.x == 0) sr0 = upper_index(word_base, R, w0);
+        __syncthreads();
+        const uint64_t r0 = sr0;
+        const uint32_t nwin = (uint32_t)min<uint64_t>(kPackThreads + 1, R - r0);
+        for (uint32_t i = threadIdx.x; i < nwin; i += blockDim.x) swb[i] = word_base[r0 + i];
+        __syncthreads();
+        const uint64_t w = w0 + threadIdx.x;
+        if (w < W) {
+            uint64_t r;
+            if (swb[nwin - 1] <= w) {
+                // many records without k-mers between: search past the window
+                const uint64_t s = r0 + nwin - 1;
+                r = s + upper_index(word_base + s, R - s, w);
+            } else {
+                uint32_t lo = 0, hi = nwin;
+                while (hi - lo > 1) {
+                    const uint32_t mid = (lo + hi) / 2;
+                    if (swb[mid] <= w) lo = mid; else hi = mid;
+                }
+                r = r0 + lo;
+            }
+            const uint64_t j = w - word_base[r];
+            const uint32_t len = rec_len[r];
+            const uint8_t *s = buf + rec_start[r] + 32 * j;
+            const int nb = (int)min<uint64_t>(32, len - 32 * j);
+            uint64_t word = 0;
+            for (int i = 0; i < nb; i++) word |= (uint64_t)base_code(s[i]) << (62 - 2 * i);
+            packed[w] = word;
+            const uint32_t nk = len - k + 1;
+            const uint32_t p0 = (uint32_t)(32 * j);
+            const uint32_t nh = p0 < nk ? min<uint32_t>(nk - p0, 32) : 0;
+            wmeta[w] = (uint64_t)(uint32_t)(occ_base[r] + p0) | ((uint64_t)nh << 32) | ((uint64_t)(j > 0) << 40) |
+                       ((uint64_t)(p0 + nh == nk) << 41);
+        }
+        __syncthreads();
     }
 }
 
@@ -198,77 +240,74 @@ __global__ void k_pack(const uint8_t *__restrict__ buf, const uint64_t *__restri
 
 struct OccCtx {
     const uint64_t *packed;
-    const uint64_t *word_base;
-    const uint64_t *occ_base;
-    const uint32_t *rec_len;
-    uint64_t R;
+    const uint64_t *wmeta;
+    uint64_t W;
     int k;
     Mixer mx;
 };
 
 // Calls f(h, mask, occ) for each occurrence starting in word w: h is the mixed
 // canonical k-mer (2k bits), mask its 8 edge bits, occ its input k-mer index.
+// Every base read lies inside the word's read: a k-mer's bases and its
+// successor base (when the read has one) end before the read does.
 template <typename F>
 __device__ __forceinline__ void for_each_occ(const OccCtx &c, uint64_t w, F &&f) {
-    const uint64_t r = upper_index(c.word_base, c.R, w);
-    const uint64_t j = w - c.word_base[r];
-    const uint32_t len = c.rec_len[r];
+    const uint64_t meta = c.wmeta[w];
+    const uint32_t nh = (uint32_t)(meta >> 32) & 63;
+    if (nh == 0) return;
+    const uint32_t occ0 = (uint32_t)meta;
+    const bool has_prev = (meta >> 40) & 1;
+    const bool ends = (meta >> 41) & 1;
     const int k = c.k;
-    const uint32_t nk = len - k + 1;
-    const uint32_t p0 = (uint32_t)(32 * j);
-    if (p0 >= nk) return;
-    const uint32_t pend = min<uint32_t>(nk, p0 + 32);
-    const uint64_t wlast = c.word_base[r] + (len + 31) / 32;  // one past the read's last word
     const uint64_t w0 = c.packed[w];
-    const uint64_t w1 = (w + 1 < wlast) ? c.packed[w + 1] : 0;
-    const uint32_t prev_base = (j > 0) ? (uint32_t)(c.packed[w - 1] & 3) : 0;
+    const uint64_t w1 = (w + 1 < c.W) ? c.packed[w + 1] : 0;
+    const uint32_t prev_base = has_prev ? (uint32_t)(c.packed[w - 1] & 3) : 0;
     const uint64_t km = kmask(k);
-    // base at offset i (0..63) from p0
+    // base at offset i (0..63) from the word start
     auto base_at = [&](int i) -> uint32_t {
         return (uint32_t)((i < 32 ? (w0 >> (62 - 2 * i)) : (w1 >> (62 - 2 * (i - 32)))) & 3);
     };
-    // first k-mer: bases [0, k)
-    uint64_t x;
-    {
-        // top 2k bits of the 128-bit w0:w1
-        x = (w0 >> (64 - 2 * k));
-    }
+    uint64_t x = w0 >> (64 - 2 * k);  // bases [0, k)
     uint64_t rcx = revcomp(x, k);
-    const uint64_t obase = c.occ_base[r];
-    for (uint32_t p = p0; p < pend; p++) {
-        const int i = (int)(p - p0);
+    for (uint32_t i = 0; i < nh; i++) {
         if (i > 0) {
-            const uint32_t b = base_at(i + k - 1);
+            const uint32_t b = base_at((int)i + k - 1);
             x = ((x << 2) | b) & km;
             rcx = (rcx >> 2) | ((uint64_t)(3 - b) << (2 * k - 2));
         }
         const bool fwd = x > rcx;
         const uint64_t X = fwd ? x : rcx;
         uint32_t m = 0;
-        if (p + 1 < nk) {
-            const uint32_t b = base_at(i + k);
+        if (!(ends && i + 1 == nh)) {
+            const uint32_t b = base_at((int)i + k);
             m |= 1u << (fwd ? b : 4 + b);
         }
-        if (p > 0) {
-            const uint32_t a = 3 - (i > 0 ? base_at(i - 1) : prev_base);
+        if (i > 0 || has_prev) {
+            const uint32_t a = 3 - (i > 0 ? base_at((int)i - 1) : prev_base);
             m |= 1u << (fwd ? 4 + a : a);
         }
-        f(mix_fwd(c.mx, X), m, obase + p);
+        f(mix_fwd(c.mx, X), m, (uint64_t)(occ0 + i));
     }
 }
 
 // Two-level radix partitioning of the occurrence records by the top bits of the
-// mixed key: 2^kL1Bits buckets first, then 2^kL2Bits sub-buckets of each, so
-// that every write goes out as a run of records staged and sorted in LDS.
+// mixed key: 2^kL1Bits buckets first, then 2^kL2Bits sub-buckets of each.
 // Level-1 records keep the key bits below the level-1 bits:
 //   rec = (h mod 2^(2k-8)) << 8 | mask        (2k <= 62, so 2k bits in all)
-// and the level-2 bucket is read back from the record.
+// and the level-2 bucket is read back from the record.  Both levels write each
+// record straight to its place: a workgroup owns a contiguous run of every
+// bucket and hands out its slots with LDS counters, so the open end of each
+// run (one partial cache line per bucket per workgroup) stays in L2 while it
+// fills.  Record order inside a partition is not used by the counting pass.
 constexpr int kL1Bits = 8;
 constexpr uint32_t kL1 = 1u << kL1Bits;
 constexpr int kL2Bits = kPartBits - kL1Bits;
 constexpr uint32_t kL2 = 1u << kL2Bits;
-constexpr int kOccThreads = 256;
-constexpr int kChunkRecs = kOccThreads * 32;  // a word starts at most 32 k-mers
+constexpr int kOccThreads = 1024;
+constexpr int kHistRowGroups = 8;     // row groups of the column sum
+constexpr int kChunkRecs = 8192;      // level-2 records per workgroup
+constexpr int kL2Threads = 1024;
+constexpr int kL2Per = kChunkRecs / kL2Threads;
 
 // Per workgroup word range (same split in the histogram and level-1 passes).
 __device__ __forceinline__ void word_range(uint64_t W, uint64_t &wb, uint64_t &we) {
@@ -278,20 +317,21 @@ __device__ __forceinline__ void word_range(uint64_t W, uint64_t &wb, uint64_t &w
     if (wb > W) wb = W;
 }
 
-// Final partition sizes (global) and per-workgroup level-1 counts.
-__global__ void __launch_bounds__(kOccThreads) k_part_hist(OccCtx c, const uint64_t *__restrict__ d_words,
-                                                            uint32_t *__restrict__ hist, uint32_t *__restrict__ hist1) {
+// Final partition sizes of the workgroup (one row per workgroup, summed by
+// k_col_sum) and its level-1 counts.
+__global__ void __launch_bounds__(kOccThreads) k_part_hist(OccCtx c, uint32_t *__restrict__ rows,
+                                                            uint32_t *__restrict__ hist1) {
     __shared__ uint32_t lh[kParts];
     for (uint32_t i = threadIdx.x; i < kParts; i += blockDim.x) lh[i] = 0;
     __syncthreads();
     uint64_t wb, we;
-    word_range(*d_words, wb, we);
+    word_range(c.W, wb, we);
     const int sh = 2 * c.k - kPartBits;
     for (uint64_t w = wb + threadIdx.x; w < we; w += blockDim.x)
         for_each_occ(c, w, [&](uint64_t h, uint32_t, uint64_t) { atomicAdd(&lh[h >> sh], 1u); });
     __syncthreads();
-    for (uint32_t i = threadIdx.x; i < kParts; i += blockDim.x)
-        if (lh[i]) atomicAdd(&hist[i], lh[i]);
+    uint32_t *row = rows + (uint64_t)blockIdx.x * kParts;
+    for (uint32_t i = threadIdx.x; i < kParts; i += blockDim.x) row[i] = lh[i];
     for (uint32_t b = threadIdx.x; b < kL1; b += blockDim.x) {
         uint32_t s1 = 0;
         for (uint32_t j = 0; j < kL2; j++) s1 += lh[b * kL2 + j];
@@ -299,79 +339,50 @@ __global__ void __launch_bounds__(kOccThreads) k_part_hist(OccCtx c, const uint6
     }
 }
 
-// Exclusive scan of n small counts held in LDS by one wave (n <= 1024).
-__device__ __forceinline__ void lds_excl_scan(const uint32_t *cnt, uint32_t *off, uint32_t n) {
-    if (threadIdx.x < 64) {
-        const int lane = threadIdx.x;
-        uint32_t run = 0;
-        for (uint32_t b0 = 0; b0 < n; b0 += 64) {
-            const uint32_t v = (b0 + lane < n) ? cnt[b0 + lane] : 0;
-            const uint32_t inc = wave_incl_sum(v);
-            if (b0 + lane < n) off[b0 + lane] = run + inc - v;
-            run += __shfl(inc, 63, 64);
-        }
-    }
+// out[col] += sum of rows[r][col] over one group of rows (out zeroed before).
+__global__ void k_col_sum(const uint32_t *__restrict__ rows, uint32_t nrows, uint32_t ncols,
+                          uint32_t *__restrict__ out) {
+    const uint32_t col = blockIdx.x * blockDim.x + threadIdx.x;
+    if (col >= ncols) return;
+    const uint32_t per = (nrows + gridDim.y - 1) / gridDim.y;
+    const uint32_t r0 = blockIdx.y * per, r1 = min(nrows, r0 + per);
+    uint32_t s = 0;
+    for (uint32_t r = r0; r < r1; r++) s += rows[(uint64_t)r * ncols + col];
+    if (s) atomicAdd(&out[col], s);
 }
 
-// Level 1: records of the workgroup's words, chunk by chunk, sorted by level-1
-// bucket in LDS and written as runs at offsets fixed by the histogram pass.
+// Level 1: every occurrence of the workgroup's words goes to the workgroup's run
+// of its level-1 bucket (run starts from the histogram pass).
 template <bool WITH_OCC>
-__global__ void __launch_bounds__(kOccThreads) k_part1(OccCtx c, const uint64_t *__restrict__ d_words,
-                                                        const uint64_t *__restrict__ base1,
+__global__ void __launch_bounds__(kOccThreads) k_part1(OccCtx c, const uint64_t *__restrict__ base1,
                                                         uint64_t *__restrict__ rec_out, uint32_t *__restrict__ occ_out) {
-    __shared__ uint64_t srec[kChunkRecs];
-    __shared__ uint32_t socc[WITH_OCC ? kChunkRecs : 1];
-    __shared__ uint8_t sbin[kChunkRecs];
-    __shared__ uint32_t gbase[kL1], lcnt[kL1], loff[kL1], lfill[kL1];
-    for (uint32_t b = threadIdx.x; b < kL1; b += blockDim.x) gbase[b] = (uint32_t)base1[(uint64_t)b * gridDim.x + blockIdx.x];
+    __shared__ uint32_t fill[kL1];
+    for (uint32_t b = threadIdx.x; b < kL1; b += blockDim.x) fill[b] = (uint32_t)base1[(uint64_t)b * gridDim.x + blockIdx.x];
+    __syncthreads();
     uint64_t wb, we;
-    word_range(*d_words, wb, we);
+    word_range(c.W, wb, we);
     const int sh1 = 2 * c.k - kL1Bits;
     const uint64_t low1 = (1ull << sh1) - 1;
-    for (uint64_t w0 = wb; w0 < we; w0 += blockDim.x) {
-        const uint64_t w = w0 + threadIdx.x;
-        for (uint32_t b = threadIdx.x; b < kL1; b += blockDim.x) { lcnt[b] = 0; lfill[b] = 0; }
-        __syncthreads();
-        if (w < we) for_each_occ(c, w, [&](uint64_t h, uint32_t, uint64_t) { atomicAdd(&lcnt[h >> sh1], 1u); });
-        __syncthreads();
-        lds_excl_scan(lcnt, loff, kL1);
-        __syncthreads();
-        if (w < we)
-            for_each_occ(c, w, [&](uint64_t h, uint32_t m, uint64_t occ) {
-                const uint32_t b = (uint32_t)(h >> sh1);
-                const uint32_t slot = loff[b] + atomicAdd(&lfill[b], 1u);
-                srec[slot] = ((h & low1) << 8) | m;
-                if (WITH_OCC) socc[slot] = (uint32_t)occ;
-                sbin[slot] = (uint8_t)b;
-            });
-        __syncthreads();
-        const uint32_t total = loff[kL1 - 1] + lcnt[kL1 - 1];
-        for (uint32_t i = threadIdx.x; i < total; i += blockDim.x) {
-            const uint32_t b = sbin[i];
-            const uint32_t g = gbase[b] + (i - loff[b]);
-            rec_out[g] = srec[i];
-            if (WITH_OCC) occ_out[g] = socc[i];
-        }
-        __syncthreads();
-        for (uint32_t b = threadIdx.x; b < kL1; b += blockDim.x) gbase[b] += lcnt[b];
-        __syncthreads();
-    }
+    for (uint64_t w = wb + threadIdx.x; w < we; w += blockDim.x)
+        for_each_occ(c, w, [&](uint64_t h, uint32_t m, uint64_t occ) {
+            const uint32_t g = atomicAdd(&fill[h >> sh1], 1u);
+            rec_out[g] = ((h & low1) << 8) | m;
+            if (WITH_OCC) occ_out[g] = (uint32_t)occ;
+        });
 }
 
-// Level 2: each workgroup takes one chunk of one level-1 bucket, sorts it by
-// level-2 bucket in LDS and appends the runs to the final partitions.
+// Level 2: each workgroup takes one chunk of one level-1 bucket, counts it by
+// level-2 bucket in LDS, claims a run of each final partition with one atomic
+// per sub-bucket and writes its records there.
 // chunk_start[b] = first chunk index of level-1 bucket b (kL1 + 1 entries).
 template <bool WITH_OCC>
-__global__ void __launch_bounds__(kOccThreads) k_part2(const uint64_t *__restrict__ rec_in,
-                                                        const uint32_t *__restrict__ occ_in,
-                                                        const uint32_t *__restrict__ l1_start,
-                                                        const uint32_t *__restrict__ chunk_start, int k,
-                                                        uint32_t *__restrict__ cursor, uint64_t *__restrict__ rec_out,
-                                                        uint32_t *__restrict__ occ_out) {
-    __shared__ uint64_t srec[kChunkRecs];
-    __shared__ uint32_t socc[WITH_OCC ? kChunkRecs : 1];
-    __shared__ uint8_t sbin[kChunkRecs];
-    __shared__ uint32_t gbase[kL2], lcnt[kL2], loff[kL2], lfill[kL2];
+__global__ void __launch_bounds__(kL2Threads) k_part2(const uint64_t *__restrict__ rec_in,
+                                                       const uint32_t *__restrict__ occ_in,
+                                                       const uint32_t *__restrict__ l1_start,
+                                                       const uint32_t *__restrict__ chunk_start, int k,
+                                                       uint32_t *__restrict__ cursor, uint64_t *__restrict__ rec_out,
+                                                       uint32_t *__restrict__ occ_out) {
+    __shared__ uint32_t lcnt[kL2];
     __shared__ uint32_t sb, sfirst;
     const uint32_t chunk = blockIdx.x;
     if (threadIdx.x == 0) {
@@ -383,7 +394,7 @@ __global__ void __launch_bounds__(kOccThreads) k_part2(const uint64_t *__restric
         sb = lo;
         sfirst = chunk_start[lo];
     }
-    for (uint32_t b = threadIdx.x; b < kL2; b += blockDim.x) { lcnt[b] = 0; lfill[b] = 0; }
+    for (uint32_t b = threadIdx.x; b < kL2; b += blockDim.x) lcnt[b] = 0;
     __syncthreads();
     if (chunk >= chunk_start[kL1]) return;
     const uint32_t b1 = sb;
@@ -391,11 +402,11 @@ __global__ void __launch_bounds__(kOccThreads) k_part2(const uint64_t *__restric
     const uint64_t e = min<uint64_t>((uint64_t)l1_start[b1 + 1], a + kChunkRecs);
     const uint32_t n = (uint32_t)(e - a);
     const int sh2 = 8 + 2 * k - kPartBits;
-    uint64_t r[kChunkRecs / kOccThreads];
-    uint32_t o[kChunkRecs / kOccThreads];
+    uint64_t r[kL2Per];
+    uint32_t o[kL2Per];
 #pragma unroll
-    for (int j = 0; j < kChunkRecs / kOccThreads; j++) {
-        const uint32_t i = j * kOccThreads + threadIdx.x;
+    for (int j = 0; j < kL2Per; j++) {
+        const uint32_t i = j * kL2Threads + threadIdx.x;
         if (i < n) {
             r[j] = rec_in[a + i];
             if (WITH_OCC) o[j] = occ_in[a + i];
@@ -403,27 +414,17 @@ __global__ void __launch_bounds__(kOccThreads) k_part2(const uint64_t *__restric
         }
     }
     __syncthreads();
-    lds_excl_scan(lcnt, loff, kL2);
-    __syncthreads();
     for (uint32_t b = threadIdx.x; b < kL2; b += blockDim.x)
-        if (lcnt[b]) gbase[b] = atomicAdd(&cursor[b1 * kL2 + b], lcnt[b]);
-#pragma unroll
-    for (int j = 0; j < kChunkRecs / kOccThreads; j++) {
-        const uint32_t i = j * kOccThreads + threadIdx.x;
-        if (i < n) {
-            const uint32_t b = (uint32_t)((r[j] >> sh2) & (kL2 - 1));
-            const uint32_t slot = loff[b] + atomicAdd(&lfill[b], 1u);
-            srec[slot] = r[j];
-            if (WITH_OCC) socc[slot] = o[j];
-            sbin[slot] = (uint8_t)b;
-        }
-    }
+        lcnt[b] = lcnt[b] ? atomicAdd(&cursor[b1 * kL2 + b], lcnt[b]) : 0;
     __syncthreads();
-    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
-        const uint32_t b = sbin[i];
-        const uint32_t g = gbase[b] + (i - loff[b]);
-        rec_out[g] = srec[i];
-        if (WITH_OCC) occ_out[g] = socc[i];
+#pragma unroll
+    for (int j = 0; j < kL2Per; j++) {
+        const uint32_t i = j * kL2Threads + threadIdx.x;
+        if (i < n) {
+            const uint32_t g = atomicAdd(&lcnt[(r[j] >> sh2) & (kL2 - 1)], 1u);
+            rec_out[g] = r[j];
+            if (WITH_OCC) occ_out[g] = o[j];
+        }
     }
 }
 
@@ -449,7 +450,7 @@ __global__ void k_l1_chunks(const uint32_t *__restrict__ pstart, uint32_t *__res
 
 constexpr int kTabBits = 12;
 constexpr uint32_t kTab = 1u << kTabBits;
-constexpr int kCountThreads = 512;
+constexpr int kCountThreads = 1024;
 constexpr uint64_t kEmpty = ~0ull;
 
 struct CountArgs {
@@ -506,6 +507,46 @@ __device__ __forceinline__ bool table_add(uint64_t *tkey, uint64_t *tcnt, uint32
     return false;
 }
 
+// Same probe, 16-bit counters: 8 per slot in 4 words, incremented with
+// non-returning LDS adds.  Used when the partition has fewer than 2^16
+// records, so no counter can wrap; min(count, 255) is taken at emit.
+template <bool WITH_OCC>
+__device__ __forceinline__ bool table_add16(uint64_t *tkey, uint32_t *tc, uint32_t *tfo, uint32_t mask, uint64_t rec,
+                                            uint32_t occ) {
+    const uint64_t key = rec >> 8;
+    uint32_t m = (uint32_t)(rec & 0xFF);
+    uint32_t s = slot_hash(key, mask);
+    for (uint32_t probe = 0; probe <= mask; probe++) {
+        uint64_t cur = tkey[s];
+        if (cur == kEmpty) {
+            const uint64_t prev = atomicCAS((unsigned long long *)&tkey[s], (unsigned long long)kEmpty,
+                                            (unsigned long long)key);
+            cur = (prev == kEmpty) ? key : prev;
+        }
+        if (cur == key) {
+            while (m) {
+                const int t = __ffs(m) - 1;
+                m &= m - 1;
+                atomicAdd(&tc[4 * s + (t >> 1)], 1u << (16 * (t & 1)));
+            }
+            if (WITH_OCC) atomicMin(&tfo[s], occ);
+            return true;
+        }
+        s = (s + 1) & mask;
+    }
+    return false;
+}
+
+__device__ __forceinline__ uint64_t counts16_to_bytes(const uint32_t *tc, uint32_t s) {
+    uint64_t out = 0;
+#pragma unroll
+    for (int t = 0; t < 8; t++) {
+        const uint32_t c = (tc[4 * s + (t >> 1)] >> (16 * (t & 1))) & 0xFFFF;
+        out |= (uint64_t)(c > 255 ? 255 : c) << (8 * t);
+    }
+    return out;
+}
+
 // Cutoff of graph.cpp:1100-1112 on one slot: multiplicities below c become 0.
 __device__ __forceinline__ uint64_t apply_cutoff(uint64_t cnt, int c) {
     uint64_t out = 0;
@@ -520,8 +561,8 @@ __device__ __forceinline__ uint64_t apply_cutoff(uint64_t cnt, int c) {
 // Emits the survivors of one partition table and (reference order) the
 // first-occurrence index of every distinct key.  Slots are taken 8 per thread
 // per round; one atomic per workgroup and round reserves the output ranges.
-template <bool WITH_OCC>
-__device__ void table_emit(const uint64_t *tkey, const uint64_t *tcnt, const uint32_t *tfo, uint32_t n_slots,
+template <bool WITH_OCC, typename CNT>
+__device__ void table_emit(const uint64_t *tkey, CNT cnt_of, const uint32_t *tfo, uint32_t n_slots,
                            uint32_t part, const CountArgs &a) {
     __shared__ uint64_t wsum[kCountThreads / 64];
     __shared__ unsigned long long base_s, base_d;
@@ -541,7 +582,7 @@ __device__ void table_emit(const uint64_t *tkey, const uint64_t *tcnt, const uin
             if (s < n_slots) {
                 keys[j] = tkey[s];
                 if (keys[j] != kEmpty) {
-                    mults[j] = apply_cutoff(tcnt[s], a.c);
+                    mults[j] = apply_cutoff(cnt_of(s), a.c);
                     if (WITH_OCC) fos[j] = tfo[s];
                     nd++;
                     ns += mults[j] != 0;
@@ -581,21 +622,28 @@ __device__ void table_emit(const uint64_t *tkey, const uint64_t *tcnt, const uin
 template <bool WITH_OCC>
 __global__ void __launch_bounds__(kCountThreads) k_count_lds(CountArgs a) {
     __shared__ uint64_t tkey[kTab];
-    __shared__ uint64_t tcnt[kTab];
+    __shared__ uint32_t tc[4 * kTab];  // 8 x u16 counters per slot, or 8 x u8 in the first 2*kTab words
     __shared__ uint32_t tfo[WITH_OCC ? kTab : 1];
     __shared__ int overflow;
     const uint32_t part = blockIdx.x;
+    const uint32_t b = a.bucket[part], e = a.bucket[part + 1];
+    const bool small = (e - b) < 0xFFFFu;
     for (uint32_t i = threadIdx.x; i < kTab; i += blockDim.x) {
         tkey[i] = kEmpty;
-        tcnt[i] = 0;
         if (WITH_OCC) tfo[i] = ~0u;
     }
+    for (uint32_t i = threadIdx.x; i < 4 * kTab; i += blockDim.x) tc[i] = 0;
     if (threadIdx.x == 0) overflow = 0;
     __syncthreads();
-    const uint32_t b = a.bucket[part], e = a.bucket[part + 1];
     bool ok = true;
-    for (uint32_t i = b + threadIdx.x; i < e && ok; i += blockDim.x)
-        ok = table_add<WITH_OCC>(tkey, tcnt, tfo, kTab - 1, a.rec[i], WITH_OCC ? a.occ[i] : 0);
+    if (small) {
+        for (uint32_t i = b + threadIdx.x; i < e && ok; i += blockDim.x)
+            ok = table_add16<WITH_OCC>(tkey, tc, tfo, kTab - 1, a.rec[i], WITH_OCC ? a.occ[i] : 0);
+    } else {
+        uint64_t *tcnt = reinterpret_cast<uint64_t *>(tc);
+        for (uint32_t i = b + threadIdx.x; i < e && ok; i += blockDim.x)
+            ok = table_add<WITH_OCC>(tkey, tcnt, tfo, kTab - 1, a.rec[i], WITH_OCC ? a.occ[i] : 0);
+    }
     if (!ok) overflow = 1;
     __syncthreads();
     if (overflow) {
@@ -605,7 +653,11 @@ __global__ void __launch_bounds__(kCountThreads) k_count_lds(CountArgs a) {
         }
         return;
     }
-    table_emit<WITH_OCC>(tkey, tcnt, tfo, kTab, part, a);
+    if (small)
+        table_emit<WITH_OCC>(tkey, [&](uint32_t s) { return counts16_to_bytes(tc, s); }, tfo, kTab, part, a);
+    else
+        table_emit<WITH_OCC>(tkey, [&](uint32_t s) { return reinterpret_cast<const uint64_t *>(tc)[s]; }, tfo, kTab,
+                             part, a);
 }
 
 // Global-memory fallback for partitions with more distinct keys than the LDS
@@ -632,7 +684,7 @@ __global__ void __launch_bounds__(kCountThreads) k_count_global(CountArgs a, con
         table_add<WITH_OCC>(tkey, tcnt, tfo, n_slots - 1, a.rec[i], WITH_OCC ? a.occ[i] : 0);
     __threadfence_block();
     __syncthreads();
-    table_emit<WITH_OCC>(tkey, tcnt, tfo, n_slots, part, a);
+    table_emit<WITH_OCC>(tkey, [&](uint32_t s) { return tcnt[s]; }, tfo, n_slots, part, a);
 }
 
 // ---------------------------------------------------------------------------
@@ -646,8 +698,9 @@ __global__ void __launch_bounds__(kCountThreads) k_count_global(CountArgs a, con
 constexpr int kSelBits = 14;
 constexpr uint32_t kSelBins = 1u << kSelBits;
 
+// One histogram row per workgroup (summed by k_col_sum).
 __global__ void k_sel_hist(const uint32_t *__restrict__ dfo, const unsigned long long *__restrict__ d_n, int shift,
-                           uint32_t *__restrict__ hist, unsigned long long *__restrict__ maxfo) {
+                           uint32_t *__restrict__ rows, unsigned long long *__restrict__ maxfo) {
     __shared__ uint32_t lh[kSelBins];
     __shared__ uint32_t lmax;
     for (uint32_t i = threadIdx.x; i < kSelBins; i += blockDim.x) lh[i] = 0;
@@ -662,8 +715,8 @@ __global__ void k_sel_hist(const uint32_t *__restrict__ dfo, const unsigned long
     }
     atomicMax(&lmax, mx);
     __syncthreads();
-    for (uint32_t i = threadIdx.x; i < kSelBins; i += blockDim.x)
-        if (lh[i]) atomicAdd(&hist[i], lh[i]);
+    uint32_t *row = rows + (uint64_t)blockIdx.x * kSelBins;
+    for (uint32_t i = threadIdx.x; i < kSelBins; i += blockDim.x) row[i] = lh[i];
     if (threadIdx.x == 0) atomicMax(maxfo, (unsigned long long)lmax);
 }
 
@@ -959,21 +1012,20 @@ int Pipeline::build(const uint8_t *d_in, uint64_t n) {
     if (N == 0) return 0;
     // --- pack
     SWAP_CHECK(packed_.reserve((W + 1) * 8));
+    SWAP_CHECK(wmeta_.reserve((W + 1) * 8));
     {
-        const unsigned grid = (unsigned)std::min<uint64_t>((W + 255) / 256, 65535);
+        const unsigned grid = (unsigned)std::min<uint64_t>((W + kPackThreads - 1) / kPackThreads, 65535);
         phase_begin(1);
-        hipLaunchKernelGGL(k_pack, dim3(grid), dim3(256), 0, st_, buf, rec_start_.as<uint64_t>(),
-                           rec_len_.as<uint32_t>(), word_base_.as<uint64_t>(), R, d_small + 5,
-                           packed_.as<uint64_t>());
+        hipLaunchKernelGGL(k_pack, dim3(grid), dim3(kPackThreads), 0, st_, buf, rec_start_.as<uint64_t>(),
+                           rec_len_.as<uint32_t>(), word_base_.as<uint64_t>(), occ_base_.as<uint64_t>(), R, k_,
+                           d_small + 5, packed_.as<uint64_t>(), wmeta_.as<uint64_t>());
         phase_end(1);
     }
     // --- partition
     OccCtx oc;
     oc.packed = packed_.as<uint64_t>();
-    oc.word_base = word_base_.as<uint64_t>();
-    oc.occ_base = occ_base_.as<uint64_t>();
-    oc.rec_len = rec_len_.as<uint32_t>();
-    oc.R = R;
+    oc.wmeta = wmeta_.as<uint64_t>();
+    oc.W = W;
     oc.k = k_;
     oc.mx = mx_;
     int ncu = 0;
@@ -986,10 +1038,13 @@ int Pipeline::build(const uint8_t *d_in, uint64_t n) {
     SWAP_CHECK(hist1_.reserve(n1 * 4));
     SWAP_CHECK(base1_.reserve((n1 + 1) * 8));
     SWAP_CHECK(l1info_.reserve((2 * kL1 + 4) * 4));
+    SWAP_CHECK(rows_.reserve((uint64_t)occ_grid * kParts * 4));
     SWAP_CHECK(hipMemsetAsync(hist_.p, 0, (kParts + 1) * 4, st_));
     phase_begin(2);
-    hipLaunchKernelGGL(k_part_hist, dim3(occ_grid), dim3(kOccThreads), 0, st_, oc, d_small + 5,
-                       hist_.as<uint32_t>(), hist1_.as<uint32_t>());
+    hipLaunchKernelGGL(k_part_hist, dim3(occ_grid), dim3(kOccThreads), 0, st_, oc, rows_.as<uint32_t>(),
+                       hist1_.as<uint32_t>());
+    hipLaunchKernelGGL(k_col_sum, dim3(kParts / 256, kHistRowGroups), dim3(256), 0, st_, rows_.as<uint32_t>(),
+                       occ_grid, kParts, hist_.as<uint32_t>());
     phase_end(2);
     SWAP_CHECK(scan_scratch_.reserve((scan_scratch_words(n1 + kParts + 1) + 16) * 8));
     scan_excl<uint32_t>(hist_.as<uint32_t>(), bucket_.as<uint64_t>(), kParts + 1, scan_scratch_.as<uint64_t>(),
@@ -1011,20 +1066,20 @@ int Pipeline::build(const uint8_t *d_in, uint64_t n) {
     }
     phase_begin(3);
     if (ref_order_)
-        hipLaunchKernelGGL(k_part1<true>, dim3(occ_grid), dim3(kOccThreads), 0, st_, oc, d_small + 5,
-                           base1_.as<uint64_t>(), rec1_.as<uint64_t>(), occ1_.as<uint32_t>());
+        hipLaunchKernelGGL(k_part1<true>, dim3(occ_grid), dim3(kOccThreads), 0, st_, oc, base1_.as<uint64_t>(),
+                           rec1_.as<uint64_t>(), occ1_.as<uint32_t>());
     else
-        hipLaunchKernelGGL(k_part1<false>, dim3(occ_grid), dim3(kOccThreads), 0, st_, oc, d_small + 5,
-                           base1_.as<uint64_t>(), rec1_.as<uint64_t>(), (uint32_t *)nullptr);
+        hipLaunchKernelGGL(k_part1<false>, dim3(occ_grid), dim3(kOccThreads), 0, st_, oc, base1_.as<uint64_t>(),
+                           rec1_.as<uint64_t>(), (uint32_t *)nullptr);
     phase_end(3);
     const unsigned grid2 = (unsigned)(N / kChunkRecs + kL1 + 1);
     phase_begin(7);
     if (ref_order_)
-        hipLaunchKernelGGL(k_part2<true>, dim3(grid2), dim3(kOccThreads), 0, st_, rec1_.as<uint64_t>(),
+        hipLaunchKernelGGL(k_part2<true>, dim3(grid2), dim3(kL2Threads), 0, st_, rec1_.as<uint64_t>(),
                            occ1_.as<uint32_t>(), l1_start, chunk_start, k_, cursor_.as<uint32_t>(),
                            rec_.as<uint64_t>(), occ_.as<uint32_t>());
     else
-        hipLaunchKernelGGL(k_part2<false>, dim3(grid2), dim3(kOccThreads), 0, st_, rec1_.as<uint64_t>(),
+        hipLaunchKernelGGL(k_part2<false>, dim3(grid2), dim3(kL2Threads), 0, st_, rec1_.as<uint64_t>(),
                            (const uint32_t *)nullptr, l1_start, chunk_start, k_, cursor_.as<uint32_t>(),
                            rec_.as<uint64_t>(), (uint32_t *)nullptr);
     phase_end(7);
@@ -1125,9 +1180,12 @@ int Pipeline::build(const uint8_t *d_in, uint64_t n) {
         SWAP_CHECK(hipMemcpyAsync(etab_.p, h_etab_, sizeof(EpochTabHost), hipMemcpyHostToDevice, st_));
         EpochTabHost *d_et = etab_.as<EpochTabHost>();
         phase_begin(6);
+        SWAP_CHECK(rows_.reserve((uint64_t)ncu * kSelBins * 4));
         hipLaunchKernelGGL(k_sel_hist, dim3(ncu), dim3(1024), 0, st_, dfo_.as<uint32_t>(),
-                           (const unsigned long long *)(d_small + 9), shift, d_hist,
+                           (const unsigned long long *)(d_small + 9), shift, rows_.as<uint32_t>(),
                            (unsigned long long *)(d_small + 12));
+        hipLaunchKernelGGL(k_col_sum, dim3(kSelBins / 256, kHistRowGroups), dim3(256), 0, st_, rows_.as<uint32_t>(),
+                           (uint32_t)ncu, kSelBins, d_hist);
         hipLaunchKernelGGL(k_sel_targets, dim3(1), dim3(1024), 0, st_, d_hist, d_starts, (int)starts.size(),
                            sel_target_.as<int>(), d_si);
         hipLaunchKernelGGL(k_sel_collect_si, dim3(ncu * 2), dim3(1024), 0, st_, dfo_.as<uint32_t>(),

@@ -90,6 +90,7 @@ public:
     int fetch_merge(MergeDev &out);
     int visit_priorities(uint64_t *d_prio);
     int table_order_device(const uint64_t *d_rank, const EpochTabHost &tab, const void *d_et, uint32_t *d_order);
+    int table_reorder_device(const uint64_t *d_rank, const void *d_et, uint32_t *d_order);
 
     // Counters of the last build / merge.
     uint64_t n_reads = 0, n_occ = 0, n_distinct = 0, n_surv = 0, n_kept = 0, n_overflow = 0;
@@ -127,7 +128,7 @@ private:
     std::string err_;
     // build buffers
     DevBuf fasta_, tile_nl_, tile_off_, nl_pos_, scan_scratch_, small_, rec_start_, rec_len_, occ_count_,
-        word_count_, occ_base_, word_base_, packed_, hist_, bucket_, cursor_, hist1_, base1_, l1info_, rec1_, occ1_, rec_, occ_, surv_key_, surv_mult_,
+        word_count_, occ_base_, word_base_, packed_, wmeta_, rows_, hist_, bucket_, cursor_, hist1_, base1_, l1info_, rec1_, occ1_, rec_, occ_, surv_key_, surv_mult_,
         surv_first_, dfo_, sel_, sel_target_, sel_out_, overflow_, gtab_key_, gtab_cnt_, gtab_fo_;
     // merge buffers
     enum {

@@ -84,6 +84,19 @@ __global__ void k_ord_ties(const uint32_t *__restrict__ bucket, uint32_t *__rest
     }
 }
 
+// Insertion epochs for new ranks (hash values and buckets unchanged).
+__global__ void k_ord_ins(const uint64_t *__restrict__ rank, uint64_t n, const EpochTab *__restrict__ etp,
+                          uint8_t *__restrict__ ins) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const EpochTab &et = *etp;
+    const uint64_t r = rank[i];
+    int e = 0;
+    for (int j = 1; j < et.n; j++)
+        if (et.start[j] <= r) e = j;
+    ins[i] = (uint8_t)e;
+}
+
 __global__ void k_ord_positions(const uint32_t *__restrict__ order, uint64_t n, uint64_t *__restrict__ pos) {
     const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) pos[order[i]] = i;
@@ -129,6 +142,18 @@ int Pipeline::table_order_device(const uint64_t *d_rank, const EpochTabHost &tab
     return 0;
 }
 
+// Same table filled again in a new insertion order (the nodes table after the
+// cutoff copy: equal bucket count, so the bucket sort of the previous call
+// stands and only the order inside each bucket is redone).
+int Pipeline::table_reorder_device(const uint64_t *d_rank, const void *d_et, uint32_t *d_order) {
+    const uint64_t S = n_surv;
+    const EpochTab *et = static_cast<const EpochTab *>(d_et);
+    hipLaunchKernelGGL(k_ord_ins, dim3(oblk(S)), dim3(256), 0, st_, d_rank, S, et, ob_[O_INS].as<uint8_t>());
+    hipLaunchKernelGGL(k_ord_ties, dim3(oblk(S)), dim3(256), 0, st_, ob_[O_BKT2].as<uint32_t>(), d_order, S, et,
+                       ob_[O_H].as<uint64_t>(), ob_[O_INS].as<uint8_t>(), d_rank);
+    return 0;
+}
+
 EpochTabHost epoch_table(uint64_t n_total, bool extra, const std::vector<uint64_t> *starts_override) {
     EpochTabHost t;
     std::vector<uint64_t> s = table_epoch_starts(n_total, extra);
@@ -159,11 +184,12 @@ int Pipeline::visit_priorities(uint64_t *d_prio) {
     *h_etabS_ = epoch_table(S, false, nullptr);
     OCHECK(etabS_.reserve(sizeof(EpochTabHost) + 64));
     OCHECK(hipMemcpyAsync(etabS_.p, h_etabS_, sizeof(EpochTabHost), hipMemcpyHostToDevice, st_));
-    for (int t = 0; t < 2; t++) {
-        hipLaunchKernelGGL(k_ord_positions, dim3(oblk(S)), dim3(256), 0, st_, ord, S, rank);
-        rc = table_order_device(rank, *h_etabS_, etabS_.p, ord);
-        if (rc) return rc;
-    }
+    hipLaunchKernelGGL(k_ord_positions, dim3(oblk(S)), dim3(256), 0, st_, ord, S, rank);
+    rc = table_order_device(rank, *h_etabS_, etabS_.p, ord);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_ord_positions, dim3(oblk(S)), dim3(256), 0, st_, ord, S, rank);
+    rc = table_reorder_device(rank, etabS_.p, ord);
+    if (rc) return rc;
     hipLaunchKernelGGL(k_ord_positions, dim3(oblk(S)), dim3(256), 0, st_, ord, S, d_prio);
     return 0;
 }
