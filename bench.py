@@ -18,6 +18,8 @@ reference's np=1 path) on a bounded prefix of the same reads.
 import argparse
 import json
 import os
+import re
+import shutil
 import subprocess
 import sys
 import tempfile
@@ -52,6 +54,69 @@ def algorithmic_bytes(phase, st, words, ref_order):
     return 0
 
 
+def sample_prefix(fa: bytes, frac: float) -> bytes:
+    """The first records of the FASTA, about frac of its bytes, ending on a record boundary."""
+    cut = int(len(fa) * frac)
+    pos = fa.rfind(b"\n>", 0, cut)
+    return fa[:pos + 1] if pos > 0 else fa
+
+
+def count_kmers(fa: bytes, k: int) -> int:
+    """Input k-mer occurrences of a one-line-per-sequence FASTA (records shorter than k give none)."""
+    lines = fa.split(b"\n")
+    n = 0
+    for i in range(1, len(lines), 2):
+        n += max(0, len(lines[i]) - k + 1)
+    return n
+
+
+def find_mpirun():
+    for cand in (shutil.which("mpirun"), "/opt/conda/bin/mpirun"):
+        if cand and os.path.exists(cand):
+            return cand
+    return None
+
+
+def reference_baseline(fa: bytes, k: int, c: int, frac=0.25, nprocs=8):
+    """The reference itself (built from its sources into oracle/_ref) under mpirun on the host cores.
+
+    Timed like the reference reports itself: the sum of its build+merge phase
+    times from logtime.txt (Constructing Graph + cutoffGraph + Constructing
+    DistGraph + Simplify DistGraph, rank-0 clocks after barriers), on the
+    first `frac` of the reads.  Returns None if the binary or mpirun is absent.
+    """
+    exe = os.path.join(REPO, "oracle", "_ref", "swap")
+    mpirun = find_mpirun()
+    if not os.path.exists(exe) or not mpirun:
+        return None
+    tmpdir = tempfile.mkdtemp(prefix="ref_base_")
+    sample = sample_prefix(fa, frac)
+    path = os.path.join(tmpdir, "s.fa")
+    with open(path, "wb") as f:
+        f.write(sample)
+    out = os.path.join(tmpdir, "out")
+    p = subprocess.run([mpirun, "-np", str(nprocs), exe, "-k", str(k), "-c", str(c), "-i", path, "-o", out, "-p"],
+                       capture_output=True, text=True, timeout=600)
+    log = os.path.join(out, "logtime.txt")
+    if p.returncode != 0 or not os.path.exists(log):
+        print("reference baseline failed (rc %d): %s" % (p.returncode, p.stderr[-400:]), file=sys.stderr)
+        return None
+    secs = 0.0
+    for line in open(log):
+        m = re.search(r"Time spend in (Constructing Graph|cutoffGraph|Constructing DistGraph|Simplify DistGraph) "
+                      r"([0-9.]+)", line)
+        if m:
+            secs += float(m.group(2))
+    kmers = count_kmers(sample, k)
+    shutil.rmtree(tmpdir, ignore_errors=True)
+    if secs <= 0:
+        return None
+    return {"value": kmers / secs, "unit": "input k-mers/s", "cores": nprocs, "kind": "reference",
+            "sample": "first %.0f%% of the reads (%d input k-mers); reference swap -p under mpirun -np %d, "
+                      "build+merge phase times from its logtime.txt, %.1f s" % (
+                          100 * len(sample) / len(fa), kmers, nprocs, secs)}
+
+
 def cpu_baseline(fa: bytes, k: int, c: int, target_s=15.0):
     """Times the oracle CLI on growing prefixes of the reads until ~target_s."""
     exe = os.path.join(REPO, "oracle", "swap_oracle")
@@ -83,6 +148,21 @@ def cpu_baseline(fa: bytes, k: int, c: int, target_s=15.0):
     return res
 
 
+def copy_probe(nbytes):
+    """One hipMemcpy device-to-device of nbytes: reads and writes nbytes once."""
+    import ctypes
+    import swapamd
+    H = swapamd.hip()
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    if H.hipMalloc(ctypes.byref(a), nbytes) or H.hipMalloc(ctypes.byref(b), nbytes):
+        raise RuntimeError("hipMalloc failed")
+    if H.hipMemset(a, 1, nbytes) or H.hipMemcpy(b, a, nbytes, 3):  # 3 = hipMemcpyDeviceToDevice
+        raise RuntimeError("copy probe failed")
+    H.hipDeviceSynchronize()
+    H.hipFree(a)
+    H.hipFree(b)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -91,6 +171,9 @@ def main():
     ap.add_argument("--workload", default="bacterial")
     ap.add_argument("--hash-order", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="first run one device-to-device copy of MIB MiB (a known-size kernel that calibrates "
+                         "hardware byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (sharding is described in DESIGN.md)",
@@ -104,6 +187,8 @@ def main():
     print("generated %s: %d bytes in %.1f s" % (args.workload, len(fa), time.time() - t0), file=sys.stderr,
           flush=True)
     dev = swapamd.DeviceBuffer(fa)
+    if args.copy_probe:
+        copy_probe(args.copy_probe << 20)
     ref_order = not args.hash_order
     a = swapamd.Assembler(k, c, hash_order=args.hash_order)
     has_merge = True
@@ -143,7 +228,9 @@ def main():
     roof = {"bound": "hbm", "kernel": PHASES[dom], "achieved": achieved / 1e9 if achieved else None,
             "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": (achieved / HBM_PEAK) if achieved else None,
             "traffic": None, "algorithmic_bytes": alg, "kernel_ms": phases[dom]}
-    cpu = None if args.no_cpu_baseline else cpu_baseline(fa, k, c)
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = reference_baseline(fa, k, c) or cpu_baseline(fa, k, c)
     line = {
         "metric": "input k-mers/s (build+merge) at 4.6Mbp/50x",
         "value": value,

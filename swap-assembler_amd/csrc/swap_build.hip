@@ -36,6 +36,18 @@ __device__ __forceinline__ uint32_t base_code(uint8_t c) {
     }
 }
 
+// The same map on four bytes at once: the 2-bit codes of bytes 0..3 (byte 0
+// first, in the top bits of the result).  c|0x20 folds case; ((u>>1)^(u>>2))&3
+// is 0,1,2,3 for a,c,g,t; any other byte gives 0.
+__device__ __forceinline__ uint32_t base_codes4(uint32_t t) {
+    const uint32_t u = t | 0x20202020u;
+    const uint32_t code = ((u >> 1) ^ (u >> 2)) & 0x03030303u;
+    const uint32_t valid = zero_bytes(u ^ 0x61616161u) | zero_bytes(u ^ 0x63636363u) |
+                           zero_bytes(u ^ 0x67676767u) | zero_bytes(u ^ 0x74747474u);
+    const uint32_t c = code & ((valid >> 7) * 3u);
+    return ((c & 3u) << 6) | (((c >> 8) & 3u) << 4) | (((c >> 16) & 3u) << 2) | ((c >> 24) & 3u);
+}
+
 // ---------------------------------------------------------------------------
 // FASTA record scan
 
@@ -182,7 +194,7 @@ __device__ __forceinline__ uint64_t upper_index(const uint64_t *__restrict__ a, 
     return lo;
 }
 
-__global__ void __launch_bounds__(kPackThreads) k_pack(const uint8_t *__restrict__ buf,
+__global__ void __launch_bounds__(kPackThreads) k_pack(const uint8_t *__restrict__ buf, uint64_t n,
                                                        const uint64_t *__restrict__ rec_start,
                                                        const uint32_t *__restrict__ rec_len,
                                                        const uint64_t *__restrict__ word_base,
@@ -216,10 +228,25 @@ __global__ void __launch_bounds__(kPackThreads) k_pack(const uint8_t *__restrict
             }
             const uint64_t j = w - word_base[r];
             const uint32_t len = rec_len[r];
-            const uint8_t *s = buf + rec_start[r] + 32 * j;
+            const uint64_t st = rec_start[r] + 32 * j;
             const int nb = (int)min<uint64_t>(32, len - 32 * j);
             uint64_t word = 0;
-            for (int i = 0; i < nb; i++) word |= (uint64_t)base_code(s[i]) << (62 - 2 * i);
+            // nine aligned dwords cover the 32 bytes from any offset
+            const uint32_t sh = (uint32_t)(((uintptr_t)buf + st) & 3);
+            const uint64_t a0 = st - sh;  // dword-aligned byte offset
+            if (a0 + 36 <= n && sh <= st) {
+                const uint32_t *d = reinterpret_cast<const uint32_t *>(buf + a0);
+                uint32_t v[9];
+#pragma unroll
+                for (int q = 0; q < 9; q++) v[q] = d[q];
+#pragma unroll
+                for (int q = 0; q < 8; q++)
+                    word |= (uint64_t)base_codes4(__builtin_amdgcn_alignbyte(v[q + 1], v[q], sh)) << (56 - 8 * q);
+                if (nb < 32) word &= ~0ull << (64 - 2 * nb);
+            } else {
+                const uint8_t *s = buf + st;
+                for (int i = 0; i < nb; i++) word |= (uint64_t)base_code(s[i]) << (62 - 2 * i);
+            }
             packed[w] = word;
             const uint32_t nk = len - k + 1;
             const uint32_t p0 = (uint32_t)(32 * j);
@@ -309,6 +336,39 @@ constexpr int kChunkRecs = 8192;      // level-2 records per workgroup
 constexpr int kL2Threads = 1024;
 constexpr int kL2Per = kChunkRecs / kL2Threads;
 
+// Occurrence records between the passes.  Reference order carries the input
+// index with each record: 12 bytes {rec lo, rec hi, occ}, moved as one
+// dwordx3 access; hash order moves the bare 8-byte record.
+struct __attribute__((aligned(4))) Rec12 {
+    uint32_t lo, hi, occ;
+};
+
+template <bool WITH_OCC>
+__device__ __forceinline__ void rec_store(uint32_t *base, uint64_t g, uint64_t rec, uint32_t occ) {
+    if (WITH_OCC) {
+        Rec12 v;
+        v.lo = (uint32_t)rec;
+        v.hi = (uint32_t)(rec >> 32);
+        v.occ = occ;
+        reinterpret_cast<Rec12 *>(base)[g] = v;
+    } else {
+        reinterpret_cast<uint64_t *>(base)[g] = rec;
+    }
+}
+
+template <bool WITH_OCC>
+__device__ __forceinline__ uint64_t rec_load(const uint32_t *base, uint64_t g, uint32_t &occ) {
+    if (WITH_OCC) {
+        const Rec12 v = reinterpret_cast<const Rec12 *>(base)[g];
+        occ = v.occ;
+        return (uint64_t)v.lo | ((uint64_t)v.hi << 32);
+    }
+    occ = 0;
+    return reinterpret_cast<const uint64_t *>(base)[g];
+}
+
+constexpr int rec_words(bool with_occ) { return with_occ ? 3 : 2; }
+
 // Per workgroup word range (same split in the histogram and level-1 passes).
 __device__ __forceinline__ void word_range(uint64_t W, uint64_t &wb, uint64_t &we) {
     const uint64_t per = (W + gridDim.x - 1) / gridDim.x;
@@ -355,7 +415,7 @@ __global__ void k_col_sum(const uint32_t *__restrict__ rows, uint32_t nrows, uin
 // of its level-1 bucket (run starts from the histogram pass).
 template <bool WITH_OCC>
 __global__ void __launch_bounds__(kOccThreads) k_part1(OccCtx c, const uint64_t *__restrict__ base1,
-                                                        uint64_t *__restrict__ rec_out, uint32_t *__restrict__ occ_out) {
+                                                        uint32_t *__restrict__ rec_out) {
     __shared__ uint32_t fill[kL1];
     for (uint32_t b = threadIdx.x; b < kL1; b += blockDim.x) fill[b] = (uint32_t)base1[(uint64_t)b * gridDim.x + blockIdx.x];
     __syncthreads();
@@ -366,8 +426,7 @@ __global__ void __launch_bounds__(kOccThreads) k_part1(OccCtx c, const uint64_t 
     for (uint64_t w = wb + threadIdx.x; w < we; w += blockDim.x)
         for_each_occ(c, w, [&](uint64_t h, uint32_t m, uint64_t occ) {
             const uint32_t g = atomicAdd(&fill[h >> sh1], 1u);
-            rec_out[g] = ((h & low1) << 8) | m;
-            if (WITH_OCC) occ_out[g] = (uint32_t)occ;
+            rec_store<WITH_OCC>(rec_out, g, ((h & low1) << 8) | m, (uint32_t)occ);
         });
 }
 
@@ -376,12 +435,10 @@ __global__ void __launch_bounds__(kOccThreads) k_part1(OccCtx c, const uint64_t 
 // per sub-bucket and writes its records there.
 // chunk_start[b] = first chunk index of level-1 bucket b (kL1 + 1 entries).
 template <bool WITH_OCC>
-__global__ void __launch_bounds__(kL2Threads) k_part2(const uint64_t *__restrict__ rec_in,
-                                                       const uint32_t *__restrict__ occ_in,
+__global__ void __launch_bounds__(kL2Threads) k_part2(const uint32_t *__restrict__ rec_in,
                                                        const uint32_t *__restrict__ l1_start,
                                                        const uint32_t *__restrict__ chunk_start, int k,
-                                                       uint32_t *__restrict__ cursor, uint64_t *__restrict__ rec_out,
-                                                       uint32_t *__restrict__ occ_out) {
+                                                       uint32_t *__restrict__ cursor, uint32_t *__restrict__ rec_out) {
     __shared__ uint32_t lcnt[kL2];
     __shared__ uint32_t sb, sfirst;
     const uint32_t chunk = blockIdx.x;
@@ -408,8 +465,7 @@ __global__ void __launch_bounds__(kL2Threads) k_part2(const uint64_t *__restrict
     for (int j = 0; j < kL2Per; j++) {
         const uint32_t i = j * kL2Threads + threadIdx.x;
         if (i < n) {
-            r[j] = rec_in[a + i];
-            if (WITH_OCC) o[j] = occ_in[a + i];
+            r[j] = rec_load<WITH_OCC>(rec_in, a + i, o[j]);
             atomicAdd(&lcnt[(r[j] >> sh2) & (kL2 - 1)], 1u);
         }
     }
@@ -422,8 +478,7 @@ __global__ void __launch_bounds__(kL2Threads) k_part2(const uint64_t *__restrict
         const uint32_t i = j * kL2Threads + threadIdx.x;
         if (i < n) {
             const uint32_t g = atomicAdd(&lcnt[(r[j] >> sh2) & (kL2 - 1)], 1u);
-            rec_out[g] = r[j];
-            if (WITH_OCC) occ_out[g] = o[j];
+            rec_store<WITH_OCC>(rec_out, g, r[j], o[j]);
         }
     }
 }
@@ -454,8 +509,7 @@ constexpr int kCountThreads = 1024;
 constexpr uint64_t kEmpty = ~0ull;
 
 struct CountArgs {
-    const uint64_t *rec;
-    const uint32_t *occ;
+    const uint32_t *rec;      // final partitions, rec_words() words per record
     const uint32_t *bucket;   // start of each final partition, kParts+1 entries
     int k;
     int c;
@@ -637,12 +691,18 @@ __global__ void __launch_bounds__(kCountThreads) k_count_lds(CountArgs a) {
     __syncthreads();
     bool ok = true;
     if (small) {
-        for (uint32_t i = b + threadIdx.x; i < e && ok; i += blockDim.x)
-            ok = table_add16<WITH_OCC>(tkey, tc, tfo, kTab - 1, a.rec[i], WITH_OCC ? a.occ[i] : 0);
+        for (uint32_t i = b + threadIdx.x; i < e && ok; i += blockDim.x) {
+            uint32_t o;
+            const uint64_t r = rec_load<WITH_OCC>(a.rec, i, o);
+            ok = table_add16<WITH_OCC>(tkey, tc, tfo, kTab - 1, r, o);
+        }
     } else {
         uint64_t *tcnt = reinterpret_cast<uint64_t *>(tc);
-        for (uint32_t i = b + threadIdx.x; i < e && ok; i += blockDim.x)
-            ok = table_add<WITH_OCC>(tkey, tcnt, tfo, kTab - 1, a.rec[i], WITH_OCC ? a.occ[i] : 0);
+        for (uint32_t i = b + threadIdx.x; i < e && ok; i += blockDim.x) {
+            uint32_t o;
+            const uint64_t r = rec_load<WITH_OCC>(a.rec, i, o);
+            ok = table_add<WITH_OCC>(tkey, tcnt, tfo, kTab - 1, r, o);
+        }
     }
     if (!ok) overflow = 1;
     __syncthreads();
@@ -680,8 +740,11 @@ __global__ void __launch_bounds__(kCountThreads) k_count_global(CountArgs a, con
     }
     __syncthreads();
     const uint32_t b = a.bucket[part], e = a.bucket[part + 1];
-    for (uint32_t i = b + threadIdx.x; i < e; i += blockDim.x)
-        table_add<WITH_OCC>(tkey, tcnt, tfo, n_slots - 1, a.rec[i], WITH_OCC ? a.occ[i] : 0);
+    for (uint32_t i = b + threadIdx.x; i < e; i += blockDim.x) {
+        uint32_t o;
+        const uint64_t r = rec_load<WITH_OCC>(a.rec, i, o);
+        table_add<WITH_OCC>(tkey, tcnt, tfo, n_slots - 1, r, o);
+    }
     __threadfence_block();
     __syncthreads();
     table_emit<WITH_OCC>(tkey, [&](uint32_t s) { return tcnt[s]; }, tfo, n_slots, part, a);
@@ -720,17 +783,14 @@ __global__ void k_sel_hist(const uint32_t *__restrict__ dfo, const unsigned long
     if (threadIdx.x == 0) atomicMax(maxfo, (unsigned long long)lmax);
 }
 
-struct SelInfo;
 // Device-side selection bookkeeping (no host round trip): bin and in-bin
-// position of each wanted rank, and the compact list of bins to collect.
+// position of each wanted rank, and the compact list of bins to mark.
 struct SelInfo {
     int n_starts;
     int n_targets;
     int tgt[32];        // target index of each start
     uint32_t pos[32];   // position of the start's rank inside its bin
     uint32_t tbin[32];  // bin of each target
-    uint32_t toff[33];  // offset of each target's values
-    uint32_t fill[32];  // append cursors
 };
 
 __global__ void __launch_bounds__(1024) k_sel_targets(const uint32_t *__restrict__ hist, const uint64_t *__restrict__ starts,
@@ -755,7 +815,6 @@ __global__ void __launch_bounds__(1024) k_sel_targets(const uint32_t *__restrict
     __syncthreads();
     if (t == 0) {
         int nt = 0;
-        uint32_t tot = 0;
         for (int j = 0; j < n_starts && j < 32; j++) {
             const uint64_t T = starts[j];
             uint32_t lo = 0, hi = kSelBins;  // last bin with cum <= T
@@ -768,65 +827,45 @@ __global__ void __launch_bounds__(1024) k_sel_targets(const uint32_t *__restrict
             if (ti < 0) {
                 ti = nt++;
                 si->tbin[ti] = lo;
-                si->toff[ti] = tot;
-                si->fill[ti] = 0;
-                tot += cum[lo + 1] - cum[lo];
                 target[lo] = ti;
             }
             si->tgt[j] = ti;
             si->pos[j] = (uint32_t)(T - cum[lo]);
         }
-        si->toff[nt] = tot;
         si->n_starts = n_starts;
         si->n_targets = nt;
     }
 }
 
-__global__ void k_sel_collect_si(const uint32_t *__restrict__ dfo, const unsigned long long *__restrict__ d_n, int shift,
-                                 const int *__restrict__ target, SelInfo *__restrict__ si, uint32_t *__restrict__ out) {
+// Marks the first occurrences that fall in a target bin in that bin's bitmap
+// (first occurrences of distinct keys are distinct, so the bitmap is exact).
+// Only a few bins are targets: the atomics are few and spread over the bitmap.
+__global__ void k_sel_mark(const uint32_t *__restrict__ dfo, const unsigned long long *__restrict__ d_n, int shift,
+                           const int *__restrict__ target, uint32_t *__restrict__ bitmap) {
     const uint64_t n = *d_n;
-    const int lane = threadIdx.x & 63;
-    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-    const uint64_t n_up = (n + stride - 1) / stride * stride;
-    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n_up; i += stride) {
-        const uint32_t f = i < n ? dfo[i] : 0;
-        int t = i < n ? target[f >> shift] : -1;
-        uint64_t pending = __ballot(t >= 0);
-        while (pending) {
-            const int leader = __ffsll((long long)pending) - 1;
-            const int lt = __shfl(t, leader, 64);
-            const uint64_t same = __ballot(t == lt);
-            uint32_t base = 0;
-            if (lane == leader) base = atomicAdd(&si->fill[lt], (uint32_t)__popcll(same));
-            base = __shfl(base, leader, 64);
-            if (t == lt) {
-                out[si->toff[lt] + base + __popcll(same & ((1ull << lane) - 1))] = f;
-                t = -1;
-            }
-            pending &= ~same;
+    const uint32_t nw = (1u << shift) >= 32 ? (1u << shift) / 32 : 1;
+    const uint32_t in_bin = (1u << shift) - 1;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t f = dfo[i];
+        const int t = target[f >> shift];
+        if (t >= 0) {
+            const uint32_t bit = f & in_bin;
+            atomicOr(&bitmap[(uint32_t)t * nw + (bit >> 5)], 1u << (bit & 31));
         }
     }
 }
 
-// Workgroup per target bin: the bin's distinct first occurrences as a bitmap
-// in LDS, then the wanted ranks by prefix popcount.  Writes the first
-// occurrence of each growth rank into the epoch table of the k-mer table.
-__global__ void __launch_bounds__(1024) k_sel_pick(const SelInfo *__restrict__ si, const uint32_t *__restrict__ vals,
+// Workgroup per target bin: the wanted ranks by prefix popcount over the bin's
+// bitmap.  Writes the first occurrence of each growth rank into the epoch
+// table of the k-mer table.
+__global__ void __launch_bounds__(1024) k_sel_pick(const SelInfo *__restrict__ si, const uint32_t *__restrict__ bitmap,
                                                    int shift, uint64_t *__restrict__ epoch_start) {
-    __shared__ uint32_t bm[1u << (18 - 5)];
     __shared__ uint32_t part[1024];
     __shared__ uint32_t found;
     const int tg = blockIdx.x;
     if (tg >= si->n_targets) return;
     const uint32_t nw = (1u << shift) >= 32 ? (1u << shift) / 32 : 1;
-    for (uint32_t i = threadIdx.x; i < nw; i += blockDim.x) bm[i] = 0;
-    __syncthreads();
-    const uint32_t base = si->tbin[tg] << shift;
-    for (uint32_t i = si->toff[tg] + threadIdx.x; i < si->toff[tg + 1]; i += blockDim.x) {
-        const uint32_t bit = vals[i] - base;
-        atomicOr(&bm[bit >> 5], 1u << (bit & 31));
-    }
-    __syncthreads();
+    const uint32_t *bm = bitmap + (uint64_t)tg * nw;
     const uint32_t per = (nw + blockDim.x - 1) / blockDim.x;
     uint32_t c = 0;
     for (uint32_t j = 0; j < per; j++) {
@@ -840,6 +879,7 @@ __global__ void __launch_bounds__(1024) k_sel_pick(const SelInfo *__restrict__ s
         for (int i = 0; i < (int)blockDim.x; i++) { const uint32_t v = part[i]; part[i] = run; run += v; }
     }
     __syncthreads();
+    const uint32_t base = si->tbin[tg] << shift;
     for (int j = 0; j < si->n_starts; j++) {
         if (si->tgt[j] != tg) continue;
         const uint32_t want = si->pos[j];  // 0-based rank inside the bin
@@ -871,12 +911,17 @@ __global__ void __launch_bounds__(1024) k_sel_pick(const SelInfo *__restrict__ s
 // Pipeline::build
 
 DevBuf::~DevBuf() {
-    if (p) hipFree(p);
+    if (p) (void)hipFree(p);
 }
 
 hipError_t DevBuf::reserve(size_t bytes) {
     if (bytes <= cap && p) return hipSuccess;
-    if (p) hipFree(p);
+    if (p) {
+        const hipError_t e = hipFree(p);
+        p = nullptr;
+        cap = 0;
+        if (e != hipSuccess) return e;
+    }
     p = nullptr;
     cap = 0;
     size_t want = bytes + bytes / 8 + 256;
@@ -893,12 +938,16 @@ static uint64_t inv_odd(uint64_t a) {
 }
 
 Pipeline::Pipeline(int device, int k, int c, bool reference_order) : dev_(device), k_(k), c_(c), ref_order_(reference_order) {
-    hipSetDevice(dev_);
-    hipStreamCreateWithFlags(&st_, hipStreamNonBlocking);
-    hipHostMalloc((void **)&h_small_, 1024 * sizeof(uint64_t), hipHostMallocDefault);
-    hipHostMalloc((void **)&h_starts_, 64 * sizeof(uint64_t), hipHostMallocDefault);
-    hipHostMalloc((void **)&h_etab_, sizeof(EpochTabHost), hipHostMallocDefault);
-    hipHostMalloc((void **)&h_etabS_, sizeof(EpochTabHost), hipHostMallocDefault);
+    // the first failure is kept in init_err_ (checked by the C API at create)
+    auto keep = [&](hipError_t e) {
+        if (e != hipSuccess && init_err_ == hipSuccess) init_err_ = e;
+    };
+    keep(hipSetDevice(dev_));
+    keep(hipStreamCreateWithFlags(&st_, hipStreamNonBlocking));
+    keep(hipHostMalloc((void **)&h_small_, 1024 * sizeof(uint64_t), hipHostMallocDefault));
+    keep(hipHostMalloc((void **)&h_starts_, 64 * sizeof(uint64_t), hipHostMallocDefault));
+    keep(hipHostMalloc((void **)&h_etab_, sizeof(EpochTabHost), hipHostMallocDefault));
+    keep(hipHostMalloc((void **)&h_etabS_, sizeof(EpochTabHost), hipHostMallocDefault));
     mx_.bits = 2 * k;
     mx_.m = kmask(k);
     mx_.a1 = 0x9E3779B97F4A7C15ull | 1;
@@ -906,30 +955,35 @@ Pipeline::Pipeline(int device, int k, int c, bool reference_order) : dev_(device
     mx_.i1 = inv_odd(mx_.a1);
     mx_.i2 = inv_odd(mx_.a2);
     mx_.s = k;  // half of the 2k bits
-    for (auto &e : ev_) hipEventCreate(&e);
+    for (auto &e : ev_) keep(hipEventCreate(&e));
 }
 
+// Phase timing is diagnostic: a failed event call leaves that phase at 0 ms.
 void Pipeline::phase_begin(int i) {
-    hipEventRecord(ev_[2 * i], st_);
-    ev_used_[i] = true;
+    ev_used_[i] = hipEventRecord(ev_[2 * i], st_) == hipSuccess;
 }
-void Pipeline::phase_end(int i) { hipEventRecord(ev_[2 * i + 1], st_); }
+void Pipeline::phase_end(int i) {
+    if (hipEventRecord(ev_[2 * i + 1], st_) != hipSuccess) ev_used_[i] = false;
+}
 void Pipeline::collect_phases() {
-    hipStreamSynchronize(st_);
+    const bool synced = hipStreamSynchronize(st_) == hipSuccess;
     for (int i = 0; i < kPhases; i++) {
         phase_ms[i] = 0;
-        if (ev_used_[i]) hipEventElapsedTime(&phase_ms[i], ev_[2 * i], ev_[2 * i + 1]);
+        if (synced && ev_used_[i] && hipEventElapsedTime(&phase_ms[i], ev_[2 * i], ev_[2 * i + 1]) != hipSuccess)
+            phase_ms[i] = 0;
     }
 }
 
 Pipeline::~Pipeline() {
-    hipSetDevice(dev_);
-    for (auto &e : ev_) if (e) hipEventDestroy(e);
-    if (h_small_) hipHostFree(h_small_);
-    if (h_starts_) hipHostFree(h_starts_);
-    if (h_etab_) hipHostFree(h_etab_);
-    if (h_etabS_) hipHostFree(h_etabS_);
-    if (st_) hipStreamDestroy(st_);
+    // teardown: nothing useful can be done about a failing release here
+    (void)hipSetDevice(dev_);
+    for (auto &e : ev_)
+        if (e) (void)hipEventDestroy(e);
+    if (h_small_) (void)hipHostFree(h_small_);
+    if (h_starts_) (void)hipHostFree(h_starts_);
+    if (h_etab_) (void)hipHostFree(h_etab_);
+    if (h_etabS_) (void)hipHostFree(h_etabS_);
+    if (st_) (void)hipStreamDestroy(st_);
 }
 
 int Pipeline::fail(int code, const char *msg) {
@@ -968,8 +1022,8 @@ int Pipeline::build(const uint8_t *d_in, uint64_t n) {
     hipLaunchKernelGGL(k_tile_count, dim3((unsigned)tiles), dim3(kTileThreads), 0, st_, buf, n,
                        tile_nl_.as<uint32_t>(), (unsigned long long *)d_small);
     SWAP_CHECK(scan_scratch_.reserve(scan_words * 8));
-    scan_excl<uint32_t>(tile_nl_.as<uint32_t>(), tile_off_.as<uint64_t>(), tiles, scan_scratch_.as<uint64_t>(),
-                        d_small + 1, st_);
+    SWAP_CHECK(scan_excl<uint32_t>(tile_nl_.as<uint32_t>(), tile_off_.as<uint64_t>(), tiles, scan_scratch_.as<uint64_t>(),
+                        d_small + 1, st_));
     SWAP_CHECK(hipMemcpyAsync(hs, d_small, 2 * 8, hipMemcpyDeviceToHost, st_));
     SWAP_CHECK(hipStreamSynchronize(st_));
     if (hs[0] == ~0ull) return fail(-3, "no '>' in the input");
@@ -996,10 +1050,10 @@ int Pipeline::build(const uint8_t *d_in, uint64_t n) {
     hipLaunchKernelGGL(k_records, dim3((unsigned)((R + 255) / 256)), dim3(256), 0, st_, nl_pos_.as<uint64_t>(),
                        d_small + 2, R, k_, rec_start_.as<uint64_t>(), rec_len_.as<uint32_t>(),
                        occ_count_.as<uint32_t>(), word_count_.as<uint32_t>(), (unsigned long long *)(d_small + 512));
-    scan_excl<uint32_t>(occ_count_.as<uint32_t>(), occ_base_.as<uint64_t>(), R, scan_scratch_.as<uint64_t>(),
-                        d_small + 4, st_);
-    scan_excl<uint32_t>(word_count_.as<uint32_t>(), word_base_.as<uint64_t>(), R, scan_scratch_.as<uint64_t>(),
-                        d_small + 5, st_);
+    SWAP_CHECK(scan_excl<uint32_t>(occ_count_.as<uint32_t>(), occ_base_.as<uint64_t>(), R, scan_scratch_.as<uint64_t>(),
+                        d_small + 4, st_));
+    SWAP_CHECK(scan_excl<uint32_t>(word_count_.as<uint32_t>(), word_base_.as<uint64_t>(), R, scan_scratch_.as<uint64_t>(),
+                        d_small + 5, st_));
     phase_end(0);
     SWAP_CHECK(hipMemcpyAsync(hs + 4, d_small + 4, 2 * 8, hipMemcpyDeviceToHost, st_));
     SWAP_CHECK(hipMemcpyAsync(hs + 512, d_small + 512, 512 * 8, hipMemcpyDeviceToHost, st_));
@@ -1016,7 +1070,7 @@ int Pipeline::build(const uint8_t *d_in, uint64_t n) {
     {
         const unsigned grid = (unsigned)std::min<uint64_t>((W + kPackThreads - 1) / kPackThreads, 65535);
         phase_begin(1);
-        hipLaunchKernelGGL(k_pack, dim3(grid), dim3(kPackThreads), 0, st_, buf, rec_start_.as<uint64_t>(),
+        hipLaunchKernelGGL(k_pack, dim3(grid), dim3(kPackThreads), 0, st_, buf, n, rec_start_.as<uint64_t>(),
                            rec_len_.as<uint32_t>(), word_base_.as<uint64_t>(), occ_base_.as<uint64_t>(), R, k_,
                            d_small + 5, packed_.as<uint64_t>(), wmeta_.as<uint64_t>());
         phase_end(1);
@@ -1029,7 +1083,7 @@ int Pipeline::build(const uint8_t *d_in, uint64_t n) {
     oc.k = k_;
     oc.mx = mx_;
     int ncu = 0;
-    hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev_);
+    SWAP_CHECK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev_));
     const unsigned occ_grid = (unsigned)std::max(8, ncu * 2);
     const uint64_t n1 = (uint64_t)kL1 * occ_grid;
     SWAP_CHECK(hist_.reserve((kParts + 1) * 4));   // final partition sizes, then starts (u32)
@@ -1047,10 +1101,10 @@ int Pipeline::build(const uint8_t *d_in, uint64_t n) {
                        occ_grid, kParts, hist_.as<uint32_t>());
     phase_end(2);
     SWAP_CHECK(scan_scratch_.reserve((scan_scratch_words(n1 + kParts + 1) + 16) * 8));
-    scan_excl<uint32_t>(hist_.as<uint32_t>(), bucket_.as<uint64_t>(), kParts + 1, scan_scratch_.as<uint64_t>(),
-                        d_small + 6, st_);
-    scan_excl<uint32_t>(hist1_.as<uint32_t>(), base1_.as<uint64_t>(), n1, scan_scratch_.as<uint64_t>(),
-                        d_small + 6, st_);
+    SWAP_CHECK(scan_excl<uint32_t>(hist_.as<uint32_t>(), bucket_.as<uint64_t>(), kParts + 1, scan_scratch_.as<uint64_t>(),
+                        d_small + 6, st_));
+    SWAP_CHECK(scan_excl<uint32_t>(hist1_.as<uint32_t>(), base1_.as<uint64_t>(), n1, scan_scratch_.as<uint64_t>(),
+                        d_small + 6, st_));
     // u32 partition starts for the count pass, and the level-2 cursors
     hipLaunchKernelGGL(k_u64_to_u32, dim3((kParts + 1 + 255) / 256), dim3(256), 0, st_, bucket_.as<uint64_t>(),
                        (uint64_t)kParts + 1, hist_.as<uint32_t>());
@@ -1058,30 +1112,25 @@ int Pipeline::build(const uint8_t *d_in, uint64_t n) {
     uint32_t *l1_start = l1info_.as<uint32_t>();
     uint32_t *chunk_start = l1_start + kL1 + 2;
     hipLaunchKernelGGL(k_l1_chunks, dim3(1), dim3(64), 0, st_, hist_.as<uint32_t>(), l1_start, chunk_start);
-    SWAP_CHECK(rec1_.reserve(N * 8));
-    SWAP_CHECK(rec_.reserve(N * 8));
-    if (ref_order_) {
-        SWAP_CHECK(occ1_.reserve(N * 4));
-        SWAP_CHECK(occ_.reserve(N * 4));
-    }
+    const uint64_t rec_bytes = N * 4 * (uint64_t)rec_words(ref_order_);
+    SWAP_CHECK(rec1_.reserve(rec_bytes));
+    SWAP_CHECK(rec_.reserve(rec_bytes));
     phase_begin(3);
     if (ref_order_)
         hipLaunchKernelGGL(k_part1<true>, dim3(occ_grid), dim3(kOccThreads), 0, st_, oc, base1_.as<uint64_t>(),
-                           rec1_.as<uint64_t>(), occ1_.as<uint32_t>());
+                           rec1_.as<uint32_t>());
     else
         hipLaunchKernelGGL(k_part1<false>, dim3(occ_grid), dim3(kOccThreads), 0, st_, oc, base1_.as<uint64_t>(),
-                           rec1_.as<uint64_t>(), (uint32_t *)nullptr);
+                           rec1_.as<uint32_t>());
     phase_end(3);
     const unsigned grid2 = (unsigned)(N / kChunkRecs + kL1 + 1);
     phase_begin(7);
     if (ref_order_)
-        hipLaunchKernelGGL(k_part2<true>, dim3(grid2), dim3(kL2Threads), 0, st_, rec1_.as<uint64_t>(),
-                           occ1_.as<uint32_t>(), l1_start, chunk_start, k_, cursor_.as<uint32_t>(),
-                           rec_.as<uint64_t>(), occ_.as<uint32_t>());
+        hipLaunchKernelGGL(k_part2<true>, dim3(grid2), dim3(kL2Threads), 0, st_, rec1_.as<uint32_t>(), l1_start,
+                           chunk_start, k_, cursor_.as<uint32_t>(), rec_.as<uint32_t>());
     else
-        hipLaunchKernelGGL(k_part2<false>, dim3(grid2), dim3(kL2Threads), 0, st_, rec1_.as<uint64_t>(),
-                           (const uint32_t *)nullptr, l1_start, chunk_start, k_, cursor_.as<uint32_t>(),
-                           rec_.as<uint64_t>(), (uint32_t *)nullptr);
+        hipLaunchKernelGGL(k_part2<false>, dim3(grid2), dim3(kL2Threads), 0, st_, rec1_.as<uint32_t>(), l1_start,
+                           chunk_start, k_, cursor_.as<uint32_t>(), rec_.as<uint32_t>());
     phase_end(7);
     // --- count + cutoff
     // survivors <= distinct <= N; reserve N to avoid a sync (grow-only buffers)
@@ -1093,8 +1142,7 @@ int Pipeline::build(const uint8_t *d_in, uint64_t n) {
     }
     SWAP_CHECK(overflow_.reserve(kParts * 4));
     CountArgs ca;
-    ca.rec = rec_.as<uint64_t>();
-    ca.occ = ref_order_ ? occ_.as<uint32_t>() : nullptr;
+    ca.rec = rec_.as<uint32_t>();
     ca.bucket = hist_.as<uint32_t>();
     ca.k = k_;
     ca.c = c_;
@@ -1172,7 +1220,10 @@ int Pipeline::build(const uint8_t *d_in, uint64_t n) {
         SWAP_CHECK(hipMemsetAsync(d_hist, 0, kSelBins * 4, st_));
         SWAP_CHECK(hipMemsetAsync(d_small + 12, 0, 8, st_));
         SWAP_CHECK(sel_target_.reserve(kSelBins * 4 + 64));
-        SWAP_CHECK(sel_out_.reserve(N * 4 + 16));
+        const uint32_t sel_words = (1u << shift) >= 32 ? (1u << shift) / 32 : 1;  // bitmap words per bin
+        SWAP_CHECK(sel_out_.reserve((uint64_t)32 * sel_words * 4 + 16));
+        SWAP_CHECK(hipMemsetAsync(sel_out_.p, 0, (uint64_t)32 * sel_words * 4, st_));
+        SWAP_CHECK(rows_.reserve((uint64_t)ncu * kSelBins * 4));
         // the k-mer table's epoch table: bucket counts from D, starts filled below
         epoch1_ = epoch_table(n_distinct, false, nullptr);
         SWAP_CHECK(etab_.reserve(sizeof(EpochTabHost) + 64));
@@ -1180,7 +1231,6 @@ int Pipeline::build(const uint8_t *d_in, uint64_t n) {
         SWAP_CHECK(hipMemcpyAsync(etab_.p, h_etab_, sizeof(EpochTabHost), hipMemcpyHostToDevice, st_));
         EpochTabHost *d_et = etab_.as<EpochTabHost>();
         phase_begin(6);
-        SWAP_CHECK(rows_.reserve((uint64_t)ncu * kSelBins * 4));
         hipLaunchKernelGGL(k_sel_hist, dim3(ncu), dim3(1024), 0, st_, dfo_.as<uint32_t>(),
                            (const unsigned long long *)(d_small + 9), shift, rows_.as<uint32_t>(),
                            (unsigned long long *)(d_small + 12));
@@ -1188,8 +1238,8 @@ int Pipeline::build(const uint8_t *d_in, uint64_t n) {
                            (uint32_t)ncu, kSelBins, d_hist);
         hipLaunchKernelGGL(k_sel_targets, dim3(1), dim3(1024), 0, st_, d_hist, d_starts, (int)starts.size(),
                            sel_target_.as<int>(), d_si);
-        hipLaunchKernelGGL(k_sel_collect_si, dim3(ncu * 2), dim3(1024), 0, st_, dfo_.as<uint32_t>(),
-                           (const unsigned long long *)(d_small + 9), shift, sel_target_.as<int>(), d_si,
+        hipLaunchKernelGGL(k_sel_mark, dim3(ncu * 4), dim3(1024), 0, st_, dfo_.as<uint32_t>(),
+                           (const unsigned long long *)(d_small + 9), shift, sel_target_.as<int>(),
                            sel_out_.as<uint32_t>());
         hipLaunchKernelGGL(k_sel_pick, dim3(32), dim3(1024), 0, st_, d_si, sel_out_.as<uint32_t>(), shift,
                            &d_et->start[0]);

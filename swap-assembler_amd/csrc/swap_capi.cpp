@@ -96,6 +96,10 @@ int swap_create(const swap_cfg *cfg, swap_ctx **out) {
     swap_ctx *c = new swap_ctx();
     c->cfg = *cfg;
     c->pipe.reset(new swapk::Pipeline(cfg->device, cfg->k, cfg->c, !(cfg->flags & SWAP_FLAG_HASH_ORDER)));
+    if (c->pipe->init_error() != hipSuccess) {
+        delete c;
+        return SWAP_E_DEVICE;
+    }
     *out = c;
     return SWAP_OK;
 }

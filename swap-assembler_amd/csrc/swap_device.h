@@ -104,6 +104,8 @@ public:
     const uint64_t *d_surv_mult() const { return surv_mult_.as<uint64_t>(); }
 
     const std::string &last_error() const { return err_; }
+    // hipSuccess, or the first HIP error met while setting up the context
+    hipError_t init_error() const { return init_err_; }
 
     int k() const { return k_; }
     int c() const { return c_; }
@@ -124,11 +126,12 @@ private:
     int k_, c_;
     bool ref_order_;
     hipStream_t st_ = nullptr;
+    hipError_t init_err_ = hipSuccess;
     Mixer mx_;
     std::string err_;
     // build buffers
     DevBuf fasta_, tile_nl_, tile_off_, nl_pos_, scan_scratch_, small_, rec_start_, rec_len_, occ_count_,
-        word_count_, occ_base_, word_base_, packed_, wmeta_, rows_, hist_, bucket_, cursor_, hist1_, base1_, l1info_, rec1_, occ1_, rec_, occ_, surv_key_, surv_mult_,
+        word_count_, occ_base_, word_base_, packed_, wmeta_, rows_, hist_, bucket_, cursor_, hist1_, base1_, l1info_, rec1_, rec_, surv_key_, surv_mult_,
         surv_first_, dfo_, sel_, sel_target_, sel_out_, overflow_, gtab_key_, gtab_cnt_, gtab_fo_;
     // merge buffers
     enum {

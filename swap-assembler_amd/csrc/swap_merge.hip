@@ -180,41 +180,62 @@ __global__ void k_rs_list(const uint32_t *__restrict__ rflag, const uint64_t *__
     }
 }
 
-// one thread per ruler: walk to the next ruler / kept node
-__global__ void k_rs_walk(const uint32_t *__restrict__ rlist, uint64_t nr, const uint8_t *__restrict__ is11,
-                          const uint32_t *__restrict__ nxt, const uint8_t *__restrict__ selfadj,
-                          const uint32_t *__restrict__ rid, uint64_t S, uint32_t *__restrict__ label_r,
-                          uint32_t *__restrict__ label_off, Walk *__restrict__ rw, uint8_t *__restrict__ rsp) {
+// Per oriented (1,1) node: its successor with what the walk does there, so a
+// walk step is one 8-byte load:  bits 0-31 successor, bits 32-33 kind
+// (0 pass through, 1 next ruler, 2 kept node or none), bit 34 the
+// successor's self-adjacency flag.
+__global__ void k_rs_prep(const uint8_t *__restrict__ is11, const uint32_t *__restrict__ nxt,
+                          const uint32_t *__restrict__ rflag, const uint8_t *__restrict__ selfadj, uint64_t S,
+                          uint64_t *__restrict__ nx) {
+    const uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= 2 * S || !is11[v >> 1]) return;
+    const uint32_t t = nxt[v];
+    uint64_t kind, sa = 0;
+    if (t == kNil || !is11[t >> 1]) {
+        kind = 2;
+    } else {
+        kind = rflag[t] ? 1 : 0;
+        sa = selfadj[t >> 1];
+    }
+    nx[v] = (uint64_t)t | (kind << 32) | (sa << 34);
+}
+
+// one thread per ruler: walk to the next ruler / kept node, labelling the
+// nodes passed with (ruler, offset)
+__global__ void k_rs_walk(const uint32_t *__restrict__ rlist, uint64_t nr, const uint64_t *__restrict__ nx,
+                          const uint8_t *__restrict__ selfadj, const uint32_t *__restrict__ rid, uint64_t S,
+                          uint2 *__restrict__ label, Walk *__restrict__ rw, uint8_t *__restrict__ rsp) {
     const uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= nr) return;
     uint32_t u = rlist[r];
-    uint8_t sp = selfadj[u >> 1];
+    uint32_t sp = selfadj[u >> 1];
     Walk x;
     x.succ = kNil;
     x.term = kNil;
     x.last = u;
     uint32_t steps = 1;
     for (uint64_t guard = 0; guard <= 2 * S; guard++) {
-        const uint32_t t = nxt[u];
-        if (t == kNil || !is11[t >> 1]) {  // arrives at a kept node
+        const uint64_t e = nx[u];
+        const uint32_t t = (uint32_t)e;
+        const uint32_t kind = (uint32_t)(e >> 32) & 3;
+        if (kind == 2) {  // arrives at a kept node
             x.term = t;
             x.last = u;
             break;
         }
-        if (rid[t] != kNil) {  // next ruler
+        if (kind == 1) {  // next ruler
             x.succ = rid[t];
             x.last = u;
             break;
         }
-        label_r[t] = (uint32_t)r;
-        label_off[t] = steps;
-        sp |= selfadj[t >> 1];
+        label[t] = make_uint2((uint32_t)r, steps);
+        sp |= (uint32_t)(e >> 34) & 1;
         u = t;
         steps++;
     }
     x.dist = steps;
     rw[r] = x;
-    rsp[r] = sp;
+    rsp[r] = (uint8_t)sp;
 }
 
 // pointer jumping over rulers: succ is a ruler index
@@ -238,7 +259,7 @@ __global__ void k_rs_jump(const Walk *__restrict__ win, const uint8_t *__restric
 
 // per oriented node: walk state from its ruler
 __global__ void k_rs_final(const uint8_t *__restrict__ is11, const uint32_t *__restrict__ rid,
-                           const uint32_t *__restrict__ label_r, const uint32_t *__restrict__ label_off,
+                           const uint2 *__restrict__ label,
                            const Walk *__restrict__ rw, const uint8_t *__restrict__ rsp, uint64_t S,
                            Walk *__restrict__ w, uint8_t *__restrict__ sp) {
     const uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -249,8 +270,9 @@ __global__ void k_rs_final(const uint8_t *__restrict__ is11, const uint32_t *__r
     if (is11[v >> 1]) {
         uint32_t r = rid[v], off = 0;
         if (r == kNil) {
-            r = label_r[v];
-            off = label_off[v];
+            const uint2 lb = label[v];
+            r = lb.x;
+            off = lb.y;
         }
         if (r == kNil) {  // a cycle no ruler reached: never terminates
             x.succ = (uint32_t)v;
@@ -683,17 +705,19 @@ int Pipeline::merge_device(bool reference_order, MergeDev &out) {
     MCHECK(mb_[M_FLAG].reserve(2 * S * 4));
     MCHECK(mb_[M_POS].reserve((2 * S + 1) * 8));
     MCHECK(mb_[M_NL].reserve(2 * S * 4));      // ruler id per oriented node
-    MCHECK(mb_[M_NR].reserve(2 * S * 4));      // label: ruler
-    MCHECK(mb_[M_PAR].reserve(2 * S * 4));     // label: offset
+    MCHECK(mb_[M_NR].reserve(2 * S * 8));      // label: (ruler, offset)
+    MCHECK(mb_[M_PAR].reserve(2 * S * 8));     // packed successor
     MCHECK(scan_scratch_.reserve((scan_scratch_words(2 * S + 1) + 16) * 8));
     Walk *wa = mb_[M_WALKA].as<Walk>(), *wb = mb_[M_WALKB].as<Walk>();
     uint8_t *spa = mb_[M_SPA].as<uint8_t>(), *spb = mb_[M_SPB].as<uint8_t>();
-    uint32_t *rid = mb_[M_NL].as<uint32_t>(), *lab_r = mb_[M_NR].as<uint32_t>(), *lab_o = mb_[M_PAR].as<uint32_t>();
+    uint32_t *rid = mb_[M_NL].as<uint32_t>();
+    uint2 *label = mb_[M_NR].as<uint2>();
+    uint64_t *nx = mb_[M_PAR].as<uint64_t>();
     phase_begin(9);
     hipLaunchKernelGGL(k_rs_mark, dim3(nblk(2 * S)), dim3(256), 0, st_, mb_[M_IS11].as<uint8_t>(),
                        mb_[M_NXT].as<uint32_t>(), S, mb_[M_FLAG].as<uint32_t>(), kRuleStride);
-    scan_excl<uint32_t>(mb_[M_FLAG].as<uint32_t>(), mb_[M_POS].as<uint64_t>(), 2 * S, scan_scratch_.as<uint64_t>(),
-                        d_small + 26, st_);
+    MCHECK(scan_excl<uint32_t>(mb_[M_FLAG].as<uint32_t>(), mb_[M_POS].as<uint64_t>(), 2 * S, scan_scratch_.as<uint64_t>(),
+                        d_small + 26, st_));
     MCHECK(hipMemcpyAsync(hs + 26, d_small + 26, 8, hipMemcpyDeviceToHost, st_));
     MCHECK(hipStreamSynchronize(st_));
     const uint64_t nr = hs[26];
@@ -703,12 +727,14 @@ int Pipeline::merge_device(bool reference_order, MergeDev &out) {
     uint32_t *rlist = mb_[M_ELEN].as<uint32_t>();
     Walk *rwa = mb_[M_CPRI].as<Walk>(), *rwb = rwa + nr + 1;
     uint8_t *rspa = mb_[M_CMUL].as<uint8_t>(), *rspb = rspa + nr + 1;
-    MCHECK(hipMemsetAsync(lab_r, 0xFF, 2 * S * 4, st_));
+    MCHECK(hipMemsetAsync(label, 0xFF, 2 * S * 8, st_));
+    hipLaunchKernelGGL(k_rs_prep, dim3(nblk(2 * S)), dim3(256), 0, st_, mb_[M_IS11].as<uint8_t>(),
+                       mb_[M_NXT].as<uint32_t>(), mb_[M_FLAG].as<uint32_t>(), mb_[M_SELF].as<uint8_t>(), S, nx);
     hipLaunchKernelGGL(k_rs_list, dim3(nblk(2 * S)), dim3(256), 0, st_, mb_[M_FLAG].as<uint32_t>(),
                        mb_[M_POS].as<uint64_t>(), 2 * S, rid, rlist);
     if (nr) {
-        hipLaunchKernelGGL(k_rs_walk, dim3(nblk(nr)), dim3(256), 0, st_, rlist, nr, mb_[M_IS11].as<uint8_t>(),
-                           mb_[M_NXT].as<uint32_t>(), mb_[M_SELF].as<uint8_t>(), rid, S, lab_r, lab_o, rwa, rspa);
+        hipLaunchKernelGGL(k_rs_walk, dim3(nblk(nr)), dim3(256), 0, st_, rlist, nr, nx, mb_[M_SELF].as<uint8_t>(),
+                           rid, S, label, rwa, rspa);
         int rounds = 1;
         while ((1ull << rounds) < nr + 2) rounds++;
         rounds++;
@@ -718,8 +744,8 @@ int Pipeline::merge_device(bool reference_order, MergeDev &out) {
             std::swap(rspa, rspb);
         }
     }
-    hipLaunchKernelGGL(k_rs_final, dim3(nblk(2 * S)), dim3(256), 0, st_, mb_[M_IS11].as<uint8_t>(), rid, lab_r,
-                       lab_o, rwa, rspa, S, wa, spa);
+    hipLaunchKernelGGL(k_rs_final, dim3(nblk(2 * S)), dim3(256), 0, st_, mb_[M_IS11].as<uint8_t>(), rid, label,
+                       rwa, rspa, S, wa, spa);
     phase_end(9);
     // --- kept nodes
     MCHECK(mb_[M_FLAG].reserve(S * 4));
@@ -728,8 +754,8 @@ int Pipeline::merge_device(bool reference_order, MergeDev &out) {
     phase_begin(10);
     hipLaunchKernelGGL(k_kept_flags, dim3(nblk(S)), dim3(256), 0, st_, mb_[M_IS11].as<uint8_t>(), S,
                        mb_[M_FLAG].as<uint32_t>());
-    scan_excl<uint32_t>(mb_[M_FLAG].as<uint32_t>(), mb_[M_POS].as<uint64_t>(), S, scan_scratch_.as<uint64_t>(),
-                        d_small + 20, st_);
+    MCHECK(scan_excl<uint32_t>(mb_[M_FLAG].as<uint32_t>(), mb_[M_POS].as<uint64_t>(), S, scan_scratch_.as<uint64_t>(),
+                        d_small + 20, st_));
     MCHECK(mb_[M_SPEC].reserve(S * 4));
     hipLaunchKernelGGL(k_special_flags, dim3(nblk(S)), dim3(256), 0, st_, mb_[M_IS11].as<uint8_t>(), wa, spa, S,
                        mb_[M_SPEC].as<uint32_t>(), (unsigned long long *)(d_small + 21));
@@ -770,10 +796,10 @@ int Pipeline::merge_device(bool reference_order, MergeDev &out) {
     MCHECK(mb_[M_EOFF].reserve((ne + 1) * 8));
     MCHECK(mb_[M_COFF].reserve((ne + 1) * 8));
     MCHECK(scan_scratch_.reserve((scan_scratch_words(ne + 1) + 16) * 8));
-    scan_excl<uint32_t>(mb_[M_ELEN].as<uint32_t>(), mb_[M_EOFF].as<uint64_t>(), ne, scan_scratch_.as<uint64_t>(),
-                        d_small + 22, st_);
-    scan_excl<uint32_t>(mb_[M_CSIZE].as<uint32_t>(), mb_[M_COFF].as<uint64_t>(), ne, scan_scratch_.as<uint64_t>(),
-                        d_small + 23, st_);
+    MCHECK(scan_excl<uint32_t>(mb_[M_ELEN].as<uint32_t>(), mb_[M_EOFF].as<uint64_t>(), ne, scan_scratch_.as<uint64_t>(),
+                        d_small + 22, st_));
+    MCHECK(scan_excl<uint32_t>(mb_[M_CSIZE].as<uint32_t>(), mb_[M_COFF].as<uint64_t>(), ne, scan_scratch_.as<uint64_t>(),
+                        d_small + 23, st_));
     MCHECK(hipMemcpyAsync(hs + 22, d_small + 22, 2 * 8, hipMemcpyDeviceToHost, st_));
     MCHECK(hipStreamSynchronize(st_));
     phase_end(10);

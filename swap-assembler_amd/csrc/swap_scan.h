@@ -72,23 +72,20 @@ static inline uint64_t scan_scratch_words(uint64_t n) {
     return w + 4;
 }
 
+// Returns the first launch or copy error (hipSuccess otherwise).
 template <typename TIn>
-static void scan_excl(const TIn *in, uint64_t *out, uint64_t n, uint64_t *scratch, uint64_t *d_total,
-                      hipStream_t st) {
-    if (n == 0) {
-        hipMemsetAsync(d_total, 0, 8, st);
-        return;
-    }
+static hipError_t scan_excl(const TIn *in, uint64_t *out, uint64_t n, uint64_t *scratch, uint64_t *d_total,
+                            hipStream_t st) {
+    if (n == 0) return hipMemsetAsync(d_total, 0, 8, st);
     const uint64_t nb = (n + kScanTile - 1) / kScanTile;
     uint64_t *sums = scratch;
     hipLaunchKernelGGL(k_scan_local<TIn>, dim3((unsigned)nb), dim3(kScanThreads), 0, st, in, out, n, sums);
-    if (nb == 1) {
-        hipMemcpyAsync(d_total, sums, 8, hipMemcpyDeviceToDevice, st);
-        return;
-    }
+    if (nb == 1) return hipMemcpyAsync(d_total, sums, 8, hipMemcpyDeviceToDevice, st);
     uint64_t *sum_scan = scratch + nb;
-    scan_excl<uint64_t>(sums, sum_scan, nb, scratch + 2 * nb, d_total, st);
+    const hipError_t e = scan_excl<uint64_t>(sums, sum_scan, nb, scratch + 2 * nb, d_total, st);
+    if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_scan_add, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, out, n, sum_scan);
+    return hipGetLastError();
 }
 
 }  // namespace swapk

@@ -228,7 +228,9 @@ def hip():
         H.hipDeviceSynchronize.argtypes = []
         H.hipSetDevice.argtypes = [ctypes.c_int]
         H.hipGetDeviceCount.argtypes = [ctypes.POINTER(ctypes.c_int)]
-        for f in ("hipMalloc", "hipFree", "hipMemcpy", "hipDeviceSynchronize", "hipSetDevice", "hipGetDeviceCount"):
+        H.hipMemset.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_size_t]
+        for f in ("hipMalloc", "hipFree", "hipMemcpy", "hipDeviceSynchronize", "hipSetDevice", "hipGetDeviceCount",
+                  "hipMemset"):
             getattr(H, f).restype = ctypes.c_int
         _hip = H
     return _hip
