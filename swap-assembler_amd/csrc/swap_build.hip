@@ -578,10 +578,21 @@ __device__ __forceinline__ bool table_add16(uint64_t *tkey, uint32_t *tc, uint32
             cur = (prev == kEmpty) ? key : prev;
         }
         if (cur == key) {
-            while (m) {
-                const int t = __ffs(m) - 1;
+            // an occurrence has at most two edge bits (one per side): straight-line adds
+            if (m) {
+                const int t0 = __ffs(m) - 1;
+                atomicAdd(&tc[4 * s + (t0 >> 1)], 1u << (16 * (t0 & 1)));
                 m &= m - 1;
-                atomicAdd(&tc[4 * s + (t >> 1)], 1u << (16 * (t & 1)));
+                if (m) {
+                    const int t1 = __ffs(m) - 1;
+                    atomicAdd(&tc[4 * s + (t1 >> 1)], 1u << (16 * (t1 & 1)));
+                    m &= m - 1;
+                    while (m) {  // not produced by the occurrence pass; kept for completeness
+                        const int t = __ffs(m) - 1;
+                        m &= m - 1;
+                        atomicAdd(&tc[4 * s + (t >> 1)], 1u << (16 * (t & 1)));
+                    }
+                }
             }
             if (WITH_OCC) atomicMin(&tfo[s], occ);
             return true;
@@ -691,10 +702,18 @@ __global__ void __launch_bounds__(kCountThreads) k_count_lds(CountArgs a) {
     __syncthreads();
     bool ok = true;
     if (small) {
-        for (uint32_t i = b + threadIdx.x; i < e && ok; i += blockDim.x) {
-            uint32_t o;
-            const uint64_t r = rec_load<WITH_OCC>(a.rec, i, o);
-            ok = table_add16<WITH_OCC>(tkey, tc, tfo, kTab - 1, r, o);
+        // two records in flight per thread: each load is issued one table
+        // update ahead of its use
+        const uint64_t st = blockDim.x, end = e;
+        uint64_t i = b + threadIdx.x;
+        uint32_t oa = 0, ob = 0;
+        uint64_t ra = i < end ? rec_load<WITH_OCC>(a.rec, i, oa) : 0;
+        uint64_t rb = i + st < end ? rec_load<WITH_OCC>(a.rec, i + st, ob) : 0;
+        for (; i < end && ok; i += 2 * st) {
+            ok = table_add16<WITH_OCC>(tkey, tc, tfo, kTab - 1, ra, oa);
+            if (i + 2 * st < end) ra = rec_load<WITH_OCC>(a.rec, i + 2 * st, oa);
+            if (i + st < end && ok) ok = table_add16<WITH_OCC>(tkey, tc, tfo, kTab - 1, rb, ob);
+            if (i + 3 * st < end) rb = rec_load<WITH_OCC>(a.rec, i + 3 * st, ob);
         }
     } else {
         uint64_t *tcnt = reinterpret_cast<uint64_t *>(tc);

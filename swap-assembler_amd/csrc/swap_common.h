@@ -12,7 +12,7 @@ namespace swapk {
 
 // Partition count for device-local hash partitioning: 2^kPartBits buckets keyed
 // by the top bits of a bijective mix of the canonical k-mer.
-constexpr int kPartBits = 14;
+constexpr int kPartBits = 15;
 constexpr uint32_t kParts = 1u << kPartBits;
 
 SWAP_HD uint64_t kmask(int k) { return (k >= 32) ? ~0ull : ((1ull << (2 * k)) - 1); }
