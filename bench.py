@@ -37,21 +37,55 @@ PHASES = {0: "record_scan", 1: "pack_2bit", 2: "partition_hist", 3: "partition_l
           14: "classify"}
 
 
+# phase index -> device kernel of that phase whose hardware byte counters are kept
+# in profiles/traffic.json (written from rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE)
+PHASE_KERNEL = {4: "swapk::k_count_lds<%s>", 3: "swapk::k_part1<%s>", 7: "swapk::k_part2<%s>",
+                2: "swapk::k_part_hist", 1: "swapk::k_pack"}
+
+
+def measured_traffic(phase, ref_order, workload):
+    """HBM bytes per launch of the phase's kernel from the committed counter summary, or None."""
+    path = os.path.join(REPO, "profiles", "traffic.json")
+    if phase not in PHASE_KERNEL or not os.path.exists(path):
+        return None
+    with open(path) as f:
+        t = json.load(f)
+    if t.get("workload") != workload:
+        return None
+    name = PHASE_KERNEL[phase]
+    if "%s" in name:
+        name = name % ("true" if ref_order else "false")
+    k = t.get("kernels", {}).get(name)
+    return k["bytes"] if k else None
+
+
 def algorithmic_bytes(phase, st, words, ref_order):
-    """Words a phase must move once (reads + writes), per launch."""
+    """Bytes a phase's kernel must read and write once, per launch.
+
+    Records are 8 bytes (key bits + edge mask), plus a 4-byte input index in
+    reference order; a packed word comes with an 8-byte occurrence descriptor.
+    """
     N, S, D, R = st["kmers"], st["survivors"], st["distinct"], st["reads"]
     occ = 4 if ref_order else 0
-    if phase == 3:   # level-1 partition: packed reads twice (count + place passes), records out
-        return 2 * 8 * words + 2 * 16 * R + N * (8 + occ)
+    if phase == 3:   # level-1 partition: packed words + descriptors in, records out
+        return 16 * words + N * (8 + occ)
     if phase == 7:   # level-2 partition: records in, records out
         return 2 * N * (8 + occ)
-    if phase == 4:   # LDS count: records in, survivors out, first-occurrence bits
+    if phase == 4:   # LDS count: records in, survivors out, first occurrence of every distinct key
         return N * (8 + occ) + S * (8 + 8 + occ) + (4 * D if ref_order else 0)
-    if phase == 2:   # partition histogram: packed reads once
-        return 8 * words + 16 * R
-    if phase == 1:   # 2-bit pack: sequence bytes in, packed words out
-        return 100 * R + 8 * words
+    if phase == 2:   # partition histogram: packed words + descriptors in
+        return 16 * words
+    if phase == 1:   # 2-bit pack: sequence bytes in, packed words + descriptors out
+        return 100 * R + 16 * words
     return 0
+
+
+def path_bytes(st, fasta_bytes, read_len, k, edge_bases):
+    """Whole-path algorithmic bytes of the build+merge (word-granular unique traffic):
+    F + 8N + 8E + 48D + 16S (build) + 16S + edge bases (merge), E = 2(L-k) per read."""
+    N, S, D, R = st["kmers"], st["survivors"], st["distinct"], st["reads"]
+    E = 2 * R * max(read_len - k, 0)
+    return fasta_bytes + 8 * N + 8 * E + 48 * D + 16 * S + 16 * S + edge_bases
 
 
 def sample_prefix(fa: bytes, frac: float) -> bytes:
@@ -163,6 +197,64 @@ def copy_probe(nbytes):
     H.hipFree(b)
 
 
+def batched_vs_per_sample(n_samples, slots, warmup=True):
+    """Metric 2: the many-sample workload (seeded genomes log-uniform in
+    [20 kbp, 2 Mbp], 150 bp reads at 20-80x, 0.5% errors, k=31, c=4), inputs
+    resident in HBM, noCEcontigs.fasta produced for every sample.  Plain path:
+    one context assembling the samples one after another.  Batched path: one
+    server with `slots` concurrent contexts.  Returns samples/s of both and
+    whether every sample's output is byte-identical between them."""
+    import concurrent.futures as cf
+    import swapamd
+    with cf.ThreadPoolExecutor(max_workers=8) as ex:
+        gen = list(ex.map(lambda i: swapamd.generate("batch", i), range(n_samples)))
+    fas = [g[0] for g in gen]
+    k, c = gen[0][1], gen[0][2]
+    bufs = [swapamd.DeviceBuffer(fa) for fa in fas]
+    order = sorted(range(n_samples), key=lambda i: -len(fas[i]))
+    noce = swapamd.SWAP_FILE_NOCE
+
+    a = swapamd.Assembler(k, c)
+    if warmup:  # grow the context's buffers to the largest sample once
+        a.build_device(bufs[order[0]].ptr.value, len(fas[order[0]]))
+        a.merge()
+    swapamd.hip().hipDeviceSynchronize()
+    t0 = time.perf_counter()
+    plain, kmers = [None] * n_samples, 0
+    for i in range(n_samples):
+        a.build_device(bufs[i].ptr.value, len(fas[i]))
+        a.merge()
+        plain[i] = a.file_bytes(noce)
+        kmers += a.stats()["kmers"]
+    swapamd.hip().hipDeviceSynchronize()
+    t_plain = time.perf_counter() - t0
+    a.close()
+
+    srv = swapamd.Server(slots=slots)
+    if warmup:  # grow every slot's buffers with the largest samples once
+        ids = srv.submit_batch([swapamd.Server.job(len(fas[i]), k, c, outputs=(noce,), device_ptr=bufs[i].ptr.value)
+                                for i in order[:slots]])
+        srv.wait_all()
+        for j in ids:
+            srv.release(j)
+    jobs = [swapamd.Server.job(len(fas[i]), k, c, outputs=(noce,), device_ptr=bufs[i].ptr.value)
+            for i in range(n_samples)]
+    swapamd.hip().hipDeviceSynchronize()
+    t0 = time.perf_counter()
+    ids = srv.submit_batch(jobs)
+    srv.wait_all()
+    t_batch = time.perf_counter() - t0
+    same = all(srv.output(ids[i], noce) == plain[i] for i in range(n_samples))
+    srv.close()
+    for b in bufs:
+        b.close()
+    return {"workload": "batch", "samples": n_samples, "input_bytes": sum(len(f) for f in fas),
+            "input_kmers": kmers, "k": k, "c": c, "slots": slots,
+            "per_sample_samples_per_s": n_samples / t_plain, "batched_samples_per_s": n_samples / t_batch,
+            "factor": t_plain / t_batch, "per_sample_s": t_plain, "batched_s": t_batch,
+            "identical_outputs": same}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -171,6 +263,9 @@ def main():
     ap.add_argument("--workload", default="bacterial")
     ap.add_argument("--hash-order", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--batch-samples", type=int, default=256,
+                    help="samples of the many-sample workload for the batched-vs-per-sample figure (0 skips it)")
+    ap.add_argument("--batch-slots", type=int, default=4)
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="first run one device-to-device copy of MIB MiB (a known-size kernel that calibrates "
                          "hardware byte counters)")
@@ -227,10 +322,25 @@ def main():
     achieved = alg / (phases[dom] * 1e-3) if alg else None
     roof = {"bound": "hbm", "kernel": PHASES[dom], "achieved": achieved / 1e9 if achieved else None,
             "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": (achieved / HBM_PEAK) if achieved else None,
-            "traffic": None, "algorithmic_bytes": alg, "kernel_ms": phases[dom]}
+            "traffic": measured_traffic(dom, ref_order, args.workload), "algorithmic_bytes": alg,
+            "kernel_ms": phases[dom]}
+    # whole-path figure: edge-string bases measured as the bases of the contig file
+    edge_bases = 0
+    if has_merge:
+        noce = a.file_bytes(swapamd.SWAP_FILE_NOCE)
+        edge_bases = len(noce) - noce.count(b"\n") * len(b">proc0: ")
+    read_len = len(fa.split(b"\n", 2)[1])
+    pb = path_bytes(st, len(fa), read_len, k, edge_bases)
+    roof["path"] = {"algorithmic_bytes": pb, "achieved": pb / (dt / args.steps) / 1e9, "unit": "GB/s",
+                    "frac": pb / (dt / args.steps) / HBM_PEAK}
     cpu = None
     if not args.no_cpu_baseline:
         cpu = reference_baseline(fa, k, c) or cpu_baseline(fa, k, c)
+    batched = None
+    if args.batch_samples > 0:
+        a.close()
+        dev.close()
+        batched = batched_vs_per_sample(args.batch_samples, args.batch_slots)
     line = {
         "metric": "input k-mers/s (build+merge) at 4.6Mbp/50x",
         "value": value,
@@ -251,6 +361,7 @@ def main():
         "cpu_baseline": cpu,
         "stats": {kk: st[kk] for kk in ("kmers", "distinct", "survivors", "kept", "overflow", "special")},
         "phase_ms": {PHASES[i]: round(v, 4) for i, v in phases.items()},
+        "batched": batched,
     }
     print(json.dumps(line))
     a.close()

@@ -24,6 +24,7 @@
  *   swap_export_kmer_graph       the kmolecules table after cutoffGraph
  *                                (graph.cpp:353, 1112)
  *   swap_export_compacted        the nodes table after the merge (graph.cpp:354)
+ *   swap_reconfigure, swap_server_*  no counterpart (many samples per process)
  *   swap_format_file /           printKmerGraph (graph.cpp:3352-3440),
  *   swap_write_file              gatherDistNodeGraph + the contigGraph.txt write
  *                                (graph.cpp:576-696, 3617-3622), printContigs
@@ -137,6 +138,71 @@ int64_t swap_table_epochs(uint64_t n_total, int extra_resize, uint64_t *starts, 
 
 const char *swap_strerror(int code);
 const char *swap_last_error(const swap_ctx *ctx);
+
+/* Re-targets a context to a new k, c and flags for its next build, keeping its
+ * device buffers (same device).  No counterpart in the reference (one run =
+ * one parameter set); used by callers that assemble many samples. */
+int swap_reconfigure(swap_ctx *ctx, const swap_cfg *cfg);
+
+/* The compacted graph after swap_merge (the reference's nodes table after the
+ * merge, graph.cpp:354): n kept nodes, in contigGraph.txt order.  Node i has key
+ * keys[i]; its edge slot t (0..7, as in swap_export_kmer_graph) is the string
+ * pool[edge_off[8*i+t] .. + edge_len[8*i+t]) of 'A','C','G','T' (length 0: no
+ * edge) with merged multiplicity edge_mult[8*i+t].  *pool_len receives the pool
+ * size.  Returns n; NULL buffers query the sizes. */
+int64_t swap_export_compacted(swap_ctx *ctx, uint64_t *keys, uint64_t *edge_off, uint32_t *edge_len,
+                              uint8_t *edge_mult, size_t cap_nodes, char *pool, size_t cap_pool,
+                              uint64_t *pool_len);
+
+/* ---------------------------------------------------------------------------
+ * Many assemblies on one GPU (no counterpart in the reference, which assembles
+ * one sample per run).  A server runs jobs on `slots` concurrent contexts (one
+ * host thread and one HIP stream each; their kernels overlap on the device).
+ * A job starts, in submission order, when a slot is free and the estimated
+ * device footprint of the running jobs plus its own fits hbm_budget (bytes;
+ * 0 = 80% of the free device memory); a job alone always starts.  Every job's
+ * outputs are byte-identical to running it alone on a fresh context. */
+#define SWAP_JOB_QUEUED 1
+#define SWAP_JOB_RUNNING 2
+#define SWAP_JOB_DEVICE_INPUT 1u /* swap_job.job_flags: fasta is device memory on the server's device */
+
+typedef struct swap_job {
+    const void *fasta;  /* FASTA bytes; must stay valid until the job ends */
+    size_t n;
+    int k, c;
+    unsigned flags;     /* SWAP_FLAG_* */
+    unsigned outputs;   /* bit (1u << SWAP_FILE_*) per file to keep: kmerGraph.txt only with
+                           SWAP_FLAG_KMER_GRAPH, the others only without it */
+    unsigned job_flags; /* SWAP_JOB_* */
+} swap_job;
+
+typedef struct swap_job_info {
+    int status;        /* SWAP_JOB_QUEUED, SWAP_JOB_RUNNING, 0 done, or a negative code */
+    double submit_ms;  /* times since the server was created */
+    double start_ms;
+    double end_ms;
+    swap_stats stats;  /* of the job's build and merge */
+} swap_job_info;
+
+typedef struct swap_server swap_server;
+
+int swap_server_create(int device, int slots, size_t hbm_budget, swap_server **out);
+/* Queues a job; returns its id (>= 0) or a negative code. */
+int64_t swap_server_submit(swap_server *s, const swap_job *job);
+/* Queues n jobs, largest input first; ids[i] receives the id of jobs[i]. */
+int swap_server_submit_batch(swap_server *s, const swap_job *jobs, size_t n, int64_t *ids);
+/* Blocks until the job has ended; returns its status (0 or a negative code). */
+int swap_server_wait(swap_server *s, int64_t id);
+/* Blocks until every queued job has ended; returns 0 or the first job error. */
+int swap_server_wait_all(swap_server *s);
+int swap_server_job_info(swap_server *s, int64_t id, swap_job_info *info);
+const char *swap_server_job_error(swap_server *s, int64_t id);
+/* Bytes of output file `which` of an ended job (buf NULL: size query). */
+int64_t swap_server_output(swap_server *s, int64_t id, int which, char *buf, size_t cap);
+/* Frees the outputs kept for an ended job. */
+int swap_server_release(swap_server *s, int64_t id);
+/* Finishes the queued and running jobs, then frees the server. */
+void swap_server_destroy(swap_server *s);
 
 #ifdef __cplusplus
 }

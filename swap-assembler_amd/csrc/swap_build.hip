@@ -967,6 +967,14 @@ Pipeline::Pipeline(int device, int k, int c, bool reference_order) : dev_(device
     keep(hipHostMalloc((void **)&h_starts_, 64 * sizeof(uint64_t), hipHostMallocDefault));
     keep(hipHostMalloc((void **)&h_etab_, sizeof(EpochTabHost), hipHostMallocDefault));
     keep(hipHostMalloc((void **)&h_etabS_, sizeof(EpochTabHost), hipHostMallocDefault));
+    reconfigure(k, c, reference_order);
+    for (auto &e : ev_) keep(hipEventCreate(&e));
+}
+
+void Pipeline::reconfigure(int k, int c, bool reference_order) {
+    k_ = k;
+    c_ = c;
+    ref_order_ = reference_order;
     mx_.bits = 2 * k;
     mx_.m = kmask(k);
     mx_.a1 = 0x9E3779B97F4A7C15ull | 1;
@@ -974,7 +982,7 @@ Pipeline::Pipeline(int device, int k, int c, bool reference_order) : dev_(device
     mx_.i1 = inv_odd(mx_.a1);
     mx_.i2 = inv_odd(mx_.a2);
     mx_.s = k;  // half of the 2k bits
-    for (auto &e : ev_) keep(hipEventCreate(&e));
+    n_reads = n_occ = n_distinct = n_surv = n_kept = n_overflow = n_special = 0;
 }
 
 // Phase timing is diagnostic: a failed event call leaves that phase at 0 ms.

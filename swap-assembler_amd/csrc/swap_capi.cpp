@@ -81,6 +81,37 @@ void kmer_graph_order(swap_ctx *c, std::vector<uint32_t> &order) {
 
 }  // namespace
 
+namespace swapk {
+
+// Output file bytes of a context (the body of swap_format_file, without the
+// caller-buffer copy).
+int format_file_string(swap_ctx *c, int which, std::string &out) {
+    out.clear();
+    if (!c) return SWAP_E_INVAL;
+    int rc = 0;
+    if (which == SWAP_FILE_KMER_GRAPH) {
+        rc = ensure_graph(c);
+        if (rc) return rc;
+        std::vector<uint32_t> order;
+        kmer_graph_order(c, order);
+        format_kmer_graph(c->cfg.k, c->graph, order, out);
+    } else if (which == SWAP_FILE_CONTIG_GRAPH || which == SWAP_FILE_NOCE) {
+        if (!c->merged) return set_err(c, SWAP_E_STATE, "no merge");
+        if (!c->have_compacted) {
+            rc = materialize_merge(*c->pipe, c->cfg, c->comp);
+            if (rc) return set_err(c, rc, c->pipe->last_error());
+            c->have_compacted = true;
+        }
+        if (which == SWAP_FILE_CONTIG_GRAPH) format_contig_graph(c->comp, out);
+        else format_noce(c->cfg.k, c->comp, out);
+    } else {
+        return set_err(c, SWAP_E_INVAL, "unknown file");
+    }
+    return 0;
+}
+
+}  // namespace swapk
+
 extern "C" {
 
 int swap_create(const swap_cfg *cfg, swap_ctx **out) {
@@ -173,25 +204,8 @@ int64_t swap_export_kmer_graph(swap_ctx *c, uint64_t *keys, uint8_t *mult, size_
 int64_t swap_format_file(swap_ctx *c, int which, char *buf, size_t cap) {
     if (!c) return SWAP_E_INVAL;
     std::string out;
-    int rc = 0;
-    if (which == SWAP_FILE_KMER_GRAPH) {
-        rc = ensure_graph(c);
-        if (rc) return rc;
-        std::vector<uint32_t> order;
-        kmer_graph_order(c, order);
-        swapk::format_kmer_graph(c->cfg.k, c->graph, order, out);
-    } else if (which == SWAP_FILE_CONTIG_GRAPH || which == SWAP_FILE_NOCE) {
-        if (!c->merged) return set_err(c, SWAP_E_STATE, "no merge");
-        if (!c->have_compacted) {
-            rc = swapk::materialize_merge(*c->pipe, c->cfg, c->comp);
-            if (rc) return set_err(c, rc, c->pipe->last_error());
-            c->have_compacted = true;
-        }
-        if (which == SWAP_FILE_CONTIG_GRAPH) swapk::format_contig_graph(c->comp, out);
-        else swapk::format_noce(c->cfg.k, c->comp, out);
-    } else {
-        return set_err(c, SWAP_E_INVAL, "unknown file");
-    }
+    const int rc = swapk::format_file_string(c, which, out);
+    if (rc) return rc;
     if (buf) {
         if (cap < out.size()) return set_err(c, SWAP_E_RANGE, "buffer too small");
         std::memcpy(buf, out.data(), out.size());
@@ -201,16 +215,48 @@ int64_t swap_format_file(swap_ctx *c, int which, char *buf, size_t cap) {
 
 int64_t swap_write_file(swap_ctx *c, int which, const char *path) {
     if (!c || !path) return SWAP_E_INVAL;
-    int64_t n = swap_format_file(c, which, nullptr, 0);
-    if (n < 0) return n;
-    std::vector<char> b((size_t)n);
-    n = swap_format_file(c, which, b.data(), b.size());
-    if (n < 0) return n;
+    std::string b;
+    const int rc = swapk::format_file_string(c, which, b);
+    if (rc) return rc;
     FILE *f = std::fopen(path, "wb");
     if (!f) return set_err(c, SWAP_E_IO, std::string("cannot open ") + path);
     const bool ok = std::fwrite(b.data(), 1, b.size(), f) == b.size();
     std::fclose(f);
-    return ok ? n : set_err(c, SWAP_E_IO, std::string("short write to ") + path);
+    return ok ? (int64_t)b.size() : set_err(c, SWAP_E_IO, std::string("short write to ") + path);
+}
+
+int swap_reconfigure(swap_ctx *c, const swap_cfg *cfg) {
+    if (!c || !cfg) return SWAP_E_INVAL;
+    if (cfg->k < 9 || cfg->k > 31 || (cfg->k % 2) == 0) return SWAP_E_INVAL;
+    if (cfg->device != c->cfg.device) return set_err(c, SWAP_E_INVAL, "a context stays on its device");
+    c->cfg = *cfg;
+    c->pipe->reconfigure(cfg->k, cfg->c, !(cfg->flags & SWAP_FLAG_HASH_ORDER));
+    c->built = c->merged = c->have_graph = c->have_compacted = false;
+    return SWAP_OK;
+}
+
+int64_t swap_export_compacted(swap_ctx *c, uint64_t *keys, uint64_t *edge_off, uint32_t *edge_len,
+                              uint8_t *edge_mult, size_t cap_nodes, char *pool, size_t cap_pool,
+                              uint64_t *pool_len) {
+    if (!c) return SWAP_E_INVAL;
+    if (!c->merged) return set_err(c, SWAP_E_STATE, "no merge");
+    if (!c->have_compacted) {
+        const int rc = swapk::materialize_merge(*c->pipe, c->cfg, c->comp);
+        if (rc) return set_err(c, rc, c->pipe->last_error());
+        c->have_compacted = true;
+    }
+    const swapk::HostCompacted &g = c->comp;
+    const size_t n = g.key.size();
+    if (pool_len) *pool_len = g.pool.size();
+    const bool want_nodes = keys || edge_off || edge_len || edge_mult;
+    if (want_nodes && cap_nodes < n) return set_err(c, SWAP_E_RANGE, "node buffers too small");
+    if (pool && cap_pool < g.pool.size()) return set_err(c, SWAP_E_RANGE, "pool buffer too small");
+    if (keys) std::memcpy(keys, g.key.data(), n * 8);
+    if (edge_off) std::memcpy(edge_off, g.off.data(), 8 * n * 8);
+    if (edge_len) std::memcpy(edge_len, g.len.data(), 8 * n * 4);
+    if (edge_mult) std::memcpy(edge_mult, g.mult.data(), 8 * n);
+    if (pool && !g.pool.empty()) std::memcpy(pool, g.pool.data(), g.pool.size());
+    return (int64_t)n;
 }
 
 int64_t swap_table_order(const uint64_t *keys, const uint64_t *ranks, size_t n, uint64_t n_total, int extra_resize,

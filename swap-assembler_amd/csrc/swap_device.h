@@ -77,6 +77,8 @@ class Pipeline {
 public:
     Pipeline(int device, int k, int c, bool reference_order);
     ~Pipeline();
+    // New k, c and order mode for the next build; device buffers are kept.
+    void reconfigure(int k, int c, bool reference_order);
     int device() const { return dev_; }
     hipStream_t stream() const { return st_; }
 

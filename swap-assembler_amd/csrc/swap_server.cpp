@@ -1,0 +1,272 @@
+// Many assemblies on one GPU: a job queue served by a fixed number of slots.
+//
+// Each slot is a host thread that owns one context (its own HIP stream and
+// grow-only device buffers) and runs whole assemblies on it: build, merge,
+// output formatting.  The slots' kernels overlap on the device and each slot's
+// host work (order replay, writers) overlaps the others' device work, so
+// small, uneven samples keep the GPU busy where one context alone would leave
+// it idle between launches and host synchronisations.
+//
+// Admission: a job starts when a slot is free and the estimated device
+// footprint of the running jobs plus its own fits the HBM budget (a job alone
+// is always admitted).  Jobs start in submission order.  Results are the same
+// as running each job alone on a fresh context: a context keeps no state from
+// one job to the next except buffer capacity.
+//
+// The reference assembles one sample per run and has no counterpart of this
+// module.
+#include "swap_hip.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <memory>
+#include <mutex>
+#include <numeric>
+#include <string>
+#include <thread>
+#include <vector>
+
+namespace swapk {
+int format_file_string(swap_ctx *c, int which, std::string &out);
+}
+
+namespace {
+
+using clk = std::chrono::steady_clock;
+
+// Device bytes per FASTA byte a running job may hold: two 12-byte occurrence
+// records per input k-mer (about one k-mer per input byte for short-read
+// FASTA), survivor arrays reserved at the occurrence count, the packed input
+// and the merge arrays.
+constexpr double kBytesPerInputByte = 36.0;
+
+struct JobRec {
+    swap_job spec{};
+    std::string files[3];
+    int status = SWAP_JOB_QUEUED;
+    std::string err;
+    swap_stats stats{};
+    double t_submit = 0, t_start = 0, t_end = 0;
+    size_t est = 0;
+};
+
+}  // namespace
+
+struct swap_server {
+    int device = 0;
+    int slots = 1;
+    size_t budget = 0;
+    std::mutex mu;
+    std::condition_variable cv_work, cv_done;
+    std::deque<int64_t> queue;
+    std::vector<std::unique_ptr<JobRec>> jobs;
+    size_t in_flight = 0;
+    int running = 0;
+    bool stopping = false;
+    std::vector<std::thread> workers;
+    clk::time_point t0 = clk::now();
+
+    double now_ms() const { return std::chrono::duration<double, std::milli>(clk::now() - t0).count(); }
+};
+
+namespace {
+
+int run_job(swap_server *s, swap_ctx *&ctx, JobRec &j, std::string &err) {
+    swap_cfg cfg;
+    cfg.k = j.spec.k;
+    cfg.c = j.spec.c;
+    cfg.device = s->device;
+    cfg.flags = j.spec.flags;
+    int rc = ctx ? swap_reconfigure(ctx, &cfg) : swap_create(&cfg, &ctx);
+    if (rc) {
+        err = ctx ? swap_last_error(ctx) : swap_strerror(rc);
+        return rc;
+    }
+    if (j.spec.job_flags & SWAP_JOB_DEVICE_INPUT)
+        rc = swap_build_device(ctx, j.spec.fasta, j.spec.n);
+    else
+        rc = swap_build(ctx, static_cast<const char *>(j.spec.fasta), j.spec.n);
+    if (rc == SWAP_OK && !(j.spec.flags & SWAP_FLAG_KMER_GRAPH)) rc = swap_merge(ctx);
+    for (int w = 0; w < 3 && rc == SWAP_OK; w++)
+        if (j.spec.outputs & (1u << w)) rc = swapk::format_file_string(ctx, w, j.files[w]);
+    if (rc == SWAP_OK) rc = swap_get_stats(ctx, &j.stats);
+    if (rc) err = swap_last_error(ctx);
+    return rc;
+}
+
+void worker(swap_server *s) {
+    swap_ctx *ctx = nullptr;
+    std::unique_lock<std::mutex> lk(s->mu);
+    for (;;) {
+        s->cv_work.wait(lk, [&] {
+            if (s->queue.empty()) return s->stopping;
+            const JobRec &j = *s->jobs[s->queue.front()];
+            return s->running == 0 || s->in_flight + j.est <= s->budget;
+        });
+        if (s->queue.empty()) break;
+        const int64_t id = s->queue.front();
+        s->queue.pop_front();
+        JobRec &j = *s->jobs[id];
+        j.status = SWAP_JOB_RUNNING;
+        j.t_start = s->now_ms();
+        s->in_flight += j.est;
+        s->running++;
+        lk.unlock();
+        std::string err;
+        const int rc = run_job(s, ctx, j, err);
+        lk.lock();
+        j.t_end = s->now_ms();
+        j.err = err;
+        j.status = rc;
+        s->in_flight -= j.est;
+        s->running--;
+        s->cv_done.notify_all();
+        s->cv_work.notify_all();
+    }
+    lk.unlock();
+    if (ctx) swap_destroy(ctx);
+}
+
+}  // namespace
+
+extern "C" {
+
+int swap_server_create(int device, int slots, size_t hbm_budget, swap_server **out) {
+    if (!out || slots < 1 || slots > 64) return SWAP_E_INVAL;
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return SWAP_E_NOGPU;
+    if (device < 0 || device >= ndev) return SWAP_E_INVAL;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) != hipSuccess) return SWAP_E_NOGPU;
+    if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0) return SWAP_E_NOGPU;
+    if (hbm_budget == 0) {
+        size_t free_b = 0, total_b = 0;
+        if (hipSetDevice(device) != hipSuccess || hipMemGetInfo(&free_b, &total_b) != hipSuccess)
+            return SWAP_E_DEVICE;
+        hbm_budget = free_b / 10 * 8;
+    }
+    swap_server *s = new swap_server();
+    s->device = device;
+    s->slots = slots;
+    s->budget = hbm_budget;
+    for (int i = 0; i < slots; i++) s->workers.emplace_back(worker, s);
+    *out = s;
+    return SWAP_OK;
+}
+
+int64_t swap_server_submit(swap_server *s, const swap_job *job) {
+    if (!s || !job || (!job->fasta && job->n)) return SWAP_E_INVAL;
+    if (job->k < 9 || job->k > 31 || (job->k % 2) == 0) return SWAP_E_INVAL;
+    if (job->flags & SWAP_FLAG_KMER_GRAPH) {
+        if (job->outputs & ~(1u << SWAP_FILE_KMER_GRAPH)) return SWAP_E_INVAL;
+    } else if (job->outputs & (1u << SWAP_FILE_KMER_GRAPH)) {
+        return SWAP_E_INVAL;
+    }
+    auto j = std::make_unique<JobRec>();
+    j->spec = *job;
+    j->est = (size_t)(kBytesPerInputByte * (double)job->n);
+    std::lock_guard<std::mutex> lk(s->mu);
+    if (s->stopping) return SWAP_E_STATE;
+    j->t_submit = s->now_ms();
+    const int64_t id = (int64_t)s->jobs.size();
+    s->jobs.push_back(std::move(j));
+    s->queue.push_back(id);
+    s->cv_work.notify_one();
+    return id;
+}
+
+int swap_server_submit_batch(swap_server *s, const swap_job *jobs, size_t n, int64_t *ids) {
+    if (!s || (!jobs && n) || (!ids && n)) return SWAP_E_INVAL;
+    // largest input first: the long jobs start early and the short ones fill in
+    std::vector<size_t> order(n);
+    std::iota(order.begin(), order.end(), (size_t)0);
+    std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) { return jobs[a].n > jobs[b].n; });
+    for (size_t i : order) {
+        const int64_t id = swap_server_submit(s, &jobs[i]);
+        if (id < 0) return (int)id;
+        ids[i] = id;
+    }
+    return SWAP_OK;
+}
+
+int swap_server_wait(swap_server *s, int64_t id) {
+    if (!s) return SWAP_E_INVAL;
+    std::unique_lock<std::mutex> lk(s->mu);
+    if (id < 0 || id >= (int64_t)s->jobs.size()) return SWAP_E_INVAL;
+    JobRec &j = *s->jobs[id];
+    s->cv_done.wait(lk, [&] { return j.status != SWAP_JOB_QUEUED && j.status != SWAP_JOB_RUNNING; });
+    return j.status;
+}
+
+int swap_server_wait_all(swap_server *s) {
+    if (!s) return SWAP_E_INVAL;
+    std::unique_lock<std::mutex> lk(s->mu);
+    s->cv_done.wait(lk, [&] { return s->queue.empty() && s->running == 0; });
+    for (const auto &j : s->jobs)
+        if (j->status < 0) return j->status;
+    return SWAP_OK;
+}
+
+int swap_server_job_info(swap_server *s, int64_t id, swap_job_info *info) {
+    if (!s || !info) return SWAP_E_INVAL;
+    std::lock_guard<std::mutex> lk(s->mu);
+    if (id < 0 || id >= (int64_t)s->jobs.size()) return SWAP_E_INVAL;
+    const JobRec &j = *s->jobs[id];
+    info->status = j.status;
+    info->submit_ms = j.t_submit;
+    info->start_ms = j.t_start;
+    info->end_ms = j.t_end;
+    info->stats = j.stats;
+    return SWAP_OK;
+}
+
+const char *swap_server_job_error(swap_server *s, int64_t id) {
+    if (!s) return "";
+    std::lock_guard<std::mutex> lk(s->mu);
+    if (id < 0 || id >= (int64_t)s->jobs.size()) return "";
+    return s->jobs[id]->err.c_str();
+}
+
+int64_t swap_server_output(swap_server *s, int64_t id, int which, char *buf, size_t cap) {
+    if (!s || which < 0 || which > 2) return SWAP_E_INVAL;
+    std::lock_guard<std::mutex> lk(s->mu);
+    if (id < 0 || id >= (int64_t)s->jobs.size()) return SWAP_E_INVAL;
+    const JobRec &j = *s->jobs[id];
+    if (j.status != SWAP_OK) return j.status < 0 ? j.status : SWAP_E_STATE;
+    if (!(j.spec.outputs & (1u << which))) return SWAP_E_STATE;
+    const std::string &f = j.files[which];
+    if (buf) {
+        if (cap < f.size()) return SWAP_E_RANGE;
+        std::memcpy(buf, f.data(), f.size());
+    }
+    return (int64_t)f.size();
+}
+
+int swap_server_release(swap_server *s, int64_t id) {
+    if (!s) return SWAP_E_INVAL;
+    std::lock_guard<std::mutex> lk(s->mu);
+    if (id < 0 || id >= (int64_t)s->jobs.size()) return SWAP_E_INVAL;
+    JobRec &j = *s->jobs[id];
+    if (j.status == SWAP_JOB_QUEUED || j.status == SWAP_JOB_RUNNING) return SWAP_E_STATE;
+    for (auto &f : j.files) std::string().swap(f);
+    return SWAP_OK;
+}
+
+void swap_server_destroy(swap_server *s) {
+    if (!s) return;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->stopping = true;
+    }
+    s->cv_work.notify_all();
+    for (auto &t : s->workers) t.join();
+    delete s;
+}
+
+}  // extern "C"

@@ -23,8 +23,15 @@ SWAP_FILE_NOCE = 2
 EXPORTS = [
     "swap_create", "swap_destroy", "swap_build", "swap_build_device", "swap_merge", "swap_get_stats",
     "swap_export_kmer_graph", "swap_format_file", "swap_write_file", "swap_strerror", "swap_last_error",
-    "swap_table_order", "swap_visit_order", "swap_table_epochs",
+    "swap_table_order", "swap_visit_order", "swap_table_epochs", "swap_reconfigure", "swap_export_compacted",
+    "swap_server_create", "swap_server_submit", "swap_server_submit_batch", "swap_server_wait",
+    "swap_server_wait_all", "swap_server_job_info", "swap_server_job_error", "swap_server_output",
+    "swap_server_release", "swap_server_destroy",
 ]
+
+SWAP_JOB_QUEUED = 1
+SWAP_JOB_RUNNING = 2
+SWAP_JOB_DEVICE_INPUT = 1
 
 
 class SwapCfg(ctypes.Structure):
@@ -36,6 +43,16 @@ class SwapStats(ctypes.Structure):
                 ("survivors", ctypes.c_uint64), ("kept", ctypes.c_uint64), ("overflow", ctypes.c_uint64),
                 ("special", ctypes.c_uint64), ("build_ms", ctypes.c_double), ("merge_ms", ctypes.c_double),
                 ("phase_ms", ctypes.c_double * 16)]
+
+
+class SwapJob(ctypes.Structure):
+    _fields_ = [("fasta", ctypes.c_void_p), ("n", ctypes.c_size_t), ("k", ctypes.c_int), ("c", ctypes.c_int),
+                ("flags", ctypes.c_uint), ("outputs", ctypes.c_uint), ("job_flags", ctypes.c_uint)]
+
+
+class SwapJobInfo(ctypes.Structure):
+    _fields_ = [("status", ctypes.c_int), ("submit_ms", ctypes.c_double), ("start_ms", ctypes.c_double),
+                ("end_ms", ctypes.c_double), ("stats", SwapStats)]
 
 
 _lib = None
@@ -76,6 +93,34 @@ def lib():
         L.swap_visit_order.restype = ctypes.c_int64
         L.swap_table_epochs.argtypes = [ctypes.c_uint64, ctypes.c_int, ctypes.c_void_p, ctypes.c_size_t]
         L.swap_table_epochs.restype = ctypes.c_int64
+        L.swap_reconfigure.argtypes = [ctypes.c_void_p, ctypes.POINTER(SwapCfg)]
+        L.swap_reconfigure.restype = ctypes.c_int
+        L.swap_export_compacted.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                            ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p, ctypes.c_size_t,
+                                            ctypes.POINTER(ctypes.c_uint64)]
+        L.swap_export_compacted.restype = ctypes.c_int64
+        L.swap_server_create.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_size_t, ctypes.POINTER(ctypes.c_void_p)]
+        L.swap_server_create.restype = ctypes.c_int
+        L.swap_server_submit.argtypes = [ctypes.c_void_p, ctypes.POINTER(SwapJob)]
+        L.swap_server_submit.restype = ctypes.c_int64
+        L.swap_server_submit_batch.argtypes = [ctypes.c_void_p, ctypes.POINTER(SwapJob), ctypes.c_size_t,
+                                               ctypes.c_void_p]
+        L.swap_server_submit_batch.restype = ctypes.c_int
+        L.swap_server_wait.argtypes = [ctypes.c_void_p, ctypes.c_int64]
+        L.swap_server_wait.restype = ctypes.c_int
+        L.swap_server_wait_all.argtypes = [ctypes.c_void_p]
+        L.swap_server_wait_all.restype = ctypes.c_int
+        L.swap_server_job_info.argtypes = [ctypes.c_void_p, ctypes.c_int64, ctypes.POINTER(SwapJobInfo)]
+        L.swap_server_job_info.restype = ctypes.c_int
+        L.swap_server_job_error.argtypes = [ctypes.c_void_p, ctypes.c_int64]
+        L.swap_server_job_error.restype = ctypes.c_char_p
+        L.swap_server_output.argtypes = [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int, ctypes.c_void_p,
+                                         ctypes.c_size_t]
+        L.swap_server_output.restype = ctypes.c_int64
+        L.swap_server_release.argtypes = [ctypes.c_void_p, ctypes.c_int64]
+        L.swap_server_release.restype = ctypes.c_int
+        L.swap_server_destroy.argtypes = [ctypes.c_void_p]
+        L.swap_server_destroy.restype = None
         _lib = L
     return _lib
 
@@ -149,6 +194,120 @@ class Assembler:
 
     def write_file(self, which, path):
         return self._check(lib().swap_write_file(self._h, which, path.encode()))
+
+
+    def reconfigure(self, k, c, kmer_graph_only=False, hash_order=False, device=0):
+        cfg = SwapCfg(k, c, device, (SWAP_FLAG_KMER_GRAPH if kmer_graph_only else 0)
+                      | (SWAP_FLAG_HASH_ORDER if hash_order else 0))
+        self._check(lib().swap_reconfigure(self._h, ctypes.byref(cfg)))
+        self.k, self.c = k, c
+
+    def compacted(self):
+        """(keys u64[n], edge_off u64[n,8], edge_len u32[n,8], edge_mult u8[n,8], pool bytes)."""
+        plen = ctypes.c_uint64()
+        n = self._check(lib().swap_export_compacted(self._h, None, None, None, None, 0, None, 0, ctypes.byref(plen)))
+        keys = np.zeros(n, dtype=np.uint64)
+        off = np.zeros((n, 8), dtype=np.uint64)
+        ln = np.zeros((n, 8), dtype=np.uint32)
+        mu = np.zeros((n, 8), dtype=np.uint8)
+        pool = ctypes.create_string_buffer(max(plen.value, 1))
+        self._check(lib().swap_export_compacted(self._h, keys.ctypes.data, off.ctypes.data, ln.ctypes.data,
+                                                mu.ctypes.data, n, pool, plen.value, ctypes.byref(plen)))
+        return keys, off, ln, mu, pool.raw[:plen.value]
+
+
+class Server:
+    """Many assemblies on one GPU: swap_server with `slots` concurrent contexts."""
+
+    def __init__(self, slots=4, device=0, hbm_budget=0):
+        h = ctypes.c_void_p()
+        rc = lib().swap_server_create(device, slots, hbm_budget, ctypes.byref(h))
+        if rc:
+            raise SwapError(rc)
+        self._h = h
+        self._keep = {}  # job id -> input bytes kept alive until the job ends
+
+    def close(self):
+        if self._h:
+            lib().swap_server_destroy(self._h)
+            self._h = None
+            self._keep.clear()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @staticmethod
+    def job(fasta, k, c, outputs=(SWAP_FILE_NOCE,), kmer_graph_only=False, hash_order=False, device_ptr=None):
+        """A SwapJob for host bytes (fasta) or device memory (device_ptr, len(fasta) ignored if fasta is int)."""
+        j = SwapJob()
+        if device_ptr is not None:
+            j.fasta = ctypes.c_void_p(device_ptr)
+            j.n = fasta if isinstance(fasta, int) else len(fasta)
+            j.job_flags = SWAP_JOB_DEVICE_INPUT
+        else:
+            j.fasta = ctypes.cast(ctypes.c_char_p(fasta), ctypes.c_void_p)
+            j.n = len(fasta)
+            j.job_flags = 0
+        j.k, j.c = k, c
+        j.flags = (SWAP_FLAG_KMER_GRAPH if kmer_graph_only else 0) | (SWAP_FLAG_HASH_ORDER if hash_order else 0)
+        j.outputs = 0
+        for w in outputs:
+            j.outputs |= 1 << w
+        j._src = fasta  # the input stays alive as long as the job object
+        return j
+
+    def submit(self, job, keep=None):
+        jid = lib().swap_server_submit(self._h, ctypes.byref(job))
+        if jid < 0:
+            raise SwapError(int(jid))
+        self._keep[jid] = (job, keep)
+        return jid
+
+    def submit_batch(self, jobs, keep=None):
+        arr = (SwapJob * len(jobs))(*jobs)
+        ids = np.zeros(len(jobs), dtype=np.int64)
+        rc = lib().swap_server_submit_batch(self._h, arr, len(jobs), ids.ctypes.data)
+        if rc:
+            raise SwapError(rc)
+        for i, jid in enumerate(ids):
+            self._keep[int(jid)] = (jobs[i], keep[i] if keep else None)
+        return [int(x) for x in ids]
+
+    def wait(self, jid):
+        rc = lib().swap_server_wait(self._h, jid)
+        if rc < 0:
+            raise SwapError(rc, lib().swap_server_job_error(self._h, jid).decode())
+        return rc
+
+    def wait_all(self):
+        rc = lib().swap_server_wait_all(self._h)
+        if rc < 0:
+            raise SwapError(rc)
+
+    def info(self, jid):
+        i = SwapJobInfo()
+        rc = lib().swap_server_job_info(self._h, jid, ctypes.byref(i))
+        if rc:
+            raise SwapError(rc)
+        return {"status": i.status, "submit_ms": i.submit_ms, "start_ms": i.start_ms, "end_ms": i.end_ms,
+                "kmers": i.stats.kmers, "survivors": i.stats.survivors, "kept": i.stats.kept}
+
+    def output(self, jid, which):
+        n = lib().swap_server_output(self._h, jid, which, None, 0)
+        if n < 0:
+            raise SwapError(int(n))
+        buf = ctypes.create_string_buffer(max(n, 1))
+        n = lib().swap_server_output(self._h, jid, which, buf, n)
+        if n < 0:
+            raise SwapError(int(n))
+        return buf.raw[:n]
+
+    def release(self, jid):
+        lib().swap_server_release(self._h, jid)
+        self._keep.pop(jid, None)
 
 
 def table_order(keys, ranks, n_total, extra_resize=False):

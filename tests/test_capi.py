@@ -39,6 +39,19 @@ def test_create_without_gpu_fails_loudly():
     assert e.value.code in (-6,)
 
 
+@pytest.mark.skipif(os.path.exists("/dev/kfd"), reason="a GPU driver is present")
+def test_server_without_gpu_fails_loudly():
+    with pytest.raises(swapamd.SwapError) as e:
+        swapamd.Server(slots=2)
+    assert e.value.code in (-6,)
+
+
+def test_server_rejects_bad_slot_count():
+    h = ctypes.c_void_p()
+    assert swapamd.lib().swap_server_create(0, 0, 0, ctypes.byref(h)) == -1
+    assert swapamd.lib().swap_server_create(0, 65, 0, ctypes.byref(h)) == -1
+
+
 def test_create_rejects_bad_k():
     cfg = swapamd.SwapCfg(20, 3, 0, 0)
     h = ctypes.c_void_p()
