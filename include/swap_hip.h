@@ -107,10 +107,12 @@ int64_t swap_export_kmer_graph(swap_ctx *ctx, uint64_t *keys, uint8_t *mult, siz
 /* Output files (byte formats of the reference):
  *   SWAP_FILE_KMER_GRAPH   kmerGraph.txt      (after a SWAP_FLAG_KMER_GRAPH build)
  *   SWAP_FILE_CONTIG_GRAPH contigGraph.txt    (after swap_merge)
- *   SWAP_FILE_NOCE         noCEcontigs.fasta  (after swap_merge) */
+ *   SWAP_FILE_NOCE         noCEcontigs.fasta  (after swap_merge)
+ *   SWAP_FILE_PERF_CONTIGS noCEcontigs.fasta as the -p mode writes it (after swap_merge) */
 #define SWAP_FILE_KMER_GRAPH 0
 #define SWAP_FILE_CONTIG_GRAPH 1
 #define SWAP_FILE_NOCE 2
+#define SWAP_FILE_PERF_CONTIGS 3 /* the -p contigs file (stringContigs, graph.cpp:3166-3238), after swap_merge */
 /* Writes the file bytes into buf; returns the byte count (buf NULL: size query). */
 int64_t swap_format_file(swap_ctx *ctx, int which, char *buf, size_t cap);
 /* Writes the file to path; returns bytes written or a negative code. */

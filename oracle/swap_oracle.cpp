@@ -132,7 +132,7 @@ struct oracle_graph {
     std::vector<uint64_t> surv_rank;              // index of the key among first occurrences
     std::vector<uint64_t> first_in_order;         // first occurrence of the i-th distinct key
     std::vector<unsigned long long> visit;        // survivors in merge visit order
-    std::string kmer_graph_txt, contig_graph_txt, noce_txt;
+    std::string kmer_graph_txt, contig_graph_txt, noce_txt, perf_txt;
 };
 
 namespace {
@@ -373,6 +373,29 @@ void print_contigs(const Ctx &cx, NodeTable &nodes, std::string &out) {
     }
 }
 
+// stringContigs (graph.cpp:3166-3238), the -p output: printContigs' selection
+// over the table the merge leaves (before the gather), each contig as
+// ">contig_NodeID<id>_Index<slot>_Len<len>_Mul<m>_<running count>\n<seq>\n".
+void print_perf_contigs(const Ctx &cx, NodeTable &nodes, std::string &out) {
+    int count = 0;
+    for (NodeTable::iterator it = nodes.begin(); it != nodes.end(); ++it) {
+        const Node &n = it->second;
+        if (!n.live) continue;
+        for (int i = 0; i < 8; i++) {
+            if (n.arcs[i].empty()) continue;
+            if ((long)n.arcs[i].size() < 100 - cx.k) continue;
+            unsigned long long id;
+            cx.neighbour(n, i, id);
+            if (n.id < id) continue;
+            count++;
+            const std::string seq = cx.str(cx.orient(n.id, i)) + n.arcs[i];
+            out += ">contig_NodeID" + std::to_string(it->first) + "_Index" + std::to_string(i) + "_Len" +
+                   std::to_string(seq.size()) + "_Mul" + std::to_string((int)n.m[i]) + "_" + std::to_string(count) +
+                   "\n" + seq + "\n";
+        }
+    }
+}
+
 // printKmerGraph (graph.cpp:3352-3440), including its neighbour encoding: when the
 // walk ends on the reverse strand the value written is that of reverse(str(Y))
 // with only the character at the slot index complemented (the loop at
@@ -449,7 +472,10 @@ oracle_graph *oracle_run(const char *fasta, uint64_t n, int k, int c, int flags)
     NodeTable nodes;
     build_nodes(kmol, nodes);
     for (NodeTable::iterator it = nodes.begin(); it != nodes.end(); ++it) g->visit.push_back(it->first);
-    if (!(flags & 1)) swap_merge(g->cx, nodes);  // graph.cpp:3576
+    if (!(flags & 1)) {
+        swap_merge(g->cx, nodes);  // graph.cpp:3576
+        print_perf_contigs(g->cx, nodes, g->perf_txt);  // what -p writes at this point (graph.cpp:3591-3611)
+    }
     gather(nodes, g->contig_graph_txt);
     g->contig_graph_txt += "\n";  // graph.cpp:3617-3622: fprintf("%s\n")
     if (flags & 1) {
@@ -511,7 +537,10 @@ uint64_t oracle_visit_order(const oracle_graph *g, uint64_t *keys, uint64_t cap)
 }
 
 uint64_t oracle_file(const oracle_graph *g, int which, char *buf, uint64_t cap) {
-    const std::string *s = which == 0 ? &g->kmer_graph_txt : which == 1 ? &g->contig_graph_txt : &g->noce_txt;
+    const std::string *s = which == 0   ? &g->kmer_graph_txt
+                           : which == 1 ? &g->contig_graph_txt
+                           : which == 2 ? &g->noce_txt
+                                        : &g->perf_txt;
     if (buf && cap >= s->size()) memcpy(buf, s->data(), s->size());
     return s->size();
 }

@@ -59,7 +59,7 @@ uint64_t oracle_visit_order(const oracle_graph *g, uint64_t *keys, uint64_t cap)
 // Output files, byte for byte, into caller buffers.  Each returns the byte
 // count; pass buf=NULL to query the size.
 //   which: 0 = kmerGraph.txt (run with flags bit0), 1 = contigGraph.txt,
-//          2 = noCEcontigs.fasta
+//          2 = noCEcontigs.fasta, 3 = noCEcontigs.fasta as the -p mode writes it
 uint64_t oracle_file(const oracle_graph *g, int which, char *buf, uint64_t cap);
 
 }  // extern "C"

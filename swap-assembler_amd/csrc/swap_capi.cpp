@@ -95,7 +95,7 @@ int format_file_string(swap_ctx *c, int which, std::string &out) {
         std::vector<uint32_t> order;
         kmer_graph_order(c, order);
         format_kmer_graph(c->cfg.k, c->graph, order, out);
-    } else if (which == SWAP_FILE_CONTIG_GRAPH || which == SWAP_FILE_NOCE) {
+    } else if (which == SWAP_FILE_CONTIG_GRAPH || which == SWAP_FILE_NOCE || which == SWAP_FILE_PERF_CONTIGS) {
         if (!c->merged) return set_err(c, SWAP_E_STATE, "no merge");
         if (!c->have_compacted) {
             rc = materialize_merge(*c->pipe, c->cfg, c->comp);
@@ -103,7 +103,8 @@ int format_file_string(swap_ctx *c, int which, std::string &out) {
             c->have_compacted = true;
         }
         if (which == SWAP_FILE_CONTIG_GRAPH) format_contig_graph(c->comp, out);
-        else format_noce(c->cfg.k, c->comp, out);
+        else if (which == SWAP_FILE_NOCE) format_noce(c->cfg.k, c->comp, out);
+        else format_perf_contigs(c->cfg.k, c->comp, out);
     } else {
         return set_err(c, SWAP_E_INVAL, "unknown file");
     }

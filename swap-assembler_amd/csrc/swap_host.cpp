@@ -323,6 +323,36 @@ void format_noce(int k, const HostCompacted &c, std::string &out) {
     }
 }
 
+// The -p contigs file (stringContigs, graph.cpp:3166-3238): the same contig
+// selection as printContigs, walked in the order of the merge's own table (the
+// contigGraph.txt order), written as ">contig_NodeID<X>_Index<slot>_Len<len>
+// _Mul<merged multiplicity>_<running count>" and the sequence on its own line.
+// Contig extension does not run in that mode, so no removed-edge records follow.
+void format_perf_contigs(int k, const HostCompacted &c, std::string &out) {
+    const size_t n = c.key.size();
+    int count = 0;
+    char head[160];
+    for (size_t i = 0; i < n; i++) {
+        const uint64_t X = c.key[i];
+        for (int s = 0; s < 8; s++) {
+            const uint32_t L = c.len[8 * i + s];
+            if (L == 0) continue;
+            if ((long)L < 100 - k) continue;
+            const char *str = c.pool.data() + c.off[8 * i + s];
+            uint64_t id;
+            terminal(X, s, str, L, k, &id);
+            if (X < id) continue;
+            count++;
+            std::snprintf(head, sizeof head, ">contig_NodeID%llu_Index%d_Len%u_Mul%d_%d\n", (unsigned long long)X, s,
+                          (unsigned)(k + L), (int)c.mult[8 * i + s], count);
+            out += head;
+            put_kmer(out, s >= 4 ? revcomp(X, k) : X, k);
+            out.append(str, L);
+            out.push_back('\n');
+        }
+    }
+}
+
 }  // namespace swapk
 
 // ---------------------------------------------------------------------------

@@ -48,6 +48,7 @@ std::vector<uint32_t> gather_order(const std::vector<uint64_t> &key, const std::
 void format_kmer_graph(int k, const HostKmerGraph &g, const std::vector<uint32_t> &order, std::string &out);
 void format_contig_graph(const HostCompacted &c, std::string &out);
 void format_noce(int k, const HostCompacted &c, std::string &out);
+void format_perf_contigs(int k, const HostCompacted &c, std::string &out);
 
 // Runs the merge (device kernels; cycles and fold-backs replayed on the host).
 // When nothing needs the host replay the results stay on the device and

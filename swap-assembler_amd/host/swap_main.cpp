@@ -1,14 +1,16 @@
 // swap -- drop-in driver for the k-mer graph build + k-molecule merge path.
 //
-//   swap -k <kmerlength> -c <cutoff> -i <reads.fa> -o <outdir> [-s] [-d] [-h] [-v]
+//   swap -k <kmerlength> -c <cutoff> -i <reads.fa> -o <outdir> [-s] [-d] [-p] [-h] [-v]
 //        [--device N] [--hash-order]
 //
 // Same flags, messages and output files as the reference's main
 // (graph.cpp:3443-3698, sequence.cpp:4-183): -s writes kmerGraph.txt and stops
 // after the k-mer graph; otherwise the graph is merged and noCEcontigs.fasta is
-// written, plus contigGraph.txt with -d.  logtime.txt holds the phase timings.
-// Contig extension (CEContig.fasta) and the -j / -p outputs are outside this
-// path.  Unlike the reference, errors exit with status 1.
+// written, plus contigGraph.txt with -d; with -p the merged graph's contigs are
+// written in the performance-mode format (stringContigs, graph.cpp:3591-3611)
+// instead and nothing else.  logtime.txt holds the phase timings.  Contig
+// extension (CEContig.fasta) and the -j output are outside this path.  Unlike
+// the reference, errors exit with status 1.
 #include <sys/stat.h>
 #include <sys/types.h>
 
@@ -28,7 +30,7 @@ static void usage(const char *argv0) {
 int main(int argc, char **argv) {
     int k = 23, c = 5, device = 0;  // sequence.cpp:8-9 defaults
     std::string in, out;
-    bool kmer_graph = false, dist_graph = false, hash_order = false;
+    bool kmer_graph = false, dist_graph = false, hash_order = false, perf = false;
     for (int i = 1; i < argc; i++) {
         const char *a = argv[i];
         if (a[0] != '-') { usage(argv[0]); return 1; }
@@ -63,7 +65,8 @@ int main(int argc, char **argv) {
                 break;
             case 's': kmer_graph = true; break;
             case 'd': dist_graph = true; break;
-            case 'j': case 'p': break;  // outputs outside this path; accepted for compatibility
+            case 'p': perf = true; break;
+            case 'j': break;  // output of the contig extension, outside this path; accepted for compatibility
             default: break;
         }
     }
@@ -118,6 +121,17 @@ int main(int argc, char **argv) {
         swap_get_stats(ctx, &st);
         if (log) fprintf(log, "Time spend in Simplify DistGraph %.5f\n", since(t2));
         auto t3 = std::chrono::steady_clock::now();
+        if (perf) {
+            int64_t w = swap_write_file(ctx, SWAP_FILE_PERF_CONTIGS, (out + "/noCEcontigs.fasta").c_str());
+            if (w < 0) { printf("Error: %s (%s)\n", swap_strerror((int)w), swap_last_error(ctx)); return 1; }
+            if (log) {
+                fprintf(log, "Time spend in printDistGraph %.5f\n", since(t3));
+                fprintf(log, "Time spend in All step %.5f\n", since(t0));
+                fclose(log);
+            }
+            swap_destroy(ctx);
+            return 0;
+        }
         if (dist_graph) {
             int64_t w = swap_write_file(ctx, SWAP_FILE_CONTIG_GRAPH, (out + "/contigGraph.txt").c_str());
             if (w < 0) { printf("Error: %s (%s)\n", swap_strerror((int)w), swap_last_error(ctx)); return 1; }

@@ -18,6 +18,7 @@ SWAP_FLAG_HASH_ORDER = 2
 SWAP_FILE_KMER_GRAPH = 0
 SWAP_FILE_CONTIG_GRAPH = 1
 SWAP_FILE_NOCE = 2
+SWAP_FILE_PERF_CONTIGS = 3
 
 # exported entry points and their signatures (mirrors include/swap_hip.h)
 EXPORTS = [

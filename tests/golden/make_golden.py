@@ -15,11 +15,17 @@ find the Node" on these inputs.  np=4 is used because the k-mer graph and the
 noCE contig set do not depend on the rank count (checked against np=3, 5 and 8
 when the cases were made).
 
-Stored per case (all gzipped text):
+Stored per case (all gzipped text, written with a zero gzip timestamp so that
+reruns reproduce the same bytes):
   <case>.fa.gz          the input
   <case>.kmerGraph.gz   kmerGraph.txt lines, sorted (line order is hash order)
   <case>.noce.gz        noCEcontigs.fasta lines, sorted
 and one JSON index, cases.json, with k, c and the line counts.
+
+The -p mode output is not stored: the reference writes it from each rank's
+table right after the distributed merge, before the rank-0 finisher, so at
+np=4 it still holds the (1,1) nodes whose locks collided and differs from the
+single-rank result in topology, not just in order.
 """
 import gzip
 import json
@@ -130,6 +136,12 @@ CASES = [
 ]
 
 
+def gz_write(path, text):
+    with open(path, "wb") as raw:
+        with gzip.GzipFile(fileobj=raw, mode="wb", mtime=0) as f:
+            f.write(text.encode())
+
+
 def run_ref(k, c, fa, flag):
     out = tempfile.mkdtemp(prefix="ref_", dir=os.path.join(REPO, "_work"))
     os.rmdir(out)
@@ -150,8 +162,7 @@ def main():
         fa = os.path.join(REPO, "_work", name + ".fa")
         with open(fa, "w") as f:
             f.write(text)
-        with gzip.open(os.path.join(HERE, name + ".fa.gz"), "wt") as f:
-            f.write(text)
+        gz_write(os.path.join(HERE, name + ".fa.gz"), text)
         out_s = run_ref(k, c, fa, "-s")
         with open(os.path.join(out_s, "kmerGraph.txt")) as f:
             kg = sorted(f.read().splitlines())
@@ -160,10 +171,8 @@ def main():
         with open(os.path.join(out_d, "noCEcontigs.fasta")) as f:
             noce = sorted(f.read().splitlines())
         shutil.rmtree(out_d)
-        with gzip.open(os.path.join(HERE, name + ".kmerGraph.gz"), "wt") as f:
-            f.write("\n".join(kg) + ("\n" if kg else ""))
-        with gzip.open(os.path.join(HERE, name + ".noce.gz"), "wt") as f:
-            f.write("\n".join(noce) + ("\n" if noce else ""))
+        gz_write(os.path.join(HERE, name + ".kmerGraph.gz"), "\n".join(kg) + ("\n" if kg else ""))
+        gz_write(os.path.join(HERE, name + ".noce.gz"), "\n".join(noce) + ("\n" if noce else ""))
         index[name] = {"k": k, "c": c, "kmer_graph_lines": len(kg), "noce_lines": len(noce), "np": 4}
         print(name, index[name])
     with open(os.path.join(HERE, "cases.json"), "w") as f:

@@ -59,6 +59,8 @@ def check_merge(fa, k, c, hash_order=False):
                   flush=True)
         assert cg == ocg
         assert noce == onoce
+        # the -p contigs file: same merged table, written in the merge's own order
+        assert a.file_bytes(swapamd.SWAP_FILE_PERF_CONTIGS) == o.file(3)
     a.close()
     return st
 

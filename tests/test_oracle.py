@@ -5,6 +5,7 @@ import gzip
 import json
 import os
 import random
+import re
 
 import numpy as np
 import pytest
@@ -138,3 +139,31 @@ def test_records_before_first_gt_are_skipped():
     fa = b"junk\nmore junk\n>a\nACGTACGTACGTACGTACGTACGTAC\n"
     o = Oracle(fa, 21, 0)
     assert o.stats()["reads"] == 1 and o.stats()["kmers"] == 6
+
+
+def decode_kmer(x, k):
+    """2-bit MSB-first value -> bases (A=0 C=1 G=2 T=3)."""
+    return "".join("ACGT"[(x >> (2 * (k - 1 - i))) & 3] for i in range(k))
+
+
+@pytest.mark.parametrize("name", sorted(CASES))
+def test_perf_contigs_are_the_noce_selection(name):
+    """The -p writer (stringContigs) selects exactly printContigs' contigs: on a
+    single rank nothing is left for the finisher on these inputs, so the
+    sequences equal the noCE set, and each header's fields describe its line."""
+    case = CASES[name]
+    o = Oracle(gold(name, "fa"), case["k"], case["c"])
+    perf = o.file(3).decode().splitlines()
+    noce = sorted(l[len(">proc0: "):] for l in o.file(2).decode().splitlines())
+    assert len(perf) % 2 == 0
+    seqs = []
+    for i in range(0, len(perf), 2):
+        m = re.match(r">contig_NodeID(\d+)_Index(\d)_Len(\d+)_Mul(\d+)_(\d+)$", perf[i])
+        assert m, perf[i]
+        node, slot, ln, mul, cnt = (int(g) for g in m.groups())
+        seq = perf[i + 1]
+        assert ln == len(seq) and cnt == i // 2 + 1 and 0 < mul <= 255
+        first = seq[:case["k"]]
+        assert (first if slot < 4 else kmer_textbook.rc(first)) == decode_kmer(node, case["k"])
+        seqs.append(seq)
+    assert sorted(seqs) == noce
