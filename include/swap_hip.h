@@ -28,7 +28,9 @@
  *   swap_format_file /           printKmerGraph (graph.cpp:3352-3440),
  *   swap_write_file              gatherDistNodeGraph + the contigGraph.txt write
  *                                (graph.cpp:576-696, 3617-3622), printContigs
- *                                (graph.cpp:1139-1202)
+ *                                (graph.cpp:1139-1202), stringContigs (-p,
+ *                                graph.cpp:3166-3238), and the contig extension
+ *                                with its writers (graph.cpp:2089-3350)
  */
 #ifndef SWAP_HIP_H
 #define SWAP_HIP_H
@@ -108,11 +110,17 @@ int64_t swap_export_kmer_graph(swap_ctx *ctx, uint64_t *keys, uint8_t *mult, siz
  *   SWAP_FILE_KMER_GRAPH   kmerGraph.txt      (after a SWAP_FLAG_KMER_GRAPH build)
  *   SWAP_FILE_CONTIG_GRAPH contigGraph.txt    (after swap_merge)
  *   SWAP_FILE_NOCE         noCEcontigs.fasta  (after swap_merge)
- *   SWAP_FILE_PERF_CONTIGS noCEcontigs.fasta as the -p mode writes it (after swap_merge) */
+ *   SWAP_FILE_PERF_CONTIGS noCEcontigs.fasta as the -p mode writes it (after swap_merge)
+ *   SWAP_FILE_CE_CONTIGS, SWAP_FILE_JUNG_ARC, SWAP_FILE_JUNG_MUL: see below */
 #define SWAP_FILE_KMER_GRAPH 0
 #define SWAP_FILE_CONTIG_GRAPH 1
 #define SWAP_FILE_NOCE 2
 #define SWAP_FILE_PERF_CONTIGS 3 /* the -p contigs file (stringContigs, graph.cpp:3166-3238), after swap_merge */
+/* after swap_merge, from the contig extension run on the host (the reference's
+ * rank-0 finisher and clean-up passes, graph.cpp:2089-3350; computed once per merge) */
+#define SWAP_FILE_CE_CONTIGS 4   /* CEContig.fasta (masterPrintContigs, graph.cpp:3240-3318) */
+#define SWAP_FILE_JUNG_ARC 5     /* JungGraph_arc.txt (-j, printJungGraph, graph.cpp:3320-3350) */
+#define SWAP_FILE_JUNG_MUL 6     /* JungGraph_mul.txt (-j) */
 /* Writes the file bytes into buf; returns the byte count (buf NULL: size query). */
 int64_t swap_format_file(swap_ctx *ctx, int which, char *buf, size_t cap);
 /* Writes the file to path; returns bytes written or a negative code. */
@@ -140,6 +148,16 @@ int64_t swap_table_epochs(uint64_t n_total, int extra_resize, uint64_t *starts, 
 
 const char *swap_strerror(int code);
 const char *swap_last_error(const swap_ctx *ctx);
+
+/* Host utility, no device: the contig extension on a contigGraph.txt (the
+ * reference reads that file back the same way, recvBufDistNodeGraph,
+ * graph.cpp:845-896), as the rank-0 part of its main runs it
+ * (graph.cpp:3636-3682).  survivors = the k-molecule count after the cutoff
+ * (it sizes the table the graph is read into, which fixes the walk order).
+ * which: SWAP_FILE_NOCE, SWAP_FILE_CE_CONTIGS, SWAP_FILE_JUNG_ARC or
+ * SWAP_FILE_JUNG_MUL.  Returns the byte count (buf NULL: size query). */
+int64_t swap_contig_extension(int k, uint64_t survivors, const char *contig_graph, size_t n, int which, char *buf,
+                              size_t cap);
 
 /* Re-targets a context to a new k, c and flags for its next build, keeping its
  * device buffers (same device).  No counterpart in the reference (one run =

@@ -29,6 +29,9 @@ struct swap_ctx {
     std::vector<uint32_t> visit;    // survivor index -> merge visit rank
     bool have_compacted = false;
     swapk::HostCompacted comp;
+    bool have_ce = false;
+    int ce_rc = 0;
+    swapk::CeFiles ce;
 };
 
 namespace {
@@ -95,16 +98,30 @@ int format_file_string(swap_ctx *c, int which, std::string &out) {
         std::vector<uint32_t> order;
         kmer_graph_order(c, order);
         format_kmer_graph(c->cfg.k, c->graph, order, out);
-    } else if (which == SWAP_FILE_CONTIG_GRAPH || which == SWAP_FILE_NOCE || which == SWAP_FILE_PERF_CONTIGS) {
+    } else if (which >= SWAP_FILE_CONTIG_GRAPH && which <= SWAP_FILE_JUNG_MUL) {
         if (!c->merged) return set_err(c, SWAP_E_STATE, "no merge");
         if (!c->have_compacted) {
             rc = materialize_merge(*c->pipe, c->cfg, c->comp);
             if (rc) return set_err(c, rc, c->pipe->last_error());
             c->have_compacted = true;
         }
-        if (which == SWAP_FILE_CONTIG_GRAPH) format_contig_graph(c->comp, out);
-        else if (which == SWAP_FILE_NOCE) format_noce(c->cfg.k, c->comp, out);
-        else format_perf_contigs(c->cfg.k, c->comp, out);
+        if (which == SWAP_FILE_CONTIG_GRAPH) {
+            format_contig_graph(c->comp, out);
+        } else if (which == SWAP_FILE_NOCE) {
+            format_noce(c->cfg.k, c->comp, out);
+        } else if (which == SWAP_FILE_PERF_CONTIGS) {
+            format_perf_contigs(c->cfg.k, c->comp, out);
+        } else {
+            if (!c->have_ce) {
+                c->ce_rc = contig_extension(c->cfg.k, c->comp, c->ce);
+                c->have_ce = true;
+            }
+            if (c->ce_rc)
+                return set_err(c, c->ce_rc,
+                               "contig extension: no edge base left to average the coverage over (the reference "
+                               "divides by zero here)");
+            out = which == SWAP_FILE_CE_CONTIGS ? c->ce.ce : which == SWAP_FILE_JUNG_ARC ? c->ce.jung_arc : c->ce.jung_mul;
+        }
     } else {
         return set_err(c, SWAP_E_INVAL, "unknown file");
     }
@@ -140,7 +157,7 @@ void swap_destroy(swap_ctx *c) { delete c; }
 
 int swap_build_device(swap_ctx *c, const void *d_fasta, size_t n) {
     if (!c || (!d_fasta && n)) return SWAP_E_INVAL;
-    c->built = c->merged = c->have_graph = c->have_compacted = false;
+    c->built = c->merged = c->have_graph = c->have_compacted = c->have_ce = false;
     if (n == 0) return set_err(c, SWAP_E_EMPTY, "empty input");
     auto t0 = clk::now();
     int rc = c->pipe->build(static_cast<const uint8_t *>(d_fasta), n);
@@ -171,6 +188,7 @@ int swap_merge(swap_ctx *c) {
     if (rc) return set_err(c, rc, c->pipe->last_error());
     c->merged = true;
     c->have_compacted = mat;
+    c->have_ce = false;
     return SWAP_OK;
 }
 
@@ -232,7 +250,7 @@ int swap_reconfigure(swap_ctx *c, const swap_cfg *cfg) {
     if (cfg->device != c->cfg.device) return set_err(c, SWAP_E_INVAL, "a context stays on its device");
     c->cfg = *cfg;
     c->pipe->reconfigure(cfg->k, cfg->c, !(cfg->flags & SWAP_FLAG_HASH_ORDER));
-    c->built = c->merged = c->have_graph = c->have_compacted = false;
+    c->built = c->merged = c->have_graph = c->have_compacted = c->have_ce = false;
     return SWAP_OK;
 }
 

@@ -45,6 +45,16 @@ struct HostCompacted {
     // nodes table in visit order); noce_order lists them in the order of the
     // table the gather re-fills, which printContigs walks
     std::vector<uint32_t> noce_order;
+    // bucket count of the reference's nodes table when the gather re-fills it
+    uint64_t node_buckets = 0;
+};
+
+// Outputs of the contig extension (host, after the merge).
+struct CeFiles {
+    std::string noce;      // noCEcontigs.fasta as printContigs writes it from the re-filled table
+    std::string ce;        // CEContig.fasta
+    std::string jung_arc;  // JungGraph_arc.txt
+    std::string jung_mul;  // JungGraph_mul.txt
 };
 
 // Growth epochs of a reference hash table: epoch e starts at insert rank

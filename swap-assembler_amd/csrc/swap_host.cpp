@@ -633,6 +633,7 @@ int materialize_merge(Pipeline &p, const swap_cfg &cfg, HostCompacted &out) {
     else
         std::sort(ord.begin(), ord.end(), [&](uint32_t a, uint32_t b) { return live[a].id < live[b].id; });
     // noCEcontigs.fasta order: the re-filled table (graph.cpp:3615, 891)
+    out.node_buckets = table_buckets(S, false);
     out.noce_order.resize(live.size());
     if (ref && P3) {
         std::vector<uint64_t> keys(live.size());

@@ -49,6 +49,10 @@ void format_kmer_graph(int k, const HostKmerGraph &g, const std::vector<uint32_t
 void format_contig_graph(const HostCompacted &c, std::string &out);
 void format_noce(int k, const HostCompacted &c, std::string &out);
 void format_perf_contigs(int k, const HostCompacted &c, std::string &out);
+// Contig extension of the merged graph (the reference's rank-0 finisher,
+// clean-up passes and writers, graph.cpp:2089-3350); 0, or SWAP_E_EMPTY where
+// the reference divides by zero (no edge base left).
+int contig_extension(int k, const HostCompacted &g, CeFiles &out);
 
 // Runs the merge (device kernels; cycles and fold-backs replayed on the host).
 // When nothing needs the host replay the results stay on the device and

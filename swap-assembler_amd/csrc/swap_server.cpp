@@ -47,7 +47,7 @@ constexpr double kBytesPerInputByte = 36.0;
 
 struct JobRec {
     swap_job spec{};
-    std::string files[4];
+    std::string files[7];
     int status = SWAP_JOB_QUEUED;
     std::string err;
     swap_stats stats{};
@@ -92,7 +92,7 @@ int run_job(swap_server *s, swap_ctx *&ctx, JobRec &j, std::string &err) {
     else
         rc = swap_build(ctx, static_cast<const char *>(j.spec.fasta), j.spec.n);
     if (rc == SWAP_OK && !(j.spec.flags & SWAP_FLAG_KMER_GRAPH)) rc = swap_merge(ctx);
-    for (int w = 0; w < 4 && rc == SWAP_OK; w++)
+    for (int w = 0; w < 7 && rc == SWAP_OK; w++)
         if (j.spec.outputs & (1u << w)) rc = swapk::format_file_string(ctx, w, j.files[w]);
     if (rc == SWAP_OK) rc = swap_get_stats(ctx, &j.stats);
     if (rc) err = swap_last_error(ctx);
@@ -234,7 +234,7 @@ const char *swap_server_job_error(swap_server *s, int64_t id) {
 }
 
 int64_t swap_server_output(swap_server *s, int64_t id, int which, char *buf, size_t cap) {
-    if (!s || which < 0 || which > 3) return SWAP_E_INVAL;
+    if (!s || which < 0 || which > 6) return SWAP_E_INVAL;
     std::lock_guard<std::mutex> lk(s->mu);
     if (id < 0 || id >= (int64_t)s->jobs.size()) return SWAP_E_INVAL;
     const JobRec &j = *s->jobs[id];

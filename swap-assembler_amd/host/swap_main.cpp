@@ -6,11 +6,12 @@
 // Same flags, messages and output files as the reference's main
 // (graph.cpp:3443-3698, sequence.cpp:4-183): -s writes kmerGraph.txt and stops
 // after the k-mer graph; otherwise the graph is merged and noCEcontigs.fasta is
-// written, plus contigGraph.txt with -d; with -p the merged graph's contigs are
-// written in the performance-mode format (stringContigs, graph.cpp:3591-3611)
-// instead and nothing else.  logtime.txt holds the phase timings.  Contig
-// extension (CEContig.fasta) and the -j output are outside this path.  Unlike
-// the reference, errors exit with status 1.
+// written, then the contig extension runs on the host and CEContig.fasta is
+// written (plus JungGraph_arc.txt / JungGraph_mul.txt with -j); contigGraph.txt
+// with -d; with -p the merged graph's contigs are written in the
+// performance-mode format (stringContigs, graph.cpp:3591-3611) instead and
+// nothing else.  logtime.txt holds the phase timings.  Unlike the reference,
+// errors exit with status 1.
 #include <sys/stat.h>
 #include <sys/types.h>
 
@@ -30,7 +31,7 @@ static void usage(const char *argv0) {
 int main(int argc, char **argv) {
     int k = 23, c = 5, device = 0;  // sequence.cpp:8-9 defaults
     std::string in, out;
-    bool kmer_graph = false, dist_graph = false, hash_order = false, perf = false;
+    bool kmer_graph = false, dist_graph = false, hash_order = false, perf = false, jung = false;
     for (int i = 1; i < argc; i++) {
         const char *a = argv[i];
         if (a[0] != '-') { usage(argv[0]); return 1; }
@@ -66,7 +67,7 @@ int main(int argc, char **argv) {
             case 's': kmer_graph = true; break;
             case 'd': dist_graph = true; break;
             case 'p': perf = true; break;
-            case 'j': break;  // output of the contig extension, outside this path; accepted for compatibility
+            case 'j': jung = true; break;
             default: break;
         }
     }
@@ -140,6 +141,15 @@ int main(int argc, char **argv) {
         if (w < 0) { printf("Error: %s (%s)\n", swap_strerror((int)w), swap_last_error(ctx)); return 1; }
         if (log) fprintf(log, "Time spend in printDistGraph %.5f\n", since(t3));
         printf("|proc:0 %llu nodes in distGraph\n", (unsigned long long)st.kept);
+        auto t4 = std::chrono::steady_clock::now();
+        w = swap_write_file(ctx, SWAP_FILE_CE_CONTIGS, (out + "/CEContig.fasta").c_str());
+        if (w < 0) { printf("Error: %s (%s)\n", swap_strerror((int)w), swap_last_error(ctx)); return 1; }
+        if (log) fprintf(log, "Time spend in Contig Extension %.5f\n", since(t4));
+        if (jung) {
+            w = swap_write_file(ctx, SWAP_FILE_JUNG_ARC, (out + "/JungGraph_arc.txt").c_str());
+            if (w >= 0) w = swap_write_file(ctx, SWAP_FILE_JUNG_MUL, (out + "/JungGraph_mul.txt").c_str());
+            if (w < 0) { printf("Error: %s (%s)\n", swap_strerror((int)w), swap_last_error(ctx)); return 1; }
+        }
     }
     if (log) {
         fprintf(log, "Time spend in All step %.5f\n", since(t0));

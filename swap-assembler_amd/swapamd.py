@@ -19,6 +19,9 @@ SWAP_FILE_KMER_GRAPH = 0
 SWAP_FILE_CONTIG_GRAPH = 1
 SWAP_FILE_NOCE = 2
 SWAP_FILE_PERF_CONTIGS = 3
+SWAP_FILE_CE_CONTIGS = 4
+SWAP_FILE_JUNG_ARC = 5
+SWAP_FILE_JUNG_MUL = 6
 
 # exported entry points and their signatures (mirrors include/swap_hip.h)
 EXPORTS = [
@@ -27,7 +30,7 @@ EXPORTS = [
     "swap_table_order", "swap_visit_order", "swap_table_epochs", "swap_reconfigure", "swap_export_compacted",
     "swap_server_create", "swap_server_submit", "swap_server_submit_batch", "swap_server_wait",
     "swap_server_wait_all", "swap_server_job_info", "swap_server_job_error", "swap_server_output",
-    "swap_server_release", "swap_server_destroy",
+    "swap_server_release", "swap_server_destroy", "swap_contig_extension",
 ]
 
 SWAP_JOB_QUEUED = 1
@@ -120,6 +123,9 @@ def lib():
         L.swap_server_output.restype = ctypes.c_int64
         L.swap_server_release.argtypes = [ctypes.c_void_p, ctypes.c_int64]
         L.swap_server_release.restype = ctypes.c_int
+        L.swap_contig_extension.argtypes = [ctypes.c_int, ctypes.c_uint64, ctypes.c_char_p, ctypes.c_size_t,
+                                            ctypes.c_int, ctypes.c_void_p, ctypes.c_size_t]
+        L.swap_contig_extension.restype = ctypes.c_int64
         L.swap_server_destroy.argtypes = [ctypes.c_void_p]
         L.swap_server_destroy.restype = None
         _lib = L
@@ -309,6 +315,18 @@ class Server:
     def release(self, jid):
         lib().swap_server_release(self._h, jid)
         self._keep.pop(jid, None)
+
+
+def contig_extension(k, survivors, contig_graph: bytes, which):
+    """One output file of the contig extension run on contigGraph.txt bytes (host only)."""
+    n = lib().swap_contig_extension(k, survivors, contig_graph, len(contig_graph), which, None, 0)
+    if n < 0:
+        raise SwapError(int(n))
+    buf = ctypes.create_string_buffer(max(n, 1))
+    n = lib().swap_contig_extension(k, survivors, contig_graph, len(contig_graph), which, buf, n)
+    if n < 0:
+        raise SwapError(int(n))
+    return buf.raw[:n]
 
 
 def table_order(keys, ranks, n_total, extra_resize=False):

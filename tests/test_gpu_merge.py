@@ -127,3 +127,31 @@ def test_random_genomes(k):
 def test_small_workload_full_size():
     fa, k, c = swapamd.generate("small")
     check_merge(fa, k, c)
+
+
+CE_CASES = json.load(open(os.path.join(GOLD, "ce_cases.json")))
+
+
+def ce_gold(name, what):
+    with gzip.open(os.path.join(GOLD, "ce_%s.%s.gz" % (name, what)), "rb") as f:
+        return f.read()
+
+
+@pytest.mark.parametrize("name", sorted(CE_CASES))
+def test_contig_extension_from_reads(name):
+    """Reads -> GPU build and merge -> host contig extension: contigGraph.txt and
+    the extension's files equal what the reference's own rank-0 code wrote for
+    that graph."""
+    case = CE_CASES[name]
+    fa = ce_gold(name, "fa")
+    print("contig extension case %s k=%d c=%d" % (name, case["k"], case["c"]), flush=True)
+    a = swapamd.Assembler(case["k"], case["c"])
+    a.build(fa)
+    a.merge()
+    assert a.stats()["survivors"] == case["survivors"]
+    assert a.file_bytes(swapamd.SWAP_FILE_CONTIG_GRAPH) == ce_gold(name, "contigGraph")
+    assert a.file_bytes(swapamd.SWAP_FILE_NOCE) == ce_gold(name, "noce")
+    assert a.file_bytes(swapamd.SWAP_FILE_CE_CONTIGS) == ce_gold(name, "cecontig")
+    assert a.file_bytes(swapamd.SWAP_FILE_JUNG_ARC) == ce_gold(name, "jungarc")
+    assert a.file_bytes(swapamd.SWAP_FILE_JUNG_MUL) == ce_gold(name, "jungmul")
+    a.close()
