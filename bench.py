@@ -255,6 +255,57 @@ def batched_vs_per_sample(n_samples, slots, warmup=True):
             "identical_outputs": same}
 
 
+def serving(n_requests, rate, slots, seed=5):
+    """The request-stream workload: requests arrive as a Poisson stream (mean
+    `rate` per second), each a seeded genome log-uniform in [50 kbp, 5 Mbp] with
+    reads of 100/150/250 bp at 20-60x, 0.5% errors, k from {21, 25, 31}, c=3,
+    FASTA bytes in host memory (the transfer to the device is part of each
+    request).  A server with `slots` contexts admits each request as a slot and
+    the HBM budget allow; its noCE contigs are produced.  Latency = submit to
+    end of the request."""
+    import concurrent.futures as cf
+    import random
+    import swapamd
+    with cf.ThreadPoolExecutor(max_workers=8) as ex:
+        reqs = list(ex.map(lambda i: swapamd.generate("request", i), range(n_requests)))
+    rng = random.Random(seed)
+    arrivals, t = [], 0.0
+    for _ in range(n_requests):
+        t += rng.expovariate(rate)
+        arrivals.append(t)
+    noce = swapamd.SWAP_FILE_NOCE
+    srv = swapamd.Server(slots=slots)
+    # warm every slot once (device buffers grow to the largest request)
+    big = sorted(range(n_requests), key=lambda i: -len(reqs[i][0]))[:slots]
+    warm = srv.submit_batch([swapamd.Server.job(reqs[i][0], reqs[i][1], reqs[i][2], outputs=(noce,)) for i in big])
+    srv.wait_all()
+    for j in warm:
+        srv.release(j)
+    ids = []
+    t0 = time.perf_counter()
+    for i, (fa, k, c) in enumerate(reqs):
+        delay = arrivals[i] - (time.perf_counter() - t0)
+        if delay > 0:
+            time.sleep(delay)
+        ids.append(srv.submit(swapamd.Server.job(fa, k, c, outputs=(noce,))))
+    srv.wait_all()
+    wall = time.perf_counter() - t0
+    infos = [srv.info(j) for j in ids]
+    lat = sorted(x["end_ms"] - x["submit_ms"] for x in infos)
+    queue = sorted(x["start_ms"] - x["submit_ms"] for x in infos)
+    ok = all(x["status"] == 0 for x in infos)
+    srv.close()
+
+    def pct(v, q):
+        return v[min(len(v) - 1, int(round(q * (len(v) - 1))))]
+
+    return {"workload": "request", "requests": n_requests, "rate_per_s": rate, "slots": slots,
+            "input_bytes": sum(len(r[0]) for r in reqs), "input_kmers": sum(x["kmers"] for x in infos),
+            "p50_ms": pct(lat, 0.50), "p99_ms": pct(lat, 0.99), "mean_ms": sum(lat) / len(lat),
+            "queue_p50_ms": pct(queue, 0.50), "queue_p99_ms": pct(queue, 0.99),
+            "samples_per_s": n_requests / wall, "all_ok": ok}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -266,6 +317,9 @@ def main():
     ap.add_argument("--batch-samples", type=int, default=256,
                     help="samples of the many-sample workload for the batched-vs-per-sample figure (0 skips it)")
     ap.add_argument("--batch-slots", type=int, default=4)
+    ap.add_argument("--serve-requests", type=int, default=60,
+                    help="requests of the serving figure (Poisson arrivals at --serve-rate; 0 skips it)")
+    ap.add_argument("--serve-rate", type=float, default=20.0)
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="first run one device-to-device copy of MIB MiB (a known-size kernel that calibrates "
                          "hardware byte counters)")
@@ -313,6 +367,18 @@ def main():
     N = st["kmers"]
     value = N / (dt / args.steps)
 
+    # PCIe-inclusive figure: FASTA bytes from host memory (swap_build stages them
+    # through pinned buffers), then the merge; one step after the timed ones
+    swapamd.hip().hipDeviceSynchronize()
+    t0 = time.perf_counter()
+    a.build(fa)
+    if has_merge:
+        a.merge()
+    swapamd.hip().hipDeviceSynchronize()
+    t_host = time.perf_counter() - t0
+    host_input = {"ms_per_step": 1000.0 * t_host, "value": a.stats()["kmers"] / t_host, "unit": "input k-mers/s",
+                  "input_gb_per_s": len(fa) / 1e9 / t_host}
+
     # dominant kernel of the last step
     reads = st["reads"]
     words = reads * ((len(fa.split(b"\n", 2)[1]) + 31) // 32)
@@ -341,6 +407,11 @@ def main():
         a.close()
         dev.close()
         batched = batched_vs_per_sample(args.batch_samples, args.batch_slots)
+    served = None
+    if args.serve_requests > 0:
+        a.close()
+        dev.close()
+        served = serving(args.serve_requests, args.serve_rate, args.batch_slots)
     line = {
         "metric": "input k-mers/s (build+merge) at 4.6Mbp/50x",
         "value": value,
@@ -361,7 +432,9 @@ def main():
         "cpu_baseline": cpu,
         "stats": {kk: st[kk] for kk in ("kmers", "distinct", "survivors", "kept", "overflow", "special")},
         "phase_ms": {PHASES[i]: round(v, 4) for i, v in phases.items()},
+        "host_input": host_input,
         "batched": batched,
+        "serving": served,
     }
     print(json.dumps(line))
     a.close()

@@ -11,15 +11,72 @@
 #include <memory>
 #include <numeric>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "swap_device.h"
 #include "swap_host.h"
 
+// Host-to-device staging of FASTA bytes: two pinned buffers; while the DMA
+// engine copies one to the device, host threads fill the other, so the
+// transfer runs at the rate of the slower of the two instead of their sum.
+struct Staging {
+    static constexpr size_t kChunk = 32u << 20;
+    static constexpr int kThreads = 4;  // host copy threads per chunk
+    char *buf[2] = {nullptr, nullptr};
+    hipEvent_t done[2] = {nullptr, nullptr};
+    bool ready = false;
+
+    hipError_t init() {
+        if (ready) return hipSuccess;
+        for (int b = 0; b < 2; b++) {
+            hipError_t e = hipHostMalloc((void **)&buf[b], kChunk, hipHostMallocDefault);
+            if (e != hipSuccess) return e;
+            e = hipEventCreateWithFlags(&done[b], hipEventDisableTiming);
+            if (e != hipSuccess) return e;
+        }
+        ready = true;
+        return hipSuccess;
+    }
+    ~Staging() {
+        for (int b = 0; b < 2; b++) {
+            if (buf[b]) (void)hipHostFree(buf[b]);
+            if (done[b]) (void)hipEventDestroy(done[b]);
+        }
+    }
+    // copies n host bytes to dst on stream st (asynchronous with respect to the
+    // host once the last chunk is queued)
+    hipError_t copy(void *dst, const char *src, size_t n, hipStream_t st) {
+        hipError_t e = init();
+        if (e != hipSuccess) return e;
+        for (int b = 0; b < 2; b++)  // a previous call's copies out of the buffers have ended
+            if ((e = hipEventSynchronize(done[b])) != hipSuccess) return e;
+        for (size_t off = 0, i = 0; off < n; off += kChunk, i++) {
+            const size_t len = std::min(kChunk, n - off);
+            const int b = (int)(i & 1);
+            if (i >= 2 && (e = hipEventSynchronize(done[b])) != hipSuccess) return e;
+            const size_t part = (len + kThreads - 1) / kThreads;
+            std::vector<std::thread> team;
+            for (int t = 1; t < kThreads; t++) {
+                const size_t lo = std::min(len, part * (size_t)t), hi = std::min(len, lo + part);
+                if (lo < hi) team.emplace_back([=] { std::memcpy(buf[b] + lo, src + off + lo, hi - lo); });
+            }
+            std::memcpy(buf[b], src + off, std::min(len, part));
+            for (auto &th : team) th.join();
+            if ((e = hipMemcpyAsync(static_cast<char *>(dst) + off, buf[b], len, hipMemcpyHostToDevice, st)) !=
+                hipSuccess)
+                return e;
+            if ((e = hipEventRecord(done[b], st)) != hipSuccess) return e;
+        }
+        return hipSuccess;
+    }
+};
+
 struct swap_ctx {
     swap_cfg cfg;
     std::unique_ptr<swapk::Pipeline> pipe;
     swapk::DevBuf input;
+    Staging staging;
     bool built = false, merged = false;
     double build_ms = 0, merge_ms = 0;
     std::string err;
@@ -172,7 +229,8 @@ int swap_build(swap_ctx *c, const char *fasta, size_t n) {
     if (n == 0) return set_err(c, SWAP_E_EMPTY, "empty input");
     if (hipSetDevice(c->cfg.device) != hipSuccess) return set_err(c, SWAP_E_DEVICE, "hipSetDevice");
     if (c->input.reserve(n + 16) != hipSuccess) return set_err(c, SWAP_E_DEVICE, "device allocation of the input");
-    if (hipMemcpy(c->input.p, fasta, n, hipMemcpyHostToDevice) != hipSuccess)
+    // the build's first kernels are queued behind the copies on the same stream
+    if (c->staging.copy(c->input.p, fasta, n, c->pipe->stream()) != hipSuccess)
         return set_err(c, SWAP_E_DEVICE, "copy of the input to the device");
     return swap_build_device(c, c->input.p, n);
 }
