@@ -325,7 +325,7 @@ def main():
                          "hardware byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path (sharding is described in DESIGN.md)",
+        print("bench.py measures one GPU: the k-mer graph path runs on one device and has no multi-GPU mode",
               file=sys.stderr)
         sys.exit(2)
 

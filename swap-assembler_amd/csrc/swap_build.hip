@@ -624,7 +624,7 @@ __device__ __forceinline__ uint64_t apply_cutoff(uint64_t cnt, int c) {
 }
 
 // Emits the survivors of one partition table and (reference order) the
-// first-occurrence index of every distinct key.  Slots are taken 8 per thread
+// first-occurrence index of every distinct key.  Slots are taken 4 per thread
 // per round; one atomic per workgroup and round reserves the output ranges.
 template <bool WITH_OCC, typename CNT>
 __device__ void table_emit(const uint64_t *tkey, CNT cnt_of, const uint32_t *tfo, uint32_t n_slots,
@@ -633,7 +633,7 @@ __device__ void table_emit(const uint64_t *tkey, CNT cnt_of, const uint32_t *tfo
     __shared__ unsigned long long base_s, base_d;
     const int hb = 2 * a.k - kL1Bits;
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
-    constexpr int kPer = 8;
+    constexpr int kPer = 4;  // one round over the 4096-slot table
     for (uint32_t r0 = 0; r0 < n_slots; r0 += blockDim.x * kPer) {
         uint64_t keys[kPer], mults[kPer];
         uint32_t fos[kPer];

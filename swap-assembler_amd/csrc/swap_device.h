@@ -63,6 +63,7 @@ struct EpochTabHost {
     int n = 0;
     uint64_t start[32] = {0};
     uint64_t P[32] = {0};
+    uint64_t M[32] = {0};  // floor((2^64 - 1) / P[e]): x mod P[e] by one high multiply (Barrett)
 };
 EpochTabHost epoch_table(uint64_t n_total, bool extra, const std::vector<uint64_t> *starts_override);
 

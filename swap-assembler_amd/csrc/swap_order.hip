@@ -21,7 +21,18 @@ struct EpochTab {
     int n;              // number of epochs
     uint64_t start[32]; // first insert rank of each epoch
     uint64_t P[32];     // bucket count during each epoch
+    uint64_t M[32];     // floor((2^64 - 1) / P[e])
 };
+
+// x mod P with M = floor((2^64 - 1) / P): q = floor(x * M / 2^64) is the
+// quotient or one less, so one conditional subtraction finishes the remainder
+// (instead of a 64-bit division)
+__device__ __forceinline__ uint64_t mod_p(uint64_t x, uint64_t P, uint64_t M) {
+    const uint64_t q = __umul64hi(x, M);
+    uint64_t r = x - q * P;
+    if (r >= P) r -= P;
+    return r;
+}
 
 __device__ __forceinline__ uint64_t dev_ref_hash(uint64_t x) {
     return (x >> 32) ^ (uint64_t)(int64_t)(int32_t)(uint32_t)(x & 0xFFFFFFFFull);
@@ -40,7 +51,7 @@ __global__ void k_ord_prep(const uint64_t *__restrict__ key, const uint64_t *__r
         if (et.start[j] <= r) e = j;
     h[i] = hv;
     ins[i] = (uint8_t)e;
-    bucket[i] = (uint32_t)(hv % et.P[et.n - 1]);
+    bucket[i] = (uint32_t)mod_p(hv, et.P[et.n - 1], et.M[et.n - 1]);
     idx[i] = (uint32_t)i;
 }
 
@@ -50,7 +61,7 @@ __device__ __forceinline__ bool ord_before(uint32_t a, uint32_t b, int e, const 
                                            const uint64_t *__restrict__ rank) {
     bool flip = false;
     for (; e >= 0; e--) {
-        const uint64_t ba = h[a] % et.P[e], bb = h[b] % et.P[e];
+        const uint64_t ba = mod_p(h[a], et.P[e], et.M[e]), bb = mod_p(h[b], et.P[e], et.M[e]);
         if (ba != bb) return (ba < bb) != flip;
         const bool ia = ins[a] == e, ib = ins[b] == e;
         if (ia && ib) return (rank[a] > rank[b]) != flip;
@@ -162,6 +173,7 @@ EpochTabHost epoch_table(uint64_t n_total, bool extra, const std::vector<uint64_
     for (int e = 0; e < t.n; e++) {
         if (e > 0) P = table_buckets(s[e] + 1, false);
         t.P[e] = P;
+        t.M[e] = ~0ull / P;
         t.start[e] = s[e];
         if (starts_override) t.start[e] = e < (int)starts_override->size() ? (*starts_override)[e] : ~0ull;
     }

@@ -6,8 +6,8 @@ its single-rank merge (from the oracle, whose files the GPU path reproduces
 byte for byte), and what the reference's own rank-0 code makes of that graph --
 its finisher, printContigs, the four clean-up rounds, masterRemoveCrossEdge,
 masterPrintContigs and printJungGraph -- run by oracle/_ref/ref_ce (built by
-`make -C oracle ref` from /root/reference).  Run here, where the reference
-exists; the GPU box only reads the fixtures.
+`make -C oracle ref` from /root/reference).  Run where the reference sources
+exist; the tests only read the fixtures.
 
 Stored per case (gzipped, zero timestamp):
   ce_<case>.fa.gz            input
