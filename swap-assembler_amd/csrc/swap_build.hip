@@ -496,10 +496,11 @@ __global__ void k_l1_chunks(const uint32_t *__restrict__ pstart, uint32_t *__res
 }
 
 // ---------------------------------------------------------------------------
-// Per-partition counting.  One workgroup owns one partition: its records go
-// into an open-addressing table in LDS (linear probing, key inserted by CAS),
-// the 8 edge counters of a slot live in one 64-bit word as saturating bytes
-// (min(count, 255) of graph.cpp:1074-1078), and the first-occurrence index is an
+// Per-partition counting.  A workgroup owns one partition at a time: its
+// records go into an open-addressing table in LDS (linear probing, key inserted
+// by CAS), the 8 edge counters of a slot are 16-bit LDS adds (partitions under
+// 2^16 records) or saturating bytes in one 64-bit word, min(count, 255) of
+// graph.cpp:1074-1078 either way, and the first-occurrence index is an
 // atomicMin.  A partition whose distinct keys overflow the LDS table is redone
 // by the global-memory variant of the same kernel.
 
@@ -684,59 +685,180 @@ __device__ void table_emit(const uint64_t *tkey, CNT cnt_of, const uint32_t *tfo
     }
 }
 
+// Records of the next partition each thread loads one partition ahead.
+// Partitions hold N / 2^15 records on average (about 4900 for a 50x bacterial
+// sample: under 5 per thread); the rest load inside the insert loop.
+constexpr int kPre = 6;
+// Partitions per workgroup whose bounds are staged in LDS (grid >= kParts / this).
+constexpr uint32_t kMaxPartsPerGroup = 512;
+
+// Persistent and software-pipelined: one workgroup per CU walks the partitions
+// blockIdx.x, blockIdx.x + gridDim.x, ...  Iteration i inserts partition i's
+// records (loaded during iteration i-1) into the LDS table; then each thread
+// stores its share of partition i-1's output (the output range reserved by one
+// device atomic per counter during iteration i-1), takes its 4 slots of
+// partition i into registers, clears them, and the workgroup's (survivors,
+// distinct) totals are scanned; thread 0 then reserves partition i's output
+// range while partition i+1's records load.  The global-memory latencies of a
+// partition (its record loads, the reservation, the output stores) thus overlap
+// the LDS work of its neighbours.  Slot j*kCountThreads + t belongs to thread t,
+// so the emit and the clear touch consecutive LDS words across a wave.
 template <bool WITH_OCC>
 __global__ void __launch_bounds__(kCountThreads) k_count_lds(CountArgs a) {
+    static_assert(kTab == 4 * kCountThreads, "the emit takes 4 slots per thread");
     __shared__ uint64_t tkey[kTab];
     __shared__ uint32_t tc[4 * kTab];  // 8 x u16 counters per slot, or 8 x u8 in the first 2*kTab words
     __shared__ uint32_t tfo[WITH_OCC ? kTab : 1];
-    __shared__ int overflow;
-    const uint32_t part = blockIdx.x;
-    const uint32_t b = a.bucket[part], e = a.bucket[part + 1];
-    const bool small = (e - b) < 0xFFFFu;
-    for (uint32_t i = threadIdx.x; i < kTab; i += blockDim.x) {
+    __shared__ uint64_t wsum[kCountThreads / 64];
+    __shared__ unsigned long long base_s, base_d;
+    __shared__ int overflow[2];  // set during the insert of iterations of each parity
+    __shared__ uint32_t bnd[kMaxPartsPerGroup + 1][2];  // [i] = bounds of this group's i-th partition
+    const uint32_t t = threadIdx.x;
+    const int lane = t & 63, w = t >> 6;
+    const int hb = 2 * a.k - kL1Bits;
+    const uint32_t total = a.bucket[kParts];  // >= 1: the host skips empty builds
+    uint64_t *tcnt = reinterpret_cast<uint64_t *>(tc);
+    for (uint32_t i = t; i < kTab; i += kCountThreads) {
         tkey[i] = kEmpty;
         if (WITH_OCC) tfo[i] = ~0u;
     }
-    for (uint32_t i = threadIdx.x; i < 4 * kTab; i += blockDim.x) tc[i] = 0;
-    if (threadIdx.x == 0) overflow = 0;
-    __syncthreads();
-    bool ok = true;
-    if (small) {
-        // two records in flight per thread: each load is issued one table
-        // update ahead of its use
-        const uint64_t st = blockDim.x, end = e;
-        uint64_t i = b + threadIdx.x;
-        uint32_t oa = 0, ob = 0;
-        uint64_t ra = i < end ? rec_load<WITH_OCC>(a.rec, i, oa) : 0;
-        uint64_t rb = i + st < end ? rec_load<WITH_OCC>(a.rec, i + st, ob) : 0;
-        for (; i < end && ok; i += 2 * st) {
-            ok = table_add16<WITH_OCC>(tkey, tc, tfo, kTab - 1, ra, oa);
-            if (i + 2 * st < end) ra = rec_load<WITH_OCC>(a.rec, i + 2 * st, oa);
-            if (i + st < end && ok) ok = table_add16<WITH_OCC>(tkey, tc, tfo, kTab - 1, rb, ob);
-            if (i + 3 * st < end) rb = rec_load<WITH_OCC>(a.rec, i + 3 * st, ob);
-        }
-    } else {
-        uint64_t *tcnt = reinterpret_cast<uint64_t *>(tc);
-        for (uint32_t i = b + threadIdx.x; i < e && ok; i += blockDim.x) {
-            uint32_t o;
-            const uint64_t r = rec_load<WITH_OCC>(a.rec, i, o);
-            ok = table_add<WITH_OCC>(tkey, tcnt, tfo, kTab - 1, r, o);
-        }
+    for (uint32_t i = t; i < 4 * kTab; i += kCountThreads) tc[i] = 0;
+    if (t < 2) overflow[t] = 0;
+    for (uint32_t i = t; i <= kMaxPartsPerGroup; i += kCountThreads) {
+        const uint64_t p = blockIdx.x + (uint64_t)i * gridDim.x;
+        bnd[i][0] = p < kParts ? a.bucket[p] : 0;
+        bnd[i][1] = p < kParts ? a.bucket[p + 1] : 0;
     }
-    if (!ok) overflow = 1;
     __syncthreads();
-    if (overflow) {
-        if (threadIdx.x == 0) {
-            const unsigned long long j = atomicAdd(&a.counters[2], 1ull);
-            a.overflow_list[j] = part;
+
+    // next partition's records: loaded from clamped indices so that every load
+    // is issued (entries past the partition's end are never used)
+    uint64_t pr[kPre];
+    uint32_t po[kPre];
+    uint32_t b = 0, e = 0;
+    auto prefetch = [&](uint32_t it) {
+        b = bnd[it][0];
+        e = bnd[it][1];
+#pragma unroll
+        for (int j = 0; j < kPre; j++) {
+            const uint64_t i = min((uint64_t)b + t + (uint64_t)j * kCountThreads, (uint64_t)total - 1);
+            pr[j] = rec_load<WITH_OCC>(a.rec, i, po[j]);
         }
-        return;
+    };
+    // previous partition, held until its output range is known
+    uint64_t keys[4], mults[4];
+    uint32_t fos[4];
+    uint64_t pv = 0, pinc = 0;
+    uint32_t prev_part = 0;
+    bool have_prev = false, prev_ovf = false;
+    unsigned long long pend_s = 0, pend_d = 0;
+#pragma unroll
+    for (int j = 0; j < 4; j++) keys[j] = kEmpty;
+
+    prefetch(0);
+    for (uint32_t part = blockIdx.x, it = 0;; part += gridDim.x, it++) {
+        const bool have = part < kParts;  // uniform over the workgroup
+        const bool small = (e - b) < 0xFFFFu;
+        bool ok = true;
+        if (have) {
+            if (small) {
+                // 16-bit counters cannot wrap below 2^16 records
+#pragma unroll
+                for (int j = 0; j < kPre; j++) {
+                    const uint64_t i = (uint64_t)b + t + (uint64_t)j * kCountThreads;
+                    if (i < e && ok) ok = table_add16<WITH_OCC>(tkey, tc, tfo, kTab - 1, pr[j], po[j]);
+                }
+                for (uint64_t i = (uint64_t)b + t + (uint64_t)kPre * kCountThreads; i < e && ok; i += kCountThreads) {
+                    uint32_t o;
+                    const uint64_t r = rec_load<WITH_OCC>(a.rec, i, o);
+                    ok = table_add16<WITH_OCC>(tkey, tc, tfo, kTab - 1, r, o);
+                }
+            } else {
+#pragma unroll
+                for (int j = 0; j < kPre; j++) {
+                    const uint64_t i = (uint64_t)b + t + (uint64_t)j * kCountThreads;
+                    if (i < e && ok) ok = table_add<WITH_OCC>(tkey, tcnt, tfo, kTab - 1, pr[j], po[j]);
+                }
+                for (uint64_t i = (uint64_t)b + t + (uint64_t)kPre * kCountThreads; i < e && ok; i += kCountThreads) {
+                    uint32_t o;
+                    const uint64_t r = rec_load<WITH_OCC>(a.rec, i, o);
+                    ok = table_add<WITH_OCC>(tkey, tcnt, tfo, kTab - 1, r, o);
+                }
+            }
+            if (!ok) overflow[it & 1] = 1;
+        }
+        if (t == 0 && have_prev && !prev_ovf) {
+            base_s = pend_s;
+            base_d = pend_d;
+        }
+        __syncthreads();  // table complete; previous output bases and wave offsets visible
+        if (have_prev && !prev_ovf) {
+            const uint64_t ex = pinc - pv + wsum[w];
+            uint64_t os = base_s + (uint32_t)ex, od = base_d + (ex >> 32);
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                if (keys[j] == kEmpty) continue;
+                if (WITH_OCC && a.dfo) a.dfo[od++] = fos[j];
+                if (mults[j]) {
+                    a.surv_key[os] = mix_inv(a.mx, ((uint64_t)(prev_part >> kL2Bits) << hb) | keys[j]);
+                    a.surv_mult[os] = mults[j];
+                    if (WITH_OCC) a.surv_first[os] = fos[j];
+                    os++;
+                }
+            }
+        }
+        if (!have) break;
+        const bool ovf = overflow[it & 1] != 0;
+        uint32_t ns = 0, nd = 0;
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const uint32_t s = (uint32_t)j * kCountThreads + t;
+            keys[j] = ovf ? kEmpty : tkey[s];
+            mults[j] = 0;
+            fos[j] = 0;
+            if (keys[j] != kEmpty) {
+                mults[j] = apply_cutoff(small ? counts16_to_bytes(tc, s) : tcnt[s], a.c);
+                if (WITH_OCC) fos[j] = tfo[s];
+                nd++;
+                ns += mults[j] != 0;
+            }
+            // a slot's key, counters and first occurrence are read only by its owner
+            tkey[s] = kEmpty;
+            if (WITH_OCC) tfo[s] = ~0u;
+            if (small) {
+#pragma unroll
+                for (int q = 0; q < 4; q++) tc[4 * s + q] = 0;
+            } else {
+                tcnt[s] = 0;
+            }
+        }
+        // workgroup exclusive scan of (survivors, distinct) packed in one u64
+        pv = ((uint64_t)nd << 32) | ns;
+        pinc = wave_incl_sum(pv);
+        if (lane == 63) wsum[w] = pinc;
+        if (part + gridDim.x < kParts) prefetch(it + 1);
+        __syncthreads();  // wave sums complete; the table is empty
+        if (t == 0) {
+            overflow[it & 1] = 0;
+            if (ovf) {
+                const unsigned long long j = atomicAdd(&a.counters[2], 1ull);
+                a.overflow_list[j] = part;
+            } else {
+                uint64_t run = 0;
+                for (int i = 0; i < kCountThreads / 64; i++) {
+                    const uint64_t x = wsum[i];
+                    wsum[i] = run;
+                    run += x;
+                }
+                const uint32_t ts = (uint32_t)run, td = (uint32_t)(run >> 32);
+                pend_s = ts ? atomicAdd(&a.counters[0], (unsigned long long)ts) : 0;
+                pend_d = td ? atomicAdd(&a.counters[1], (unsigned long long)td) : 0;
+            }
+        }
+        have_prev = true;
+        prev_ovf = ovf;
+        prev_part = part;
     }
-    if (small)
-        table_emit<WITH_OCC>(tkey, [&](uint32_t s) { return counts16_to_bytes(tc, s); }, tfo, kTab, part, a);
-    else
-        table_emit<WITH_OCC>(tkey, [&](uint32_t s) { return reinterpret_cast<const uint64_t *>(tc)[s]; }, tfo, kTab,
-                             part, a);
 }
 
 // Global-memory fallback for partitions with more distinct keys than the LDS
@@ -1181,10 +1303,14 @@ int Pipeline::build(const uint8_t *d_in, uint64_t n) {
     ca.overflow_list = overflow_.as<uint32_t>();
     ca.dfo = ref_order_ ? dfo_.as<uint32_t>() : nullptr;
     phase_begin(4);
+    // one resident workgroup per CU (the LDS table takes most of a CU's LDS),
+    // and at least the number whose partition bounds fit the kernel's LDS list
+    const unsigned count_grid = (unsigned)std::min<int>(
+        (int)kParts, std::max(ncu, (int)((kParts + kMaxPartsPerGroup - 1) / kMaxPartsPerGroup)));
     if (ref_order_)
-        hipLaunchKernelGGL(k_count_lds<true>, dim3(kParts), dim3(kCountThreads), 0, st_, ca);
+        hipLaunchKernelGGL(k_count_lds<true>, dim3(count_grid), dim3(kCountThreads), 0, st_, ca);
     else
-        hipLaunchKernelGGL(k_count_lds<false>, dim3(kParts), dim3(kCountThreads), 0, st_, ca);
+        hipLaunchKernelGGL(k_count_lds<false>, dim3(count_grid), dim3(kCountThreads), 0, st_, ca);
     phase_end(4);
     SWAP_CHECK(hipGetLastError());
     SWAP_CHECK(hipMemcpyAsync(hs + 8, d_small + 8, 3 * 8, hipMemcpyDeviceToHost, st_));
