@@ -519,6 +519,8 @@ struct CountArgs {
     uint64_t *surv_mult;
     uint32_t *surv_first;
     unsigned long long *counters;  // [0]=survivors [1]=distinct [2]=overflow partitions
+    uint32_t *part_surv;           // survivors of each partition (the survivor index sizes its regions by them)
+    uint32_t *part_first;          // first survivor of each partition's run, ~0u if not one run
     uint32_t *overflow_list;
     uint32_t *dfo;                 // first-occurrence index of every distinct key (reference order), or null
 };
@@ -666,6 +668,7 @@ __device__ void table_emit(const uint64_t *tkey, CNT cnt_of, const uint32_t *tfo
             const uint32_t ts = (uint32_t)run, td = (uint32_t)(run >> 32);
             base_s = ts ? atomicAdd(&a.counters[0], (unsigned long long)ts) : 0;
             base_d = td ? atomicAdd(&a.counters[1], (unsigned long long)td) : 0;
+            if (ts) atomicAdd(&a.part_surv[part], ts);
         }
         __syncthreads();
         const uint64_t ex = inc - v + wsum[w];
@@ -843,6 +846,8 @@ __global__ void __launch_bounds__(kCountThreads) k_count_lds(CountArgs a) {
             if (ovf) {
                 const unsigned long long j = atomicAdd(&a.counters[2], 1ull);
                 a.overflow_list[j] = part;
+                a.part_surv[part] = 0;  // counted by the global-memory pass
+                a.part_first[part] = ~0u;
             } else {
                 uint64_t run = 0;
                 for (int i = 0; i < kCountThreads / 64; i++) {
@@ -853,6 +858,8 @@ __global__ void __launch_bounds__(kCountThreads) k_count_lds(CountArgs a) {
                 const uint32_t ts = (uint32_t)run, td = (uint32_t)(run >> 32);
                 pend_s = ts ? atomicAdd(&a.counters[0], (unsigned long long)ts) : 0;
                 pend_d = td ? atomicAdd(&a.counters[1], (unsigned long long)td) : 0;
+                a.part_surv[part] = ts;
+                a.part_first[part] = (uint32_t)pend_s;
             }
         }
         have_prev = true;
@@ -1290,6 +1297,8 @@ int Pipeline::build(const uint8_t *d_in, uint64_t n) {
         SWAP_CHECK(dfo_.reserve(N * 4));
     }
     SWAP_CHECK(overflow_.reserve(kParts * 4));
+    SWAP_CHECK(part_surv_.reserve(kParts * 4));
+    SWAP_CHECK(part_first_.reserve(kParts * 4));
     CountArgs ca;
     ca.rec = rec_.as<uint32_t>();
     ca.bucket = hist_.as<uint32_t>();
@@ -1301,6 +1310,8 @@ int Pipeline::build(const uint8_t *d_in, uint64_t n) {
     ca.surv_first = ref_order_ ? surv_first_.as<uint32_t>() : nullptr;
     ca.counters = (unsigned long long *)(d_small + 8);
     ca.overflow_list = overflow_.as<uint32_t>();
+    ca.part_surv = part_surv_.as<uint32_t>();
+    ca.part_first = part_first_.as<uint32_t>();
     ca.dfo = ref_order_ ? dfo_.as<uint32_t>() : nullptr;
     phase_begin(4);
     // one resident workgroup per CU (the LDS table takes most of a CU's LDS),

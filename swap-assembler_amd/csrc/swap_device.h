@@ -145,10 +145,10 @@ private:
     // build buffers
     DevBuf fasta_, tile_nl_, tile_off_, nl_pos_, scan_scratch_, small_, rec_start_, rec_len_, occ_count_,
         word_count_, occ_base_, word_base_, packed_, wmeta_, rows_, hist_, bucket_, cursor_, hist1_, base1_, l1info_, rec1_, rec_, surv_key_, surv_mult_,
-        surv_first_, dfo_, sel_, sel_target_, sel_out_, overflow_, gtab_key_, gtab_cnt_, gtab_fo_;
+        surv_first_, dfo_, sel_, sel_target_, sel_out_, overflow_, part_surv_, part_first_, gtab_key_, gtab_cnt_, gtab_fo_;
     // merge buffers
     enum {
-        M_IKEY, M_IVAL, M_PRIO, M_IS11, M_NXT, M_EX, M_SELF, M_WALKA, M_WALKB, M_SPA, M_SPB, M_FLAG, M_POS,
+        M_IENT, M_IBASE, M_PRIO, M_IS11, M_NXT, M_EX, M_SELF, M_WALKA, M_WALKB, M_SPA, M_SPB, M_FLAG, M_POS,
         M_KEPTOF, M_KEPTSRC, M_ELEN, M_EMULT, M_EOTHER, M_CSIZE, M_ESPEC, M_EOFF, M_COFF, M_POOL, M_CPRI,
         M_CMUL, M_CEDGE, M_LEV, M_LEVOFF, M_PAR, M_NEED, M_NL, M_NR, M_INL, M_INR, M_CNT, M_SPEC, M_COUNT
     };

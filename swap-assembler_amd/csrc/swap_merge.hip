@@ -45,34 +45,109 @@ constexpr uint64_t kIdxEmpty = ~0ull;
 // ---------------------------------------------------------------------------
 // survivor index
 
-__device__ __forceinline__ uint32_t idx_hash(uint64_t key, uint32_t mask) {
-    return (uint32_t)((key * 0xD6E8FEB86659FD93ull) >> 32) & mask;
+// The survivors of count partition p (p = top kPartBits bits of the mixed key,
+// the partition the count kernel took them from) own one open-addressing
+// region of entries [base[p], base[p+1]); its size is a power of two >= 1.5x
+// their number (0 for a partition without survivors).  An entry is 16 bytes
+// (key, survivor id), read by one load.
+
+struct SurvIndex {
+    const uint64_t *ent;   // 2 words per entry: key (kIdxEmpty if free), survivor id
+    const uint64_t *base;  // kParts + 1 region starts
+    Mixer mx;
+    int shift;             // 2k - kPartBits
+};
+
+__device__ __forceinline__ uint32_t region_size(uint32_t ns) {
+    if (!ns) return 0;
+    const uint32_t want = ns + (ns >> 1) + 1;
+    return 1u << (32 - __clz(want - 1));
 }
 
-__global__ void k_index_build(const uint64_t *__restrict__ key, uint64_t n, uint64_t *ikey, uint32_t *ival,
-                              uint32_t mask) {
+__global__ void k_index_sizes(const uint32_t *__restrict__ part_surv, uint32_t *__restrict__ size) {
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p < kParts) size[p] = region_size(part_surv[p]);
+}
+
+// Regions built in LDS, one workgroup per partition whose survivors form one
+// run of the survivor array and whose region fits kIdxLds entries: the region
+// is written whole (free entries included) with coalesced stores.  Other
+// regions are only cleared here and filled by k_index_loose.
+constexpr uint32_t kIdxLds = 4096;
+
+__global__ void __launch_bounds__(256) k_index_part(const uint64_t *__restrict__ key,
+                                                    const uint32_t *__restrict__ part_surv,
+                                                    const uint32_t *__restrict__ part_first, SurvIndex ix,
+                                                    uint64_t *__restrict__ ent) {
+    __shared__ unsigned long long lkey[kIdxLds];
+    __shared__ uint32_t lid[kIdxLds];
+    const uint32_t p = blockIdx.x;
+    const uint64_t b = ix.base[p];
+    const uint32_t size = (uint32_t)(ix.base[p + 1] - b);
+    if (!size) return;
+    const uint32_t first = part_first[p];
+    if (first == ~0u || size > kIdxLds) {
+        for (uint32_t s = threadIdx.x; s < 2 * size; s += blockDim.x) ent[2 * b + s] = kIdxEmpty;
+        return;
+    }
+    for (uint32_t s = threadIdx.x; s < size; s += blockDim.x) lkey[s] = kIdxEmpty;
+    __syncthreads();
+    const uint32_t ns = part_surv[p], mask = size - 1;
+    for (uint32_t i = threadIdx.x; i < ns; i += blockDim.x) {
+        const uint64_t k = key[first + i];
+        uint32_t s = (uint32_t)mix_fwd(ix.mx, k) & mask;
+        for (uint32_t probe = 0; probe <= mask; probe++) {  // the region is at most 2/3 full
+            const unsigned long long prev = atomicCAS(&lkey[s], (unsigned long long)kIdxEmpty, (unsigned long long)k);
+            if (prev == kIdxEmpty) {
+                lid[s] = first + i;
+                break;
+            }
+            s = (s + 1) & mask;
+        }
+    }
+    __syncthreads();
+    for (uint32_t s = threadIdx.x; s < size; s += blockDim.x) {
+        const uint64_t k = lkey[s];
+        reinterpret_cast<ulonglong2 *>(ent)[b + s] = make_ulonglong2(k, k == kIdxEmpty ? kIdxEmpty : (uint64_t)lid[s]);
+    }
+}
+
+// Survivors of the regions k_index_part left empty: device-memory inserts.
+__global__ void k_index_loose(const uint64_t *__restrict__ key, uint64_t n, const uint32_t *__restrict__ part_first,
+                              SurvIndex ix, uint64_t *ent) {
     const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const uint64_t k = key[i];
-    uint32_t s = idx_hash(k, mask);
-    for (uint32_t probe = 0; probe <= mask; probe++) {  // the table is at most half full
-        const uint64_t prev = atomicCAS((unsigned long long *)&ikey[s], (unsigned long long)kIdxEmpty,
+    const uint64_t h = mix_fwd(ix.mx, k);
+    const uint64_t p = h >> ix.shift;
+    const uint64_t b = ix.base[p];
+    const uint64_t size = ix.base[p + 1] - b;  // the region holds key's partition: size >= 2
+    if (part_first[p] != ~0u && size <= kIdxLds) return;
+    const uint64_t mask = size - 1;
+    uint64_t s = h & mask;
+    for (uint64_t probe = 0; probe <= mask; probe++) {  // the region is at most 2/3 full
+        const uint64_t prev = atomicCAS((unsigned long long *)&ent[2 * (b + s)], (unsigned long long)kIdxEmpty,
                                         (unsigned long long)k);
-        if (prev == kIdxEmpty || prev == k) {
-            ival[s] = (uint32_t)i;
+        if (prev == kIdxEmpty) {
+            ent[2 * (b + s) + 1] = i;
             return;
         }
         s = (s + 1) & mask;
     }
 }
 
-__device__ __forceinline__ uint32_t index_find(const uint64_t *__restrict__ ikey, const uint32_t *__restrict__ ival,
-                                               uint32_t mask, uint64_t k) {
-    uint32_t s = idx_hash(k, mask);
-    for (uint32_t probe = 0; probe <= mask; probe++) {
-        const uint64_t cur = ikey[s];
-        if (cur == k) return ival[s];
-        if (cur == kIdxEmpty) return kNil;
+__device__ __forceinline__ uint32_t index_find(const SurvIndex &ix, uint64_t k) {
+    const uint64_t h = mix_fwd(ix.mx, k);
+    const uint64_t p = h >> ix.shift;
+    const uint64_t b = ix.base[p];
+    const uint64_t size = ix.base[p + 1] - b;
+    if (!size) return kNil;
+    const uint64_t mask = size - 1;
+    uint64_t s = h & mask;
+    for (uint64_t probe = 0; probe <= mask; probe++) {
+        const ulonglong2 e = reinterpret_cast<const ulonglong2 *>(ix.ent)[b + s];
+        if (e.x == k) return (uint32_t)e.y;
+        if (e.x == kIdxEmpty) return kNil;
         s = (s + 1) & mask;
     }
     return kNil;
@@ -86,9 +161,7 @@ struct MergeArgs {
     const uint64_t *mult;
     uint64_t S;
     int k;
-    const uint64_t *ikey;
-    const uint32_t *ival;
-    uint32_t imask;
+    SurvIndex ix;
 };
 
 __device__ __forceinline__ uint64_t orient_kmer(uint64_t X, int slot, int k) { return slot >= 4 ? revcomp(X, k) : X; }
@@ -99,7 +172,7 @@ __device__ __forceinline__ uint32_t hop(const MergeArgs &a, uint64_t X, int s) {
     const uint64_t t = ((orient_kmer(X, s, a.k) << 2) | (uint64_t)(s & 3)) & km;
     const uint64_t r = revcomp(t, a.k);
     const uint64_t Y = t > r ? t : r;
-    const uint32_t j = index_find(a.ikey, a.ival, a.imask, Y);
+    const uint32_t j = index_find(a.ix, Y);
     if (j == kNil) return kNil;
     return (j << 1) | (t == Y ? 0u : 1u);
 }
@@ -659,15 +732,25 @@ int Pipeline::merge_device(bool reference_order, MergeDev &out) {
     uint64_t *hs = h_small_;
     MCHECK(hipMemsetAsync(d_small + 16, 0, 16 * 8, st_));
     unsigned long long *cnt = (unsigned long long *)(d_small + 16);
-    // --- survivor index
-    uint32_t cap = 1024;
-    while (cap < 2 * S) cap <<= 1;
-    MCHECK(mb_[M_IKEY].reserve((uint64_t)cap * 8));
-    MCHECK(mb_[M_IVAL].reserve((uint64_t)cap * 4));
-    MCHECK(hipMemsetAsync(mb_[M_IKEY].p, 0xFF, (uint64_t)cap * 8, st_));
+    // --- survivor index: one region per count partition
+    const uint64_t ent_bound = 3 * S + 2 * (uint64_t)kParts;  // region_size(n) <= 3n + 2
+    MCHECK(mb_[M_IBASE].reserve((2 * (uint64_t)kParts + 1) * 8));
+    MCHECK(mb_[M_IENT].reserve(ent_bound * 16));
+    MCHECK(scan_scratch_.reserve((scan_scratch_words(kParts) + 16) * 8));
+    uint64_t *ibase = mb_[M_IBASE].as<uint64_t>();
+    uint32_t *isize = reinterpret_cast<uint32_t *>(ibase + kParts + 1);
+    SurvIndex ix;
+    ix.ent = mb_[M_IENT].as<uint64_t>();
+    ix.base = ibase;
+    ix.mx = mx_;
+    ix.shift = 2 * k_ - kPartBits;
     phase_begin(8);
-    hipLaunchKernelGGL(k_index_build, dim3(nblk(S)), dim3(256), 0, st_, surv_key_.as<uint64_t>(), S,
-                       mb_[M_IKEY].as<uint64_t>(), mb_[M_IVAL].as<uint32_t>(), cap - 1);
+    hipLaunchKernelGGL(k_index_sizes, dim3(kParts / 256), dim3(256), 0, st_, part_surv_.as<uint32_t>(), isize);
+    MCHECK(scan_excl<uint32_t>(isize, ibase, kParts, scan_scratch_.as<uint64_t>(), ibase + kParts, st_));
+    hipLaunchKernelGGL(k_index_part, dim3(kParts), dim3(256), 0, st_, surv_key_.as<uint64_t>(),
+                       part_surv_.as<uint32_t>(), part_first_.as<uint32_t>(), ix, mb_[M_IENT].as<uint64_t>());
+    hipLaunchKernelGGL(k_index_loose, dim3(nblk(S)), dim3(256), 0, st_, surv_key_.as<uint64_t>(), S,
+                       part_first_.as<uint32_t>(), ix, mb_[M_IENT].as<uint64_t>());
     // --- priorities (visit time)
     MCHECK(mb_[M_PRIO].reserve(S * 8));
     phase_end(8);
@@ -687,9 +770,7 @@ int Pipeline::merge_device(bool reference_order, MergeDev &out) {
     ma.mult = surv_mult_.as<uint64_t>();
     ma.S = S;
     ma.k = k_;
-    ma.ikey = mb_[M_IKEY].as<uint64_t>();
-    ma.ival = mb_[M_IVAL].as<uint32_t>();
-    ma.imask = cap - 1;
+    ma.ix = ix;
     MCHECK(mb_[M_IS11].reserve(S));
     MCHECK(mb_[M_NXT].reserve(2 * S * 4));
     MCHECK(mb_[M_EX].reserve(2 * S));
