@@ -60,6 +60,9 @@ extern "C" {
                                     replaying the reference's table order; topology,
                                     edge strings and contigs are the same, merged
                                     multiplicities follow the different order */
+#define SWAP_FLAG_TABLE_SPILL 4u /* testing aid: every count partition is recounted in the
+                                    device-memory table that partitions too large for the
+                                    on-chip table use; same results, slower */
 
 typedef struct swap_cfg {
     int k;          /* odd k-mer length, 9..31 (the reference CLI accepts 19..31) */

@@ -522,6 +522,7 @@ struct CountArgs {
     uint32_t *part_surv;           // survivors of each partition (the survivor index sizes its regions by them)
     uint32_t *part_first;          // first survivor of each partition's run, ~0u if not one run
     uint32_t *overflow_list;
+    int spill;                     // send every partition to the device-memory table
     uint32_t *dfo;                 // first-occurrence index of every distinct key (reference order), or null
 };
 
@@ -811,7 +812,7 @@ __global__ void __launch_bounds__(kCountThreads) k_count_lds(CountArgs a) {
             }
         }
         if (!have) break;
-        const bool ovf = overflow[it & 1] != 0;
+        const bool ovf = overflow[it & 1] != 0 || a.spill;
         uint32_t ns = 0, nd = 0;
 #pragma unroll
         for (int j = 0; j < 4; j++) {
@@ -1312,6 +1313,7 @@ int Pipeline::build(const uint8_t *d_in, uint64_t n) {
     ca.overflow_list = overflow_.as<uint32_t>();
     ca.part_surv = part_surv_.as<uint32_t>();
     ca.part_first = part_first_.as<uint32_t>();
+    ca.spill = table_spill_ ? 1 : 0;
     ca.dfo = ref_order_ ? dfo_.as<uint32_t>() : nullptr;
     phase_begin(4);
     // one resident workgroup per CU (the LDS table takes most of a CU's LDS),
@@ -1338,7 +1340,7 @@ int Pipeline::build(const uint8_t *d_in, uint64_t n) {
         for (uint64_t j = 0; j < n_overflow; j++) {
             const uint32_t p = list[j];
             const uint64_t cnt = (uint64_t)starts[p + 1] - starts[p];
-            uint32_t b = kTabBits + 1;
+            uint32_t b = 6;  // at least 2x the records (> 2x the distinct keys)
             while ((1ull << b) < 2 * cnt) b++;
             bits[j] = b;
             off[j] = tot;

@@ -206,6 +206,7 @@ int swap_create(const swap_cfg *cfg, swap_ctx **out) {
         delete c;
         return SWAP_E_DEVICE;
     }
+    c->pipe->set_table_spill((cfg->flags & SWAP_FLAG_TABLE_SPILL) != 0);
     *out = c;
     return SWAP_OK;
 }
@@ -308,6 +309,7 @@ int swap_reconfigure(swap_ctx *c, const swap_cfg *cfg) {
     if (cfg->device != c->cfg.device) return set_err(c, SWAP_E_INVAL, "a context stays on its device");
     c->cfg = *cfg;
     c->pipe->reconfigure(cfg->k, cfg->c, !(cfg->flags & SWAP_FLAG_HASH_ORDER));
+    c->pipe->set_table_spill((cfg->flags & SWAP_FLAG_TABLE_SPILL) != 0);
     c->built = c->merged = c->have_graph = c->have_compacted = c->have_ce = false;
     return SWAP_OK;
 }

@@ -90,6 +90,8 @@ public:
     ~Pipeline();
     // New k, c and order mode for the next build; device buffers are kept.
     void reconfigure(int k, int c, bool reference_order);
+    // every count partition through the device-memory table (a testing aid)
+    void set_table_spill(bool on) { table_spill_ = on; }
     int device() const { return dev_; }
     hipStream_t stream() const { return st_; }
 
@@ -138,6 +140,7 @@ private:
     int dev_;
     int k_, c_;
     bool ref_order_;
+    bool table_spill_ = false;
     hipStream_t st_ = nullptr;
     hipError_t init_err_ = hipSuccess;
     Mixer mx_;

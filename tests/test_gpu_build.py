@@ -22,10 +22,11 @@ def gold_fa(name):
         return f.read()
 
 
-def check_build(fa, k, c, hash_order=False, check_file=True):
-    print("case k=%d c=%d bytes=%d hash_order=%s" % (k, c, len(fa), hash_order), flush=True)
+def check_build(fa, k, c, hash_order=False, check_file=True, table_spill=False):
+    print("case k=%d c=%d bytes=%d hash_order=%s table_spill=%s" % (k, c, len(fa), hash_order, table_spill),
+          flush=True)
     o = Oracle(fa, k, c, kmer_graph_only=True)
-    a = swapamd.Assembler(k, c, kmer_graph_only=True, hash_order=hash_order)
+    a = swapamd.Assembler(k, c, kmer_graph_only=True, hash_order=hash_order, table_spill=table_spill)
     a.build(fa)
     st = a.stats()
     ost = o.stats()
@@ -58,6 +59,34 @@ def test_golden_inputs(name):
 def test_k_values(k):
     fa = swapamd.generate_params(30000, 90, 20.0, 0.01, 100 + k)
     check_build(fa, k, 2)
+
+
+@pytest.mark.parametrize("name", sorted(CASES))
+def test_device_memory_table(name):
+    # every partition counted by the device-memory table that partitions with
+    # more distinct keys than the on-chip table take
+    case = CASES[name]
+    check_build(gold_fa(name), case["k"], case["c"], table_spill=True)
+
+
+def poly_a_mix(seed):
+    """1000 poly-A reads (70,000 occurrences of one k-mer at k=31: a partition
+    past 2^16 records, counted with saturating 8-bit counters) among reads of a
+    random genome."""
+    rng = np.random.default_rng(seed)
+    g = "".join("ACGT"[x] for x in rng.integers(0, 4, 20000))
+    out = []
+    for i in range(1000):
+        out.append(">a%d\n%s\n" % (i, "A" * 100))
+    for i in range(4000):
+        p = int(rng.integers(0, len(g) - 100))
+        out.append(">r%d\n%s\n" % (i, g[p:p + 100]))
+    return "".join(out).encode()
+
+
+def test_partition_past_16bit_counters():
+    check_build(poly_a_mix(7), 31, 2)
+    check_build(poly_a_mix(7), 31, 2, table_spill=True)
 
 
 def test_hash_order_same_graph():

@@ -35,10 +35,11 @@ def strip_mult(text):
     return sorted(out)
 
 
-def check_merge(fa, k, c, hash_order=False):
-    print("merge case k=%d c=%d bytes=%d hash_order=%s" % (k, c, len(fa), hash_order), flush=True)
+def check_merge(fa, k, c, hash_order=False, table_spill=False):
+    print("merge case k=%d c=%d bytes=%d hash_order=%s table_spill=%s" % (k, c, len(fa), hash_order, table_spill),
+          flush=True)
     o = Oracle(fa, k, c)
-    a = swapamd.Assembler(k, c, hash_order=hash_order)
+    a = swapamd.Assembler(k, c, hash_order=hash_order, table_spill=table_spill)
     a.build(fa)
     a.merge()
     cg = a.file_bytes(swapamd.SWAP_FILE_CONTIG_GRAPH)
@@ -75,6 +76,19 @@ def test_golden_inputs(name):
 def test_golden_inputs_hash_order(name):
     case = CASES[name]
     check_merge(gold_fa(name), case["k"], case["c"], hash_order=True)
+
+
+@pytest.mark.parametrize("name", ["repeats_k25_c3", "uniform_k31_c3"])
+def test_device_memory_table(name):
+    # survivors counted by the device-memory table come in several runs per
+    # partition: the survivor index builds those regions with device atomics
+    case = CASES[name]
+    check_merge(gold_fa(name), case["k"], case["c"], table_spill=True)
+
+
+def test_partition_past_16bit_counters():
+    from test_gpu_build import poly_a_mix
+    check_merge(poly_a_mix(7), 31, 2)
 
 
 def circular_reads(seed, G, L, n, err):
