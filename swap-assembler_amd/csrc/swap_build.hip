@@ -497,11 +497,11 @@ __global__ void k_l1_chunks(const uint32_t *__restrict__ pstart, uint32_t *__res
 
 // ---------------------------------------------------------------------------
 // Per-partition counting.  A workgroup owns one partition at a time: its
-// records go into an open-addressing table in LDS (linear probing, key inserted
-// by CAS), the 8 edge counters of a slot are 16-bit LDS adds (partitions under
-// 2^16 records) or saturating bytes in one 64-bit word, min(count, 255) of
-// graph.cpp:1074-1078 either way, and the first-occurrence index is an
-// atomicMin.  A partition whose distinct keys overflow the LDS table is redone
+// records go into an open-addressing table in LDS (key inserted by CAS), the 8
+// edge counters of a slot are 16-bit LDS adds with 4-slot buckets probed in
+// order (partitions under 2^16 records) or saturating bytes in one 64-bit word
+// with linear probing, min(count, 255) of graph.cpp:1074-1078 either way, and
+// the first-occurrence index is an atomicMin.  A partition whose distinct keys overflow the LDS table is redone
 // by the global-memory variant of the same kernel.
 
 constexpr int kTabBits = 12;
@@ -565,45 +565,92 @@ __device__ __forceinline__ bool table_add(uint64_t *tkey, uint64_t *tcnt, uint32
     return false;
 }
 
-// Same probe, 16-bit counters: 8 per slot in 4 words, incremented with
-// non-returning LDS adds.  Used when the partition has fewer than 2^16
-// records, so no counter can wrap; min(count, 255) is taken at emit.
+// Edge counters of one occurrence on slot s (16-bit counters, non-returning
+// LDS adds).  An occurrence has at most two edge bits (one per side).
+__device__ __forceinline__ void add_edges16(uint32_t *tc, uint32_t s, uint32_t m) {
+    if (!m) return;
+    const int t0 = __ffs(m) - 1;
+    atomicAdd(&tc[4 * s + (t0 >> 1)], 1u << (16 * (t0 & 1)));
+    m &= m - 1;
+    if (!m) return;
+    const int t1 = __ffs(m) - 1;
+    atomicAdd(&tc[4 * s + (t1 >> 1)], 1u << (16 * (t1 & 1)));
+    m &= m - 1;
+    while (m) {  // not produced by the occurrence pass; kept for completeness
+        const int t = __ffs(m) - 1;
+        m &= m - 1;
+        atomicAdd(&tc[4 * s + (t >> 1)], 1u << (16 * (t & 1)));
+    }
+}
+
+// Bucketed probing for the 16-bit table: a key's home is a 4-slot bucket (32
+// bytes of keys, read by one thread in two 16-byte LDS reads), buckets are
+// probed in order.  Keys enter the first empty slot by CAS, so the filled
+// slots of the table form a prefix of every key's probe sequence and a key is
+// never stored twice.
+constexpr uint32_t kBucketSlots = 4;
+constexpr uint32_t kBuckets = kTab / kBucketSlots;
+
+__device__ __forceinline__ uint32_t home_bucket(uint64_t key) {
+    return (uint32_t)((key * 0x9E3779B97F4A7C15ull) >> 40) & (kBuckets - 1);
+}
+
+// Finds or inserts key starting at bucket b0 (fresh reads, one bucket at a
+// time); returns its slot, or ~0u when the table is full.
+__device__ uint32_t bucket_probe(uint64_t *tkey, uint64_t key, uint32_t b0) {
+    for (uint32_t n = 0; n < kBuckets; n++) {
+        const uint32_t s0 = ((b0 + n) & (kBuckets - 1)) * kBucketSlots;
+        const ulonglong2 v01 = *reinterpret_cast<const ulonglong2 *>(&tkey[s0]);
+        const ulonglong2 v23 = *reinterpret_cast<const ulonglong2 *>(&tkey[s0 + 2]);
+        const uint64_t v[4] = {v01.x, v01.y, v23.x, v23.y};
+#pragma unroll
+        for (uint32_t q = 0; q < kBucketSlots; q++) {
+            if (v[q] == key) return s0 + q;
+            if (v[q] == kEmpty) {
+                const uint64_t prev = atomicCAS((unsigned long long *)&tkey[s0 + q], (unsigned long long)kEmpty,
+                                                (unsigned long long)key);
+                if (prev == kEmpty || prev == key) return s0 + q;
+            }
+        }
+    }
+    return ~0u;
+}
+
+// Inserts / updates one record in the 16-bit table: one read of its home
+// bucket, a CAS on the bucket's first empty slot when the key is not there,
+// and bucket_probe only when the bucket is full of other keys or the CAS lost
+// to another key.  Returns false when the table is full.
 template <bool WITH_OCC>
-__device__ __forceinline__ bool table_add16(uint64_t *tkey, uint32_t *tc, uint32_t *tfo, uint32_t mask, uint64_t rec,
+__device__ __forceinline__ bool table_add16(uint64_t *tkey, uint32_t *tc, uint32_t *tfo, uint64_t rec,
                                             uint32_t occ) {
     const uint64_t key = rec >> 8;
-    uint32_t m = (uint32_t)(rec & 0xFF);
-    uint32_t s = slot_hash(key, mask);
-    for (uint32_t probe = 0; probe <= mask; probe++) {
-        uint64_t cur = tkey[s];
-        if (cur == kEmpty) {
-            const uint64_t prev = atomicCAS((unsigned long long *)&tkey[s], (unsigned long long)kEmpty,
-                                            (unsigned long long)key);
-            cur = (prev == kEmpty) ? key : prev;
+    const uint32_t hb = home_bucket(key), s0 = hb * kBucketSlots;
+    const ulonglong2 v01 = *reinterpret_cast<const ulonglong2 *>(&tkey[s0]);
+    const ulonglong2 v23 = *reinterpret_cast<const ulonglong2 *>(&tkey[s0 + 2]);
+    const uint64_t v[4] = {v01.x, v01.y, v23.x, v23.y};
+    // first slot of the bucket that holds the key or is empty
+    uint32_t slot = ~0u;
+    int cq = -1;
+#pragma unroll
+    for (int q = kBucketSlots - 1; q >= 0; q--) {
+        if (v[q] == key) {
+            slot = s0 + q;
+            cq = -1;
+        } else if (v[q] == kEmpty) {
+            slot = ~0u;
+            cq = q;
         }
-        if (cur == key) {
-            // an occurrence has at most two edge bits (one per side): straight-line adds
-            if (m) {
-                const int t0 = __ffs(m) - 1;
-                atomicAdd(&tc[4 * s + (t0 >> 1)], 1u << (16 * (t0 & 1)));
-                m &= m - 1;
-                if (m) {
-                    const int t1 = __ffs(m) - 1;
-                    atomicAdd(&tc[4 * s + (t1 >> 1)], 1u << (16 * (t1 & 1)));
-                    m &= m - 1;
-                    while (m) {  // not produced by the occurrence pass; kept for completeness
-                        const int t = __ffs(m) - 1;
-                        m &= m - 1;
-                        atomicAdd(&tc[4 * s + (t >> 1)], 1u << (16 * (t & 1)));
-                    }
-                }
-            }
-            if (WITH_OCC) atomicMin(&tfo[s], occ);
-            return true;
-        }
-        s = (s + 1) & mask;
     }
-    return false;
+    if (cq >= 0) {
+        const uint64_t prev = atomicCAS((unsigned long long *)&tkey[s0 + cq], (unsigned long long)kEmpty,
+                                        (unsigned long long)key);
+        if (prev == kEmpty || prev == key) slot = s0 + cq;
+    }
+    if (slot == ~0u) slot = bucket_probe(tkey, key, hb);
+    if (slot == ~0u) return false;
+    if (WITH_OCC) atomicMin(&tfo[slot], occ);
+    add_edges16(tc, slot, (uint32_t)(rec & 0xFF));
+    return true;
 }
 
 __device__ __forceinline__ uint64_t counts16_to_bytes(const uint32_t *tc, uint32_t s) {
@@ -710,7 +757,7 @@ constexpr uint32_t kMaxPartsPerGroup = 512;
 template <bool WITH_OCC>
 __global__ void __launch_bounds__(kCountThreads) k_count_lds(CountArgs a) {
     static_assert(kTab == 4 * kCountThreads, "the emit takes 4 slots per thread");
-    __shared__ uint64_t tkey[kTab];
+    __shared__ __align__(16) uint64_t tkey[kTab];
     __shared__ uint32_t tc[4 * kTab];  // 8 x u16 counters per slot, or 8 x u8 in the first 2*kTab words
     __shared__ uint32_t tfo[WITH_OCC ? kTab : 1];
     __shared__ uint64_t wsum[kCountThreads / 64];
@@ -770,12 +817,12 @@ __global__ void __launch_bounds__(kCountThreads) k_count_lds(CountArgs a) {
 #pragma unroll
                 for (int j = 0; j < kPre; j++) {
                     const uint64_t i = (uint64_t)b + t + (uint64_t)j * kCountThreads;
-                    if (i < e && ok) ok = table_add16<WITH_OCC>(tkey, tc, tfo, kTab - 1, pr[j], po[j]);
+                    if (i < e && ok) ok = table_add16<WITH_OCC>(tkey, tc, tfo, pr[j], po[j]);
                 }
                 for (uint64_t i = (uint64_t)b + t + (uint64_t)kPre * kCountThreads; i < e && ok; i += kCountThreads) {
                     uint32_t o;
                     const uint64_t r = rec_load<WITH_OCC>(a.rec, i, o);
-                    ok = table_add16<WITH_OCC>(tkey, tc, tfo, kTab - 1, r, o);
+                    ok = table_add16<WITH_OCC>(tkey, tc, tfo, r, o);
                 }
             } else {
 #pragma unroll
