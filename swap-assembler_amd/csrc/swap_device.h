@@ -153,7 +153,7 @@ private:
     enum {
         M_IENT, M_IBASE, M_PRIO, M_IS11, M_NXT, M_EX, M_SELF, M_WALKA, M_WALKB, M_SPA, M_SPB, M_FLAG, M_POS,
         M_KEPTOF, M_KEPTSRC, M_ELEN, M_EMULT, M_EOTHER, M_CSIZE, M_ESPEC, M_EOFF, M_COFF, M_POOL, M_CPRI,
-        M_CMUL, M_CEDGE, M_LEV, M_LEVOFF, M_PAR, M_NEED, M_NL, M_NR, M_INL, M_INR, M_CNT, M_SPEC, M_COUNT
+        M_CMUL, M_CEDGE, M_LEV, M_LEVOFF, M_PAR, M_NEED, M_NL, M_NR, M_INL, M_INR, M_CNT, M_SPEC, M_PROPS, M_COUNT
     };
     DevBuf mb_[M_COUNT];
     enum { O_H, O_INS, O_BKT, O_BKT2, O_IDX, O_TMP, O_RANK, O_ORD, O_COUNT };

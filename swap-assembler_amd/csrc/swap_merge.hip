@@ -229,16 +229,22 @@ struct Walk {
 // (ruler, offset); pointer jumping then runs over the rulers only.
 constexpr uint32_t kRuleStride = 16;
 
-__global__ void k_rs_mark(const uint8_t *__restrict__ is11, const uint32_t *__restrict__ nxt, uint64_t S,
-                          uint32_t *__restrict__ rflag, uint32_t stride) {
+// rflag[v] (ruler or not, for the ruler numbering scan) and props[v]: bit 0
+// (1,1) node, bit 1 ruler, bit 2 self-adjacent, so that k_rs_prep reads one
+// byte of each successor.
+__global__ void k_rs_mark(const uint8_t *__restrict__ is11, const uint32_t *__restrict__ nxt,
+                          const uint8_t *__restrict__ selfadj, uint64_t S, uint32_t *__restrict__ rflag,
+                          uint8_t *__restrict__ props, uint32_t stride) {
     const uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= 2 * S) return;
     uint32_t f = 0;
-    if (is11[v >> 1]) {
+    const bool one = is11[v >> 1];
+    if (one) {
         const uint32_t back = nxt[v ^ 1];
         f = (back == kNil || !is11[back >> 1]) ? 1u : ((uint32_t)((v * 0x9E3779B97F4A7C15ull) >> 32) % stride == 0);
     }
     rflag[v] = f;
+    props[v] = (uint8_t)((one ? 1u : 0u) | (f << 1) | (one && selfadj[v >> 1] ? 4u : 0u));
 }
 
 __global__ void k_rs_list(const uint32_t *__restrict__ rflag, const uint64_t *__restrict__ rpos, uint64_t n2,
@@ -257,18 +263,18 @@ __global__ void k_rs_list(const uint32_t *__restrict__ rflag, const uint64_t *__
 // walk step is one 8-byte load:  bits 0-31 successor, bits 32-33 kind
 // (0 pass through, 1 next ruler, 2 kept node or none), bit 34 the
 // successor's self-adjacency flag.
-__global__ void k_rs_prep(const uint8_t *__restrict__ is11, const uint32_t *__restrict__ nxt,
-                          const uint32_t *__restrict__ rflag, const uint8_t *__restrict__ selfadj, uint64_t S,
+__global__ void k_rs_prep(const uint8_t *__restrict__ props, const uint32_t *__restrict__ nxt, uint64_t S,
                           uint64_t *__restrict__ nx) {
     const uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (v >= 2 * S || !is11[v >> 1]) return;
+    if (v >= 2 * S || !(props[v] & 1)) return;
     const uint32_t t = nxt[v];
+    const uint32_t pt = t == kNil ? 0u : props[t];
     uint64_t kind, sa = 0;
-    if (t == kNil || !is11[t >> 1]) {
+    if (!(pt & 1)) {
         kind = 2;
     } else {
-        kind = rflag[t] ? 1 : 0;
-        sa = selfadj[t >> 1];
+        kind = (pt >> 1) & 1;
+        sa = (pt >> 2) & 1;
     }
     nx[v] = (uint64_t)t | (kind << 32) | (sa << 34);
 }
@@ -795,8 +801,10 @@ int Pipeline::merge_device(bool reference_order, MergeDev &out) {
     uint2 *label = mb_[M_NR].as<uint2>();
     uint64_t *nx = mb_[M_PAR].as<uint64_t>();
     phase_begin(9);
+    MCHECK(mb_[M_PROPS].reserve(2 * S));
     hipLaunchKernelGGL(k_rs_mark, dim3(nblk(2 * S)), dim3(256), 0, st_, mb_[M_IS11].as<uint8_t>(),
-                       mb_[M_NXT].as<uint32_t>(), S, mb_[M_FLAG].as<uint32_t>(), kRuleStride);
+                       mb_[M_NXT].as<uint32_t>(), mb_[M_SELF].as<uint8_t>(), S, mb_[M_FLAG].as<uint32_t>(),
+                       mb_[M_PROPS].as<uint8_t>(), kRuleStride);
     MCHECK(scan_excl<uint32_t>(mb_[M_FLAG].as<uint32_t>(), mb_[M_POS].as<uint64_t>(), 2 * S, scan_scratch_.as<uint64_t>(),
                         d_small + 26, st_));
     MCHECK(hipMemcpyAsync(hs + 26, d_small + 26, 8, hipMemcpyDeviceToHost, st_));
@@ -809,8 +817,8 @@ int Pipeline::merge_device(bool reference_order, MergeDev &out) {
     Walk *rwa = mb_[M_CPRI].as<Walk>(), *rwb = rwa + nr + 1;
     uint8_t *rspa = mb_[M_CMUL].as<uint8_t>(), *rspb = rspa + nr + 1;
     MCHECK(hipMemsetAsync(label, 0xFF, 2 * S * 8, st_));
-    hipLaunchKernelGGL(k_rs_prep, dim3(nblk(2 * S)), dim3(256), 0, st_, mb_[M_IS11].as<uint8_t>(),
-                       mb_[M_NXT].as<uint32_t>(), mb_[M_FLAG].as<uint32_t>(), mb_[M_SELF].as<uint8_t>(), S, nx);
+    hipLaunchKernelGGL(k_rs_prep, dim3(nblk(2 * S)), dim3(256), 0, st_, mb_[M_PROPS].as<uint8_t>(),
+                       mb_[M_NXT].as<uint32_t>(), S, nx);
     hipLaunchKernelGGL(k_rs_list, dim3(nblk(2 * S)), dim3(256), 0, st_, mb_[M_FLAG].as<uint32_t>(),
                        mb_[M_POS].as<uint64_t>(), 2 * S, rid, rlist);
     if (nr) {
