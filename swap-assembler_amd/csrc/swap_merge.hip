@@ -73,9 +73,10 @@ __global__ void k_index_sizes(const uint32_t *__restrict__ part_surv, uint32_t *
 // run of the survivor array and whose region fits kIdxLds entries: the region
 // is written whole (free entries included) with coalesced stores.  Other
 // regions are only cleared here and filled by k_index_loose.
-constexpr uint32_t kIdxLds = 4096;
+constexpr uint32_t kIdxLds = 1024;
+constexpr uint32_t kIdxThreads = 128;
 
-__global__ void __launch_bounds__(256) k_index_part(const uint64_t *__restrict__ key,
+__global__ void __launch_bounds__(kIdxThreads) k_index_part(const uint64_t *__restrict__ key,
                                                     const uint32_t *__restrict__ part_surv,
                                                     const uint32_t *__restrict__ part_first, SurvIndex ix,
                                                     uint64_t *__restrict__ ent) {
@@ -753,7 +754,7 @@ int Pipeline::merge_device(bool reference_order, MergeDev &out) {
     phase_begin(8);
     hipLaunchKernelGGL(k_index_sizes, dim3(kParts / 256), dim3(256), 0, st_, part_surv_.as<uint32_t>(), isize);
     MCHECK(scan_excl<uint32_t>(isize, ibase, kParts, scan_scratch_.as<uint64_t>(), ibase + kParts, st_));
-    hipLaunchKernelGGL(k_index_part, dim3(kParts), dim3(256), 0, st_, surv_key_.as<uint64_t>(),
+    hipLaunchKernelGGL(k_index_part, dim3(kParts), dim3(kIdxThreads), 0, st_, surv_key_.as<uint64_t>(),
                        part_surv_.as<uint32_t>(), part_first_.as<uint32_t>(), ix, mb_[M_IENT].as<uint64_t>());
     hipLaunchKernelGGL(k_index_loose, dim3(nblk(S)), dim3(256), 0, st_, surv_key_.as<uint64_t>(), S,
                        part_first_.as<uint32_t>(), ix, mb_[M_IENT].as<uint64_t>());

@@ -38,33 +38,47 @@ __device__ __forceinline__ uint64_t dev_ref_hash(uint64_t x) {
     return (x >> 32) ^ (uint64_t)(int64_t)(int32_t)(uint32_t)(x & 0xFFFFFFFFull);
 }
 
+// What the order of one key in a table depends on, gathered in one 16-byte
+// load: its hash, its insertion rank (< 2^32: an input k-mer index or a
+// position in another table) and the growth epoch it was inserted in.
+struct OrdKey {
+    uint64_t h;
+    uint32_t rank;
+    uint32_t ins;
+};
+
+__device__ __forceinline__ int insert_epoch(const EpochTab &et, uint64_t r) {
+    int e = 0;
+    for (int j = 1; j < et.n; j++)
+        if (et.start[j] <= r) e = j;
+    return e;
+}
+
 __global__ void k_ord_prep(const uint64_t *__restrict__ key, const uint64_t *__restrict__ rank, uint64_t n,
-                           const EpochTab *__restrict__ etp, uint64_t *__restrict__ h, uint8_t *__restrict__ ins,
-                           uint32_t *__restrict__ bucket, uint32_t *__restrict__ idx) {
+                           const EpochTab *__restrict__ etp, OrdKey *__restrict__ ok, uint32_t *__restrict__ bucket,
+                           uint32_t *__restrict__ idx) {
     const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const EpochTab &et = *etp;
     const uint64_t hv = dev_ref_hash(key[i]);
     const uint64_t r = rank[i];
-    int e = 0;
-    for (int j = 1; j < et.n; j++)
-        if (et.start[j] <= r) e = j;
-    h[i] = hv;
-    ins[i] = (uint8_t)e;
+    OrdKey o;
+    o.h = hv;
+    o.rank = (uint32_t)r;
+    o.ins = (uint32_t)insert_epoch(et, r);
+    ok[i] = o;
     bucket[i] = (uint32_t)mod_p(hv, et.P[et.n - 1], et.M[et.n - 1]);
     idx[i] = (uint32_t)i;
 }
 
 // a precedes b at the end of epoch e (see the host restatement in swap_host.cpp)
-__device__ __forceinline__ bool ord_before(uint32_t a, uint32_t b, int e, const EpochTab &et,
-                                           const uint64_t *__restrict__ h, const uint8_t *__restrict__ ins,
-                                           const uint64_t *__restrict__ rank) {
+__device__ __forceinline__ bool ord_before(const OrdKey &a, const OrdKey &b, int e, const EpochTab &et) {
     bool flip = false;
     for (; e >= 0; e--) {
-        const uint64_t ba = mod_p(h[a], et.P[e], et.M[e]), bb = mod_p(h[b], et.P[e], et.M[e]);
+        const uint64_t ba = mod_p(a.h, et.P[e], et.M[e]), bb = mod_p(b.h, et.P[e], et.M[e]);
         if (ba != bb) return (ba < bb) != flip;
-        const bool ia = ins[a] == e, ib = ins[b] == e;
-        if (ia && ib) return (rank[a] > rank[b]) != flip;
+        const bool ia = a.ins == (uint32_t)e, ib = b.ins == (uint32_t)e;
+        if (ia && ib) return (a.rank > b.rank) != flip;
         if (ia != ib) return ia != flip;
         flip = !flip;
     }
@@ -73,8 +87,7 @@ __device__ __forceinline__ bool ord_before(uint32_t a, uint32_t b, int e, const 
 
 // One thread per bucket run of the sorted array: insertion sort of its keys.
 __global__ void k_ord_ties(const uint32_t *__restrict__ bucket, uint32_t *__restrict__ idx, uint64_t n,
-                           const EpochTab *__restrict__ etp, const uint64_t *__restrict__ h,
-                           const uint8_t *__restrict__ ins, const uint64_t *__restrict__ rank) {
+                           const EpochTab *__restrict__ etp, const OrdKey *__restrict__ ok) {
     const uint64_t q = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= n) return;
     const EpochTab &et = *etp;
@@ -86,8 +99,9 @@ __global__ void k_ord_ties(const uint32_t *__restrict__ bucket, uint32_t *__rest
     const int f = et.n - 1;
     for (uint64_t x = q + 1; x < e; x++) {
         const uint32_t v = idx[x];
+        const OrdKey kv = ok[v];
         uint64_t y = x;
-        while (y > q && ord_before(v, idx[y - 1], f, et, h, ins, rank)) {
+        while (y > q && ord_before(kv, ok[idx[y - 1]], f, et)) {
             idx[y] = idx[y - 1];
             y--;
         }
@@ -97,15 +111,12 @@ __global__ void k_ord_ties(const uint32_t *__restrict__ bucket, uint32_t *__rest
 
 // Insertion epochs for new ranks (hash values and buckets unchanged).
 __global__ void k_ord_ins(const uint64_t *__restrict__ rank, uint64_t n, const EpochTab *__restrict__ etp,
-                          uint8_t *__restrict__ ins) {
+                          OrdKey *__restrict__ ok) {
     const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    const EpochTab &et = *etp;
     const uint64_t r = rank[i];
-    int e = 0;
-    for (int j = 1; j < et.n; j++)
-        if (et.start[j] <= r) e = j;
-    ins[i] = (uint8_t)e;
+    ok[i].rank = (uint32_t)r;
+    ok[i].ins = (uint32_t)insert_epoch(*etp, r);
 }
 
 __global__ void k_ord_positions(const uint32_t *__restrict__ order, uint64_t n, uint64_t *__restrict__ pos) {
@@ -132,13 +143,12 @@ static inline unsigned oblk(uint64_t n) { return (unsigned)((n + 255) / 256); }
 int Pipeline::table_order_device(const uint64_t *d_rank, const EpochTabHost &tab, const void *d_et, uint32_t *d_order) {
     const uint64_t S = n_surv;
     const EpochTab *et = static_cast<const EpochTab *>(d_et);
-    OCHECK(ob_[O_H].reserve(S * 8));
-    OCHECK(ob_[O_INS].reserve(S));
+    OCHECK(ob_[O_KEY].reserve(S * sizeof(OrdKey)));
     OCHECK(ob_[O_BKT].reserve(S * 4));
     OCHECK(ob_[O_BKT2].reserve(S * 4));
     OCHECK(ob_[O_IDX].reserve(S * 4));
     hipLaunchKernelGGL(k_ord_prep, dim3(oblk(S)), dim3(256), 0, st_, surv_key_.as<uint64_t>(), d_rank, S, et,
-                       ob_[O_H].as<uint64_t>(), ob_[O_INS].as<uint8_t>(), ob_[O_BKT].as<uint32_t>(),
+                       ob_[O_KEY].as<OrdKey>(), ob_[O_BKT].as<uint32_t>(),
                        ob_[O_IDX].as<uint32_t>());
     int bits = 1;
     while (bits < 32 && (1ull << bits) < tab.P[tab.n - 1]) bits++;
@@ -149,7 +159,7 @@ int Pipeline::table_order_device(const uint64_t *d_rank, const EpochTabHost &tab
     OCHECK(rocprim::radix_sort_pairs(ob_[O_TMP].p, tb, ob_[O_BKT].as<uint32_t>(), ob_[O_BKT2].as<uint32_t>(),
                                      ob_[O_IDX].as<uint32_t>(), d_order, (size_t)S, 0, bits, st_));
     hipLaunchKernelGGL(k_ord_ties, dim3(oblk(S)), dim3(256), 0, st_, ob_[O_BKT2].as<uint32_t>(), d_order, S, et,
-                       ob_[O_H].as<uint64_t>(), ob_[O_INS].as<uint8_t>(), d_rank);
+                       ob_[O_KEY].as<OrdKey>());
     return 0;
 }
 
@@ -159,9 +169,9 @@ int Pipeline::table_order_device(const uint64_t *d_rank, const EpochTabHost &tab
 int Pipeline::table_reorder_device(const uint64_t *d_rank, const void *d_et, uint32_t *d_order) {
     const uint64_t S = n_surv;
     const EpochTab *et = static_cast<const EpochTab *>(d_et);
-    hipLaunchKernelGGL(k_ord_ins, dim3(oblk(S)), dim3(256), 0, st_, d_rank, S, et, ob_[O_INS].as<uint8_t>());
+    hipLaunchKernelGGL(k_ord_ins, dim3(oblk(S)), dim3(256), 0, st_, d_rank, S, et, ob_[O_KEY].as<OrdKey>());
     hipLaunchKernelGGL(k_ord_ties, dim3(oblk(S)), dim3(256), 0, st_, ob_[O_BKT2].as<uint32_t>(), d_order, S, et,
-                       ob_[O_H].as<uint64_t>(), ob_[O_INS].as<uint8_t>(), d_rank);
+                       ob_[O_KEY].as<OrdKey>());
     return 0;
 }
 
