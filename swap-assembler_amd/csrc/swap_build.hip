@@ -332,8 +332,8 @@ constexpr int kL2Bits = kPartBits - kL1Bits;
 constexpr uint32_t kL2 = 1u << kL2Bits;
 constexpr int kOccThreads = 1024;
 constexpr int kHistRowGroups = 8;     // row groups of the column sum
-constexpr int kChunkRecs = 8192;      // level-2 records per workgroup
-constexpr int kL2Threads = 1024;
+constexpr int kChunkRecs = 4096;      // level-2 records per workgroup
+constexpr int kL2Threads = 512;
 constexpr int kL2Per = kChunkRecs / kL2Threads;
 
 // Occurrence records between the passes.  Reference order carries the input
@@ -432,7 +432,8 @@ __global__ void __launch_bounds__(kOccThreads) k_part1(OccCtx c, const uint64_t 
 
 // Level 2: each workgroup takes one chunk of one level-1 bucket, counts it by
 // level-2 bucket in LDS, claims a run of each final partition with one atomic
-// per sub-bucket and writes its records there.
+// per sub-bucket, orders the chunk by sub-bucket in LDS and writes it out in
+// that order, so consecutive lanes store consecutive records of one run.
 // chunk_start[b] = first chunk index of level-1 bucket b (kL1 + 1 entries).
 template <bool WITH_OCC>
 __global__ void __launch_bounds__(kL2Threads) k_part2(const uint32_t *__restrict__ rec_in,
@@ -440,6 +441,10 @@ __global__ void __launch_bounds__(kL2Threads) k_part2(const uint32_t *__restrict
                                                        const uint32_t *__restrict__ chunk_start, int k,
                                                        uint32_t *__restrict__ cursor, uint32_t *__restrict__ rec_out) {
     __shared__ uint32_t lcnt[kL2];
+    __shared__ uint32_t lstart[kL2];
+    __shared__ uint32_t gbase[kL2];
+    __shared__ uint32_t slo[kChunkRecs], shi[kChunkRecs];  // the chunk in sub-bucket order
+    __shared__ uint32_t socc[WITH_OCC ? kChunkRecs : 1];
     __shared__ uint32_t sb, sfirst;
     const uint32_t chunk = blockIdx.x;
     if (threadIdx.x == 0) {
@@ -460,26 +465,43 @@ __global__ void __launch_bounds__(kL2Threads) k_part2(const uint32_t *__restrict
     const uint32_t n = (uint32_t)(e - a);
     const int sh2 = 8 + 2 * k - kPartBits;
     uint64_t r[kL2Per];
-    uint32_t o[kL2Per];
+    uint32_t o[kL2Per], rank[kL2Per];
 #pragma unroll
     for (int j = 0; j < kL2Per; j++) {
         const uint32_t i = j * kL2Threads + threadIdx.x;
         if (i < n) {
             r[j] = rec_load<WITH_OCC>(rec_in, a + i, o[j]);
-            atomicAdd(&lcnt[(r[j] >> sh2) & (kL2 - 1)], 1u);
+            rank[j] = atomicAdd(&lcnt[(r[j] >> sh2) & (kL2 - 1)], 1u);
         }
     }
     __syncthreads();
-    for (uint32_t b = threadIdx.x; b < kL2; b += blockDim.x)
-        lcnt[b] = lcnt[b] ? atomicAdd(&cursor[b1 * kL2 + b], lcnt[b]) : 0;
+    static_assert(kL2 == 128, "the sub-bucket scan takes two entries per lane of one wave");
+    if (threadIdx.x < 64) {
+        const uint32_t c0 = lcnt[2 * threadIdx.x], c1 = lcnt[2 * threadIdx.x + 1];
+        const uint32_t ex = wave_incl_sum(c0 + c1) - c0 - c1;
+        lstart[2 * threadIdx.x] = ex;
+        lstart[2 * threadIdx.x + 1] = ex + c0;
+    }
+    if (threadIdx.x < kL2) {
+        const uint32_t c = lcnt[threadIdx.x];
+        gbase[threadIdx.x] = c ? atomicAdd(&cursor[b1 * kL2 + threadIdx.x], c) : 0;
+    }
     __syncthreads();
 #pragma unroll
     for (int j = 0; j < kL2Per; j++) {
         const uint32_t i = j * kL2Threads + threadIdx.x;
         if (i < n) {
-            const uint32_t g = atomicAdd(&lcnt[(r[j] >> sh2) & (kL2 - 1)], 1u);
-            rec_store<WITH_OCC>(rec_out, g, r[j], o[j]);
+            const uint32_t pos = lstart[(r[j] >> sh2) & (kL2 - 1)] + rank[j];
+            slo[pos] = (uint32_t)r[j];
+            shi[pos] = (uint32_t)(r[j] >> 32);
+            if (WITH_OCC) socc[pos] = o[j];
         }
+    }
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < n; i += kL2Threads) {
+        const uint64_t rec = (uint64_t)slo[i] | ((uint64_t)shi[i] << 32);
+        const uint32_t b = (uint32_t)(rec >> sh2) & (kL2 - 1);
+        rec_store<WITH_OCC>(rec_out, (uint64_t)gbase[b] + (i - lstart[b]), rec, WITH_OCC ? socc[i] : 0u);
     }
 }
 
