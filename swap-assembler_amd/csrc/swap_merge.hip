@@ -228,7 +228,7 @@ struct Walk {
 // (so cycles and long chains are cut into short pieces).  Each ruler walks to
 // the next ruler or to a kept node, labelling the nodes it passes with
 // (ruler, offset); pointer jumping then runs over the rulers only.
-constexpr uint32_t kRuleStride = 16;
+constexpr uint32_t kRuleStride = 32;
 
 // rflag[v] (ruler or not, for the ruler numbering scan) and props[v]: bit 0
 // (1,1) node, bit 1 ruler, bit 2 self-adjacent, so that k_rs_prep reads one
@@ -547,13 +547,44 @@ __global__ void k_fill_last(const uint32_t *__restrict__ elen, const uint32_t *_
 // ---------------------------------------------------------------------------
 // nearest larger priority (max hierarchy) and Cartesian-tree evaluation
 
-__global__ void k_max_level(const uint64_t *__restrict__ in, uint64_t n_in, uint64_t *__restrict__ out,
-                            uint64_t n_out) {
-    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n_out) return;
-    const uint64_t a = 2 * i < n_in ? in[2 * i] : 0;
-    const uint64_t b = 2 * i + 1 < n_in ? in[2 * i + 1] : 0;
-    out[i] = a > b ? a : b;
+// Max hierarchy: level l + 1 holds the max of each pair of level l (0 past
+// the end).  One workgroup reduces kMaxTile elements of level l0 through up
+// to kMaxSteps levels in LDS; src is level l0 (level 0 is copied into the
+// hierarchy on the way).
+constexpr int kMaxSteps = 11;
+constexpr uint32_t kMaxTile = 1u << kMaxSteps;
+constexpr int kMaxThreads = 256;
+
+struct LevTab {
+    uint64_t off[64];
+    uint64_t len[64];
+};
+
+__global__ void __launch_bounds__(kMaxThreads) k_max_levels(const uint64_t *__restrict__ src, uint64_t *__restrict__ lev,
+                                                            LevTab lt, int l0, int l1) {
+    __shared__ uint64_t buf[2][kMaxTile];
+    const uint64_t base = (uint64_t)blockIdx.x * kMaxTile;
+    const uint64_t n0 = lt.len[l0];
+    for (uint32_t j = threadIdx.x; j < kMaxTile; j += kMaxThreads) {
+        const uint64_t i = base + j;
+        const uint64_t v = i < n0 ? src[i] : 0;
+        buf[0][j] = v;
+        if (l0 == 0 && i < n0) lev[i] = v;
+    }
+    __syncthreads();
+    int cur = 0;
+    for (int l = l0 + 1; l <= l1; l++) {
+        const uint32_t size = kMaxTile >> (l - l0);
+        const uint64_t lbase = base >> (l - l0);
+        for (uint32_t j = threadIdx.x; j < size; j += kMaxThreads) {
+            const uint64_t a = buf[cur][2 * j], b = buf[cur][2 * j + 1];
+            const uint64_t m = a > b ? a : b;
+            buf[cur ^ 1][j] = m;
+            if (lbase + j < lt.len[l]) lev[lt.off[l] + lbase + j] = m;
+        }
+        cur ^= 1;
+        __syncthreads();
+    }
 }
 
 struct TreeArgs {
@@ -942,13 +973,21 @@ int Pipeline::merge_device(bool reference_order, MergeDev &out) {
         MCHECK(mb_[M_LEV].reserve(tot * 8 + 16));
         MCHECK(mb_[M_LEVOFF].reserve(loff.size() * 8 + 16));
         uint64_t *lev = mb_[M_LEV].as<uint64_t>();
-        MCHECK(hipMemcpyAsync(lev, fa.cpri, M * 8, hipMemcpyDeviceToDevice, st_));
         MCHECK(hipMemcpyAsync(mb_[M_LEVOFF].p, loff.data(), loff.size() * 8, hipMemcpyHostToDevice, st_));
+        if (nlev > 64) return fail(-4, "chain array too long for the max hierarchy");
+        LevTab lt;
         len = M;
-        for (int l = 1; l < nlev; l++) {
-            const uint64_t nl = (len + 1) / 2;
-            hipLaunchKernelGGL(k_max_level, dim3(nblk(nl)), dim3(256), 0, st_, lev + loff[l - 1], len, lev + loff[l], nl);
-            len = nl;
+        for (int l = 0; l < nlev; l++) {
+            lt.off[l] = loff[l];
+            lt.len[l] = len;
+            len = (len + 1) / 2;
+        }
+        for (int l0 = 0; l0 == 0 || l0 < nlev - 1; l0 += kMaxSteps) {
+            const int l1 = std::min(nlev - 1, l0 + kMaxSteps);
+            const uint64_t grid = (lt.len[l0] + kMaxTile - 1) / kMaxTile;
+            hipLaunchKernelGGL(k_max_levels, dim3((unsigned)grid), dim3(kMaxThreads), 0, st_,
+                               l0 == 0 ? (const uint64_t *)fa.cpri : (const uint64_t *)(lev + loff[l0]), lev, lt, l0,
+                               l1);
         }
         MCHECK(mb_[M_PAR].reserve(M * 4));
         MCHECK(mb_[M_NEED].reserve(M));
