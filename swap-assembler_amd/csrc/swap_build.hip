@@ -863,6 +863,7 @@ __global__ void __launch_bounds__(kCountThreads) k_count_lds(CountArgs a) {
         if (t == 0 && have_prev && !prev_ovf) {
             base_s = pend_s;
             base_d = pend_d;
+            a.part_first[prev_part] = (uint32_t)pend_s;
         }
         __syncthreads();  // table complete; previous output bases and wave offsets visible
         if (have_prev && !prev_ovf) {
@@ -886,7 +887,8 @@ __global__ void __launch_bounds__(kCountThreads) k_count_lds(CountArgs a) {
 #pragma unroll
         for (int j = 0; j < 4; j++) {
             const uint32_t s = (uint32_t)j * kCountThreads + t;
-            keys[j] = ovf ? kEmpty : tkey[s];
+            const uint64_t kk = tkey[s];
+            keys[j] = ovf ? kEmpty : kk;
             mults[j] = 0;
             fos[j] = 0;
             if (keys[j] != kEmpty) {
@@ -895,14 +897,18 @@ __global__ void __launch_bounds__(kCountThreads) k_count_lds(CountArgs a) {
                 nd++;
                 ns += mults[j] != 0;
             }
-            // a slot's key, counters and first occurrence are read only by its owner
-            tkey[s] = kEmpty;
-            if (WITH_OCC) tfo[s] = ~0u;
-            if (small) {
+            // a slot's key, counters and first occurrence are read only by its
+            // owner; counters and first occurrence change only in slots that
+            // hold a key
+            if (kk != kEmpty) {
+                tkey[s] = kEmpty;
+                if (WITH_OCC) tfo[s] = ~0u;
+                if (small) {
 #pragma unroll
-                for (int q = 0; q < 4; q++) tc[4 * s + q] = 0;
-            } else {
-                tcnt[s] = 0;
+                    for (int q = 0; q < 4; q++) tc[4 * s + q] = 0;
+                } else {
+                    tcnt[s] = 0;
+                }
             }
         }
         // workgroup exclusive scan of (survivors, distinct) packed in one u64
@@ -926,10 +932,14 @@ __global__ void __launch_bounds__(kCountThreads) k_count_lds(CountArgs a) {
                     run += x;
                 }
                 const uint32_t ts = (uint32_t)run, td = (uint32_t)(run >> 32);
-                pend_s = ts ? atomicAdd(&a.counters[0], (unsigned long long)ts) : 0;
-                pend_d = td ? atomicAdd(&a.counters[1], (unsigned long long)td) : 0;
+                // a per-lane (zero) index keeps the compiler from turning these
+                // into wave-aggregated atomics, whose result it would wait for
+                // here; the results are first used after the next insert
+                uint32_t zero;
+                asm volatile("v_mov_b32 %0, 0" : "=v"(zero));
+                pend_s = ts ? atomicAdd(&a.counters[zero], (unsigned long long)ts) : 0;
+                pend_d = td ? atomicAdd(&a.counters[1 + zero], (unsigned long long)td) : 0;
                 a.part_surv[part] = ts;
-                a.part_first[part] = (uint32_t)pend_s;
             }
         }
         have_prev = true;
