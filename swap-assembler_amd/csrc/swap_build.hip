@@ -917,29 +917,32 @@ __global__ void __launch_bounds__(kCountThreads) k_count_lds(CountArgs a) {
         if (lane == 63) wsum[w] = pinc;
         if (part + gridDim.x < kParts) prefetch(it + 1);
         __syncthreads();  // wave sums complete; the table is empty
-        if (t == 0) {
-            overflow[it & 1] = 0;
-            if (ovf) {
-                const unsigned long long j = atomicAdd(&a.counters[2], 1ull);
-                a.overflow_list[j] = part;
-                a.part_surv[part] = 0;  // counted by the global-memory pass
-                a.part_first[part] = ~0u;
-            } else {
-                uint64_t run = 0;
-                for (int i = 0; i < kCountThreads / 64; i++) {
-                    const uint64_t x = wsum[i];
-                    wsum[i] = run;
-                    run += x;
+        if (w == 0) {
+            // wave offsets: exclusive scan of the per-wave totals by wave 0
+            constexpr int kWaves = kCountThreads / 64;
+            const uint64_t x = (!ovf && lane < kWaves) ? wsum[lane] : 0;
+            const uint64_t incw = wave_incl_sum(x);
+            if (!ovf && lane < kWaves) wsum[lane] = incw - x;
+            const uint64_t run = ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(incw >> 32), kWaves - 1) << 32) |
+                                 (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)incw, kWaves - 1);
+            if (lane == 0) {
+                overflow[it & 1] = 0;
+                if (ovf) {
+                    const unsigned long long j = atomicAdd(&a.counters[2], 1ull);
+                    a.overflow_list[j] = part;
+                    a.part_surv[part] = 0;  // counted by the global-memory pass
+                    a.part_first[part] = ~0u;
+                } else {
+                    const uint32_t ts = (uint32_t)run, td = (uint32_t)(run >> 32);
+                    // a per-lane (zero) index keeps the compiler from turning these
+                    // into wave-aggregated atomics, whose result it would wait for
+                    // here; the results are first used after the next insert
+                    uint32_t zero;
+                    asm volatile("v_mov_b32 %0, 0" : "=v"(zero));
+                    pend_s = ts ? atomicAdd(&a.counters[zero], (unsigned long long)ts) : 0;
+                    pend_d = td ? atomicAdd(&a.counters[1 + zero], (unsigned long long)td) : 0;
+                    a.part_surv[part] = ts;
                 }
-                const uint32_t ts = (uint32_t)run, td = (uint32_t)(run >> 32);
-                // a per-lane (zero) index keeps the compiler from turning these
-                // into wave-aggregated atomics, whose result it would wait for
-                // here; the results are first used after the next insert
-                uint32_t zero;
-                asm volatile("v_mov_b32 %0, 0" : "=v"(zero));
-                pend_s = ts ? atomicAdd(&a.counters[zero], (unsigned long long)ts) : 0;
-                pend_d = td ? atomicAdd(&a.counters[1 + zero], (unsigned long long)td) : 0;
-                a.part_surv[part] = ts;
             }
         }
         have_prev = true;

@@ -5,14 +5,37 @@
 
 namespace swapk {
 
+// Value of lane (i - shift) within each row of 16 lanes (row_shr), or of the
+// last lane of the previous row / of lane 31 (row_bcast15 / row_bcast31) for
+// the rows in row_mask; 0 where there is no such lane or the row is masked.
+// DPP moves stay in the VALU: no LDS round trip per step.
+template <int Ctrl, int RowMask>
+__device__ __forceinline__ uint32_t dpp_u32(uint32_t v) {
+    return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, Ctrl, RowMask, 0xf, false);
+}
+
+template <int Ctrl, int RowMask>
+__device__ __forceinline__ uint64_t dpp_u64(uint64_t v) {
+    const uint32_t lo = dpp_u32<Ctrl, RowMask>((uint32_t)v), hi = dpp_u32<Ctrl, RowMask>((uint32_t)(v >> 32));
+    return (uint64_t)lo | ((uint64_t)hi << 32);
+}
+
+template <int Ctrl, int RowMask>
+__device__ __forceinline__ uint32_t dpp_t(uint32_t v) { return dpp_u32<Ctrl, RowMask>(v); }
+template <int Ctrl, int RowMask>
+__device__ __forceinline__ uint64_t dpp_t(uint64_t v) { return dpp_u64<Ctrl, RowMask>(v); }
+
+// Inclusive prefix sum over the 64 lanes of a wave (all lanes active):
+// Hillis-Steele inside each row of 16, then the row totals carried across
+// rows by two broadcasts.
 template <typename T>
 __device__ __forceinline__ T wave_incl_sum(T v) {
-    const int lane = threadIdx.x & 63;
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-        T o = __shfl_up(v, d, 64);
-        if (lane >= d) v += o;
-    }
+    v += dpp_t<0x111, 0xf>(v);  // row_shr:1
+    v += dpp_t<0x112, 0xf>(v);  // row_shr:2
+    v += dpp_t<0x114, 0xf>(v);  // row_shr:4
+    v += dpp_t<0x118, 0xf>(v);  // row_shr:8
+    v += dpp_t<0x142, 0xa>(v);  // row_bcast:15 into rows 1 and 3
+    v += dpp_t<0x143, 0xc>(v);  // row_bcast:31 into rows 2 and 3
     return v;
 }
 
