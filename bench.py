@@ -34,7 +34,7 @@ HBM_PEAK = 8.0e12  # MI355X HBM3E, bytes/s
 PHASES = {0: "record_scan", 1: "pack_2bit", 2: "partition_hist", 3: "partition_level1", 4: "count_lds_cutoff",
           5: "count_global_overflow", 6: "growth_rank_select", 7: "partition_level2", 8: "survivor_index",
           9: "list_ranking", 10: "kept_edges", 11: "edge_strings", 12: "cartesian_trees", 13: "visit_order",
-          14: "classify"}
+          14: "classify", 15: "survivor_keys"}
 
 
 # phase index -> device kernel of that phase whose hardware byte counters are kept

@@ -675,14 +675,31 @@ __device__ __forceinline__ bool table_add16(uint64_t *tkey, uint32_t *tc, uint32
     return true;
 }
 
-__device__ __forceinline__ uint64_t counts16_to_bytes(const uint32_t *tc, uint32_t s) {
-    uint64_t out = 0;
+// min(count, 255) and the cutoff of graph.cpp:1100-1112 on the 16-bit counters
+// of one slot, two counters per packed 16-bit operation: a counter survives
+// when min(count, 255) + (0x8000 - c) has its top bit set, i.e. when it is >= c.
+typedef unsigned short u16x2 __attribute__((ext_vector_type(2)));
+typedef short i16x2 __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ uint32_t bias16(int c) {
+    const uint32_t b = 0x8000u - (uint32_t)min(max(c, 0), 256);
+    return b | (b << 16);
+}
+
+__device__ __forceinline__ uint64_t cut16_to_bytes(const uint32_t *tc, uint32_t s, uint32_t bias) {
+    uint32_t w[4];
 #pragma unroll
-    for (int t = 0; t < 8; t++) {
-        const uint32_t c = (tc[4 * s + (t >> 1)] >> (16 * (t & 1))) & 0xFFFF;
-        out |= (uint64_t)(c > 255 ? 255 : c) << (8 * t);
+    for (int q = 0; q < 4; q++) {
+        u16x2 x = __builtin_bit_cast(u16x2, tc[4 * s + q]);
+        x = __builtin_elementwise_min(x, (u16x2)(255));
+        const u16x2 t = x + __builtin_bit_cast(u16x2, bias);
+        const i16x2 keep = __builtin_bit_cast(i16x2, t) >> 15;
+        w[q] = __builtin_bit_cast(uint32_t, x & __builtin_bit_cast(u16x2, keep));
     }
-    return out;
+    // counter 2q is byte 0 of w[q], counter 2q + 1 its byte 2
+    const uint32_t lo = __builtin_amdgcn_perm(w[1], w[0], 0x06040200u);
+    const uint32_t hi = __builtin_amdgcn_perm(w[3], w[2], 0x06040200u);
+    return (uint64_t)lo | ((uint64_t)hi << 32);
 }
 
 // Cutoff of graph.cpp:1100-1112 on one slot: multiplicities below c become 0.
@@ -748,7 +765,7 @@ __device__ void table_emit(const uint64_t *tkey, CNT cnt_of, const uint32_t *tfo
             if (keys[j] == kEmpty) continue;
             if (WITH_OCC && a.dfo) a.dfo[od++] = fos[j];
             if (mults[j]) {
-                a.surv_key[os] = mix_inv(a.mx, ((uint64_t)(part >> kL2Bits) << hb) | keys[j]);
+                a.surv_key[os] = ((uint64_t)(part >> kL2Bits) << hb) | keys[j];  // mixed; k_unmix inverts
                 a.surv_mult[os] = mults[j];
                 if (WITH_OCC) a.surv_first[os] = fos[j];
                 os++;
@@ -790,6 +807,7 @@ __global__ void __launch_bounds__(kCountThreads) k_count_lds(CountArgs a) {
     const int lane = t & 63, w = t >> 6;
     const int hb = 2 * a.k - kL1Bits;
     const uint32_t total = a.bucket[kParts];  // >= 1: the host skips empty builds
+    const uint32_t bias = bias16(a.c);
     uint64_t *tcnt = reinterpret_cast<uint64_t *>(tc);
     for (uint32_t i = t; i < kTab; i += kCountThreads) {
         tkey[i] = kEmpty;
@@ -874,7 +892,7 @@ __global__ void __launch_bounds__(kCountThreads) k_count_lds(CountArgs a) {
                 if (keys[j] == kEmpty) continue;
                 if (WITH_OCC && a.dfo) a.dfo[od++] = fos[j];
                 if (mults[j]) {
-                    a.surv_key[os] = mix_inv(a.mx, ((uint64_t)(prev_part >> kL2Bits) << hb) | keys[j]);
+                    a.surv_key[os] = ((uint64_t)(prev_part >> kL2Bits) << hb) | keys[j];  // mixed; k_unmix inverts
                     a.surv_mult[os] = mults[j];
                     if (WITH_OCC) a.surv_first[os] = fos[j];
                     os++;
@@ -892,7 +910,7 @@ __global__ void __launch_bounds__(kCountThreads) k_count_lds(CountArgs a) {
             mults[j] = 0;
             fos[j] = 0;
             if (keys[j] != kEmpty) {
-                mults[j] = apply_cutoff(small ? counts16_to_bytes(tc, s) : tcnt[s], a.c);
+                mults[j] = small ? cut16_to_bytes(tc, s, bias) : apply_cutoff(tcnt[s], a.c);
                 if (WITH_OCC) fos[j] = tfo[s];
                 nd++;
                 ns += mults[j] != 0;
@@ -949,6 +967,14 @@ __global__ void __launch_bounds__(kCountThreads) k_count_lds(CountArgs a) {
         prev_ovf = ovf;
         prev_part = part;
     }
+}
+
+// The count kernels store survivors by their mixed key h; one pass turns them
+// into k-mers (kept out of the count kernel, where it would run once per
+// slot on every wave).
+__global__ void k_unmix(uint64_t *__restrict__ key, uint64_t n, Mixer mx) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) key[i] = mix_inv(mx, key[i]);
 }
 
 // Global-memory fallback for partitions with more distinct keys than the LDS
@@ -1452,6 +1478,13 @@ int Pipeline::build(const uint8_t *d_in, uint64_t n) {
     }
     n_surv = hs[8];
     n_distinct = hs[9];
+    if (n_surv) {
+        phase_begin(15);
+        hipLaunchKernelGGL(k_unmix, dim3((unsigned)((n_surv + 255) / 256)), dim3(256), 0, st_, surv_key_.as<uint64_t>(),
+                           n_surv, mx_);
+        phase_end(15);
+        SWAP_CHECK(hipGetLastError());
+    }
     if (ref_order_ && n_distinct) {
         // growth ranks of the reference's k-mer table: the first occurrence of
         // the key of each growth rank, selected on the device
