@@ -1215,10 +1215,9 @@ void Pipeline::reconfigure(int k, int c, bool reference_order) {
     ref_order_ = reference_order;
     mx_.bits = 2 * k;
     mx_.m = kmask(k);
-    mx_.a1 = 0x9E3779B97F4A7C15ull | 1;
-    mx_.a2 = 0xC2B2AE3D27D4EB4Full | 1;
-    mx_.i1 = inv_odd(mx_.a1);
-    mx_.i2 = inv_odd(mx_.a2);
+    // Fibonacci hashing on 2k bits: 2^2k / golden ratio, odd
+    mx_.a = (0x9E3779B97F4A7C15ull >> (64 - 2 * k)) | 1;
+    mx_.ia = inv_odd(mx_.a);
     mx_.s = k;  // half of the 2k bits
     n_reads = n_occ = n_distinct = n_surv = n_kept = n_overflow = n_special = 0;
 }

@@ -30,30 +30,27 @@ SWAP_HD uint64_t revcomp(uint64_t x, int k) {
 }
 
 // Bijective mixer on the 2k-bit k-mer space used for partitioning: odd
-// multiply mod 2^2k, xor-shift, odd multiply.  The inverse is applied to the
-// bits kept in the partition records to recover the k-mer.
+// multiply mod 2^2k by 2^2k / golden ratio (Fibonacci hashing: the top bits,
+// which pick the partition, spread even runs of consecutive keys), then an
+// xor-shift by half the width so the low bits depend on the high ones.  The inverse is
+// applied to the bits kept in the partition records to recover the k-mer.
 struct Mixer {
-    int bits;          // 2k
-    uint64_t m;        // 2k-bit mask
-    uint64_t a1, a2;   // odd multipliers
-    uint64_t i1, i2;   // their inverses mod 2^64 (truncated to 2k bits on use)
-    int s;             // xor-shift amount
+    int bits;     // 2k
+    uint64_t m;   // 2k-bit mask
+    uint64_t a;   // odd multiplier
+    uint64_t ia;  // its inverse mod 2^64 (truncated to 2k bits on use)
+    int s;        // xor-shift amount, >= bits / 2
 };
 
 SWAP_HD uint64_t mix_fwd(const Mixer &mx, uint64_t x) {
-    x = (x * mx.a1) & mx.m;
-    x ^= x >> mx.s;
-    x = (x * mx.a2) & mx.m;
-    return x;
+    x = (x * mx.a) & mx.m;
+    return x ^ (x >> mx.s);
 }
 
 SWAP_HD uint64_t mix_inv(const Mixer &mx, uint64_t y) {
-    y = (y * mx.i2) & mx.m;
-    uint64_t x = y;
-    for (int sh = mx.s; sh < mx.bits; sh += mx.s) x = y ^ (x >> mx.s);
-    x &= mx.m;
-    x = (x * mx.i1) & mx.m;
-    return x;
+    // y = x ^ (x >> s) with 2s >= bits: x = y ^ (y >> s)
+    const uint64_t x = (y ^ (y >> mx.s)) & mx.m;
+    return (x * mx.ia) & mx.m;
 }
 
 // Saturating add of per-byte increments (each byte of inc is 0 or 1).
