@@ -1348,7 +1348,7 @@ int Pipeline::build(const uint8_t *d_in, uint64_t n) {
     oc.mx = mx_;
     int ncu = 0;
     SWAP_CHECK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev_));
-    const unsigned occ_grid = (unsigned)std::max(8, ncu * 2);
+    const unsigned occ_grid = (unsigned)std::max(8, ncu);
     const uint64_t n1 = (uint64_t)kL1 * occ_grid;
     SWAP_CHECK(hist_.reserve((kParts + 1) * 4));   // final partition sizes, then starts (u32)
     SWAP_CHECK(bucket_.reserve((kParts + 1) * 8));
