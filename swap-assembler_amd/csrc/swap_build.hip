@@ -822,17 +822,19 @@ __global__ void __launch_bounds__(kCountThreads) k_count_lds(CountArgs a) {
     }
     __syncthreads();
 
-    // next partition's records: loaded from clamped indices so that every load
-    // is issued (entries past the partition's end are never used)
+    // next partition's records: loaded from indices clamped to the partition's
+    // last record, so that every load is issued and none reaches into the next
+    // partition (entries past the partition's end are never used)
     uint64_t pr[kPre];
     uint32_t po[kPre];
     uint32_t b = 0, e = 0;
     auto prefetch = [&](uint32_t it) {
         b = bnd[it][0];
         e = bnd[it][1];
+        const uint64_t last = e > b ? (uint64_t)e - 1 : min((uint64_t)b, (uint64_t)total - 1);
 #pragma unroll
         for (int j = 0; j < kPre; j++) {
-            const uint64_t i = min((uint64_t)b + t + (uint64_t)j * kCountThreads, (uint64_t)total - 1);
+            const uint64_t i = min((uint64_t)b + t + (uint64_t)j * kCountThreads, last);
             pr[j] = rec_load<WITH_OCC>(a.rec, i, po[j]);
         }
     };
