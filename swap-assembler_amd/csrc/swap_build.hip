@@ -96,26 +96,26 @@ __global__ void k_tile_count(const uint8_t *__restrict__ buf, uint64_t n, uint32
     }
 }
 
-// Writes the position of every '\n' in file order.
+// Writes the position of every '\n' in file order.  A thread's 16 bytes give
+// a 16-bit mask of newline bytes; it stores one position per set bit.
 __global__ void k_tile_emit(const uint8_t *__restrict__ buf, uint64_t n, const uint64_t *__restrict__ tile_off,
                             uint64_t *__restrict__ nl_pos) {
     __shared__ uint32_t wsum[kTileThreads / 64];
     const uint64_t base = (uint64_t)blockIdx.x * kTileBytes + (uint64_t)threadIdx.x * 16;
-    uint8_t b[16];
+    uint32_t m = 0;
     if (base + 16 <= n) {
         const uint4 v = *reinterpret_cast<const uint4 *>(buf + base);
         const uint32_t w[4] = {v.x, v.y, v.z, v.w};
 #pragma unroll
-        for (int i = 0; i < 4; i++)
-#pragma unroll
-            for (int j = 0; j < 4; j++) b[4 * i + j] = (uint8_t)(w[i] >> (8 * j));
+        for (int i = 0; i < 4; i++) {
+            const uint32_t z = zero_bytes(w[i] ^ 0x0A0A0A0Au);  // 0x80 in each newline byte
+            m |= (((z >> 7) & 1u) | ((z >> 14) & 2u) | ((z >> 21) & 4u) | ((z >> 28) & 8u)) << (4 * i);
+        }
     } else {
-#pragma unroll
-        for (int i = 0; i < 16; i++) b[i] = (base + i < n) ? buf[base + i] : 0;
+        for (int i = 0; i < 16; i++)
+            if (base + i < n && buf[base + i] == '\n') m |= 1u << i;
     }
-    uint32_t cnt = 0;
-#pragma unroll
-    for (int i = 0; i < 16; i++) cnt += (b[i] == '\n');
+    const uint32_t cnt = __popc(m);
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     uint32_t inc = wave_incl_sum(cnt);
     if (lane == 63) wsum[w] = inc;
@@ -123,9 +123,10 @@ __global__ void k_tile_emit(const uint8_t *__restrict__ buf, uint64_t n, const u
     uint32_t off = inc - cnt;
     for (int i = 0; i < w; i++) off += wsum[i];
     uint64_t o = tile_off[blockIdx.x] + off;
-#pragma unroll
-    for (int i = 0; i < 16; i++)
-        if (b[i] == '\n') nl_pos[o++] = base + i;
+    while (m) {
+        nl_pos[o++] = base + (uint32_t)(__ffs(m) - 1);
+        m &= m - 1;
+    }
 }
 
 // Records are line pairs starting at the first '>' (kmerGraph.cpp:173-181): the
