@@ -195,20 +195,26 @@ __device__ __forceinline__ uint64_t upper_index(const uint64_t *__restrict__ a, 
     return lo;
 }
 
+// block_rec[b] = the record holding word b * kPackThreads (the record r with
+// word_base[r] <= b * kPackThreads < word_base[r + 1]).
+__global__ void k_block_rec(const uint64_t *__restrict__ word_base, uint64_t R, uint64_t W,
+                            uint64_t *__restrict__ block_rec) {
+    const uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= R) return;
+    const uint64_t wb = word_base[r], we = r + 1 < R ? word_base[r + 1] : W;
+    for (uint64_t b = (wb + kPackThreads - 1) / kPackThreads; b * kPackThreads < we; b++) block_rec[b] = r;
+}
+
 __global__ void __launch_bounds__(kPackThreads) k_pack(const uint8_t *__restrict__ buf, uint64_t n,
                                                        const uint64_t *__restrict__ rec_start,
                                                        const uint32_t *__restrict__ rec_len,
                                                        const uint64_t *__restrict__ word_base,
                                                        const uint64_t *__restrict__ occ_base, uint64_t R, int k,
-                                                       const uint64_t *__restrict__ d_words,
+                                                       const uint64_t *__restrict__ block_rec, uint64_t W,
                                                        uint64_t *__restrict__ packed, uint64_t *__restrict__ wmeta) {
     __shared__ uint64_t swb[kPackThreads + 1];
-    __shared__ uint64_t sr0;
-    const uint64_t W = *d_words;
     for (uint64_t w0 = (uint64_t)blockIdx.x * kPackThreads; w0 < W; w0 += (uint64_t)gridDim.x * kPackThreads) {
-        if (threadIdx.x == 0) sr0 = upper_index(word_base, R, w0);
-        __syncthreads();
-        const uint64_t r0 = sr0;
+        const uint64_t r0 = block_rec[w0 / kPackThreads];
         const uint32_t nwin = (uint32_t)min<uint64_t>(kPackThreads + 1, R - r0);
         for (uint32_t i = threadIdx.x; i < nwin; i += blockDim.x) swb[i] = word_base[r0 + i];
         __syncthreads();
@@ -1335,11 +1341,15 @@ int Pipeline::build(const uint8_t *d_in, uint64_t n) {
     SWAP_CHECK(packed_.reserve((W + 1) * 8));
     SWAP_CHECK(wmeta_.reserve((W + 1) * 8));
     {
-        const unsigned grid = (unsigned)std::min<uint64_t>((W + kPackThreads - 1) / kPackThreads, 65535);
+        const uint64_t nblocks = (W + kPackThreads - 1) / kPackThreads;
+        const unsigned grid = (unsigned)std::min<uint64_t>(nblocks, 65535);
+        SWAP_CHECK(block_rec_.reserve(nblocks * 8 + 8));
         phase_begin(1);
+        hipLaunchKernelGGL(k_block_rec, dim3((unsigned)((R + 255) / 256)), dim3(256), 0, st_, word_base_.as<uint64_t>(),
+                           R, W, block_rec_.as<uint64_t>());
         hipLaunchKernelGGL(k_pack, dim3(grid), dim3(kPackThreads), 0, st_, buf, n, rec_start_.as<uint64_t>(),
                            rec_len_.as<uint32_t>(), word_base_.as<uint64_t>(), occ_base_.as<uint64_t>(), R, k_,
-                           d_small + 5, packed_.as<uint64_t>(), wmeta_.as<uint64_t>());
+                           block_rec_.as<uint64_t>(), W, packed_.as<uint64_t>(), wmeta_.as<uint64_t>());
         phase_end(1);
     }
     // --- partition

@@ -147,7 +147,7 @@ private:
     std::string err_;
     // build buffers
     DevBuf fasta_, tile_nl_, tile_off_, nl_pos_, scan_scratch_, small_, rec_start_, rec_len_, occ_count_,
-        word_count_, occ_base_, word_base_, packed_, wmeta_, rows_, hist_, bucket_, cursor_, hist1_, base1_, l1info_, rec1_, rec_, surv_key_, surv_mult_,
+        word_count_, occ_base_, word_base_, block_rec_, packed_, wmeta_, rows_, hist_, bucket_, cursor_, hist1_, base1_, l1info_, rec1_, rec_, surv_key_, surv_mult_,
         surv_first_, dfo_, sel_, sel_target_, sel_out_, overflow_, part_surv_, part_first_, gtab_key_, gtab_cnt_, gtab_fo_;
     // merge buffers
     enum {
