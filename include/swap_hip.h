@@ -63,6 +63,8 @@ extern "C" {
 #define SWAP_FLAG_TABLE_SPILL 4u /* testing aid: every count partition is recounted in the
                                     device-memory table that partitions too large for the
                                     on-chip table use; same results, slower */
+#define SWAP_FLAG_WIDE_PARTITIONS 8u /* testing aid: at least 2^18 count partitions, the
+                                        path inputs of billions of k-mers take; same results */
 
 typedef struct swap_cfg {
     int k;          /* odd k-mer length, 9..31 (the reference CLI accepts 19..31) */

@@ -325,7 +325,8 @@ __device__ __forceinline__ void for_each_occ(const OccCtx &c, uint64_t w, F &&f)
 }
 
 // Two-level radix partitioning of the occurrence records by the top bits of the
-// mixed key: 2^kL1Bits buckets first, then 2^kL2Bits sub-buckets of each.
+// mixed key: 2^kL1Bits buckets first, then 2^l2bits sub-buckets of each
+// (l2bits = pbits - kL1Bits, 7 for inputs up to ~2^28 occurrences).
 // Level-1 records keep the key bits below the level-1 bits:
 //   rec = (h mod 2^(2k-8)) << 8 | mask        (2k <= 62, so 2k bits in all)
 // and the level-2 bucket is read back from the record.  Both levels write each
@@ -335,8 +336,12 @@ __device__ __forceinline__ void for_each_occ(const OccCtx &c, uint64_t w, F &&f)
 // fills.  Record order inside a partition is not used by the counting pass.
 constexpr int kL1Bits = 8;
 constexpr uint32_t kL1 = 1u << kL1Bits;
-constexpr int kL2Bits = kPartBits - kL1Bits;
-constexpr uint32_t kL2 = 1u << kL2Bits;
+// the histogram pass bins by the top kPartBitsMin bits (level-1 bucket and the
+// next kHistL2Bits); finer partitions are counted after level 1
+constexpr uint32_t kHistBins = 1u << kPartBitsMin;
+constexpr int kHistL2Bits = kPartBitsMin - kL1Bits;
+constexpr uint32_t kHistL2 = 1u << kHistL2Bits;
+constexpr int kL2MaxBits = kPartBitsMax - kL1Bits;
 constexpr int kOccThreads = 1024;
 constexpr int kHistRowGroups = 8;     // row groups of the column sum
 constexpr int kChunkRecs = 4096;      // level-2 records per workgroup
@@ -388,20 +393,20 @@ __device__ __forceinline__ void word_range(uint64_t W, uint64_t &wb, uint64_t &w
 // k_col_sum) and its level-1 counts.
 __global__ void __launch_bounds__(kOccThreads) k_part_hist(OccCtx c, uint32_t *__restrict__ rows,
                                                             uint32_t *__restrict__ hist1) {
-    __shared__ uint32_t lh[kParts];
-    for (uint32_t i = threadIdx.x; i < kParts; i += blockDim.x) lh[i] = 0;
+    __shared__ uint32_t lh[kHistBins];
+    for (uint32_t i = threadIdx.x; i < kHistBins; i += blockDim.x) lh[i] = 0;
     __syncthreads();
     uint64_t wb, we;
     word_range(c.W, wb, we);
-    const int sh = 2 * c.k - kPartBits;
+    const int sh = 2 * c.k - kPartBitsMin;
     for (uint64_t w = wb + threadIdx.x; w < we; w += blockDim.x)
         for_each_occ(c, w, [&](uint64_t h, uint32_t, uint64_t) { atomicAdd(&lh[h >> sh], 1u); });
     __syncthreads();
-    uint32_t *row = rows + (uint64_t)blockIdx.x * kParts;
-    for (uint32_t i = threadIdx.x; i < kParts; i += blockDim.x) row[i] = lh[i];
+    uint32_t *row = rows + (uint64_t)blockIdx.x * kHistBins;
+    for (uint32_t i = threadIdx.x; i < kHistBins; i += blockDim.x) row[i] = lh[i];
     for (uint32_t b = threadIdx.x; b < kL1; b += blockDim.x) {
         uint32_t s1 = 0;
-        for (uint32_t j = 0; j < kL2; j++) s1 += lh[b * kL2 + j];
+        for (uint32_t j = 0; j < kHistL2; j++) s1 += lh[b * kHistL2 + j];
         hist1[(uint64_t)b * gridDim.x + blockIdx.x] = s1;
     }
 }
@@ -437,40 +442,70 @@ __global__ void __launch_bounds__(kOccThreads) k_part1(OccCtx c, const uint64_t 
         });
 }
 
+// Level-1 bucket b and its first record of the chunk `chunk` (chunk_start[b] =
+// first chunk index of bucket b, kL1 + 1 entries; chunks of `recs` records).
+__device__ __forceinline__ bool chunk_range(uint32_t chunk, const uint32_t *__restrict__ l1_start,
+                                            const uint32_t *__restrict__ chunk_start, uint32_t recs, uint32_t &b1,
+                                            uint64_t &a, uint64_t &e) {
+    if (chunk >= chunk_start[kL1]) return false;
+    uint32_t lo = 0, hi = kL1;  // last b with chunk_start[b] <= chunk
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) / 2;
+        if (chunk_start[mid] <= chunk) lo = mid; else hi = mid;
+    }
+    b1 = lo;
+    a = (uint64_t)l1_start[lo] + (uint64_t)(chunk - chunk_start[lo]) * recs;
+    e = min<uint64_t>((uint64_t)l1_start[lo + 1], a + recs);
+    return true;
+}
+
+// Exclusive scan of cnt[0..n) into start[] by the whole workgroup (n <= items
+// per thread x blockDim.x; blockDim.x a multiple of 64, at most 1024).
+template <int kItems>
+__device__ __forceinline__ void block_scan_excl(const uint32_t *cnt, uint32_t *start, uint32_t n) {
+    __shared__ uint32_t wtot[16];
+    const uint32_t t = threadIdx.x, base = t * kItems;
+    uint32_t v[kItems], sum = 0;
+#pragma unroll
+    for (int j = 0; j < kItems; j++) {
+        v[j] = base + j < n ? cnt[base + j] : 0;
+        sum += v[j];
+    }
+    const uint32_t inc = wave_incl_sum(sum);
+    if ((t & 63) == 63) wtot[t >> 6] = inc;
+    __syncthreads();
+    uint32_t off = inc - sum;
+    for (uint32_t w = 0; w < (t >> 6); w++) off += wtot[w];
+#pragma unroll
+    for (int j = 0; j < kItems; j++) {
+        if (base + j < n) start[base + j] = off;
+        off += v[j];
+    }
+}
+
 // Level 2: each workgroup takes one chunk of one level-1 bucket, counts it by
 // level-2 bucket in LDS, claims a run of each final partition with one atomic
 // per sub-bucket, orders the chunk by sub-bucket in LDS and writes it out in
 // that order, so consecutive lanes store consecutive records of one run.
-// chunk_start[b] = first chunk index of level-1 bucket b (kL1 + 1 entries).
-template <bool WITH_OCC>
+// L2MAX bounds 2^l2bits (the LDS counters are sized by it).
+template <bool WITH_OCC, int L2MAX>
 __global__ void __launch_bounds__(kL2Threads) k_part2(const uint32_t *__restrict__ rec_in,
                                                        const uint32_t *__restrict__ l1_start,
-                                                       const uint32_t *__restrict__ chunk_start, int k,
+                                                       const uint32_t *__restrict__ chunk_start, int k, int l2bits,
                                                        uint32_t *__restrict__ cursor, uint32_t *__restrict__ rec_out) {
-    __shared__ uint32_t lcnt[kL2];
-    __shared__ uint32_t lstart[kL2];
-    __shared__ uint32_t gbase[kL2];
+    __shared__ uint32_t lcnt[L2MAX];
+    __shared__ uint32_t lstart[L2MAX];
+    __shared__ uint32_t gbase[L2MAX];
     __shared__ uint32_t slo[kChunkRecs], shi[kChunkRecs];  // the chunk in sub-bucket order
     __shared__ uint32_t socc[WITH_OCC ? kChunkRecs : 1];
-    __shared__ uint32_t sb, sfirst;
-    const uint32_t chunk = blockIdx.x;
-    if (threadIdx.x == 0) {
-        uint32_t lo = 0, hi = kL1;  // last b with chunk_start[b] <= chunk
-        while (hi - lo > 1) {
-            const uint32_t mid = (lo + hi) / 2;
-            if (chunk_start[mid] <= chunk) lo = mid; else hi = mid;
-        }
-        sb = lo;
-        sfirst = chunk_start[lo];
-    }
-    for (uint32_t b = threadIdx.x; b < kL2; b += blockDim.x) lcnt[b] = 0;
+    const uint32_t nl2 = 1u << l2bits, m2 = nl2 - 1;
+    uint32_t b1;
+    uint64_t a, e;
+    if (!chunk_range(blockIdx.x, l1_start, chunk_start, kChunkRecs, b1, a, e)) return;
+    for (uint32_t b = threadIdx.x; b < nl2; b += blockDim.x) lcnt[b] = 0;
     __syncthreads();
-    if (chunk >= chunk_start[kL1]) return;
-    const uint32_t b1 = sb;
-    const uint64_t a = (uint64_t)l1_start[b1] + (uint64_t)(chunk - sfirst) * kChunkRecs;
-    const uint64_t e = min<uint64_t>((uint64_t)l1_start[b1 + 1], a + kChunkRecs);
     const uint32_t n = (uint32_t)(e - a);
-    const int sh2 = 8 + 2 * k - kPartBits;
+    const int sh2 = 8 + 2 * k - (kL1Bits + l2bits);
     uint64_t r[kL2Per];
     uint32_t o[kL2Per], rank[kL2Per];
 #pragma unroll
@@ -478,27 +513,21 @@ __global__ void __launch_bounds__(kL2Threads) k_part2(const uint32_t *__restrict
         const uint32_t i = j * kL2Threads + threadIdx.x;
         if (i < n) {
             r[j] = rec_load<WITH_OCC>(rec_in, a + i, o[j]);
-            rank[j] = atomicAdd(&lcnt[(r[j] >> sh2) & (kL2 - 1)], 1u);
+            rank[j] = atomicAdd(&lcnt[(r[j] >> sh2) & m2], 1u);
         }
     }
     __syncthreads();
-    static_assert(kL2 == 128, "the sub-bucket scan takes two entries per lane of one wave");
-    if (threadIdx.x < 64) {
-        const uint32_t c0 = lcnt[2 * threadIdx.x], c1 = lcnt[2 * threadIdx.x + 1];
-        const uint32_t ex = wave_incl_sum(c0 + c1) - c0 - c1;
-        lstart[2 * threadIdx.x] = ex;
-        lstart[2 * threadIdx.x + 1] = ex + c0;
-    }
-    if (threadIdx.x < kL2) {
-        const uint32_t c = lcnt[threadIdx.x];
-        gbase[threadIdx.x] = c ? atomicAdd(&cursor[b1 * kL2 + threadIdx.x], c) : 0;
+    block_scan_excl<(L2MAX + kL2Threads - 1) / kL2Threads>(lcnt, lstart, nl2);
+    for (uint32_t b = threadIdx.x; b < nl2; b += blockDim.x) {
+        const uint32_t c = lcnt[b];
+        gbase[b] = c ? atomicAdd(&cursor[((uint64_t)b1 << l2bits) + b], c) : 0;
     }
     __syncthreads();
 #pragma unroll
     for (int j = 0; j < kL2Per; j++) {
         const uint32_t i = j * kL2Threads + threadIdx.x;
         if (i < n) {
-            const uint32_t pos = lstart[(r[j] >> sh2) & (kL2 - 1)] + rank[j];
+            const uint32_t pos = lstart[(r[j] >> sh2) & m2] + rank[j];
             slo[pos] = (uint32_t)r[j];
             shi[pos] = (uint32_t)(r[j] >> 32);
             if (WITH_OCC) socc[pos] = o[j];
@@ -507,21 +536,51 @@ __global__ void __launch_bounds__(kL2Threads) k_part2(const uint32_t *__restrict
     __syncthreads();
     for (uint32_t i = threadIdx.x; i < n; i += kL2Threads) {
         const uint64_t rec = (uint64_t)slo[i] | ((uint64_t)shi[i] << 32);
-        const uint32_t b = (uint32_t)(rec >> sh2) & (kL2 - 1);
+        const uint32_t b = (uint32_t)(rec >> sh2) & m2;
         rec_store<WITH_OCC>(rec_out, (uint64_t)gbase[b] + (i - lstart[b]), rec, WITH_OCC ? socc[i] : 0u);
     }
 }
 
-// chunk_start[b] for the level-2 pass from the final partition starts.
-__global__ void k_l1_chunks(const uint32_t *__restrict__ pstart, uint32_t *__restrict__ l1_start,
-                            uint32_t *__restrict__ chunk_start) {
+// Level-1 bucket starts l1_start[b] = pstart[b << l2bits] (pstart: partition
+// starts at l2bits sub-buckets per bucket), and the chunk table of chunks of
+// `recs` records.
+__global__ void k_l1_chunks(const uint32_t *__restrict__ pstart, int l2bits, uint32_t recs,
+                            uint32_t *__restrict__ l1_start, uint32_t *__restrict__ chunk_start) {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
     uint32_t ch = 0;
     for (uint32_t b = 0; b <= kL1; b++) {
-        l1_start[b] = pstart[b * kL2];
+        l1_start[b] = pstart[(uint64_t)b << l2bits];
         chunk_start[b] = ch;
-        if (b < kL1) ch += (pstart[(b + 1) * kL2] - pstart[b * kL2] + kChunkRecs - 1) / kChunkRecs;
+        if (b < kL1) ch += (pstart[(uint64_t)(b + 1) << l2bits] - pstart[(uint64_t)b << l2bits] + recs - 1) / recs;
     }
+}
+
+// Final partition sizes when they are finer than the histogram pass's bins:
+// each workgroup counts one chunk of one level-1 bucket by sub-bucket in LDS
+// and adds its non-zero counts (large chunks keep the device atomics few).
+constexpr uint32_t kFineChunkRecs = 1u << 16;
+
+template <bool WITH_OCC>
+__global__ void __launch_bounds__(1024) k_fine_hist(const uint32_t *__restrict__ rec_in,
+                                                    const uint32_t *__restrict__ l1_start,
+                                                    const uint32_t *__restrict__ chunk_start, int k, int l2bits,
+                                                    uint32_t *__restrict__ sizes) {
+    __shared__ uint32_t lh[1u << kL2MaxBits];
+    const uint32_t nl2 = 1u << l2bits, m2 = nl2 - 1;
+    uint32_t b1;
+    uint64_t a, e;
+    if (!chunk_range(blockIdx.x, l1_start, chunk_start, kFineChunkRecs, b1, a, e)) return;
+    for (uint32_t b = threadIdx.x; b < nl2; b += blockDim.x) lh[b] = 0;
+    __syncthreads();
+    const int sh2 = 8 + 2 * k - (kL1Bits + l2bits);
+    for (uint64_t i = a + threadIdx.x; i < e; i += blockDim.x) {
+        uint32_t o;
+        const uint64_t r = rec_load<WITH_OCC>(rec_in, i, o);
+        atomicAdd(&lh[(r >> sh2) & m2], 1u);
+    }
+    __syncthreads();
+    for (uint32_t b = threadIdx.x; b < nl2; b += blockDim.x)
+        if (lh[b]) atomicAdd(&sizes[((uint64_t)b1 << l2bits) + b], lh[b]);
 }
 
 // ---------------------------------------------------------------------------
@@ -540,7 +599,9 @@ constexpr uint64_t kEmpty = ~0ull;
 
 struct CountArgs {
     const uint32_t *rec;      // final partitions, rec_words() words per record
-    const uint32_t *bucket;   // start of each final partition, kParts+1 entries
+    const uint32_t *bucket;   // start of each final partition, nparts + 1 entries
+    uint32_t nparts;          // 2^pbits
+    int l2bits;               // pbits - kL1Bits
     int k;
     int c;
     Mixer mx;
@@ -772,7 +833,7 @@ __device__ void table_emit(const uint64_t *tkey, CNT cnt_of, const uint32_t *tfo
             if (keys[j] == kEmpty) continue;
             if (WITH_OCC && a.dfo) a.dfo[od++] = fos[j];
             if (mults[j]) {
-                a.surv_key[os] = ((uint64_t)(part >> kL2Bits) << hb) | keys[j];  // mixed; k_unmix inverts
+                a.surv_key[os] = ((uint64_t)(part >> a.l2bits) << hb) | keys[j];  // mixed; k_unmix inverts
                 a.surv_mult[os] = mults[j];
                 if (WITH_OCC) a.surv_first[os] = fos[j];
                 os++;
@@ -786,8 +847,11 @@ __device__ void table_emit(const uint64_t *tkey, CNT cnt_of, const uint32_t *tfo
 // Partitions hold N / 2^15 records on average (about 4900 for a 50x bacterial
 // sample: under 5 per thread); the rest load inside the insert loop.
 constexpr int kPre = 6;
-// Partitions per workgroup whose bounds are staged in LDS (grid >= kParts / this).
+// Partitions per workgroup whose bounds are staged in LDS (grid >= nparts / this).
 constexpr uint32_t kMaxPartsPerGroup = 512;
+// Records per partition the partition count aims at (distinct keys are a
+// fraction of them; the LDS table holds 4096).
+constexpr uint64_t kTargetPartRecs = 8192;
 
 // Persistent and software-pipelined: one workgroup per CU walks the partitions
 // blockIdx.x, blockIdx.x + gridDim.x, ...  Iteration i inserts partition i's
@@ -813,7 +877,8 @@ __global__ void __launch_bounds__(kCountThreads) k_count_lds(CountArgs a) {
     const uint32_t t = threadIdx.x;
     const int lane = t & 63, w = t >> 6;
     const int hb = 2 * a.k - kL1Bits;
-    const uint32_t total = a.bucket[kParts];  // >= 1: the host skips empty builds
+    const uint32_t nparts = a.nparts;
+    const uint32_t total = a.bucket[nparts];  // >= 1: the host skips empty builds
     const uint32_t bias = bias16(a.c);
     uint64_t *tcnt = reinterpret_cast<uint64_t *>(tc);
     for (uint32_t i = t; i < kTab; i += kCountThreads) {
@@ -824,8 +889,8 @@ __global__ void __launch_bounds__(kCountThreads) k_count_lds(CountArgs a) {
     if (t < 2) overflow[t] = 0;
     for (uint32_t i = t; i <= kMaxPartsPerGroup; i += kCountThreads) {
         const uint64_t p = blockIdx.x + (uint64_t)i * gridDim.x;
-        bnd[i][0] = p < kParts ? a.bucket[p] : 0;
-        bnd[i][1] = p < kParts ? a.bucket[p + 1] : 0;
+        bnd[i][0] = p < nparts ? a.bucket[p] : 0;
+        bnd[i][1] = p < nparts ? a.bucket[p + 1] : 0;
     }
     __syncthreads();
 
@@ -857,7 +922,7 @@ __global__ void __launch_bounds__(kCountThreads) k_count_lds(CountArgs a) {
 
     prefetch(0);
     for (uint32_t part = blockIdx.x, it = 0;; part += gridDim.x, it++) {
-        const bool have = part < kParts;  // uniform over the workgroup
+        const bool have = part < nparts;  // uniform over the workgroup
         const bool small = (e - b) < 0xFFFFu;
         bool ok = true;
         if (have) {
@@ -901,7 +966,7 @@ __global__ void __launch_bounds__(kCountThreads) k_count_lds(CountArgs a) {
                 if (keys[j] == kEmpty) continue;
                 if (WITH_OCC && a.dfo) a.dfo[od++] = fos[j];
                 if (mults[j]) {
-                    a.surv_key[os] = ((uint64_t)(prev_part >> kL2Bits) << hb) | keys[j];  // mixed; k_unmix inverts
+                    a.surv_key[os] = ((uint64_t)(prev_part >> a.l2bits) << hb) | keys[j];  // mixed; k_unmix inverts
                     a.surv_mult[os] = mults[j];
                     if (WITH_OCC) a.surv_first[os] = fos[j];
                     os++;
@@ -942,7 +1007,7 @@ __global__ void __launch_bounds__(kCountThreads) k_count_lds(CountArgs a) {
         pv = ((uint64_t)nd << 32) | ns;
         pinc = wave_incl_sum(pv);
         if (lane == 63) wsum[w] = pinc;
-        if (part + gridDim.x < kParts) prefetch(it + 1);
+        if (part + gridDim.x < nparts) prefetch(it + 1);
         __syncthreads();  // wave sums complete; the table is empty
         if (w == 0) {
             // wave offsets: exclusive scan of the per-wave totals by wave 0
@@ -1363,32 +1428,45 @@ int Pipeline::build(const uint8_t *d_in, uint64_t n) {
     SWAP_CHECK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev_));
     const unsigned occ_grid = (unsigned)std::max(8, ncu);
     const uint64_t n1 = (uint64_t)kL1 * occ_grid;
-    SWAP_CHECK(hist_.reserve((kParts + 1) * 4));   // final partition sizes, then starts (u32)
-    SWAP_CHECK(bucket_.reserve((kParts + 1) * 8));
-    SWAP_CHECK(cursor_.reserve((kParts + 1) * 4));
+    // partition bits: about kTargetPartRecs records per partition, so that its
+    // distinct keys fit the 4096-slot LDS table; at most 2k (the key bits)
+    int pbits = kPartBitsMin;
+    while (pbits < kPartBitsMax && pbits < 2 * k_ && (N >> pbits) > kTargetPartRecs) pbits++;
+    if (wide_parts_) pbits = std::max(pbits, std::min(kPartBitsMax - 2, 2 * k_));
+    pbits_ = pbits;
+    const int l2bits = pbits - kL1Bits;
+    const uint64_t nparts = 1ull << pbits;
+    SWAP_CHECK(hist_.reserve((nparts + 1) * 4));   // final partition sizes, then starts (u32)
+    SWAP_CHECK(bucket_.reserve((nparts + 1) * 8));
+    SWAP_CHECK(cursor_.reserve((nparts + 1) * 4));
     SWAP_CHECK(hist1_.reserve(n1 * 4));
     SWAP_CHECK(base1_.reserve((n1 + 1) * 8));
-    SWAP_CHECK(l1info_.reserve((2 * kL1 + 4) * 4));
-    SWAP_CHECK(rows_.reserve((uint64_t)occ_grid * kParts * 4));
-    SWAP_CHECK(hipMemsetAsync(hist_.p, 0, (kParts + 1) * 4, st_));
+    SWAP_CHECK(l1info_.reserve((3 * kL1 + 8) * 4));
+    SWAP_CHECK(rows_.reserve((uint64_t)occ_grid * kHistBins * 4));
+    SWAP_CHECK(hipMemsetAsync(hist_.p, 0, (kHistBins + 1) * 4, st_));
     phase_begin(2);
     hipLaunchKernelGGL(k_part_hist, dim3(occ_grid), dim3(kOccThreads), 0, st_, oc, rows_.as<uint32_t>(),
                        hist1_.as<uint32_t>());
-    hipLaunchKernelGGL(k_col_sum, dim3(kParts / 256, kHistRowGroups), dim3(256), 0, st_, rows_.as<uint32_t>(),
-                       occ_grid, kParts, hist_.as<uint32_t>());
+    hipLaunchKernelGGL(k_col_sum, dim3(kHistBins / 256, kHistRowGroups), dim3(256), 0, st_, rows_.as<uint32_t>(),
+                       occ_grid, kHistBins, hist_.as<uint32_t>());
     phase_end(2);
-    SWAP_CHECK(scan_scratch_.reserve((scan_scratch_words(n1 + kParts + 1) + 16) * 8));
-    SWAP_CHECK(scan_excl<uint32_t>(hist_.as<uint32_t>(), bucket_.as<uint64_t>(), kParts + 1, scan_scratch_.as<uint64_t>(),
-                        d_small + 6, st_));
+    SWAP_CHECK(scan_scratch_.reserve((scan_scratch_words(std::max<uint64_t>(n1, nparts + 1)) + 16) * 8));
+    SWAP_CHECK(scan_excl<uint32_t>(hist_.as<uint32_t>(), bucket_.as<uint64_t>(), kHistBins + 1,
+                                   scan_scratch_.as<uint64_t>(), d_small + 6, st_));
     SWAP_CHECK(scan_excl<uint32_t>(hist1_.as<uint32_t>(), base1_.as<uint64_t>(), n1, scan_scratch_.as<uint64_t>(),
                         d_small + 6, st_));
-    // u32 partition starts for the count pass, and the level-2 cursors
-    hipLaunchKernelGGL(k_u64_to_u32, dim3((kParts + 1 + 255) / 256), dim3(256), 0, st_, bucket_.as<uint64_t>(),
-                       (uint64_t)kParts + 1, hist_.as<uint32_t>());
-    SWAP_CHECK(hipMemcpyAsync(cursor_.p, hist_.p, (kParts + 1) * 4, hipMemcpyDeviceToDevice, st_));
+    // u32 starts of the histogram's bins: the final partition starts (and the
+    // level-2 cursors) when the partitions are these bins
+    hipLaunchKernelGGL(k_u64_to_u32, dim3((kHistBins + 1 + 255) / 256), dim3(256), 0, st_, bucket_.as<uint64_t>(),
+                       (uint64_t)kHistBins + 1, hist_.as<uint32_t>());
     uint32_t *l1_start = l1info_.as<uint32_t>();
     uint32_t *chunk_start = l1_start + kL1 + 2;
-    hipLaunchKernelGGL(k_l1_chunks, dim3(1), dim3(64), 0, st_, hist_.as<uint32_t>(), l1_start, chunk_start);
+    uint32_t *fchunk_start = chunk_start + kL1 + 2;
+    hipLaunchKernelGGL(k_l1_chunks, dim3(1), dim3(64), 0, st_, hist_.as<uint32_t>(), kHistL2Bits, (uint32_t)kChunkRecs,
+                       l1_start, chunk_start);
+    if (pbits > kPartBitsMin)
+        hipLaunchKernelGGL(k_l1_chunks, dim3(1), dim3(64), 0, st_, hist_.as<uint32_t>(), kHistL2Bits,
+                           kFineChunkRecs, l1_start, fchunk_start);
     const uint64_t rec_bytes = N * 4 * (uint64_t)rec_words(ref_order_);
     SWAP_CHECK(rec1_.reserve(rec_bytes));
     SWAP_CHECK(rec_.reserve(rec_bytes));
@@ -1400,14 +1478,41 @@ int Pipeline::build(const uint8_t *d_in, uint64_t n) {
         hipLaunchKernelGGL(k_part1<false>, dim3(occ_grid), dim3(kOccThreads), 0, st_, oc, base1_.as<uint64_t>(),
                            rec1_.as<uint32_t>());
     phase_end(3);
+    if (pbits > kPartBitsMin) {
+        // finer partitions: their sizes from the level-1 output
+        SWAP_CHECK(hipMemsetAsync(hist_.p, 0, (nparts + 1) * 4, st_));
+        const unsigned gridf = (unsigned)(N / kFineChunkRecs + kL1 + 1);
+        if (ref_order_)
+            hipLaunchKernelGGL(k_fine_hist<true>, dim3(gridf), dim3(1024), 0, st_, rec1_.as<uint32_t>(), l1_start,
+                               fchunk_start, k_, l2bits, hist_.as<uint32_t>());
+        else
+            hipLaunchKernelGGL(k_fine_hist<false>, dim3(gridf), dim3(1024), 0, st_, rec1_.as<uint32_t>(), l1_start,
+                               fchunk_start, k_, l2bits, hist_.as<uint32_t>());
+        SWAP_CHECK(scan_excl<uint32_t>(hist_.as<uint32_t>(), bucket_.as<uint64_t>(), nparts + 1,
+                                       scan_scratch_.as<uint64_t>(), d_small + 6, st_));
+        hipLaunchKernelGGL(k_u64_to_u32, dim3((unsigned)((nparts + 1 + 255) / 256)), dim3(256), 0, st_,
+                           bucket_.as<uint64_t>(), nparts + 1, hist_.as<uint32_t>());
+    }
+    SWAP_CHECK(hipMemcpyAsync(cursor_.p, hist_.p, (nparts + 1) * 4, hipMemcpyDeviceToDevice, st_));
     const unsigned grid2 = (unsigned)(N / kChunkRecs + kL1 + 1);
     phase_begin(7);
-    if (ref_order_)
-        hipLaunchKernelGGL(k_part2<true>, dim3(grid2), dim3(kL2Threads), 0, st_, rec1_.as<uint32_t>(), l1_start,
-                           chunk_start, k_, cursor_.as<uint32_t>(), rec_.as<uint32_t>());
-    else
-        hipLaunchKernelGGL(k_part2<false>, dim3(grid2), dim3(kL2Threads), 0, st_, rec1_.as<uint32_t>(), l1_start,
-                           chunk_start, k_, cursor_.as<uint32_t>(), rec_.as<uint32_t>());
+    if (pbits == kPartBitsMin) {
+        if (ref_order_)
+            hipLaunchKernelGGL((k_part2<true, kHistL2>), dim3(grid2), dim3(kL2Threads), 0, st_, rec1_.as<uint32_t>(),
+                               l1_start, chunk_start, k_, l2bits, cursor_.as<uint32_t>(), rec_.as<uint32_t>());
+        else
+            hipLaunchKernelGGL((k_part2<false, kHistL2>), dim3(grid2), dim3(kL2Threads), 0, st_, rec1_.as<uint32_t>(),
+                               l1_start, chunk_start, k_, l2bits, cursor_.as<uint32_t>(), rec_.as<uint32_t>());
+    } else {
+        if (ref_order_)
+            hipLaunchKernelGGL((k_part2<true, (1 << kL2MaxBits)>), dim3(grid2), dim3(kL2Threads), 0, st_,
+                               rec1_.as<uint32_t>(), l1_start, chunk_start, k_, l2bits, cursor_.as<uint32_t>(),
+                               rec_.as<uint32_t>());
+        else
+            hipLaunchKernelGGL((k_part2<false, (1 << kL2MaxBits)>), dim3(grid2), dim3(kL2Threads), 0, st_,
+                               rec1_.as<uint32_t>(), l1_start, chunk_start, k_, l2bits, cursor_.as<uint32_t>(),
+                               rec_.as<uint32_t>());
+    }
     phase_end(7);
     // --- count + cutoff
     // survivors <= distinct <= N; reserve N to avoid a sync (grow-only buffers)
@@ -1417,12 +1522,14 @@ int Pipeline::build(const uint8_t *d_in, uint64_t n) {
         SWAP_CHECK(surv_first_.reserve(N * 4));
         SWAP_CHECK(dfo_.reserve(N * 4));
     }
-    SWAP_CHECK(overflow_.reserve(kParts * 4));
-    SWAP_CHECK(part_surv_.reserve(kParts * 4));
-    SWAP_CHECK(part_first_.reserve(kParts * 4));
+    SWAP_CHECK(overflow_.reserve(nparts * 4));
+    SWAP_CHECK(part_surv_.reserve(nparts * 4));
+    SWAP_CHECK(part_first_.reserve(nparts * 4));
     CountArgs ca;
     ca.rec = rec_.as<uint32_t>();
     ca.bucket = hist_.as<uint32_t>();
+    ca.nparts = (uint32_t)nparts;
+    ca.l2bits = l2bits;
     ca.k = k_;
     ca.c = c_;
     ca.mx = mx_;
@@ -1439,7 +1546,7 @@ int Pipeline::build(const uint8_t *d_in, uint64_t n) {
     // one resident workgroup per CU (the LDS table takes most of a CU's LDS),
     // and at least the number whose partition bounds fit the kernel's LDS list
     const unsigned count_grid = (unsigned)std::min<int>(
-        (int)kParts, std::max(ncu, (int)((kParts + kMaxPartsPerGroup - 1) / kMaxPartsPerGroup)));
+        (int)nparts, std::max(ncu, (int)((nparts + kMaxPartsPerGroup - 1) / kMaxPartsPerGroup)));
     if (ref_order_)
         hipLaunchKernelGGL(k_count_lds<true>, dim3(count_grid), dim3(kCountThreads), 0, st_, ca);
     else
@@ -1451,9 +1558,9 @@ int Pipeline::build(const uint8_t *d_in, uint64_t n) {
     n_overflow = hs[10];
     if (n_overflow) {
         // size a global table per overflowing partition from its record count
-        std::vector<uint32_t> list(n_overflow), starts(kParts + 1);
+        std::vector<uint32_t> list(n_overflow), starts(nparts + 1);
         SWAP_CHECK(hipMemcpy(list.data(), overflow_.p, n_overflow * 4, hipMemcpyDeviceToHost));
-        SWAP_CHECK(hipMemcpy(starts.data(), hist_.p, (kParts + 1) * 4, hipMemcpyDeviceToHost));
+        SWAP_CHECK(hipMemcpy(starts.data(), hist_.p, (nparts + 1) * 4, hipMemcpyDeviceToHost));
         std::vector<uint64_t> off(n_overflow);
         std::vector<uint32_t> bits(n_overflow);
         uint64_t tot = 0;

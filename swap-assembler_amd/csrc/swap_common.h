@@ -10,10 +10,12 @@
 
 namespace swapk {
 
-// Partition count for device-local hash partitioning: 2^kPartBits buckets keyed
-// by the top bits of a bijective mix of the canonical k-mer.
-constexpr int kPartBits = 15;
-constexpr uint32_t kParts = 1u << kPartBits;
+// Device-local hash partitioning: 2^pbits partitions keyed by the top bits of
+// a bijective mix of the canonical k-mer, pbits chosen per build from the input
+// size within [kPartBitsMin, kPartBitsMax] so that a partition's distinct keys
+// fit the on-chip counting table.
+constexpr int kPartBitsMin = 15;
+constexpr int kPartBitsMax = 20;
 
 SWAP_HD uint64_t kmask(int k) { return (k >= 32) ? ~0ull : ((1ull << (2 * k)) - 1); }
 

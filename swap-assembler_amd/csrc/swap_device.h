@@ -92,6 +92,8 @@ public:
     void reconfigure(int k, int c, bool reference_order);
     // every count partition through the device-memory table (a testing aid)
     void set_table_spill(bool on) { table_spill_ = on; }
+    // at least 2^18 partitions on any input (a testing aid for large inputs)
+    void set_wide_parts(bool on) { wide_parts_ = on; }
     int device() const { return dev_; }
     hipStream_t stream() const { return st_; }
 
@@ -141,6 +143,8 @@ private:
     int k_, c_;
     bool ref_order_;
     bool table_spill_ = false;
+    bool wide_parts_ = false;
+    int pbits_ = kPartBitsMin;  // partition bits of the last build
     hipStream_t st_ = nullptr;
     hipError_t init_err_ = hipSuccess;
     Mixer mx_;

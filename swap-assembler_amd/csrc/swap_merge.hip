@@ -45,7 +45,7 @@ constexpr uint64_t kIdxEmpty = ~0ull;
 // ---------------------------------------------------------------------------
 // survivor index
 
-// The survivors of count partition p (p = top kPartBits bits of the mixed key,
+// The survivors of count partition p (p = top pbits bits of the mixed key,
 // the partition the count kernel took them from) own one open-addressing
 // region of entries [base[p], base[p+1]); its size is a power of two >= 1.5x
 // their number (0 for a partition without survivors).  An entry is 16 bytes
@@ -53,9 +53,9 @@ constexpr uint64_t kIdxEmpty = ~0ull;
 
 struct SurvIndex {
     const uint64_t *ent;   // 2 words per entry: key (kIdxEmpty if free), survivor id
-    const uint64_t *base;  // kParts + 1 region starts
+    const uint64_t *base;  // nparts + 1 region starts
     Mixer mx;
-    int shift;             // 2k - kPartBits
+    int shift;             // 2k - pbits
 };
 
 __device__ __forceinline__ uint32_t region_size(uint32_t ns) {
@@ -64,9 +64,9 @@ __device__ __forceinline__ uint32_t region_size(uint32_t ns) {
     return 1u << (32 - __clz(want - 1));
 }
 
-__global__ void k_index_sizes(const uint32_t *__restrict__ part_surv, uint32_t *__restrict__ size) {
+__global__ void k_index_sizes(const uint32_t *__restrict__ part_surv, uint32_t nparts, uint32_t *__restrict__ size) {
     const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p < kParts) size[p] = region_size(part_surv[p]);
+    if (p < nparts) size[p] = region_size(part_surv[p]);
 }
 
 // Regions built in LDS, one workgroup per partition whose survivors form one
@@ -771,21 +771,23 @@ int Pipeline::merge_device(bool reference_order, MergeDev &out) {
     MCHECK(hipMemsetAsync(d_small + 16, 0, 16 * 8, st_));
     unsigned long long *cnt = (unsigned long long *)(d_small + 16);
     // --- survivor index: one region per count partition
-    const uint64_t ent_bound = 3 * S + 2 * (uint64_t)kParts;  // region_size(n) <= 3n + 2
-    MCHECK(mb_[M_IBASE].reserve((2 * (uint64_t)kParts + 1) * 8));
+    const uint64_t nparts = 1ull << pbits_;
+    const uint64_t ent_bound = 3 * S + 2 * nparts;  // region_size(n) <= 3n + 2
+    MCHECK(mb_[M_IBASE].reserve((2 * nparts + 1) * 8));
     MCHECK(mb_[M_IENT].reserve(ent_bound * 16));
-    MCHECK(scan_scratch_.reserve((scan_scratch_words(kParts) + 16) * 8));
+    MCHECK(scan_scratch_.reserve((scan_scratch_words(nparts) + 16) * 8));
     uint64_t *ibase = mb_[M_IBASE].as<uint64_t>();
-    uint32_t *isize = reinterpret_cast<uint32_t *>(ibase + kParts + 1);
+    uint32_t *isize = reinterpret_cast<uint32_t *>(ibase + nparts + 1);
     SurvIndex ix;
     ix.ent = mb_[M_IENT].as<uint64_t>();
     ix.base = ibase;
     ix.mx = mx_;
-    ix.shift = 2 * k_ - kPartBits;
+    ix.shift = 2 * k_ - pbits_;
     phase_begin(8);
-    hipLaunchKernelGGL(k_index_sizes, dim3(kParts / 256), dim3(256), 0, st_, part_surv_.as<uint32_t>(), isize);
-    MCHECK(scan_excl<uint32_t>(isize, ibase, kParts, scan_scratch_.as<uint64_t>(), ibase + kParts, st_));
-    hipLaunchKernelGGL(k_index_part, dim3(kParts), dim3(kIdxThreads), 0, st_, surv_key_.as<uint64_t>(),
+    hipLaunchKernelGGL(k_index_sizes, dim3((unsigned)(nparts / 256)), dim3(256), 0, st_, part_surv_.as<uint32_t>(),
+                       (uint32_t)nparts, isize);
+    MCHECK(scan_excl<uint32_t>(isize, ibase, nparts, scan_scratch_.as<uint64_t>(), ibase + nparts, st_));
+    hipLaunchKernelGGL(k_index_part, dim3((unsigned)nparts), dim3(kIdxThreads), 0, st_, surv_key_.as<uint64_t>(),
                        part_surv_.as<uint32_t>(), part_first_.as<uint32_t>(), ix, mb_[M_IENT].as<uint64_t>());
     hipLaunchKernelGGL(k_index_loose, dim3(nblk(S)), dim3(256), 0, st_, surv_key_.as<uint64_t>(), S,
                        part_first_.as<uint32_t>(), ix, mb_[M_IENT].as<uint64_t>());

@@ -16,6 +16,7 @@ GEN_PATH = os.path.join(HERE, "lib", "libswapgen.so")
 SWAP_FLAG_KMER_GRAPH = 1
 SWAP_FLAG_HASH_ORDER = 2
 SWAP_FLAG_TABLE_SPILL = 4
+SWAP_FLAG_WIDE_PARTITIONS = 8
 SWAP_FILE_KMER_GRAPH = 0
 SWAP_FILE_CONTIG_GRAPH = 1
 SWAP_FILE_NOCE = 2
@@ -143,11 +144,13 @@ class SwapError(RuntimeError):
 class Assembler:
     """One assembly context on one device (a thin wrapper of swap_ctx)."""
 
-    def __init__(self, k, c, device=0, kmer_graph_only=False, hash_order=False, table_spill=False):
+    def __init__(self, k, c, device=0, kmer_graph_only=False, hash_order=False, table_spill=False,
+                 wide_partitions=False):
         self.k, self.c = k, c
         cfg = SwapCfg(k, c, device, (SWAP_FLAG_KMER_GRAPH if kmer_graph_only else 0)
                       | (SWAP_FLAG_HASH_ORDER if hash_order else 0)
-                      | (SWAP_FLAG_TABLE_SPILL if table_spill else 0))
+                      | (SWAP_FLAG_TABLE_SPILL if table_spill else 0)
+                      | (SWAP_FLAG_WIDE_PARTITIONS if wide_partitions else 0))
         h = ctypes.c_void_p()
         rc = lib().swap_create(ctypes.byref(cfg), ctypes.byref(h))
         if rc:

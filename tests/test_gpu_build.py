@@ -22,11 +22,12 @@ def gold_fa(name):
         return f.read()
 
 
-def check_build(fa, k, c, hash_order=False, check_file=True, table_spill=False):
-    print("case k=%d c=%d bytes=%d hash_order=%s table_spill=%s" % (k, c, len(fa), hash_order, table_spill),
-          flush=True)
+def check_build(fa, k, c, hash_order=False, check_file=True, table_spill=False, wide=False):
+    print("case k=%d c=%d bytes=%d hash_order=%s table_spill=%s wide=%s" % (k, c, len(fa), hash_order, table_spill,
+                                                                          wide), flush=True)
     o = Oracle(fa, k, c, kmer_graph_only=True)
-    a = swapamd.Assembler(k, c, kmer_graph_only=True, hash_order=hash_order, table_spill=table_spill)
+    a = swapamd.Assembler(k, c, kmer_graph_only=True, hash_order=hash_order, table_spill=table_spill,
+                          wide_partitions=wide)
     a.build(fa)
     st = a.stats()
     ost = o.stats()
@@ -67,6 +68,20 @@ def test_device_memory_table(name):
     # more distinct keys than the on-chip table take
     case = CASES[name]
     check_build(gold_fa(name), case["k"], case["c"], table_spill=True)
+
+
+@pytest.mark.parametrize("name", sorted(CASES))
+def test_wide_partitions(name):
+    # 2^18 partitions: the sizes come from the level-1 output and level 2
+    # scatters into 1024 sub-buckets, the path of inputs of billions of k-mers
+    case = CASES[name]
+    check_build(gold_fa(name), case["k"], case["c"], wide=True)
+
+
+def test_wide_partitions_small_k_and_spill():
+    fa = swapamd.generate_params(30000, 90, 20.0, 0.01, 109)
+    check_build(fa, 9, 2, wide=True)
+    check_build(gold_fa("uniform_k21_c2"), 21, 2, wide=True, table_spill=True)
 
 
 def poly_a_mix(seed):

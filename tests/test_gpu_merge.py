@@ -35,11 +35,11 @@ def strip_mult(text):
     return sorted(out)
 
 
-def check_merge(fa, k, c, hash_order=False, table_spill=False):
-    print("merge case k=%d c=%d bytes=%d hash_order=%s table_spill=%s" % (k, c, len(fa), hash_order, table_spill),
-          flush=True)
+def check_merge(fa, k, c, hash_order=False, table_spill=False, wide=False):
+    print("merge case k=%d c=%d bytes=%d hash_order=%s table_spill=%s wide=%s" % (k, c, len(fa), hash_order,
+                                                                                table_spill, wide), flush=True)
     o = Oracle(fa, k, c)
-    a = swapamd.Assembler(k, c, hash_order=hash_order, table_spill=table_spill)
+    a = swapamd.Assembler(k, c, hash_order=hash_order, table_spill=table_spill, wide_partitions=wide)
     a.build(fa)
     a.merge()
     cg = a.file_bytes(swapamd.SWAP_FILE_CONTIG_GRAPH)
@@ -84,6 +84,18 @@ def test_device_memory_table(name):
     # partition: the survivor index builds those regions with device atomics
     case = CASES[name]
     check_merge(gold_fa(name), case["k"], case["c"], table_spill=True)
+
+
+@pytest.mark.parametrize("name", ["repeats_k25_c3", "uniform_k31_c3", "dirty_k21_c2"])
+def test_wide_partitions(name):
+    # 2^18 count partitions: the survivor index has as many regions
+    case = CASES[name]
+    check_merge(gold_fa(name), case["k"], case["c"], wide=True)
+
+
+def test_wide_partitions_hash_order():
+    case = CASES["uniform_k21_c2"]
+    check_merge(gold_fa("uniform_k21_c2"), case["k"], case["c"], hash_order=True, wide=True)
 
 
 def test_partition_past_16bit_counters():
