@@ -38,8 +38,9 @@ PHASES = {0: "record_scan", 1: "pack_2bit", 2: "partition_hist", 3: "partition_l
 
 
 # phase index -> device kernel of that phase whose hardware byte counters are kept
+# (the count of the bacterial workload takes one partition per table fill)
 # in profiles/traffic.json (written from rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE)
-PHASE_KERNEL = {4: "swapk::k_count_lds<%s>", 3: "swapk::k_part1<%s>", 7: "swapk::k_part2<%s>",
+PHASE_KERNEL = {4: "swapk::k_count_lds<%s, false>", 3: "swapk::k_part1<%s>", 7: "swapk::k_part2<%s>",
                 2: "swapk::k_part_hist", 1: "swapk::k_pack"}
 
 

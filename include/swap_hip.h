@@ -65,6 +65,9 @@ extern "C" {
                                     on-chip table use; same results, slower */
 #define SWAP_FLAG_WIDE_PARTITIONS 8u /* testing aid: at least 2^18 count partitions, the
                                         path inputs of billions of k-mers take; same results */
+#define SWAP_FLAG_SINGLE_FILL 16u /* testing aid: one count partition per fill of the on-chip
+                                     table, as on large inputs, also where small inputs'
+                                     partitions would share fills; same results */
 
 typedef struct swap_cfg {
     int k;          /* odd k-mer length, 9..31 (the reference CLI accepts 19..31) */

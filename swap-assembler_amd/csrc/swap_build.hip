@@ -614,6 +614,7 @@ struct CountArgs {
     uint32_t *overflow_list;
     int spill;                     // send every partition to the device-memory table
     uint32_t *dfo;                 // first-occurrence index of every distinct key (reference order), or null
+    int gbits;                     // 2^gbits consecutive partitions share one fill of the LDS table
 };
 
 __device__ __forceinline__ uint32_t slot_hash(uint64_t key, uint32_t mask) {
@@ -685,11 +686,24 @@ __device__ __forceinline__ uint32_t home_bucket(uint64_t key) {
     return (uint32_t)((key * 0x9E3779B97F4A7C15ull) >> 40) & (kBuckets - 1);
 }
 
-// Finds or inserts key starting at bucket b0 (fresh reads, one bucket at a
-// time); returns its slot, or ~0u when the table is full.
-__device__ uint32_t bucket_probe(uint64_t *tkey, uint64_t key, uint32_t b0) {
-    for (uint32_t n = 0; n < kBuckets; n++) {
-        const uint32_t s0 = ((b0 + n) & (kBuckets - 1)) * kBucketSlots;
+// When 2^g partitions share the table, partition q of them owns sub-table q:
+// the buckets whose slots belong to threads [q, q + 1) * kCountThreads / 2^g
+// in every round of the emit (bucket b holds slots 4b..4b+3, owned by threads
+// 4b mod kCountThreads...), so each partition's output is a run of whole
+// waves.  Bucket of local index l (0 <= l < kBuckets / 2^g) of sub-table q:
+constexpr int kRoundBucketBits = 8;  // log2(kCountThreads / kBucketSlots)
+__device__ __forceinline__ uint32_t sub_bucket(uint32_t l, uint32_t q, int g) {
+    const int rb = kRoundBucketBits - g;
+    return ((l >> rb) << kRoundBucketBits) | (q << rb) | (l & ((1u << rb) - 1));
+}
+
+// Finds or inserts key starting at local bucket l0 of sub-table q (fresh
+// reads, one bucket at a time); returns its slot, or ~0u when the sub-table
+// is full.
+__device__ uint32_t bucket_probe(uint64_t *tkey, uint64_t key, uint32_t l0, uint32_t q, int g) {
+    const uint32_t nsub = kBuckets >> g;
+    for (uint32_t n = 0; n < nsub; n++) {
+        const uint32_t s0 = sub_bucket((l0 + n) & (nsub - 1), q, g) * kBucketSlots;
         const ulonglong2 v01 = *reinterpret_cast<const ulonglong2 *>(&tkey[s0]);
         const ulonglong2 v23 = *reinterpret_cast<const ulonglong2 *>(&tkey[s0 + 2]);
         const uint64_t v[4] = {v01.x, v01.y, v23.x, v23.y};
@@ -712,9 +726,10 @@ __device__ uint32_t bucket_probe(uint64_t *tkey, uint64_t key, uint32_t b0) {
 // to another key.  Returns false when the table is full.
 template <bool WITH_OCC>
 __device__ __forceinline__ bool table_add16(uint64_t *tkey, uint32_t *tc, uint32_t *tfo, uint64_t rec,
-                                            uint32_t occ) {
+                                            uint32_t occ, uint32_t q, int g) {
     const uint64_t key = rec >> 8;
-    const uint32_t hb = home_bucket(key), s0 = hb * kBucketSlots;
+    const uint32_t l0 = home_bucket(key) & ((kBuckets >> g) - 1);
+    const uint32_t s0 = sub_bucket(l0, q, g) * kBucketSlots;
     const ulonglong2 v01 = *reinterpret_cast<const ulonglong2 *>(&tkey[s0]);
     const ulonglong2 v23 = *reinterpret_cast<const ulonglong2 *>(&tkey[s0 + 2]);
     const uint64_t v[4] = {v01.x, v01.y, v23.x, v23.y};
@@ -736,7 +751,7 @@ __device__ __forceinline__ bool table_add16(uint64_t *tkey, uint32_t *tc, uint32
                                         (unsigned long long)key);
         if (prev == kEmpty || prev == key) slot = s0 + cq;
     }
-    if (slot == ~0u) slot = bucket_probe(tkey, key, hb);
+    if (slot == ~0u) slot = bucket_probe(tkey, key, l0, q, g);
     if (slot == ~0u) return false;
     if (WITH_OCC) atomicMin(&tfo[slot], occ);
     add_edges16(tc, slot, (uint32_t)(rec & 0xFF));
@@ -847,24 +862,33 @@ __device__ void table_emit(const uint64_t *tkey, CNT cnt_of, const uint32_t *tfo
 // Partitions hold N / 2^15 records on average (about 4900 for a 50x bacterial
 // sample: under 5 per thread); the rest load inside the insert loop.
 constexpr int kPre = 6;
-// Partitions per workgroup whose bounds are staged in LDS (grid >= nparts / this).
-constexpr uint32_t kMaxPartsPerGroup = 512;
+// Table fills per workgroup whose bounds are staged in LDS (grid >= fills / this).
+constexpr uint32_t kMaxFills = 512;
+// Partitions that may share one table fill (2^kMaxFillBits), and the records a
+// shared fill aims at: under half the slots of a sub-table even when every
+// record is a distinct key.
+constexpr int kMaxFillBits = 3;
+constexpr uint64_t kFillRecs = 2048;
 // Records per partition the partition count aims at (distinct keys are a
 // fraction of them; the LDS table holds 4096).
 constexpr uint64_t kTargetPartRecs = 8192;
 
-// Persistent and software-pipelined: one workgroup per CU walks the partitions
-// blockIdx.x, blockIdx.x + gridDim.x, ...  Iteration i inserts partition i's
-// records (loaded during iteration i-1) into the LDS table; then each thread
-// stores its share of partition i-1's output (the output range reserved by one
-// device atomic per counter during iteration i-1), takes its 4 slots of
-// partition i into registers, clears them, and the workgroup's (survivors,
-// distinct) totals are scanned; thread 0 then reserves partition i's output
-// range while partition i+1's records load.  The global-memory latencies of a
-// partition (its record loads, the reservation, the output stores) thus overlap
-// the LDS work of its neighbours.  Slot j*kCountThreads + t belongs to thread t,
-// so the emit and the clear touch consecutive LDS words across a wave.
-template <bool WITH_OCC>
+// Persistent and software-pipelined: one workgroup per CU walks the table
+// fills blockIdx.x, blockIdx.x + gridDim.x, ...; fill f counts partitions
+// f * 2^gbits ... (f + 1) * 2^gbits - 1 (consecutive records), each in its own
+// sub-table.  Iteration i inserts fill i's records (loaded during iteration
+// i-1) into the LDS table; then each thread stores its share of fill i-1's
+// output (the output range reserved by one device atomic during iteration
+// i-1), takes its 4 slots of fill i into registers, clears them, and the
+// workgroup's (survivors, distinct) totals are scanned; thread 0 then reserves
+// fill i's output range while fill i+1's records load.  The global-memory
+// latencies of a fill (its record loads, the reservation, the output stores)
+// thus overlap the LDS work of its neighbours.  Slot j*kCountThreads + t
+// belongs to thread t, so the emit and the clear touch consecutive LDS words
+// across a wave.  Sharing a fill among the small partitions of a small input
+// divides the per-fill costs (barriers, the reservation on one device-wide
+// counter, which every workgroup's fills serialise on) by 2^gbits.
+template <bool WITH_OCC, bool SHARED>
 __global__ void __launch_bounds__(kCountThreads) k_count_lds(CountArgs a) {
     static_assert(kTab == 4 * kCountThreads, "the emit takes 4 slots per thread");
     __shared__ __align__(16) uint64_t tkey[kTab];
@@ -873,11 +897,17 @@ __global__ void __launch_bounds__(kCountThreads) k_count_lds(CountArgs a) {
     __shared__ uint64_t wsum[kCountThreads / 64];
     __shared__ unsigned long long base_s, base_d;
     __shared__ int overflow[2];  // set during the insert of iterations of each parity
-    __shared__ uint32_t bnd[kMaxPartsPerGroup + 1][2];  // [i] = bounds of this group's i-th partition
+    __shared__ uint32_t bnd[kMaxFills + 1][2];  // [i] = record bounds of this workgroup's i-th fill
     const uint32_t t = threadIdx.x;
     const int lane = t & 63, w = t >> 6;
     const int hb = 2 * a.k - kL1Bits;
     const uint32_t nparts = a.nparts;
+    const int g = SHARED ? a.gbits : 0;  // single fills: the sub-table arithmetic folds away
+    const uint32_t nfills = nparts >> g, qmask = (1u << g) - 1;
+    // record bits that select the level-2 sub-bucket, i.e. the partition
+    const int qsh = 8 + 2 * a.k - (kL1Bits + a.l2bits);
+    constexpr int kWaves = kCountThreads / 64;
+    const int wpp = kWaves >> g;  // waves per partition of a fill
     const uint32_t total = a.bucket[nparts];  // >= 1: the host skips empty builds
     const uint32_t bias = bias16(a.c);
     uint64_t *tcnt = reinterpret_cast<uint64_t *>(tc);
@@ -887,16 +917,16 @@ __global__ void __launch_bounds__(kCountThreads) k_count_lds(CountArgs a) {
     }
     for (uint32_t i = t; i < 4 * kTab; i += kCountThreads) tc[i] = 0;
     if (t < 2) overflow[t] = 0;
-    for (uint32_t i = t; i <= kMaxPartsPerGroup; i += kCountThreads) {
-        const uint64_t p = blockIdx.x + (uint64_t)i * gridDim.x;
-        bnd[i][0] = p < nparts ? a.bucket[p] : 0;
-        bnd[i][1] = p < nparts ? a.bucket[p + 1] : 0;
+    for (uint32_t i = t; i <= kMaxFills; i += kCountThreads) {
+        const uint64_t f = blockIdx.x + (uint64_t)i * gridDim.x;
+        bnd[i][0] = f < nfills ? a.bucket[f << g] : 0;
+        bnd[i][1] = f < nfills ? a.bucket[(f + 1) << g] : 0;
     }
     __syncthreads();
 
-    // next partition's records: loaded from indices clamped to the partition's
-    // last record, so that every load is issued and none reaches into the next
-    // partition (entries past the partition's end are never used)
+    // next fill's records: loaded from indices clamped to the fill's last
+    // record, so that every load is issued and none reaches into the next fill
+    // (entries past the fill's end are never used)
     uint64_t pr[kPre];
     uint32_t po[kPre];
     uint32_t b = 0, e = 0;
@@ -910,19 +940,21 @@ __global__ void __launch_bounds__(kCountThreads) k_count_lds(CountArgs a) {
             pr[j] = rec_load<WITH_OCC>(a.rec, i, po[j]);
         }
     };
-    // previous partition, held until its output range is known
+    // previous fill, held until its output range is known
     uint64_t keys[4], mults[4];
     uint32_t fos[4];
     uint64_t pv = 0, pinc = 0;
-    uint32_t prev_part = 0;
+    uint32_t prev_p0 = 0;   // the previous fill's first partition
+    uint32_t prev_off = 0;  // wave 0, lane q: survivors of the previous fill before its partition q
     bool have_prev = false, prev_ovf = false;
-    unsigned long long pend_s = 0, pend_d = 0;
+    unsigned long long pend = 0;  // thread 0: the output base reserved for the previous fill
 #pragma unroll
     for (int j = 0; j < 4; j++) keys[j] = kEmpty;
 
     prefetch(0);
-    for (uint32_t part = blockIdx.x, it = 0;; part += gridDim.x, it++) {
-        const bool have = part < nparts;  // uniform over the workgroup
+    for (uint32_t fill = blockIdx.x, it = 0;; fill += gridDim.x, it++) {
+        const bool have = fill < nfills;  // uniform over the workgroup
+        const uint32_t p0 = fill << g;
         const bool small = (e - b) < 0xFFFFu;
         bool ok = true;
         if (have) {
@@ -931,13 +963,16 @@ __global__ void __launch_bounds__(kCountThreads) k_count_lds(CountArgs a) {
 #pragma unroll
                 for (int j = 0; j < kPre; j++) {
                     const uint64_t i = (uint64_t)b + t + (uint64_t)j * kCountThreads;
-                    if (i < e && ok) ok = table_add16<WITH_OCC>(tkey, tc, tfo, pr[j], po[j]);
+                    if (i < e && ok)
+                        ok = table_add16<WITH_OCC>(tkey, tc, tfo, pr[j], po[j], (uint32_t)(pr[j] >> qsh) & qmask, g);
                 }
                 for (uint64_t i = (uint64_t)b + t + (uint64_t)kPre * kCountThreads; i < e && ok; i += kCountThreads) {
                     uint32_t o;
                     const uint64_t r = rec_load<WITH_OCC>(a.rec, i, o);
-                    ok = table_add16<WITH_OCC>(tkey, tc, tfo, r, o);
+                    ok = table_add16<WITH_OCC>(tkey, tc, tfo, r, o, (uint32_t)(r >> qsh) & qmask, g);
                 }
+            } else if (g > 0) {
+                ok = false;  // a shared fill of 2^16 records: its partitions go to the device-memory table
             } else {
 #pragma unroll
                 for (int j = 0; j < kPre; j++) {
@@ -952,10 +987,15 @@ __global__ void __launch_bounds__(kCountThreads) k_count_lds(CountArgs a) {
             }
             if (!ok) overflow[it & 1] = 1;
         }
-        if (t == 0 && have_prev && !prev_ovf) {
-            base_s = pend_s;
-            base_d = pend_d;
-            a.part_first[prev_part] = (uint32_t)pend_s;
+        if (w == 0 && have_prev && !prev_ovf) {
+            // the reservation returns (distinct << 32) | survivors before this fill
+            const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)pend);
+            const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(pend >> 32));
+            if (lane == 0) {
+                base_s = lo;
+                base_d = hi;
+            }
+            if (lane <= qmask) a.part_first[prev_p0 + lane] = lo + prev_off;
         }
         __syncthreads();  // table complete; previous output bases and wave offsets visible
         if (have_prev && !prev_ovf) {
@@ -966,7 +1006,7 @@ __global__ void __launch_bounds__(kCountThreads) k_count_lds(CountArgs a) {
                 if (keys[j] == kEmpty) continue;
                 if (WITH_OCC && a.dfo) a.dfo[od++] = fos[j];
                 if (mults[j]) {
-                    a.surv_key[os] = ((uint64_t)(prev_part >> a.l2bits) << hb) | keys[j];  // mixed; k_unmix inverts
+                    a.surv_key[os] = ((uint64_t)(prev_p0 >> a.l2bits) << hb) | keys[j];  // mixed; k_unmix inverts
                     a.surv_mult[os] = mults[j];
                     if (WITH_OCC) a.surv_first[os] = fos[j];
                     os++;
@@ -1007,39 +1047,58 @@ __global__ void __launch_bounds__(kCountThreads) k_count_lds(CountArgs a) {
         pv = ((uint64_t)nd << 32) | ns;
         pinc = wave_incl_sum(pv);
         if (lane == 63) wsum[w] = pinc;
-        if (part + gridDim.x < nparts) prefetch(it + 1);
+        if (fill + gridDim.x < nfills) prefetch(it + 1);
         __syncthreads();  // wave sums complete; the table is empty
         if (w == 0) {
             // wave offsets: exclusive scan of the per-wave totals by wave 0
-            constexpr int kWaves = kCountThreads / 64;
             const uint64_t x = (!ovf && lane < kWaves) ? wsum[lane] : 0;
             const uint64_t incw = wave_incl_sum(x);
             if (!ovf && lane < kWaves) wsum[lane] = incw - x;
             const uint64_t run = ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(incw >> 32), kWaves - 1) << 32) |
                                  (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)incw, kWaves - 1);
-            if (lane == 0) {
-                overflow[it & 1] = 0;
+            // survivors of the fill's partitions: partition q holds waves
+            // [q, q + 1) * wpp, lane q takes the inclusive sums before and at its end
+            const uint32_t inc_s = (uint32_t)incw;
+            const int q = lane & (kWaves - 1);
+            const uint32_t before = q ? (uint32_t)__shfl((int)inc_s, (q * wpp - 1) & (kWaves - 1)) : 0u;
+            const uint32_t at_end = (uint32_t)__shfl((int)inc_s, ((q + 1) * wpp - 1) & (kWaves - 1));
+            if (lane <= qmask) {
                 if (ovf) {
                     const unsigned long long j = atomicAdd(&a.counters[2], 1ull);
-                    a.overflow_list[j] = part;
-                    a.part_surv[part] = 0;  // counted by the global-memory pass
-                    a.part_first[part] = ~0u;
+                    a.overflow_list[j] = p0 + lane;
+                    a.part_surv[p0 + lane] = 0;  // counted by the global-memory pass
+                    a.part_first[p0 + lane] = ~0u;
                 } else {
-                    const uint32_t ts = (uint32_t)run, td = (uint32_t)(run >> 32);
-                    // a per-lane (zero) index keeps the compiler from turning these
-                    // into wave-aggregated atomics, whose result it would wait for
-                    // here; the results are first used after the next insert
+                    a.part_surv[p0 + lane] = at_end - before;
+                    prev_off = before;  // part_first follows with the fill's output base
+                }
+            }
+            if (lane == 0) {
+                overflow[it & 1] = 0;
+                if (!ovf) {
+                    // one reservation of (distinct << 32) | survivors; a per-lane
+                    // (zero) index keeps the compiler from turning it into a
+                    // wave-aggregated atomic, whose result it would wait for here;
+                    // the result is first used after the next insert
                     uint32_t zero;
                     asm volatile("v_mov_b32 %0, 0" : "=v"(zero));
-                    pend_s = ts ? atomicAdd(&a.counters[zero], (unsigned long long)ts) : 0;
-                    pend_d = td ? atomicAdd(&a.counters[1 + zero], (unsigned long long)td) : 0;
-                    a.part_surv[part] = ts;
+                    pend = run ? atomicAdd(&a.counters[3 + zero], (unsigned long long)run) : 0;
                 }
             }
         }
         have_prev = true;
         prev_ovf = ovf;
-        prev_part = part;
+        prev_p0 = p0;
+    }
+    // the last workgroup to finish publishes the totals of the reservations as
+    // the survivor and distinct counters (the device-memory pass appends to them)
+    if (t == 0) {
+        __threadfence();
+        if (atomicAdd(&a.counters[5], 1ull) == gridDim.x - 1) {
+            const unsigned long long tot = atomicAdd(&a.counters[3], 0ull);
+            a.counters[0] = (uint32_t)tot;
+            a.counters[1] = tot >> 32;
+        }
     }
 }
 
@@ -1542,15 +1601,26 @@ int Pipeline::build(const uint8_t *d_in, uint64_t n) {
     ca.part_first = part_first_.as<uint32_t>();
     ca.spill = table_spill_ ? 1 : 0;
     ca.dfo = ref_order_ ? dfo_.as<uint32_t>() : nullptr;
+    // small partitions share table fills: 2^gbits of them per fill, while a
+    // fill of average partitions stays within kFillRecs records
+    int gbits = 0;
+    while (!single_fill_ && gbits < kMaxFillBits && gbits < l2bits && ((N >> pbits) << (gbits + 1)) <= kFillRecs)
+        gbits++;
+    ca.gbits = gbits;
+    const uint64_t nfills = nparts >> gbits;
     phase_begin(4);
     // one resident workgroup per CU (the LDS table takes most of a CU's LDS),
-    // and at least the number whose partition bounds fit the kernel's LDS list
+    // and at least the number whose fill bounds fit the kernel's LDS list
     const unsigned count_grid = (unsigned)std::min<int>(
-        (int)nparts, std::max(ncu, (int)((nparts + kMaxPartsPerGroup - 1) / kMaxPartsPerGroup)));
-    if (ref_order_)
-        hipLaunchKernelGGL(k_count_lds<true>, dim3(count_grid), dim3(kCountThreads), 0, st_, ca);
+        (int)nfills, std::max(ncu, (int)((nfills + kMaxFills - 1) / kMaxFills)));
+    if (ref_order_ && gbits)
+        hipLaunchKernelGGL((k_count_lds<true, true>), dim3(count_grid), dim3(kCountThreads), 0, st_, ca);
+    else if (ref_order_)
+        hipLaunchKernelGGL((k_count_lds<true, false>), dim3(count_grid), dim3(kCountThreads), 0, st_, ca);
+    else if (gbits)
+        hipLaunchKernelGGL((k_count_lds<false, true>), dim3(count_grid), dim3(kCountThreads), 0, st_, ca);
     else
-        hipLaunchKernelGGL(k_count_lds<false>, dim3(count_grid), dim3(kCountThreads), 0, st_, ca);
+        hipLaunchKernelGGL((k_count_lds<false, false>), dim3(count_grid), dim3(kCountThreads), 0, st_, ca);
     phase_end(4);
     SWAP_CHECK(hipGetLastError());
     SWAP_CHECK(hipMemcpyAsync(hs + 8, d_small + 8, 3 * 8, hipMemcpyDeviceToHost, st_));

@@ -208,6 +208,7 @@ int swap_create(const swap_cfg *cfg, swap_ctx **out) {
     }
     c->pipe->set_table_spill((cfg->flags & SWAP_FLAG_TABLE_SPILL) != 0);
     c->pipe->set_wide_parts((cfg->flags & SWAP_FLAG_WIDE_PARTITIONS) != 0);
+    c->pipe->set_single_fill((cfg->flags & SWAP_FLAG_SINGLE_FILL) != 0);
     *out = c;
     return SWAP_OK;
 }
@@ -312,6 +313,7 @@ int swap_reconfigure(swap_ctx *c, const swap_cfg *cfg) {
     c->pipe->reconfigure(cfg->k, cfg->c, !(cfg->flags & SWAP_FLAG_HASH_ORDER));
     c->pipe->set_table_spill((cfg->flags & SWAP_FLAG_TABLE_SPILL) != 0);
     c->pipe->set_wide_parts((cfg->flags & SWAP_FLAG_WIDE_PARTITIONS) != 0);
+    c->pipe->set_single_fill((cfg->flags & SWAP_FLAG_SINGLE_FILL) != 0);
     c->built = c->merged = c->have_graph = c->have_compacted = c->have_ce = false;
     return SWAP_OK;
 }

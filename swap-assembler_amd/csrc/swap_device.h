@@ -94,6 +94,8 @@ public:
     void set_table_spill(bool on) { table_spill_ = on; }
     // at least 2^18 partitions on any input (a testing aid for large inputs)
     void set_wide_parts(bool on) { wide_parts_ = on; }
+    // one partition per on-chip table fill on any input (a testing aid)
+    void set_single_fill(bool on) { single_fill_ = on; }
     int device() const { return dev_; }
     hipStream_t stream() const { return st_; }
 
@@ -144,6 +146,7 @@ private:
     bool ref_order_;
     bool table_spill_ = false;
     bool wide_parts_ = false;
+    bool single_fill_ = false;
     int pbits_ = kPartBitsMin;  // partition bits of the last build
     hipStream_t st_ = nullptr;
     hipError_t init_err_ = hipSuccess;

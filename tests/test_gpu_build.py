@@ -22,12 +22,12 @@ def gold_fa(name):
         return f.read()
 
 
-def check_build(fa, k, c, hash_order=False, check_file=True, table_spill=False, wide=False):
-    print("case k=%d c=%d bytes=%d hash_order=%s table_spill=%s wide=%s" % (k, c, len(fa), hash_order, table_spill,
-                                                                          wide), flush=True)
+def check_build(fa, k, c, hash_order=False, check_file=True, table_spill=False, wide=False, single_fill=False):
+    print("case k=%d c=%d bytes=%d hash_order=%s table_spill=%s wide=%s single_fill=%s" % (
+        k, c, len(fa), hash_order, table_spill, wide, single_fill), flush=True)
     o = Oracle(fa, k, c, kmer_graph_only=True)
     a = swapamd.Assembler(k, c, kmer_graph_only=True, hash_order=hash_order, table_spill=table_spill,
-                          wide_partitions=wide)
+                          wide_partitions=wide, single_fill=single_fill)
     a.build(fa)
     st = a.stats()
     ost = o.stats()
@@ -76,6 +76,23 @@ def test_wide_partitions(name):
     # scatters into 1024 sub-buckets, the path of inputs of billions of k-mers
     case = CASES[name]
     check_build(gold_fa(name), case["k"], case["c"], wide=True)
+
+
+@pytest.mark.parametrize("name", sorted(CASES))
+def test_single_fill(name):
+    # one partition per on-chip table fill, the path of large inputs (small
+    # inputs share fills among 2^3 partitions)
+    case = CASES[name]
+    check_build(gold_fa(name), case["k"], case["c"], single_fill=True)
+    check_build(gold_fa(name), case["k"], case["c"], hash_order=True, single_fill=True)
+
+
+def test_shared_fills_mid_size():
+    # partitions of about 400 and 800 records (13M and 26M k-mers over 2^15
+    # partitions): fills shared by 2^2 and 2^1 of them
+    for genome, seed in ((300000, 21), (600000, 22)):
+        fa = swapamd.generate_params(genome, 90, 60.0, 0.01, seed)
+        check_build(fa, 25, 3, check_file=False)
 
 
 def test_wide_partitions_small_k_and_spill():

@@ -35,11 +35,12 @@ def strip_mult(text):
     return sorted(out)
 
 
-def check_merge(fa, k, c, hash_order=False, table_spill=False, wide=False):
-    print("merge case k=%d c=%d bytes=%d hash_order=%s table_spill=%s wide=%s" % (k, c, len(fa), hash_order,
-                                                                                table_spill, wide), flush=True)
+def check_merge(fa, k, c, hash_order=False, table_spill=False, wide=False, single_fill=False):
+    print("merge case k=%d c=%d bytes=%d hash_order=%s table_spill=%s wide=%s single_fill=%s" % (
+        k, c, len(fa), hash_order, table_spill, wide, single_fill), flush=True)
     o = Oracle(fa, k, c)
-    a = swapamd.Assembler(k, c, hash_order=hash_order, table_spill=table_spill, wide_partitions=wide)
+    a = swapamd.Assembler(k, c, hash_order=hash_order, table_spill=table_spill, wide_partitions=wide,
+                          single_fill=single_fill)
     a.build(fa)
     a.merge()
     cg = a.file_bytes(swapamd.SWAP_FILE_CONTIG_GRAPH)
@@ -91,6 +92,11 @@ def test_wide_partitions(name):
     # 2^18 count partitions: the survivor index has as many regions
     case = CASES[name]
     check_merge(gold_fa(name), case["k"], case["c"], wide=True)
+
+
+def test_single_fill():
+    case = CASES["uniform_k21_c2"]
+    check_merge(gold_fa("uniform_k21_c2"), case["k"], case["c"], single_fill=True)
 
 
 def test_wide_partitions_hash_order():
