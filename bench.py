@@ -38,9 +38,9 @@ PHASES = {0: "record_scan", 1: "pack_2bit", 2: "partition_hist", 3: "partition_l
 
 
 # phase index -> device kernel of that phase whose hardware byte counters are kept
-# (the count of the bacterial workload takes one partition per table fill)
+# (the bacterial workload: one partition per count table fill, 2^7 level-2 sub-buckets)
 # in profiles/traffic.json (written from rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE)
-PHASE_KERNEL = {4: "swapk::k_count_lds<%s, false>", 3: "swapk::k_part1<%s>", 7: "swapk::k_part2<%s>",
+PHASE_KERNEL = {4: "swapk::k_count_lds<%s, false>", 3: "swapk::k_part1<%s>", 7: "swapk::k_part2<%s, 128>",
                 2: "swapk::k_part_hist", 1: "swapk::k_pack"}
 
 
@@ -317,7 +317,9 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--batch-samples", type=int, default=256,
                     help="samples of the many-sample workload for the batched-vs-per-sample figure (0 skips it)")
-    ap.add_argument("--batch-slots", type=int, default=4)
+    ap.add_argument("--batch-slots", type=int, default=12,
+                    help="concurrent contexts of the batched and serving figures (12: the best of 4/8/12/16 "
+                         "on the batch workload)")
     ap.add_argument("--serve-requests", type=int, default=60,
                     help="requests of the serving figure (Poisson arrivals at --serve-rate; 0 skips it)")
     ap.add_argument("--serve-rate", type=float, default=20.0)
