@@ -67,3 +67,46 @@ def graph(fasta: bytes, k: int, c: int):
         if any(m):
             out[enc(X)] = m
     return out
+
+
+def rc_codes(x, k):
+    """Reverse complements of 2-bit MSB-first k-mer codes (numpy uint64 array)."""
+    import numpy as np
+    x = ~np.asarray(x, dtype=np.uint64)  # complement: 3 - a on every 2-bit digit
+    m2, m4 = np.uint64(0x3333333333333333), np.uint64(0x0F0F0F0F0F0F0F0F)
+    x = ((x >> np.uint64(2)) & m2) | ((x & m2) << np.uint64(2))
+    x = ((x >> np.uint64(4)) & m4) | ((x & m4) << np.uint64(4))
+    return x.byteswap() >> np.uint64(64 - 2 * k)
+
+
+def reciprocal_violations(keys, mult, k, rows=None):
+    """Edges of a k-molecule graph whose reciprocal edge is missing or carries a
+    different multiplicity.  Both ends of an edge count the same (k+1)-mer
+    occurrences (the formulas above), so on a correct graph this is 0 for any
+    cutoff: edge b of X (X followed by b) is edge 4 + comp(X[0]) of
+    Y = X[1:] b when Y is canonical, else edge comp(X[0]) of Y = rc(X[1:] b);
+    edges 4..7 are the same with rc(X) in place of X.  keys: sorted unique
+    uint64 codes; mult: uint8 [n, 8]; rows: the records whose edges are checked
+    (default all)."""
+    import numpy as np
+    keys = np.asarray(keys, dtype=np.uint64)
+    mult = np.asarray(mult, dtype=np.uint8)
+    mask = np.uint64((1 << (2 * k)) - 1)
+    bad = 0
+    rows = np.arange(len(keys)) if rows is None else np.asarray(rows)
+    for t in range(8):
+        sel = rows[np.nonzero(mult[rows, t])[0]]
+        if not len(sel):
+            continue
+        src = keys[sel] if t < 4 else rc_codes(keys[sel], k)
+        first = src >> np.uint64(2 * k - 2)
+        nxt = ((src << np.uint64(2)) | np.uint64(t & 3)) & mask
+        rn = rc_codes(nxt, k)
+        y = np.maximum(nxt, rn)
+        slot = np.where(y == nxt, 4 + (3 - first), 3 - first).astype(np.int64)
+        idx = np.searchsorted(keys, y)
+        idx_c = np.minimum(idx, len(keys) - 1)
+        found = keys[idx_c] == y
+        ok = found & (mult[idx_c, slot] == mult[sel, t])
+        bad += int(len(sel) - np.count_nonzero(ok))
+    return bad
