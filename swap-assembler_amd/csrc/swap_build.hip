@@ -875,10 +875,13 @@ constexpr int kPre = 6;
 // Table fills per workgroup whose bounds are staged in LDS (grid >= fills / this).
 constexpr uint32_t kMaxFills = 512;
 // Partitions that may share one table fill (2^kMaxFillBits), and the records a
-// shared fill aims at: under half the slots of a sub-table even when every
-// record is a distinct key.
+// shared fill aims at: at most one record per sub-table slot on average.
+// Reads at 20x coverage or more carry one distinct key per 3-5 records, so
+// sub-tables run under a third full (2048, half a slot per record, measured
+// 4% fewer samples/s on the many-sample workload); an input of nearly all
+// distinct keys overflows shared fills to the device-memory table instead.
 constexpr int kMaxFillBits = 3;
-constexpr uint64_t kFillRecs = 2048;
+constexpr uint64_t kFillRecs = 4096;
 // Records per partition the partition count aims at (distinct keys are a
 // fraction of them; the LDS table holds 4096).
 constexpr uint64_t kTargetPartRecs = 8192;

@@ -88,9 +88,10 @@ def test_single_fill(name):
 
 
 def test_shared_fills_mid_size():
-    # partitions of about 400 and 800 records (13M and 26M k-mers over 2^15
-    # partitions): fills shared by 2^2 and 2^1 of them
-    for genome, seed in ((300000, 21), (600000, 22)):
+    # partitions of about 800 and 1600 records (26M and 52M k-mers over 2^15
+    # partitions): fills shared by 2^2 and 2^1 of them (the golden inputs'
+    # small partitions share fills 2^3 at a time)
+    for genome, seed in ((600000, 22), (1200000, 23)):
         fa = swapamd.generate_params(genome, 90, 60.0, 0.01, seed)
         check_build(fa, 25, 3, check_file=False)
 
