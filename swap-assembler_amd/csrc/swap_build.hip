@@ -543,15 +543,25 @@ __global__ void __launch_bounds__(kL2Threads) k_part2(const uint32_t *__restrict
 
 // Level-1 bucket starts l1_start[b] = pstart[b << l2bits] (pstart: partition
 // starts at l2bits sub-buckets per bucket), and the chunk table of chunks of
-// `recs` records.
-__global__ void k_l1_chunks(const uint32_t *__restrict__ pstart, int l2bits, uint32_t recs,
-                            uint32_t *__restrict__ l1_start, uint32_t *__restrict__ chunk_start) {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    uint32_t ch = 0;
-    for (uint32_t b = 0; b <= kL1; b++) {
-        l1_start[b] = pstart[(uint64_t)b << l2bits];
-        chunk_start[b] = ch;
-        if (b < kL1) ch += (pstart[(uint64_t)(b + 1) << l2bits] - pstart[(uint64_t)b << l2bits] + recs - 1) / recs;
+// `recs` records: one thread per bucket (blockDim.x == kL1), chunk counts
+// scanned across the block.
+__global__ void __launch_bounds__(kL1) k_l1_chunks(const uint32_t *__restrict__ pstart, int l2bits, uint32_t recs,
+                                                   uint32_t *__restrict__ l1_start,
+                                                   uint32_t *__restrict__ chunk_start) {
+    __shared__ uint32_t wtot[kL1 / 64];
+    const uint32_t b = threadIdx.x;
+    const uint32_t s0 = pstart[(uint64_t)b << l2bits], s1 = pstart[(uint64_t)(b + 1) << l2bits];
+    const uint32_t n = (s1 - s0 + recs - 1) / recs;
+    const uint32_t inc = wave_incl_sum(n);
+    if ((b & 63) == 63) wtot[b >> 6] = inc;
+    __syncthreads();
+    uint32_t off = inc - n;
+    for (uint32_t w = 0; w < (b >> 6); w++) off += wtot[w];
+    l1_start[b] = s0;
+    chunk_start[b] = off;
+    if (b == kL1 - 1) {
+        l1_start[kL1] = s1;
+        chunk_start[kL1] = off + n;
     }
 }
 
@@ -1534,10 +1544,10 @@ int Pipeline::build(const uint8_t *d_in, uint64_t n) {
     uint32_t *l1_start = l1info_.as<uint32_t>();
     uint32_t *chunk_start = l1_start + kL1 + 2;
     uint32_t *fchunk_start = chunk_start + kL1 + 2;
-    hipLaunchKernelGGL(k_l1_chunks, dim3(1), dim3(64), 0, st_, hist_.as<uint32_t>(), kHistL2Bits, (uint32_t)kChunkRecs,
+    hipLaunchKernelGGL(k_l1_chunks, dim3(1), dim3(kL1), 0, st_, hist_.as<uint32_t>(), kHistL2Bits, (uint32_t)kChunkRecs,
                        l1_start, chunk_start);
     if (pbits > kPartBitsMin)
-        hipLaunchKernelGGL(k_l1_chunks, dim3(1), dim3(64), 0, st_, hist_.as<uint32_t>(), kHistL2Bits,
+        hipLaunchKernelGGL(k_l1_chunks, dim3(1), dim3(kL1), 0, st_, hist_.as<uint32_t>(), kHistL2Bits,
                            kFineChunkRecs, l1_start, fchunk_start);
     const uint64_t rec_bytes = N * 4 * (uint64_t)rec_words(ref_order_);
     SWAP_CHECK(rec1_.reserve(rec_bytes));

@@ -158,7 +158,7 @@ private:
         surv_first_, dfo_, sel_, sel_target_, sel_out_, overflow_, part_surv_, part_first_, gtab_key_, gtab_cnt_, gtab_fo_;
     // merge buffers
     enum {
-        M_IENT, M_IBASE, M_PRIO, M_IS11, M_NXT, M_EX, M_SELF, M_WALKA, M_WALKB, M_SPA, M_SPB, M_FLAG, M_POS,
+        M_IENT, M_IBASE, M_PRIO, M_IS11, M_NXT, M_EX, M_SELF, M_WALKA, M_WALKB, M_SPA, M_SPB, M_BCNT, M_BOFF,
         M_KEPTOF, M_KEPTSRC, M_ELEN, M_EMULT, M_EOTHER, M_CSIZE, M_ESPEC, M_EOFF, M_COFF, M_POOL, M_CPRI,
         M_CMUL, M_CEDGE, M_LEV, M_LEVOFF, M_PAR, M_NEED, M_NL, M_NR, M_INL, M_INR, M_CNT, M_SPEC, M_PROPS, M_COUNT
     };

@@ -230,12 +230,12 @@ struct Walk {
 // (ruler, offset); pointer jumping then runs over the rulers only.
 constexpr uint32_t kRuleStride = 32;
 
-// rflag[v] (ruler or not, for the ruler numbering scan) and props[v]: bit 0
-// (1,1) node, bit 1 ruler, bit 2 self-adjacent, so that k_rs_prep reads one
-// byte of each successor.
+// props[v]: bit 0 (1,1) node, bit 1 ruler, bit 2 self-adjacent, so that
+// k_rs_prep reads one byte of each successor (and the ruler numbering one byte
+// per oriented node).
 __global__ void k_rs_mark(const uint8_t *__restrict__ is11, const uint32_t *__restrict__ nxt,
-                          const uint8_t *__restrict__ selfadj, uint64_t S, uint32_t *__restrict__ rflag,
-                          uint8_t *__restrict__ props, uint32_t stride) {
+                          const uint8_t *__restrict__ selfadj, uint64_t S, uint8_t *__restrict__ props,
+                          uint32_t stride) {
     const uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= 2 * S) return;
     uint32_t f = 0;
@@ -244,21 +244,19 @@ __global__ void k_rs_mark(const uint8_t *__restrict__ is11, const uint32_t *__re
         const uint32_t back = nxt[v ^ 1];
         f = (back == kNil || !is11[back >> 1]) ? 1u : ((uint32_t)((v * 0x9E3779B97F4A7C15ull) >> 32) % stride == 0);
     }
-    rflag[v] = f;
     props[v] = (uint8_t)((one ? 1u : 0u) | (f << 1) | (one && selfadj[v >> 1] ? 4u : 0u));
 }
 
-__global__ void k_rs_list(const uint32_t *__restrict__ rflag, const uint64_t *__restrict__ rpos, uint64_t n2,
-                          uint32_t *__restrict__ rid, uint32_t *__restrict__ rlist) {
-    const uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (v >= n2) return;
-    if (rflag[v]) {
-        rid[v] = (uint32_t)rpos[v];
-        rlist[rpos[v]] = (uint32_t)v;
-    } else {
-        rid[v] = kNil;
-    }
-}
+// Rulers (numbered in oriented-node order) and kept nodes (in survivor order):
+// the predicates of the two compactions.
+struct RulerPred {
+    const uint8_t *props;
+    __device__ bool operator()(uint64_t v) const { return (props[v] & 2) != 0; }
+};
+struct KeptPred {
+    const uint8_t *is11;
+    __device__ bool operator()(uint64_t i) const { return !is11[i]; }
+};
 
 // Per oriented (1,1) node: its successor with what the walk does there, so a
 // walk step is one 8-byte load:  bits 0-31 successor, bits 32-33 kind
@@ -396,21 +394,6 @@ struct EdgeArgs {
     uint8_t *especial;  // 1 if the edge enters a cycle / fold-back component
     unsigned long long *counters;  // [2] special edges
 };
-
-__global__ void k_kept_flags(const uint8_t *__restrict__ is11, uint64_t S, uint32_t *__restrict__ flag) {
-    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < S) flag[i] = is11[i] ? 0u : 1u;
-}
-
-__global__ void k_kept_list(const uint8_t *__restrict__ is11, const uint64_t *__restrict__ pos, uint64_t S,
-                            uint32_t *__restrict__ kept_of, uint32_t *__restrict__ kept_src) {
-    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= S) return;
-    if (!is11[i]) {
-        kept_of[i] = (uint32_t)pos[i];
-        kept_src[pos[i]] = (uint32_t)i;
-    }
-}
 
 __global__ void k_kept_edges(EdgeArgs e) {
     const uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -823,12 +806,13 @@ int Pipeline::merge_device(bool reference_order, MergeDev &out) {
     MCHECK(mb_[M_WALKB].reserve(2 * S * sizeof(Walk)));
     MCHECK(mb_[M_SPA].reserve(2 * S));
     MCHECK(mb_[M_SPB].reserve(2 * S));
-    MCHECK(mb_[M_FLAG].reserve(2 * S * 4));
-    MCHECK(mb_[M_POS].reserve((2 * S + 1) * 8));
+    const uint64_t nbc = compact_blocks(2 * S);
+    MCHECK(mb_[M_BCNT].reserve(nbc * 4 + 4));
+    MCHECK(mb_[M_BOFF].reserve(nbc * 8 + 8));
     MCHECK(mb_[M_NL].reserve(2 * S * 4));      // ruler id per oriented node
     MCHECK(mb_[M_NR].reserve(2 * S * 8));      // label: (ruler, offset)
     MCHECK(mb_[M_PAR].reserve(2 * S * 8));     // packed successor
-    MCHECK(scan_scratch_.reserve((scan_scratch_words(2 * S + 1) + 16) * 8));
+    MCHECK(scan_scratch_.reserve((scan_scratch_words(nbc + 1) + 16) * 8));
     Walk *wa = mb_[M_WALKA].as<Walk>(), *wb = mb_[M_WALKB].as<Walk>();
     uint8_t *spa = mb_[M_SPA].as<uint8_t>(), *spb = mb_[M_SPB].as<uint8_t>();
     uint32_t *rid = mb_[M_NL].as<uint32_t>();
@@ -837,10 +821,11 @@ int Pipeline::merge_device(bool reference_order, MergeDev &out) {
     phase_begin(9);
     MCHECK(mb_[M_PROPS].reserve(2 * S));
     hipLaunchKernelGGL(k_rs_mark, dim3(nblk(2 * S)), dim3(256), 0, st_, mb_[M_IS11].as<uint8_t>(),
-                       mb_[M_NXT].as<uint32_t>(), mb_[M_SELF].as<uint8_t>(), S, mb_[M_FLAG].as<uint32_t>(),
-                       mb_[M_PROPS].as<uint8_t>(), kRuleStride);
-    MCHECK(scan_excl<uint32_t>(mb_[M_FLAG].as<uint32_t>(), mb_[M_POS].as<uint64_t>(), 2 * S, scan_scratch_.as<uint64_t>(),
-                        d_small + 26, st_));
+                       mb_[M_NXT].as<uint32_t>(), mb_[M_SELF].as<uint8_t>(), S, mb_[M_PROPS].as<uint8_t>(),
+                       kRuleStride);
+    const RulerPred ruler{mb_[M_PROPS].as<uint8_t>()};
+    MCHECK(compact_count_scan(2 * S, ruler, mb_[M_BCNT].as<uint32_t>(), mb_[M_BOFF].as<uint64_t>(),
+                              scan_scratch_.as<uint64_t>(), d_small + 26, st_));
     MCHECK(hipMemcpyAsync(hs + 26, d_small + 26, 8, hipMemcpyDeviceToHost, st_));
     MCHECK(hipStreamSynchronize(st_));
     const uint64_t nr = hs[26];
@@ -853,8 +838,7 @@ int Pipeline::merge_device(bool reference_order, MergeDev &out) {
     MCHECK(hipMemsetAsync(label, 0xFF, 2 * S * 8, st_));
     hipLaunchKernelGGL(k_rs_prep, dim3(nblk(2 * S)), dim3(256), 0, st_, mb_[M_PROPS].as<uint8_t>(),
                        mb_[M_NXT].as<uint32_t>(), S, nx);
-    hipLaunchKernelGGL(k_rs_list, dim3(nblk(2 * S)), dim3(256), 0, st_, mb_[M_FLAG].as<uint32_t>(),
-                       mb_[M_POS].as<uint64_t>(), 2 * S, rid, rlist);
+    compact_scatter<true>(2 * S, ruler, mb_[M_BOFF].as<uint64_t>(), rid, rlist, st_);
     if (nr) {
         hipLaunchKernelGGL(k_rs_walk, dim3(nblk(nr)), dim3(256), 0, st_, rlist, nr, nx, mb_[M_SELF].as<uint8_t>(),
                            rid, S, label, rwa, rspa);
@@ -871,14 +855,10 @@ int Pipeline::merge_device(bool reference_order, MergeDev &out) {
                        rwa, rspa, S, wa, spa);
     phase_end(9);
     // --- kept nodes
-    MCHECK(mb_[M_FLAG].reserve(S * 4));
-    MCHECK(mb_[M_POS].reserve((S + 1) * 8));
-    MCHECK(scan_scratch_.reserve((scan_scratch_words(S + 1) + 16) * 8));
     phase_begin(10);
-    hipLaunchKernelGGL(k_kept_flags, dim3(nblk(S)), dim3(256), 0, st_, mb_[M_IS11].as<uint8_t>(), S,
-                       mb_[M_FLAG].as<uint32_t>());
-    MCHECK(scan_excl<uint32_t>(mb_[M_FLAG].as<uint32_t>(), mb_[M_POS].as<uint64_t>(), S, scan_scratch_.as<uint64_t>(),
-                        d_small + 20, st_));
+    const KeptPred kept{mb_[M_IS11].as<uint8_t>()};
+    MCHECK(compact_count_scan(S, kept, mb_[M_BCNT].as<uint32_t>(), mb_[M_BOFF].as<uint64_t>(),
+                              scan_scratch_.as<uint64_t>(), d_small + 20, st_));
     MCHECK(mb_[M_SPEC].reserve(S * 4));
     hipLaunchKernelGGL(k_special_flags, dim3(nblk(S)), dim3(256), 0, st_, mb_[M_IS11].as<uint8_t>(), wa, spa, S,
                        mb_[M_SPEC].as<uint32_t>(), (unsigned long long *)(d_small + 21));
@@ -890,8 +870,8 @@ int Pipeline::merge_device(bool reference_order, MergeDev &out) {
     out.n_kept = nk;
     MCHECK(mb_[M_KEPTOF].reserve(S * 4));
     MCHECK(mb_[M_KEPTSRC].reserve(nk * 4 + 4));
-    hipLaunchKernelGGL(k_kept_list, dim3(nblk(S)), dim3(256), 0, st_, mb_[M_IS11].as<uint8_t>(),
-                       mb_[M_POS].as<uint64_t>(), S, mb_[M_KEPTOF].as<uint32_t>(), mb_[M_KEPTSRC].as<uint32_t>());
+    compact_scatter<false>(S, kept, mb_[M_BOFF].as<uint64_t>(), mb_[M_KEPTOF].as<uint32_t>(),
+                           mb_[M_KEPTSRC].as<uint32_t>(), st_);
     // --- kept edges
     const uint64_t ne = 8 * nk;
     MCHECK(mb_[M_ELEN].reserve(ne * 4 + 4));

@@ -111,4 +111,85 @@ static hipError_t scan_excl(const TIn *in, uint64_t *out, uint64_t n, uint64_t *
     return hipGetLastError();
 }
 
+// ---------------------------------------------------------------------------
+// Stream compaction of a predicate over [0, n): list[j] = the j-th index i
+// (ascending) with pred(i), and pos_of[i] = j for the selected i (FILL_NONE:
+// ~0u for the others).  Blocks of kCompactItems indices: a count pass, an
+// exclusive scan of the block counts (compact_count_scan), then a scatter
+// pass that re-evaluates the predicate (compact_scatter), so a caller can
+// size the list from the total in between.
+
+constexpr int kCompactThreads = 256;
+constexpr int kCompactPer = 16;
+constexpr int kCompactItems = kCompactThreads * kCompactPer;
+
+static inline uint64_t compact_blocks(uint64_t n) { return (n + kCompactItems - 1) / kCompactItems; }
+
+template <typename Pred>
+static __global__ void __launch_bounds__(kCompactThreads) k_compact_count(uint64_t n, Pred pred,
+                                                                         uint32_t *__restrict__ blk_cnt) {
+    __shared__ uint32_t wtot[kCompactThreads / 64];
+    const uint64_t base = (uint64_t)blockIdx.x * kCompactItems + (uint64_t)threadIdx.x * kCompactPer;
+    uint32_t c = 0;
+#pragma unroll
+    for (int j = 0; j < kCompactPer; j++) c += (base + j < n && pred(base + j)) ? 1u : 0u;
+    const uint32_t inc = wave_incl_sum(c);
+    if ((threadIdx.x & 63) == 63) wtot[threadIdx.x >> 6] = inc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t sum = 0;
+        for (int w = 0; w < kCompactThreads / 64; w++) sum += wtot[w];
+        blk_cnt[blockIdx.x] = sum;
+    }
+}
+
+template <bool FILL_NONE, typename Pred>
+static __global__ void __launch_bounds__(kCompactThreads) k_compact_scatter(uint64_t n, Pred pred,
+                                                                           const uint64_t *__restrict__ blk_off,
+                                                                           uint32_t *__restrict__ pos_of,
+                                                                           uint32_t *__restrict__ list) {
+    __shared__ uint32_t wtot[kCompactThreads / 64];
+    const uint64_t base = (uint64_t)blockIdx.x * kCompactItems + (uint64_t)threadIdx.x * kCompactPer;
+    uint32_t f = 0;  // bit j: index base + j selected
+#pragma unroll
+    for (int j = 0; j < kCompactPer; j++)
+        if (base + j < n && pred(base + j)) f |= 1u << j;
+    const uint32_t c = __popc(f);
+    const uint32_t inc = wave_incl_sum(c);
+    if ((threadIdx.x & 63) == 63) wtot[threadIdx.x >> 6] = inc;
+    __syncthreads();
+    uint64_t off = blk_off[blockIdx.x] + inc - c;
+    for (uint32_t w = 0; w < (threadIdx.x >> 6); w++) off += wtot[w];
+#pragma unroll
+    for (int j = 0; j < kCompactPer; j++) {
+        if (base + j >= n) break;
+        if ((f >> j) & 1) {
+            list[off] = (uint32_t)(base + j);
+            pos_of[base + j] = (uint32_t)off;
+            off++;
+        } else if (FILL_NONE) {
+            pos_of[base + j] = ~0u;
+        }
+    }
+}
+
+// Count pass and block-offset scan; the number selected goes to d_total.
+// blk_cnt / blk_off hold compact_blocks(n) entries; scratch as for scan_excl.
+template <typename Pred>
+static hipError_t compact_count_scan(uint64_t n, Pred pred, uint32_t *blk_cnt, uint64_t *blk_off, uint64_t *scratch,
+                                     uint64_t *d_total, hipStream_t st) {
+    const uint64_t nb = compact_blocks(n);
+    if (nb == 0) return hipMemsetAsync(d_total, 0, 8, st);
+    hipLaunchKernelGGL(k_compact_count<Pred>, dim3((unsigned)nb), dim3(kCompactThreads), 0, st, n, pred, blk_cnt);
+    return scan_excl<uint32_t>(blk_cnt, blk_off, nb, scratch, d_total, st);
+}
+
+template <bool FILL_NONE, typename Pred>
+static void compact_scatter(uint64_t n, Pred pred, const uint64_t *blk_off, uint32_t *pos_of, uint32_t *list,
+                            hipStream_t st) {
+    const uint64_t nb = compact_blocks(n);
+    if (nb) hipLaunchKernelGGL((k_compact_scatter<FILL_NONE, Pred>), dim3((unsigned)nb), dim3(kCompactThreads), 0, st,
+                               n, pred, blk_off, pos_of, list);
+}
+
 }  // namespace swapk
