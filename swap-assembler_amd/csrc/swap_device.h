@@ -163,7 +163,7 @@ private:
         M_CMUL, M_CEDGE, M_LEV, M_LEVOFF, M_PAR, M_NEED, M_NL, M_NR, M_INL, M_INR, M_CNT, M_SPEC, M_PROPS, M_COUNT
     };
     DevBuf mb_[M_COUNT];
-    enum { O_KEY, O_BKT, O_BKT2, O_IDX, O_TMP, O_RANK, O_ORD, O_COUNT };
+    enum { O_KEY, O_BKT, O_BKT2, O_IDX, O_TMP, O_RANK, O_ORD, O_BCNT, O_BOFF, O_BSTART, O_SK, O_SV, O_COUNT };
     DevBuf ob_[O_COUNT];
     // host mirrors of small results
     uint64_t *h_small_ = nullptr;

@@ -7,13 +7,15 @@
 // bucket it depends on the history of inserts (head insertion) and rehashes
 // (which reverse each chain), and that history only depends on each key's
 // insertion rank and on the ranks at which the table grew.  So each table
-// order is: sort by final bucket (radix sort), then order each bucket's few
-// keys with a comparator that walks back through the growth epochs.
+// order is: sort by final bucket, then order each bucket's few keys with a
+// comparator that walks back through the growth epochs.
+#include <algorithm>
 #include <cstring>
 
-#include <rocprim/device/device_radix_sort.hpp>
+#include <rocprim/device/device_segmented_radix_sort.hpp>
 
 #include "swap_host.h"
+#include "swap_scan.h"
 
 namespace swapk {
 
@@ -85,6 +87,55 @@ __device__ __forceinline__ bool ord_before(const OrdKey &a, const OrdKey &b, int
     return false;
 }
 
+// Bucket sort of a table's keys in two steps: bins by the top bits of the
+// bucket (a count pass writing one histogram column per workgroup, a scan of
+// the columns, a scatter pass whose slots come from LDS counters), then a
+// segmented radix sort of every bin by the remaining bits.  Keys of one bucket
+// come out in no set order: k_ord_ties orders every run of equal buckets.  (A
+// whole-array radix sort spends a pass per 8 bits of the 23-27-bit buckets,
+// each with its own state clearing, on the same data.)
+constexpr int kBinThreads = 1024;
+// 2^10 bins at most: with 2^12, each workgroup's keys of a bin were runs of
+// ~4 on the 4.6M-key table, one partial line per store (scatter 0.12 ms); with
+// 2^8, the segmented sort of 18K-key bins took longer than the scatter saved
+constexpr int kMaxBinBits = 10;
+
+__global__ void __launch_bounds__(kBinThreads) k_bin_count(const uint32_t *__restrict__ bucket, uint64_t n,
+                                                           uint64_t per, int shift, uint32_t nbins,
+                                                           uint32_t *__restrict__ cnt) {
+    __shared__ uint32_t h[1u << kMaxBinBits];
+    for (uint32_t i = threadIdx.x; i < nbins; i += blockDim.x) h[i] = 0;
+    __syncthreads();
+    const uint64_t b = (uint64_t)blockIdx.x * per, e = min(n, b + per);
+    for (uint64_t i = b + threadIdx.x; i < e; i += blockDim.x) atomicAdd(&h[bucket[i] >> shift], 1u);
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < nbins; i += blockDim.x) cnt[(uint64_t)i * gridDim.x + blockIdx.x] = h[i];
+}
+
+// off[bin * gridDim.x + block]: first slot of the block's keys of that bin.
+// Workgroup 0 also writes the bin bounds for the segmented sort.
+__global__ void __launch_bounds__(kBinThreads) k_bin_scatter(const uint32_t *__restrict__ bucket,
+                                                             const uint32_t *__restrict__ idx, uint64_t n,
+                                                             uint64_t per, int shift, uint32_t nbins,
+                                                             const uint64_t *__restrict__ off,
+                                                             uint32_t *__restrict__ kout, uint32_t *__restrict__ vout,
+                                                             uint32_t *__restrict__ bin_start) {
+    __shared__ uint32_t base[1u << kMaxBinBits];
+    for (uint32_t i = threadIdx.x; i < nbins; i += blockDim.x) {
+        base[i] = (uint32_t)off[(uint64_t)i * gridDim.x + blockIdx.x];
+        if (blockIdx.x == 0) bin_start[i] = base[i];
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) bin_start[nbins] = (uint32_t)n;
+    __syncthreads();
+    const uint64_t b = (uint64_t)blockIdx.x * per, e = min(n, b + per);
+    for (uint64_t i = b + threadIdx.x; i < e; i += blockDim.x) {
+        const uint32_t k = bucket[i];
+        const uint32_t slot = atomicAdd(&base[k >> shift], 1u);
+        kout[slot] = k;
+        vout[slot] = idx[i];
+    }
+}
+
 // One thread per bucket run of the sorted array: insertion sort of its keys.
 __global__ void k_ord_ties(const uint32_t *__restrict__ bucket, uint32_t *__restrict__ idx, uint64_t n,
                            const EpochTab *__restrict__ etp, const OrdKey *__restrict__ ok) {
@@ -142,6 +193,7 @@ static inline unsigned oblk(uint64_t n) { return (unsigned)((n + 255) / 256); }
 // counts are needed for the sort width; the starts may differ).
 int Pipeline::table_order_device(const uint64_t *d_rank, const EpochTabHost &tab, const void *d_et, uint32_t *d_order) {
     const uint64_t S = n_surv;
+    if (!S) return 0;
     const EpochTab *et = static_cast<const EpochTab *>(d_et);
     OCHECK(ob_[O_KEY].reserve(S * sizeof(OrdKey)));
     OCHECK(ob_[O_BKT].reserve(S * 4));
@@ -152,12 +204,43 @@ int Pipeline::table_order_device(const uint64_t *d_rank, const EpochTabHost &tab
                        ob_[O_IDX].as<uint32_t>());
     int bits = 1;
     while (bits < 32 && (1ull << bits) < tab.P[tab.n - 1]) bits++;
-    size_t tb = 0;
-    OCHECK(rocprim::radix_sort_pairs(nullptr, tb, ob_[O_BKT].as<uint32_t>(), ob_[O_BKT2].as<uint32_t>(),
-                                     ob_[O_IDX].as<uint32_t>(), d_order, (size_t)S, 0, bits, st_));
-    OCHECK(ob_[O_TMP].reserve(tb + 16));
-    OCHECK(rocprim::radix_sort_pairs(ob_[O_TMP].p, tb, ob_[O_BKT].as<uint32_t>(), ob_[O_BKT2].as<uint32_t>(),
-                                     ob_[O_IDX].as<uint32_t>(), d_order, (size_t)S, 0, bits, st_));
+    // bins of at least ~1024 keys; the segmented sort orders each by the low bits
+    int binb = 0;
+    while (binb < kMaxBinBits && (S >> (binb + 1)) >= 1024) binb++;
+    binb = std::min(binb, bits);
+    const int shift = bits - binb;
+    const uint32_t nbins = 1u << binb;
+    int ncu = 0;
+    OCHECK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev_));
+    const uint64_t nblk = std::max<uint64_t>(1, std::min<uint64_t>((uint64_t)ncu, (S + 8191) / 8192));
+    const uint64_t per = (S + nblk - 1) / nblk;
+    OCHECK(ob_[O_BCNT].reserve((uint64_t)nbins * nblk * 4 + 4));
+    OCHECK(ob_[O_BOFF].reserve((uint64_t)nbins * nblk * 8 + 8));
+    OCHECK(ob_[O_BSTART].reserve((uint64_t)(nbins + 1) * 4));
+    OCHECK(scan_scratch_.reserve((scan_scratch_words((uint64_t)nbins * nblk) + 16) * 8));
+    const bool direct = shift == 0;  // one bucket per bin: the scatter is the sort
+    if (!direct) {
+        OCHECK(ob_[O_SK].reserve(S * 4));
+        OCHECK(ob_[O_SV].reserve(S * 4));
+    }
+    uint32_t *kout = direct ? ob_[O_BKT2].as<uint32_t>() : ob_[O_SK].as<uint32_t>();
+    uint32_t *vout = direct ? d_order : ob_[O_SV].as<uint32_t>();
+    hipLaunchKernelGGL(k_bin_count, dim3((unsigned)nblk), dim3(kBinThreads), 0, st_, ob_[O_BKT].as<uint32_t>(), S, per,
+                       shift, nbins, ob_[O_BCNT].as<uint32_t>());
+    OCHECK(scan_excl<uint32_t>(ob_[O_BCNT].as<uint32_t>(), ob_[O_BOFF].as<uint64_t>(), (uint64_t)nbins * nblk,
+                               scan_scratch_.as<uint64_t>(), ob_[O_BOFF].as<uint64_t>() + (uint64_t)nbins * nblk, st_));
+    hipLaunchKernelGGL(k_bin_scatter, dim3((unsigned)nblk), dim3(kBinThreads), 0, st_, ob_[O_BKT].as<uint32_t>(),
+                       ob_[O_IDX].as<uint32_t>(), S, per, shift, nbins, ob_[O_BOFF].as<uint64_t>(), kout, vout,
+                       ob_[O_BSTART].as<uint32_t>());
+    if (!direct) {
+        const uint32_t *bs = ob_[O_BSTART].as<uint32_t>();
+        size_t tb = 0;
+        OCHECK(rocprim::segmented_radix_sort_pairs(nullptr, tb, kout, ob_[O_BKT2].as<uint32_t>(), vout, d_order,
+                                                   (unsigned)S, nbins, bs, bs + 1, 0u, (unsigned)shift, st_));
+        OCHECK(ob_[O_TMP].reserve(tb + 16));
+        OCHECK(rocprim::segmented_radix_sort_pairs(ob_[O_TMP].p, tb, kout, ob_[O_BKT2].as<uint32_t>(), vout, d_order,
+                                                   (unsigned)S, nbins, bs, bs + 1, 0u, (unsigned)shift, st_));
+    }
     hipLaunchKernelGGL(k_ord_ties, dim3(oblk(S)), dim3(256), 0, st_, ob_[O_BKT2].as<uint32_t>(), d_order, S, et,
                        ob_[O_KEY].as<OrdKey>());
     return 0;
