@@ -297,30 +297,27 @@ __device__ __forceinline__ void for_each_occ(const OccCtx &c, uint64_t w, F &&f)
     const uint64_t w1 = (w + 1 < c.W) ? c.packed[w + 1] : 0;
     const uint32_t prev_base = has_prev ? (uint32_t)(c.packed[w - 1] & 3) : 0;
     const uint64_t km = kmask(k);
-    // base at offset i (0..63) from the word start
-    auto base_at = [&](int i) -> uint32_t {
-        return (uint32_t)((i < 32 ? (w0 >> (62 - 2 * i)) : (w1 >> (62 - 2 * (i - 32)))) & 3);
-    };
     uint64_t x = w0 >> (64 - 2 * k);  // bases [0, k)
     uint64_t rcx = revcomp(x, k);
+    // bases k, k+1, ... at the top of a shift register: the base after k-mer i
+    // is taken from its top, the base before it is the top of k-mer i-1
+    uint64_t st = (w0 << (2 * k)) | (k < 32 ? w1 >> (64 - 2 * k) : 0);
+    uint32_t before = prev_base;
     for (uint32_t i = 0; i < nh; i++) {
-        if (i > 0) {
-            const uint32_t b = base_at((int)i + k - 1);
-            x = ((x << 2) | b) & km;
-            rcx = (rcx >> 2) | ((uint64_t)(3 - b) << (2 * k - 2));
-        }
+        const uint32_t next = (uint32_t)(st >> 62);
+        st <<= 2;
         const bool fwd = x > rcx;
         const uint64_t X = fwd ? x : rcx;
         uint32_t m = 0;
-        if (!(ends && i + 1 == nh)) {
-            const uint32_t b = base_at((int)i + k);
-            m |= 1u << (fwd ? b : 4 + b);
-        }
+        if (!(ends && i + 1 == nh)) m |= 1u << (fwd ? next : 4 + next);
         if (i > 0 || has_prev) {
-            const uint32_t a = 3 - (i > 0 ? base_at((int)i - 1) : prev_base);
+            const uint32_t a = 3 - before;
             m |= 1u << (fwd ? 4 + a : a);
         }
         f(mix_fwd(c.mx, X), m, (uint64_t)(occ0 + i));
+        before = (uint32_t)(x >> (2 * k - 2));
+        x = ((x << 2) | next) & km;
+        rcx = (rcx >> 2) | ((uint64_t)(3 - next) << (2 * k - 2));
     }
 }
 
