@@ -1252,12 +1252,23 @@ __global__ void __launch_bounds__(1024) k_sel_targets(const uint32_t *__restrict
 // Only a few bins are targets: the atomics are few and spread over the bitmap.
 __global__ void k_sel_mark(const uint32_t *__restrict__ dfo, const unsigned long long *__restrict__ d_n, int shift,
                            const int *__restrict__ target, uint32_t *__restrict__ bitmap) {
+    // the few target bins as a bit per bin in LDS: the target table is read
+    // only for the first occurrences that fall in one
+    __shared__ uint32_t is_target[kSelBins / 32];
+    for (uint32_t i = threadIdx.x; i < kSelBins / 32; i += blockDim.x) {
+        uint32_t w = 0;
+        for (uint32_t j = 0; j < 32; j++) w |= (target[32 * i + j] >= 0 ? 1u : 0u) << j;
+        is_target[i] = w;
+    }
+    __syncthreads();
     const uint64_t n = *d_n;
     const uint32_t nw = (1u << shift) >= 32 ? (1u << shift) / 32 : 1;
     const uint32_t in_bin = (1u << shift) - 1;
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
         const uint32_t f = dfo[i];
-        const int t = target[f >> shift];
+        const uint32_t bin = f >> shift;
+        if (!((is_target[bin >> 5] >> (bin & 31)) & 1)) continue;
+        const int t = target[bin];
         if (t >= 0) {
             const uint32_t bit = f & in_bin;
             atomicOr(&bitmap[(uint32_t)t * nw + (bit >> 5)], 1u << (bit & 31));
