@@ -1724,7 +1724,10 @@ int Pipeline::build(const uint8_t *d_in, uint64_t n) {
         const uint32_t sel_words = (1u << shift) >= 32 ? (1u << shift) / 32 : 1;  // bitmap words per bin
         SWAP_CHECK(sel_out_.reserve((uint64_t)32 * sel_words * 4 + 16));
         SWAP_CHECK(hipMemsetAsync(sel_out_.p, 0, (uint64_t)32 * sel_words * 4, st_));
-        SWAP_CHECK(rows_.reserve((uint64_t)ncu * kSelBins * 4));
+        // two histogram workgroups per CU (64 KB of LDS each): twice the waves
+        // reading the first occurrences
+        const unsigned sel_rows = 2 * (unsigned)ncu;
+        SWAP_CHECK(rows_.reserve((uint64_t)sel_rows * kSelBins * 4));
         // the k-mer table's epoch table: bucket counts from D, starts filled below
         epoch1_ = epoch_table(n_distinct, false, nullptr);
         SWAP_CHECK(etab_.reserve(sizeof(EpochTabHost) + 64));
@@ -1732,11 +1735,11 @@ int Pipeline::build(const uint8_t *d_in, uint64_t n) {
         SWAP_CHECK(hipMemcpyAsync(etab_.p, h_etab_, sizeof(EpochTabHost), hipMemcpyHostToDevice, st_));
         EpochTabHost *d_et = etab_.as<EpochTabHost>();
         phase_begin(6);
-        hipLaunchKernelGGL(k_sel_hist, dim3(ncu), dim3(1024), 0, st_, dfo_.as<uint32_t>(),
+        hipLaunchKernelGGL(k_sel_hist, dim3(sel_rows), dim3(1024), 0, st_, dfo_.as<uint32_t>(),
                            (const unsigned long long *)(d_small + 9), shift, rows_.as<uint32_t>(),
                            (unsigned long long *)(d_small + 12));
         hipLaunchKernelGGL(k_col_sum, dim3(kSelBins / 256, kHistRowGroups), dim3(256), 0, st_, rows_.as<uint32_t>(),
-                           (uint32_t)ncu, kSelBins, d_hist);
+                           sel_rows, kSelBins, d_hist);
         hipLaunchKernelGGL(k_sel_targets, dim3(1), dim3(1024), 0, st_, d_hist, d_starts, (int)starts.size(),
                            sel_target_.as<int>(), d_si);
         hipLaunchKernelGGL(k_sel_mark, dim3(ncu * 4), dim3(1024), 0, st_, dfo_.as<uint32_t>(),
