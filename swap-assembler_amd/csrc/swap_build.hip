@@ -284,18 +284,33 @@ struct OccCtx {
 // canonical k-mer (2k bits), mask its 8 edge bits, occ its input k-mer index.
 // Every base read lies inside the word's read: a k-mer's bases and its
 // successor base (when the read has one) end before the read does.
+// One word's inputs, loaded by the caller one word ahead of their use so that
+// the loads of word w + blockDim.x overlap the k-mers of word w.
+struct OccWord {
+    uint64_t meta, w0, w1;
+    uint32_t prev_base;  // last base of the previous word (used when the read continues)
+};
+
+__device__ __forceinline__ OccWord load_word(const OccCtx &c, uint64_t w) {
+    OccWord o;
+    o.meta = c.wmeta[w];
+    o.w0 = c.packed[w];
+    o.w1 = (w + 1 < c.W) ? c.packed[w + 1] : 0;
+    o.prev_base = w > 0 ? (uint32_t)(c.packed[w - 1] & 3) : 0;
+    return o;
+}
+
 template <typename F>
-__device__ __forceinline__ void for_each_occ(const OccCtx &c, uint64_t w, F &&f) {
-    const uint64_t meta = c.wmeta[w];
+__device__ __forceinline__ void for_each_occ_word(const OccCtx &c, const OccWord &o, F &&f) {
+    const uint64_t meta = o.meta;
     const uint32_t nh = (uint32_t)(meta >> 32) & 63;
     if (nh == 0) return;
     const uint32_t occ0 = (uint32_t)meta;
     const bool has_prev = (meta >> 40) & 1;
     const bool ends = (meta >> 41) & 1;
     const int k = c.k;
-    const uint64_t w0 = c.packed[w];
-    const uint64_t w1 = (w + 1 < c.W) ? c.packed[w + 1] : 0;
-    const uint32_t prev_base = has_prev ? (uint32_t)(c.packed[w - 1] & 3) : 0;
+    const uint64_t w0 = o.w0, w1 = o.w1;
+    const uint32_t prev_base = has_prev ? o.prev_base : 0;
     const uint64_t km = kmask(k);
     uint64_t x = w0 >> (64 - 2 * k);  // bases [0, k)
     uint64_t rcx = revcomp(x, k);
@@ -318,6 +333,22 @@ __device__ __forceinline__ void for_each_occ(const OccCtx &c, uint64_t w, F &&f)
         before = (uint32_t)(x >> (2 * k - 2));
         x = ((x << 2) | next) & km;
         rcx = (rcx >> 2) | ((uint64_t)(3 - next) << (2 * k - 2));
+    }
+}
+
+// f over the occurrences of words wb + threadIdx.x, wb + threadIdx.x +
+// blockDim.x, ... below we, each word's loads issued one word early.
+template <typename F>
+__device__ __forceinline__ void for_each_occ_range(const OccCtx &c, uint64_t wb, uint64_t we, F &&f) {
+    uint64_t w = wb + threadIdx.x;
+    if (w >= we) return;
+    OccWord cur = load_word(c, w);
+    for (; w < we; w += blockDim.x) {
+        const uint64_t wn = w + blockDim.x;
+        OccWord nxt = cur;
+        if (wn < we) nxt = load_word(c, wn);
+        for_each_occ_word(c, cur, f);
+        cur = nxt;
     }
 }
 
@@ -396,8 +427,7 @@ __global__ void __launch_bounds__(kOccThreads) k_part_hist(OccCtx c, uint32_t *_
     uint64_t wb, we;
     word_range(c.W, wb, we);
     const int sh = 2 * c.k - kPartBitsMin;
-    for (uint64_t w = wb + threadIdx.x; w < we; w += blockDim.x)
-        for_each_occ(c, w, [&](uint64_t h, uint32_t, uint64_t) { atomicAdd(&lh[h >> sh], 1u); });
+    for_each_occ_range(c, wb, we, [&](uint64_t h, uint32_t, uint64_t) { atomicAdd(&lh[h >> sh], 1u); });
     __syncthreads();
     uint32_t *row = rows + (uint64_t)blockIdx.x * kHistBins;
     for (uint32_t i = threadIdx.x; i < kHistBins; i += blockDim.x) row[i] = lh[i];
@@ -432,11 +462,10 @@ __global__ void __launch_bounds__(kOccThreads) k_part1(OccCtx c, const uint64_t 
     word_range(c.W, wb, we);
     const int sh1 = 2 * c.k - kL1Bits;
     const uint64_t low1 = (1ull << sh1) - 1;
-    for (uint64_t w = wb + threadIdx.x; w < we; w += blockDim.x)
-        for_each_occ(c, w, [&](uint64_t h, uint32_t m, uint64_t occ) {
-            const uint32_t g = atomicAdd(&fill[h >> sh1], 1u);
-            rec_store<WITH_OCC>(rec_out, g, ((h & low1) << 8) | m, (uint32_t)occ);
-        });
+    for_each_occ_range(c, wb, we, [&](uint64_t h, uint32_t m, uint64_t occ) {
+        const uint32_t g = atomicAdd(&fill[h >> sh1], 1u);
+        rec_store<WITH_OCC>(rec_out, g, ((h & low1) << 8) | m, (uint32_t)occ);
+    });
 }
 
 // Level-1 bucket b and its first record of the chunk `chunk` (chunk_start[b] =
