@@ -316,16 +316,22 @@ __global__ void k_rs_walk(const uint32_t *__restrict__ rlist, uint64_t nr, const
     rsp[r] = (uint8_t)sp;
 }
 
-// pointer jumping over rulers: succ is a ruler index
+// pointer jumping over rulers (succ is a ruler index), two hops of the
+// previous round's pointers per round: a ruler's span grows at least threefold
+// per launch until its walk ends, so ceil(log3(rulers + 1)) launches finish
+// every walk
 __global__ void k_rs_jump(const Walk *__restrict__ win, const uint8_t *__restrict__ spin, uint64_t nr,
                           Walk *__restrict__ wout, uint8_t *__restrict__ spout) {
     const uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= nr) return;
     Walk x = win[v];
     uint8_t f = spin[v];
-    if (x.succ != kNil) {
-        const Walk y = win[x.succ];
-        f |= spin[x.succ];
+#pragma unroll
+    for (int hop = 0; hop < 2; hop++) {
+        if (x.succ == kNil) break;
+        const uint32_t s = x.succ;
+        const Walk y = win[s];
+        f |= spin[s];
         x.dist += y.dist;
         x.succ = y.succ;
         x.term = y.term;
@@ -843,7 +849,7 @@ int Pipeline::merge_device(bool reference_order, MergeDev &out) {
         hipLaunchKernelGGL(k_rs_walk, dim3(nblk(nr)), dim3(256), 0, st_, rlist, nr, nx, mb_[M_SELF].as<uint8_t>(),
                            rid, S, label, rwa, rspa);
         int rounds = 1;
-        while ((1ull << rounds) < nr + 2) rounds++;
+        for (uint64_t span = 3; span < nr + 2; span *= 3) rounds++;
         rounds++;
         for (int r = 0; r < rounds; r++) {
             hipLaunchKernelGGL(k_rs_jump, dim3(nblk(nr)), dim3(256), 0, st_, rwa, rspa, nr, rwb, rspb);
