@@ -47,9 +47,10 @@ constexpr int kScanThreads = 256;
 constexpr int kScanItems = 8;
 constexpr int kScanTile = kScanThreads * kScanItems;
 
+// total (may be null): where a single-block scan also leaves the sum
 template <typename TIn>
 static __global__ void k_scan_local(const TIn *__restrict__ in, uint64_t *__restrict__ out, uint64_t n,
-                             uint64_t *__restrict__ block_sums) {
+                             uint64_t *__restrict__ block_sums, uint64_t *__restrict__ total) {
     __shared__ uint64_t wsum[kScanThreads / 64];
     const uint64_t base = (uint64_t)blockIdx.x * kScanTile + (uint64_t)threadIdx.x * kScanItems;
     uint64_t v[kScanItems];
@@ -71,7 +72,10 @@ static __global__ void k_scan_local(const TIn *__restrict__ in, uint64_t *__rest
         if (base + i < n) out[base + i] = run;
         run += v[i];
     }
-    if (threadIdx.x == kScanThreads - 1) block_sums[blockIdx.x] = run;
+    if (threadIdx.x == kScanThreads - 1) {
+        block_sums[blockIdx.x] = run;
+        if (total) *total = run;
+    }
 }
 
 static __global__ void k_scan_add(uint64_t *__restrict__ out, uint64_t n, const uint64_t *__restrict__ offs) {
@@ -102,8 +106,9 @@ static hipError_t scan_excl(const TIn *in, uint64_t *out, uint64_t n, uint64_t *
     if (n == 0) return hipMemsetAsync(d_total, 0, 8, st);
     const uint64_t nb = (n + kScanTile - 1) / kScanTile;
     uint64_t *sums = scratch;
-    hipLaunchKernelGGL(k_scan_local<TIn>, dim3((unsigned)nb), dim3(kScanThreads), 0, st, in, out, n, sums);
-    if (nb == 1) return hipMemcpyAsync(d_total, sums, 8, hipMemcpyDeviceToDevice, st);
+    hipLaunchKernelGGL(k_scan_local<TIn>, dim3((unsigned)nb), dim3(kScanThreads), 0, st, in, out, n, sums,
+                       nb == 1 ? d_total : (uint64_t *)nullptr);
+    if (nb == 1) return hipGetLastError();
     uint64_t *sum_scan = scratch + nb;
     const hipError_t e = scan_excl<uint64_t>(sums, sum_scan, nb, scratch + 2 * nb, d_total, st);
     if (e != hipSuccess) return e;
