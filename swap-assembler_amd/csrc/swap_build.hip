@@ -1577,7 +1577,8 @@ int Pipeline::build(const uint8_t *d_in, uint64_t n) {
     // u32 starts of the histogram's bins: the final partition starts (and the
     // level-2 cursors) when the partitions are these bins
     hipLaunchKernelGGL(k_u64_to_u32, dim3((kHistBins + 1 + 255) / 256), dim3(256), 0, st_, bucket_.as<uint64_t>(),
-                       (uint64_t)kHistBins + 1, hist_.as<uint32_t>());
+                       (uint64_t)kHistBins + 1, hist_.as<uint32_t>(),
+                       pbits == kPartBitsMin ? cursor_.as<uint32_t>() : (uint32_t *)nullptr);
     uint32_t *l1_start = l1info_.as<uint32_t>();
     uint32_t *chunk_start = l1_start + kL1 + 2;
     uint32_t *fchunk_start = chunk_start + kL1 + 2;
@@ -1610,9 +1611,10 @@ int Pipeline::build(const uint8_t *d_in, uint64_t n) {
         SWAP_CHECK(scan_excl<uint32_t>(hist_.as<uint32_t>(), bucket_.as<uint64_t>(), nparts + 1,
                                        scan_scratch_.as<uint64_t>(), d_small + 6, st_));
         hipLaunchKernelGGL(k_u64_to_u32, dim3((unsigned)((nparts + 1 + 255) / 256)), dim3(256), 0, st_,
-                           bucket_.as<uint64_t>(), nparts + 1, hist_.as<uint32_t>());
+                           bucket_.as<uint64_t>(), nparts + 1, hist_.as<uint32_t>(), cursor_.as<uint32_t>());
     }
-    SWAP_CHECK(hipMemcpyAsync(cursor_.p, hist_.p, (nparts + 1) * 4, hipMemcpyDeviceToDevice, st_));
+    // (the level-2 cursors start as copies of the partition starts, written
+    // by the conversion above)
     const unsigned grid2 = (unsigned)(N / kChunkRecs + kL1 + 1);
     phase_begin(7);
     if (pbits == kPartBitsMin) {

@@ -235,7 +235,7 @@ constexpr uint32_t kRuleStride = 32;
 // per oriented node).
 __global__ void k_rs_mark(const uint8_t *__restrict__ is11, const uint32_t *__restrict__ nxt,
                           const uint8_t *__restrict__ selfadj, uint64_t S, uint8_t *__restrict__ props,
-                          uint32_t stride) {
+                          uint32_t stride, uint2 *__restrict__ label) {
     const uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= 2 * S) return;
     uint32_t f = 0;
@@ -245,6 +245,7 @@ __global__ void k_rs_mark(const uint8_t *__restrict__ is11, const uint32_t *__re
         f = (back == kNil || !is11[back >> 1]) ? 1u : ((uint32_t)((v * 0x9E3779B97F4A7C15ull) >> 32) % stride == 0);
     }
     props[v] = (uint8_t)((one ? 1u : 0u) | (f << 1) | (one && selfadj[v >> 1] ? 4u : 0u));
+    label[v] = make_uint2(kNil, kNil);  // not reached by a ruler's walk (yet)
 }
 
 // Rulers (numbered in oriented-node order) and kept nodes (in survivor order):
@@ -828,7 +829,7 @@ int Pipeline::merge_device(bool reference_order, MergeDev &out) {
     MCHECK(mb_[M_PROPS].reserve(2 * S));
     hipLaunchKernelGGL(k_rs_mark, dim3(nblk(2 * S)), dim3(256), 0, st_, mb_[M_IS11].as<uint8_t>(),
                        mb_[M_NXT].as<uint32_t>(), mb_[M_SELF].as<uint8_t>(), S, mb_[M_PROPS].as<uint8_t>(),
-                       kRuleStride);
+                       kRuleStride, label);
     const RulerPred ruler{mb_[M_PROPS].as<uint8_t>()};
     MCHECK(compact_count_scan(2 * S, ruler, mb_[M_BCNT].as<uint32_t>(), mb_[M_BOFF].as<uint64_t>(),
                               scan_scratch_.as<uint64_t>(), d_small + 26, st_));
@@ -841,7 +842,6 @@ int Pipeline::merge_device(bool reference_order, MergeDev &out) {
     uint32_t *rlist = mb_[M_ELEN].as<uint32_t>();
     Walk *rwa = mb_[M_CPRI].as<Walk>(), *rwb = rwa + nr + 1;
     uint8_t *rspa = mb_[M_CMUL].as<uint8_t>(), *rspb = rspa + nr + 1;
-    MCHECK(hipMemsetAsync(label, 0xFF, 2 * S * 8, st_));
     hipLaunchKernelGGL(k_rs_prep, dim3(nblk(2 * S)), dim3(256), 0, st_, mb_[M_PROPS].as<uint8_t>(),
                        mb_[M_NXT].as<uint32_t>(), S, nx);
     compact_scatter<true>(2 * S, ruler, mb_[M_BOFF].as<uint64_t>(), rid, rlist, st_);

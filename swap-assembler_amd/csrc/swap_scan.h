@@ -83,9 +83,14 @@ static __global__ void k_scan_add(uint64_t *__restrict__ out, uint64_t n, const 
     if (i < n) out[i] += offs[i / kScanTile];
 }
 
-static __global__ void k_u64_to_u32(const uint64_t *__restrict__ in, uint64_t n, uint32_t *__restrict__ out) {
+// out (and out2 when given) <- in, narrowed
+static __global__ void k_u64_to_u32(const uint64_t *__restrict__ in, uint64_t n, uint32_t *__restrict__ out,
+                                    uint32_t *__restrict__ out2) {
     const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) out[i] = (uint32_t)in[i];
+    if (i >= n) return;
+    const uint32_t v = (uint32_t)in[i];
+    out[i] = v;
+    if (out2) out2[i] = v;
 }
 
 // total <- sum, out <- exclusive prefix.  scratch must hold scan_scratch_words(n).
