@@ -44,7 +44,7 @@ extern "C" {
 
 /* return codes */
 #define SWAP_OK 0
-#define SWAP_E_INVAL (-1)     /* bad argument (k not odd in [9,31], NULL pointer, wrong state) */
+#define SWAP_E_INVAL (-1)     /* bad argument (k not odd in [9,31], a merge with k < 19, NULL pointer) */
 #define SWAP_E_EMPTY (-2)     /* empty input buffer */
 #define SWAP_E_FORMAT (-3)    /* no '>' in the input (reference: assert at kmerGraph.cpp:114) */
 #define SWAP_E_TOOBIG (-4)    /* more than 2^32-16 k-mers in one build */
@@ -70,7 +70,7 @@ extern "C" {
                                      partitions would share fills; same results */
 
 typedef struct swap_cfg {
-    int k;          /* odd k-mer length, 9..31 (the reference CLI accepts 19..31) */
+    int k;          /* odd k-mer length, 9..31; swap_merge needs 19..31 (the reference CLI's range) */
     int c;          /* cutoff: edge multiplicities below c are removed */
     int device;     /* HIP device ordinal */
     unsigned flags; /* SWAP_FLAG_* */
@@ -105,7 +105,8 @@ int swap_build(swap_ctx *ctx, const char *fasta, size_t n);
 /* Build from FASTA bytes already in device memory on the context's device. */
 int swap_build_device(swap_ctx *ctx, const void *d_fasta, size_t n);
 
-/* Merge the k-mer graph of the last build into unitig edges. */
+/* Merge the k-mer graph of the last build into unitig edges (k >= 19, else
+ * SWAP_E_INVAL). */
 int swap_merge(swap_ctx *ctx);
 
 int swap_get_stats(const swap_ctx *ctx, swap_stats *out);

@@ -242,6 +242,9 @@ int swap_merge(swap_ctx *c) {
     if (!c) return SWAP_E_INVAL;
     if (!c->built) return set_err(c, SWAP_E_STATE, "merge before build");
     if (c->cfg.flags & SWAP_FLAG_KMER_GRAPH) return set_err(c, SWAP_E_STATE, "context was created for -s (no merge)");
+    if (c->cfg.k < swapk::kMergeMinK)
+        return set_err(c, SWAP_E_INVAL, "merge needs k >= 19 (the reference's range, sequence.cpp:44); k-mer graphs "
+                                         "of shorter k are built with SWAP_FLAG_KMER_GRAPH");
     auto t0 = clk::now();
     bool mat = false;
     int rc = swapk::host_merge(*c->pipe, c->cfg, &mat, c->comp);

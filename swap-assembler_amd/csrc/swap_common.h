@@ -17,6 +17,10 @@ namespace swapk {
 constexpr int kPartBitsMin = 15;
 constexpr int kPartBitsMax = 20;
 
+// Smallest k the merge accepts: the reference's command line allows odd k in
+// 19..31 (sequence.cpp:44).  Shorter k-mer graphs are built (-s) but not merged.
+constexpr int kMergeMinK = 19;
+
 SWAP_HD uint64_t kmask(int k) { return (k >= 32) ? ~0ull : ((1ull << (2 * k)) - 1); }
 
 // Reverse complement of a 2k-bit k-mer in O(1): complement every 2-bit group,

@@ -283,7 +283,8 @@ __global__ void k_rs_prep(const uint8_t *__restrict__ props, const uint32_t *__r
 // nodes passed with (ruler, offset)
 __global__ void k_rs_walk(const uint32_t *__restrict__ rlist, uint64_t nr, const uint64_t *__restrict__ nx,
                           const uint8_t *__restrict__ selfadj, const uint32_t *__restrict__ rid, uint64_t S,
-                          uint2 *__restrict__ label, Walk *__restrict__ rw, uint8_t *__restrict__ rsp) {
+                          uint2 *__restrict__ label, Walk *__restrict__ rw, uint8_t *__restrict__ rsp,
+                          unsigned int *__restrict__ err) {
     const uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= nr) return;
     uint32_t u = rlist[r];
@@ -293,6 +294,7 @@ __global__ void k_rs_walk(const uint32_t *__restrict__ rlist, uint64_t nr, const
     x.term = kNil;
     x.last = u;
     uint32_t steps = 1;
+    bool ended = false;
     for (uint64_t guard = 0; guard <= 2 * S; guard++) {
         const uint64_t e = nx[u];
         const uint32_t t = (uint32_t)e;
@@ -300,11 +302,13 @@ __global__ void k_rs_walk(const uint32_t *__restrict__ rlist, uint64_t nr, const
         if (kind == 2) {  // arrives at a kept node
             x.term = t;
             x.last = u;
+            ended = true;
             break;
         }
         if (kind == 1) {  // next ruler
             x.succ = rid[t];
             x.last = u;
+            ended = true;
             break;
         }
         label[t] = make_uint2((uint32_t)r, steps);
@@ -312,6 +316,8 @@ __global__ void k_rs_walk(const uint32_t *__restrict__ rlist, uint64_t nr, const
         u = t;
         steps++;
     }
+    // 2S + 1 steps pass every oriented node: a walk still going revisits one
+    if (!ended) atomicOr(err, 2u);
     x.dist = steps;
     rw[r] = x;
     rsp[r] = (uint8_t)sp;
@@ -588,7 +594,8 @@ struct TreeArgs {
     uint8_t *need;            // children to wait for (0..2); 255 for sentinels
     uint32_t *nl;
     uint32_t *nr;
-    unsigned int *err;        // set when a search leaves the array (a chain without its sentinel)
+    unsigned int *err;        // bit 0: a search left the array (a chain without its sentinel); bit 2: a
+                              // climb passed more parents than positions
 };
 
 __device__ __forceinline__ uint64_t levv(const TreeArgs &t, int l, uint64_t i) { return t.lev[t.lev_off[l] + i]; }
@@ -694,6 +701,7 @@ __global__ void k_tree_eval(TreeArgs t, unsigned long long *word, uint8_t *emult
         m = eval(p, mL, mR);
         q = p;
     }
+    atomicOr(t.err, 4u);  // more than M parents: the parent links hold a cycle
 }
 
 // ---------------------------------------------------------------------------
@@ -847,7 +855,7 @@ int Pipeline::merge_device(bool reference_order, MergeDev &out) {
     compact_scatter<true>(2 * S, ruler, mb_[M_BOFF].as<uint64_t>(), rid, rlist, st_);
     if (nr) {
         hipLaunchKernelGGL(k_rs_walk, dim3(nblk(nr)), dim3(256), 0, st_, rlist, nr, nx, mb_[M_SELF].as<uint8_t>(),
-                           rid, S, label, rwa, rspa);
+                           rid, S, label, rwa, rspa, (unsigned int *)(d_small + 25));
         int rounds = 1;
         for (uint64_t span = 3; span < nr + 2; span *= 3) rounds++;
         rounds++;
@@ -1003,7 +1011,10 @@ int Pipeline::merge_device(bool reference_order, MergeDev &out) {
     MCHECK(hipGetLastError());
     MCHECK(hipMemcpyAsync(hs + 25, d_small + 25, 8, hipMemcpyDeviceToHost, st_));
     MCHECK(hipStreamSynchronize(st_));
-    if ((uint32_t)hs[25]) return fail(-5, "merge: a chain array lost its end marker");
+    const uint32_t merr = (uint32_t)hs[25];
+    if (merr & 1u) return fail(-5, "merge: a chain array lost its end marker");
+    if (merr & 2u) return fail(-5, "merge: a ruler walk passed every oriented node without ending");
+    if (merr & 4u) return fail(-5, "merge: a Cartesian-tree climb passed more parents than chain positions");
     collect_phases();
     n_kept = nk;
     n_special = nspec;

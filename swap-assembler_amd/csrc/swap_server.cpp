@@ -16,6 +16,7 @@
 // The reference assembles one sample per run and has no counterpart of this
 // module.
 #include "swap_hip.h"
+#include "swap_common.h"
 
 #include <hip/hip_runtime.h>
 
@@ -165,8 +166,8 @@ int64_t swap_server_submit(swap_server *s, const swap_job *job) {
     if (job->k < 9 || job->k > 31 || (job->k % 2) == 0) return SWAP_E_INVAL;
     if (job->flags & SWAP_FLAG_KMER_GRAPH) {
         if (job->outputs & ~(1u << SWAP_FILE_KMER_GRAPH)) return SWAP_E_INVAL;
-    } else if (job->outputs & (1u << SWAP_FILE_KMER_GRAPH)) {
-        return SWAP_E_INVAL;
+    } else if ((job->outputs & (1u << SWAP_FILE_KMER_GRAPH)) || job->k < swapk::kMergeMinK) {
+        return SWAP_E_INVAL;  // a merge job needs k >= 19
     }
     auto j = std::make_unique<JobRec>();
     j->spec = *job;

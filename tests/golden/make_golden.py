@@ -142,9 +142,9 @@ def gz_write(path, text):
             f.write(text.encode())
 
 
-def run_ref(k, c, fa, flag):
-    out = tempfile.mkdtemp(prefix="ref_", dir=os.path.join(REPO, "_work"))
-    os.rmdir(out)
+def run_ref(k, c, fa, flag, scratch):
+    out = tempfile.mkdtemp(prefix="ref_", dir=scratch)
+    os.rmdir(out)  # the reference creates its output directory itself
     cmd = [MPIRUN, "-np", "4", REF, "-k", str(k), "-c", str(c), "-i", fa, "-o", out]
     if flag:
         cmd.append(flag)
@@ -155,19 +155,19 @@ def run_ref(k, c, fa, flag):
 def main():
     if not os.path.exists(REF):
         sys.exit("build the reference first: make -C oracle ref")
-    os.makedirs(os.path.join(REPO, "_work"), exist_ok=True)
+    scratch = tempfile.mkdtemp(prefix="swap_golden_")  # outside the repository
     index = {}
     for name, k, c, gen in CASES:
         text = gen()
-        fa = os.path.join(REPO, "_work", name + ".fa")
+        fa = os.path.join(scratch, name + ".fa")
         with open(fa, "w") as f:
             f.write(text)
         gz_write(os.path.join(HERE, name + ".fa.gz"), text)
-        out_s = run_ref(k, c, fa, "-s")
+        out_s = run_ref(k, c, fa, "-s", scratch)
         with open(os.path.join(out_s, "kmerGraph.txt")) as f:
             kg = sorted(f.read().splitlines())
         shutil.rmtree(out_s)
-        out_d = run_ref(k, c, fa, None)
+        out_d = run_ref(k, c, fa, None, scratch)
         with open(os.path.join(out_d, "noCEcontigs.fasta")) as f:
             noce = sorted(f.read().splitlines())
         shutil.rmtree(out_d)
@@ -175,6 +175,7 @@ def main():
         gz_write(os.path.join(HERE, name + ".noce.gz"), "\n".join(noce) + ("\n" if noce else ""))
         index[name] = {"k": k, "c": c, "kmer_graph_lines": len(kg), "noce_lines": len(noce), "np": 4}
         print(name, index[name])
+    shutil.rmtree(scratch)
     with open(os.path.join(HERE, "cases.json"), "w") as f:
         json.dump(index, f, indent=1, sort_keys=True)
 
