@@ -253,13 +253,14 @@ void build_nodes(ArcTable &kmol, NodeTable &nodes) {
 // completes and the unlock is only applied when the next lock request enters
 // the service loop.  A node visited while it is still locked that way fails its
 // own trylock (graph.cpp:1422-1427) and is skipped by this pass.
-void swap_merge(const Ctx &cx, NodeTable &nodes) {
+void swap_merge(const Ctx &cx, NodeTable &nodes, int flags) {
+    const bool plant_eager_unlock = flags & 2, plant_double_lock = flags & 4;
     bool pending = false;
     unsigned long long pendingId = 0;
     for (NodeTable::iterator it = nodes.begin(); it != nodes.end(); ++it) {
         Node &M = it->second;
         if (!M.live) continue;
-        if (pending && pendingId == M.id) continue;  // still locked: trylock fails
+        if (pending && pendingId == M.id && !plant_eager_unlock) continue;  // still locked: trylock fails
         int nl = 0, li = -1, nr = 0, ri = -1;
         for (int i = 4; i < 8; i++) if (!M.arcs[i].empty()) { nl++; li = i; }
         for (int i = 0; i < 4; i++) if (!M.arcs[i].empty()) { nr++; ri = i; }
@@ -269,7 +270,7 @@ void swap_merge(const Ctx &cx, NodeTable &nodes) {
         cx.neighbour(M, ri, R);
         pending = false;  // the first lock request of this visit serves the late unlock
         const bool lockL = (L != M.id);
-        const bool lockR = (R != L) ? (R != M.id) : lockL;  // graph.cpp:1464-1475
+        const bool lockR = (R != L) ? (R != M.id) : (plant_double_lock ? false : lockL);  // graph.cpp:1464-1475
         if (!(lockL && lockR)) {
             if (lockL || lockR) { pending = true; pendingId = lockL ? L : R; }
             continue;
@@ -473,7 +474,7 @@ oracle_graph *oracle_run(const char *fasta, uint64_t n, int k, int c, int flags)
     build_nodes(kmol, nodes);
     for (NodeTable::iterator it = nodes.begin(); it != nodes.end(); ++it) g->visit.push_back(it->first);
     if (!(flags & 1)) {
-        swap_merge(g->cx, nodes);  // graph.cpp:3576
+        swap_merge(g->cx, nodes, flags);  // graph.cpp:3576
         print_perf_contigs(g->cx, nodes, g->perf_txt);  // what -p writes at this point (graph.cpp:3591-3611)
     }
     gather(nodes, g->contig_graph_txt);

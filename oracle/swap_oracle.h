@@ -6,19 +6,18 @@
 // product (libswaphip.so and the swap driver) never links or calls it.
 //
 // Pinning (what checks each part):
+//   every output file of the single-rank path -- kmerGraph.txt, contigGraph.txt
+//       (merged multiplicities, padding and line order included),
+//       noCEcontigs.fasta and the -p file -- byte for byte against the
+//       unmodified reference run as one rank (oracle/ref_swap1.cpp) on 23
+//       committed inputs and the 1 Mbp / 4.6 Mbp workloads
+//       (tests/golden/make_golden_np1.py, tests/test_oracle.py).  The inputs
+//       include isolated cycles, self-neighbours, two-node cycles, fold-backs
+//       and the late unlock of a half-locked neighbour; the planted-mistake
+//       flags below must each break at least one of them.
 //   parse / encode / canonical / edge bits / counts / cutoff
-//       closed forms (max-canonical, (k+1)-mer occurrence counts saturating at
-//       255), brute force on tiny inputs, and the unmodified reference run at
-//       np>=3 (kmerGraph.txt, compared as sorted lines) on committed fixtures.
-//   compaction topology + edge strings + noCEcontigs.fasta
-//       the reference run at np>=3 (noCEcontigs set), textbook unitig
-//       compaction on random genomes (the genome itself comes back).
-//   merged multiplicities (order-dependent integer averages) and the
-//       reference's output line order: parity unpinned by a reference run --
-//       the reference deadlocks at np=1 in this MPI build (blocking self-send,
-//       kmerGraph.cpp:121-122) and its multi-rank merge is order-racy.  Checked
-//       by invariants only (ends of one unitig agree, min<=m<=max of the unit
-//       edges, exact on hand-worked chains).
+//       also closed forms (max-canonical, (k+1)-mer occurrence counts
+//       saturating at 255) and brute force on tiny inputs.
 #pragma once
 #include <cstdint>
 #include <cstddef>
@@ -29,6 +28,10 @@ typedef struct oracle_graph oracle_graph;
 
 // Runs the whole single-rank path on an in-memory FASTA buffer.
 //   flags: bit0 = stop after the k-mer graph (like -s, no merge)
+//          bit1 = PLANTED MISTAKE for tests: the unlock after a half-successful
+//                 lock takes effect at once instead of at the next lock request
+//          bit2 = PLANTED MISTAKE for tests: a neighbour that is both left and
+//                 right is locked twice (the second lock fails)
 // Returns NULL on malformed input (no '>' in the buffer) or bad k.
 oracle_graph *oracle_run(const char *fasta, uint64_t n, int k, int c, int flags);
 void oracle_free(oracle_graph *g);

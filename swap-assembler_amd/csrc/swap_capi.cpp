@@ -165,7 +165,7 @@ int format_file_string(swap_ctx *c, int which, std::string &out) {
         if (which == SWAP_FILE_CONTIG_GRAPH) {
             format_contig_graph(c->comp, out);
         } else if (which == SWAP_FILE_NOCE) {
-            format_noce(c->cfg.k, c->comp, out);
+            format_noce(c->cfg.k, c->comp.finished ? *c->comp.finished : c->comp, out);
         } else if (which == SWAP_FILE_PERF_CONTIGS) {
             format_perf_contigs(c->cfg.k, c->comp, out);
         } else {

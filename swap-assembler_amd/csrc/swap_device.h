@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 #include <cstdint>
 #include <string>
+#include <memory>
 #include <vector>
 
 #include "swap_common.h"
@@ -47,6 +48,11 @@ struct HostCompacted {
     std::vector<uint32_t> noce_order;
     // bucket count of the reference's nodes table when the gather re-fills it
     uint64_t node_buckets = 0;
+    // The table after the single-rank finisher (masterSimplifyNodeGraph), which
+    // noCEcontigs.fasta is written from: same rows and order as above, a node
+    // the finisher absorbed has no edges left.  Null when the finisher changes
+    // nothing (no node of exactly one left and one right edge survived the merge).
+    std::shared_ptr<const HostCompacted> finished;
 };
 
 // Outputs of the contig extension (host, after the merge).

@@ -438,7 +438,7 @@ int materialize_merge(Pipeline &p, const swap_cfg &cfg, HostCompacted &out) {
         std::string arcs[8];
         uint8_t m[8];
     };
-    std::vector<Out> live;
+    std::vector<Out> live, finished_live;
     live.reserve(nk + md.special.size());
     std::unordered_map<uint64_t, size_t> live_of;  // key -> index in live (only kept nodes next to specials)
     for (size_t i = 0; i < nk; i++) {
@@ -561,7 +561,28 @@ int materialize_merge(Pipeline &p, const swap_cfg &cfg, HostCompacted &out) {
             }
             M.live = false;
         }
-        // masterSimplifyNodeGraph (graph.cpp:2089-2192) over the live (1,1) nodes left
+        // The table as the merge leaves it: contigGraph.txt, the -p file and the
+        // contig extension's input are written from this state (graph.cpp:3591-3622).
+        auto collect = [&](std::vector<Out> &dst) {
+            for (auto &kv : R.nodes) {
+                HNode &n = kv.second;
+                if (n.kept >= 0) {
+                    Out &o = dst[(size_t)n.kept];
+                    for (int s = 0; s < 8; s++)
+                        if (md.especial[8 * (size_t)n.kept + s]) { o.arcs[s] = n.arcs[s]; o.m[s] = n.m[s]; }
+                } else if (n.live) {
+                    Out o;
+                    o.id = n.id;
+                    o.prio = n.prio;
+                    for (int s = 0; s < 8; s++) { o.arcs[s] = n.arcs[s]; o.m[s] = n.arcs[s].empty() ? 0 : n.m[s]; }
+                    dst.push_back(std::move(o));
+                }
+            }
+        };
+        std::vector<Out> after_swap = live;
+        collect(after_swap);
+        // masterSimplifyNodeGraph (graph.cpp:2089-2192) over the live (1,1) nodes
+        // left, in the order of the table the gather re-fills (graph.cpp:3614-3615)
         std::vector<HNode *> rest;
         for (auto &kv : R.nodes)
             if (kv.second.kept < 0 && kv.second.live) rest.push_back(&kv.second);
@@ -574,6 +595,7 @@ int materialize_merge(Pipeline &p, const swap_cfg &cfg, HostCompacted &out) {
         } else {
             std::sort(rest.begin(), rest.end(), [](const HNode *a, const HNode *b) { return a->prio < b->prio; });
         }
+        bool finisher_merged = false;
         for (HNode *Mp : rest) {
             HNode &M = *Mp;
             if (!M.live) continue;
@@ -597,22 +619,13 @@ int materialize_merge(Pipeline &p, const swap_cfg &cfg, HostCompacted &out) {
             Ln.m[lr] = splice_master(Ln.m[lr], Ln.arcs[lr].size(), M.m[right], M.arcs[right].size());
             Ln.arcs[lr] += M.arcs[right];
             M.live = false;
+            finisher_merged = true;
         }
-        // results: kept neighbours take the replayed strings on their component slots
-        for (auto &kv : R.nodes) {
-            HNode &n = kv.second;
-            if (n.kept >= 0) {
-                Out &o = live[(size_t)n.kept];
-                for (int s = 0; s < 8; s++)
-                    if (md.especial[8 * (size_t)n.kept + s]) { o.arcs[s] = n.arcs[s]; o.m[s] = n.m[s]; }
-            } else if (n.live) {
-                Out o;
-                o.id = n.id;
-                o.prio = n.prio;
-                for (int s = 0; s < 8; s++) { o.arcs[s] = n.arcs[s]; o.m[s] = n.arcs[s].empty() ? 0 : n.m[s]; }
-                live.push_back(std::move(o));
-            }
+        if (finisher_merged) {
+            finished_live = live;
+            collect(finished_live);
         }
+        live = std::move(after_swap);
     }
     // drop nodes without edges (graph.cpp:587-590)
     {
@@ -643,23 +656,45 @@ int materialize_merge(Pipeline &p, const swap_cfg &cfg, HostCompacted &out) {
     } else {
         std::iota(out.noce_order.begin(), out.noce_order.end(), 0u);
     }
-    size_t pool_n = 0;
-    for (auto &o : live) for (int s = 0; s < 8; s++) pool_n += o.arcs[s].size();
-    out.key.resize(live.size());
-    out.off.resize(8 * live.size());
-    out.len.resize(8 * live.size());
-    out.mult.resize(8 * live.size());
-    out.pool.resize(pool_n);
-    size_t at = 0;
-    for (size_t r = 0; r < ord.size(); r++) {
-        const Out &o = live[ord[r]];
-        out.key[r] = o.id;
-        for (int s = 0; s < 8; s++) {
-            out.off[8 * r + s] = at;
-            out.len[8 * r + s] = (uint32_t)o.arcs[s].size();
-            out.mult[8 * r + s] = o.arcs[s].empty() ? 0 : o.m[s];
-            if (!o.arcs[s].empty()) { memcpy(out.pool.data() + at, o.arcs[s].data(), o.arcs[s].size()); at += o.arcs[s].size(); }
+    // rows of a state in contigGraph.txt order
+    auto fill = [&](HostCompacted &dst, const std::vector<const Out *> &rows) {
+        size_t pool_n = 0;
+        for (const Out *o : rows)
+            if (o) for (int s = 0; s < 8; s++) pool_n += o->arcs[s].size();
+        dst.key.resize(rows.size());
+        dst.off.resize(8 * rows.size());
+        dst.len.resize(8 * rows.size());
+        dst.mult.resize(8 * rows.size());
+        dst.pool.resize(pool_n);
+        size_t at = 0;
+        for (size_t r = 0; r < rows.size(); r++) {
+            dst.key[r] = live[ord[r]].id;
+            for (int s = 0; s < 8; s++) {
+                const std::string empty;
+                const std::string &a = rows[r] ? rows[r]->arcs[s] : empty;
+                dst.off[8 * r + s] = at;
+                dst.len[8 * r + s] = (uint32_t)a.size();
+                dst.mult[8 * r + s] = a.empty() ? 0 : rows[r]->m[s];
+                if (!a.empty()) { memcpy(dst.pool.data() + at, a.data(), a.size()); at += a.size(); }
+            }
         }
+    };
+    std::vector<const Out *> rows(ord.size());
+    for (size_t r = 0; r < ord.size(); r++) rows[r] = &live[ord[r]];
+    fill(out, rows);
+    if (!finished_live.empty()) {
+        // the finisher's table: same rows; absorbed nodes keep no edge
+        std::unordered_map<uint64_t, const Out *> fin_of;
+        for (const Out &o : finished_live) fin_of[o.id] = &o;
+        for (size_t r = 0; r < ord.size(); r++) {
+            auto it = fin_of.find(live[ord[r]].id);
+            rows[r] = it == fin_of.end() ? nullptr : it->second;
+        }
+        auto fin = std::make_shared<HostCompacted>();
+        fill(*fin, rows);
+        fin->noce_order = out.noce_order;
+        fin->node_buckets = out.node_buckets;
+        out.finished = std::move(fin);
     }
     p.n_kept = live.size();
     return 0;

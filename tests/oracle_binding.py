@@ -37,8 +37,13 @@ def _lib():
 
 
 class Oracle:
-    def __init__(self, fasta: bytes, k: int, c: int, kmer_graph_only=False):
-        self._h = _lib().oracle_run(fasta, len(fasta), k, c, 1 if kmer_graph_only else 0)
+    # planted mistakes (oracle flags bits 1 and 2), for tests that show a
+    # fixture detects them
+    PLANT_EAGER_UNLOCK = 2
+    PLANT_DOUBLE_LOCK = 4
+
+    def __init__(self, fasta: bytes, k: int, c: int, kmer_graph_only=False, plant=0):
+        self._h = _lib().oracle_run(fasta, len(fasta), k, c, (1 if kmer_graph_only else 0) | plant)
         if not self._h:
             raise ValueError("oracle rejected the input")
 

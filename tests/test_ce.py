@@ -47,3 +47,24 @@ def test_graph_without_edges_is_reported():
     with pytest.raises(swapamd.SwapError) as e:
         swapamd.contig_extension(21, 10, b"", swapamd.SWAP_FILE_CE_CONTIGS)
     assert e.value.code == -2
+
+
+# The reference run end to end as one rank (tests/golden/make_golden_np1.py):
+# its contigGraph.txt (equal to the oracle's, tests/test_oracle.py) into the
+# library's contig extension, against the files that run wrote.
+from test_oracle import NP1, NP1_SMALL, np1_input, same_as_np1  # noqa: E402
+from oracle_binding import Oracle  # noqa: E402
+
+NP1_FILES = [("noce", swapamd.SWAP_FILE_NOCE), ("cecontig", swapamd.SWAP_FILE_CE_CONTIGS),
+             ("jungarc", swapamd.SWAP_FILE_JUNG_ARC), ("jungmul", swapamd.SWAP_FILE_JUNG_MUL)]
+
+
+@pytest.mark.parametrize("name", NP1_SMALL)
+def test_extension_matches_single_rank_reference(name):
+    case = NP1[name]
+    o = Oracle(np1_input(name), case["k"], case["c"])
+    graph = o.file(1)
+    assert same_as_np1(name, "contigGraph", graph)
+    for what, which in NP1_FILES:
+        got = swapamd.contig_extension(case["k"], o.stats()["survivors"], graph, which)
+        assert same_as_np1(name, what, got), what

@@ -9,6 +9,8 @@ import pytest
 
 import kmer_textbook
 import swapamd
+from test_oracle import NP1
+from test_gpu_merge import check_single_rank_reference
 
 FILES = (swapamd.SWAP_FILE_CONTIG_GRAPH, swapamd.SWAP_FILE_NOCE)
 
@@ -69,6 +71,20 @@ def check_full_size(fa, k, c, n_kmers, single_fill_or_wide, sample=None):
     assert np.array_equal(hk, keys) and np.array_equal(hm, mult)
     assert sorted(h.file_bytes(swapamd.SWAP_FILE_NOCE).splitlines()) == sorted(files[1].splitlines())
     h.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("workload", ["small", "bacterial"])
+def test_workload_equals_single_rank_reference(workload):
+    """The 1 Mbp and 4.6 Mbp workloads at full size: every output file equals,
+    byte for byte, what the unmodified reference wrote running them as one rank
+    (digests from tests/golden/make_golden_np1.py)."""
+    name = "workload_" + workload
+    if name not in NP1:
+        pytest.skip("no single-rank reference digests for the %s workload" % workload)
+    fa, k, c = swapamd.generate(workload)
+    assert (k, c) == (NP1[name]["k"], NP1[name]["c"])
+    check_single_rank_reference(name, fa)
 
 
 @pytest.mark.gpu

@@ -13,6 +13,8 @@ import swapamd
 from oracle_binding import Oracle
 import kmer_textbook
 
+from test_oracle import NP1, NP1_SMALL, np1_input, same_as_np1
+
 GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 CASES = json.load(open(os.path.join(GOLD, "cases.json")))
 
@@ -187,3 +189,33 @@ def test_contig_extension_from_reads(name):
     assert a.file_bytes(swapamd.SWAP_FILE_JUNG_ARC) == ce_gold(name, "jungarc")
     assert a.file_bytes(swapamd.SWAP_FILE_JUNG_MUL) == ce_gold(name, "jungmul")
     a.close()
+
+
+NP1_FILES = [("contigGraph", swapamd.SWAP_FILE_CONTIG_GRAPH), ("noce", swapamd.SWAP_FILE_NOCE),
+             ("perf", swapamd.SWAP_FILE_PERF_CONTIGS), ("cecontig", swapamd.SWAP_FILE_CE_CONTIGS),
+             ("jungarc", swapamd.SWAP_FILE_JUNG_ARC), ("jungmul", swapamd.SWAP_FILE_JUNG_MUL)]
+
+
+def check_single_rank_reference(name, fa):
+    """Every file of the GPU path equals what the unmodified reference wrote as
+    one rank for this input (fixture digests; exact bytes, line order and merged
+    multiplicities included)."""
+    case = NP1[name]
+    k, c = case["k"], case["c"]
+    print("single-rank reference case %s k=%d c=%d bytes=%d" % (name, k, c, len(fa)), flush=True)
+    a = swapamd.Assembler(k, c)
+    a.build(fa)
+    a.merge()
+    bad = [what for what, which in NP1_FILES if not same_as_np1(name, what, a.file_bytes(which))]
+    a.close()
+    s = swapamd.Assembler(k, c, kmer_graph_only=True)
+    s.build(fa)
+    if not same_as_np1(name, "kmerGraph", s.file_bytes(swapamd.SWAP_FILE_KMER_GRAPH)):
+        bad.append("kmerGraph")
+    s.close()
+    assert bad == []
+
+
+@pytest.mark.parametrize("name", NP1_SMALL)
+def test_single_rank_reference(name):
+    check_single_rank_reference(name, np1_input(name))

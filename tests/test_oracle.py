@@ -167,3 +167,112 @@ def test_perf_contigs_are_the_noce_selection(name):
         assert (first if slot < 4 else kmer_textbook.rc(first)) == decode_kmer(node, case["k"])
         seqs.append(seq)
     assert sorted(seqs) == noce
+
+
+# ---------------------------------------------------------------------------
+# The unmodified reference run as one rank (tests/golden/make_golden_np1.py):
+# exact bytes of every file, line order and merged multiplicities included.
+
+NP1 = json.load(open(os.path.join(GOLD, "np1_cases.json")))
+NP1_SMALL = sorted(n for n, c in NP1.items() if not c["input"].startswith("workload:"))
+NP1_SPECIAL = sorted(n for n in NP1_SMALL if NP1[n]["input"].startswith("np1_"))
+
+
+def np1_input(name):
+    with gzip.open(os.path.join(GOLD, NP1[name]["input"]), "rb") as f:
+        return f.read()
+
+
+def np1_file(name, suffix):
+    with gzip.open(os.path.join(GOLD, "np1_%s.%s.gz" % (name, suffix)), "rb") as f:
+        return f.read()
+
+
+def sha(b):
+    import hashlib
+    return hashlib.sha256(b).hexdigest()
+
+
+def same_as_np1(name, suffix, data):
+    want = NP1[name]["files"][suffix]
+    if want is None:
+        return False
+    if sha(data) == want["sha256"]:
+        return True
+    if want.get("stored"):
+        g, w = data.decode().splitlines(), np1_file(name, suffix).decode().splitlines()
+        first = next((i for i, (a, b) in enumerate(zip(g, w)) if a != b), min(len(g), len(w)))
+        print("%s %s: first difference at line %d of %d/%d" % (name, suffix, first, len(g), len(w)))
+    return False
+
+
+@pytest.mark.parametrize("name", NP1_SMALL)
+def test_oracle_equals_single_rank_reference(name):
+    case = NP1[name]
+    fa = np1_input(name)
+    o = Oracle(fa, case["k"], case["c"])
+    assert same_as_np1(name, "contigGraph", o.file(1))
+    assert same_as_np1(name, "noce", o.file(2))
+    assert same_as_np1(name, "perf", o.file(3))
+    s = Oracle(fa, case["k"], case["c"], kmer_graph_only=True)
+    assert same_as_np1(name, "kmerGraph", s.file(0))
+
+
+def test_oracle_equals_single_rank_reference_1mbp_workload():
+    import swapamd
+    fa, k, c = swapamd.generate("small")
+    name = "workload_small"
+    assert (k, c) == (NP1[name]["k"], NP1[name]["c"])
+    o = Oracle(fa, k, c)
+    assert same_as_np1(name, "contigGraph", o.file(1))
+    assert same_as_np1(name, "noce", o.file(2))
+    assert same_as_np1(name, "perf", o.file(3))
+
+
+def _canon_neighbour(X, slot, edge, k):
+    s = decode_kmer(X, k)
+    if slot >= 4:
+        s = kmer_textbook.rc(s)
+    t = (s + edge)[-k:]
+    t = max(t, kmer_textbook.rc(t))
+    return int("".join("%d" % "ACGT".index(ch) for ch in t), 4)
+
+
+def _one_one_after_merge(name):
+    """(1,1) nodes of the reference's contigGraph.txt (the table right after its
+    merge) and, per node, whether a neighbour is the node itself."""
+    k = NP1[name]["k"]
+    out = []
+    for line in np1_file(name, "contigGraph").decode().splitlines():
+        if not line.strip():
+            continue
+        parts = line.split("#")
+        head = parts[0].split(" ")
+        X = int(head[0])
+        edges = [head[1]] + [p.strip() for p in parts[1:8]]
+        right = [i for i in range(4) if edges[i]]
+        left = [i for i in range(4, 8) if edges[i]]
+        if len(right) == 1 and len(left) == 1:
+            nb = {_canon_neighbour(X, i, edges[i], k) for i in right + left}
+            out.append(X in nb)
+    return out
+
+
+def test_single_rank_fixtures_reach_the_special_merges():
+    # the reference left (1,1) nodes behind its merge: ones that are their own
+    # neighbour (cycles, fold-backs) and ones that are not (skipped while still
+    # locked by the previous visit)
+    self_nb = sum(sum(_one_one_after_merge(n)) for n in NP1_SPECIAL)
+    other = sum(len(v) - sum(v) for v in map(_one_one_after_merge, NP1_SPECIAL))
+    assert self_nb >= 10 and other >= 2
+
+
+@pytest.mark.parametrize("plant", [Oracle.PLANT_EAGER_UNLOCK, Oracle.PLANT_DOUBLE_LOCK])
+def test_single_rank_fixtures_catch_planted_merge_mistakes(plant):
+    broken = []
+    for name in NP1_SPECIAL:
+        case = NP1[name]
+        o = Oracle(np1_input(name), case["k"], case["c"], plant=plant)
+        if not same_as_np1(name, "contigGraph", o.file(1)):
+            broken.append(name)
+    assert broken, "no fixture detects the planted mistake"
