@@ -47,7 +47,8 @@ extern "C" {
 #define SWAP_E_INVAL (-1)     /* bad argument (k not odd in [9,31], a merge with k < 19, NULL pointer) */
 #define SWAP_E_EMPTY (-2)     /* empty input buffer */
 #define SWAP_E_FORMAT (-3)    /* no '>' in the input (reference: assert at kmerGraph.cpp:114) */
-#define SWAP_E_TOOBIG (-4)    /* more than 2^32-16 k-mers in one build */
+#define SWAP_E_TOOBIG (-4)    /* more than 2^32-16 k-mers in one build; for swap_assemble_static, an input
+                                 beyond its static scratch */
 #define SWAP_E_DEVICE (-5)    /* HIP runtime error (see swap_last_error) */
 #define SWAP_E_NOGPU (-6)     /* no usable gfx950 device */
 #define SWAP_E_IO (-7)        /* file could not be written */
@@ -108,6 +109,21 @@ int swap_build_device(swap_ctx *ctx, const void *d_fasta, size_t n);
 /* Merge the k-mer graph of the last build into unitig edges (k >= 19, else
  * SWAP_E_INVAL). */
 int swap_merge(swap_ctx *ctx);
+
+/* swap_build_device followed by swap_merge (build only for a SWAP_FLAG_KMER_GRAPH
+ * context) in one call that allocates nothing: every device buffer of the call
+ * is carved from one fixed scratch array per GPU (512 MiB, held by the library
+ * from the first context created on that GPU until the process ends), and no
+ * hipMalloc / hipFree or other allocating routine is called.  No counterpart in
+ * the reference: for callers that cannot allocate device memory (work issued
+ * under HIP graph capture, tight serving loops, harnesses that forbid it).
+ * Inputs up to 3 MiB of FASTA; larger ones, or an input whose buffers would
+ * not fit, return SWAP_E_TOOBIG.  Results are read with the usual export and
+ * file calls; they live in the scratch and stay valid until the next
+ * swap_assemble_static call on the same GPU from any context.  Calls that
+ * overlap on one GPU race on the scratch: the caller serialises them.
+ * swap_merge after this call returns SWAP_E_STATE. */
+int swap_assemble_static(swap_ctx *ctx, const void *d_fasta, size_t n);
 
 int swap_get_stats(const swap_ctx *ctx, swap_stats *out);
 

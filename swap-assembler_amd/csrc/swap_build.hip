@@ -1360,25 +1360,60 @@ __global__ void __launch_bounds__(1024) k_sel_pick(const SelInfo *__restrict__ s
 // ---------------------------------------------------------------------------
 // Pipeline::build
 
+static thread_local Arena *t_arena = nullptr;
+
+Arena *current_arena() { return t_arena; }
+ArenaScope::ArenaScope(Arena *a) : prev(t_arena) { t_arena = a; }
+ArenaScope::~ArenaScope() { t_arena = prev; }
+
 DevBuf::~DevBuf() {
-    if (p) (void)hipFree(p);
+    if (heap) (void)hipFree(heap);
 }
 
 hipError_t DevBuf::reserve(size_t bytes) {
-    if (bytes <= cap && p) return hipSuccess;
-    if (p) {
-        const hipError_t e = hipFree(p);
-        p = nullptr;
-        cap = 0;
-        if (e != hipSuccess) return e;
+    if (Arena *a = t_arena) {
+        if (arena_epoch == a->epoch && p && bytes <= cap) return hipSuccess;
+        const size_t want = (std::max<size_t>(bytes, 1) + 255) & ~(size_t)255;
+        a->wanted += want;
+        if (a->used + want > a->size) return hipErrorOutOfMemory;
+        p = a->base + a->used;
+        cap = want;
+        a->used += want;
+        arena_epoch = a->epoch;
+        return hipSuccess;
+    }
+    if (bytes <= cap && p) return hipSuccess;  // own memory, or a static call's space still being read
+    arena_epoch = 0;
+    if (heap && bytes <= heap_cap) {
+        p = heap;
+        cap = heap_cap;
+        return hipSuccess;
     }
     p = nullptr;
     cap = 0;
+    if (heap) {
+        const hipError_t e = hipFree(heap);
+        heap = nullptr;
+        heap_cap = 0;
+        if (e != hipSuccess) return e;
+    }
     size_t want = bytes + bytes / 8 + 256;
-    hipError_t e = hipMalloc(&p, want);
-    if (e != hipSuccess) { p = nullptr; return e; }
-    cap = want;
+    hipError_t e = hipMalloc(&heap, want);
+    if (e != hipSuccess) { heap = nullptr; return e; }
+    p = heap;
+    cap = heap_cap = want;
     return hipSuccess;
+}
+
+void Pipeline::leave_arena() {
+    DevBuf *bufs[] = {&fasta_, &tile_nl_, &tile_off_, &nl_pos_, &scan_scratch_, &small_, &rec_start_, &rec_len_,
+                      &occ_count_, &word_count_, &occ_base_, &word_base_, &block_rec_, &packed_, &wmeta_, &rows_,
+                      &hist_, &bucket_, &cursor_, &hist1_, &base1_, &l1info_, &rec1_, &rec_, &surv_key_, &surv_mult_,
+                      &surv_first_, &dfo_, &sel_, &sel_target_, &sel_out_, &overflow_, &part_surv_, &part_first_,
+                      &gtab_key_, &gtab_cnt_, &gtab_fo_, &etab_, &etabS_};
+    for (DevBuf *b : bufs) b->leave_arena();
+    for (DevBuf &b : mb_) b.leave_arena();
+    for (DevBuf &b : ob_) b.leave_arena();
 }
 
 static uint64_t inv_odd(uint64_t a) {
@@ -1456,6 +1491,7 @@ int Pipeline::fail(int code, const char *msg) {
 
 int Pipeline::build(const uint8_t *d_in, uint64_t n) {
     SWAP_CHECK(hipSetDevice(dev_));
+    if (!current_arena()) leave_arena();  // an ordinary build never writes a static call's space
     n_reads = n_occ = n_distinct = n_surv = n_kept = n_overflow = n_special = 0;
     if (n == 0) return fail(-2, "empty input");
     const uint8_t *buf = d_in;

@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cstdio>
 #include <cstring>
@@ -89,6 +90,7 @@ struct swap_ctx {
     bool have_ce = false;
     int ce_rc = 0;
     swapk::CeFiles ce;
+    bool from_static = false;  // the last build ran in the static scratch (swap_assemble_static)
 };
 
 namespace {
@@ -209,6 +211,9 @@ int swap_create(const swap_cfg *cfg, swap_ctx **out) {
     c->pipe->set_table_spill((cfg->flags & SWAP_FLAG_TABLE_SPILL) != 0);
     c->pipe->set_wide_parts((cfg->flags & SWAP_FLAG_WIDE_PARTITIONS) != 0);
     c->pipe->set_single_fill((cfg->flags & SWAP_FLAG_SINGLE_FILL) != 0);
+    // the static scratch's device address, once per device (the Pipeline made
+    // this device current); a failure only disables swap_assemble_static
+    (void)swapk::static_scratch_setup(cfg->device);
     *out = c;
     return SWAP_OK;
 }
@@ -218,6 +223,7 @@ void swap_destroy(swap_ctx *c) { delete c; }
 int swap_build_device(swap_ctx *c, const void *d_fasta, size_t n) {
     if (!c || (!d_fasta && n)) return SWAP_E_INVAL;
     c->built = c->merged = c->have_graph = c->have_compacted = c->have_ce = false;
+    c->from_static = false;
     if (n == 0) return set_err(c, SWAP_E_EMPTY, "empty input");
     auto t0 = clk::now();
     int rc = c->pipe->build(static_cast<const uint8_t *>(d_fasta), n);
@@ -242,6 +248,8 @@ int swap_merge(swap_ctx *c) {
     if (!c) return SWAP_E_INVAL;
     if (!c->built) return set_err(c, SWAP_E_STATE, "merge before build");
     if (c->cfg.flags & SWAP_FLAG_KMER_GRAPH) return set_err(c, SWAP_E_STATE, "context was created for -s (no merge)");
+    if (c->from_static && !swapk::current_arena())
+        return set_err(c, SWAP_E_STATE, "the last build ran in the static scratch and is merged already");
     if (c->cfg.k < swapk::kMergeMinK)
         return set_err(c, SWAP_E_INVAL, "merge needs k >= 19 (the reference's range, sequence.cpp:44); k-mer graphs "
                                          "of shorter k are built with SWAP_FLAG_KMER_GRAPH");
@@ -254,6 +262,40 @@ int swap_merge(swap_ctx *c) {
     c->have_compacted = mat;
     c->have_ce = false;
     return SWAP_OK;
+}
+
+int swap_assemble_static(swap_ctx *c, const void *d_fasta, size_t n) {
+    static std::atomic<uint64_t> epochs{0};
+    if (!c || (!d_fasta && n)) return SWAP_E_INVAL;
+    c->built = c->merged = c->have_graph = c->have_compacted = c->have_ce = false;
+    c->from_static = false;
+    if (n == 0) return set_err(c, SWAP_E_EMPTY, "empty input");
+    if (n > swapk::kStaticMaxInput)
+        return set_err(c, SWAP_E_TOOBIG,
+                       "input of " + std::to_string(n) + " bytes exceeds the static scratch's limit of " +
+                           std::to_string(swapk::kStaticMaxInput) + " bytes (use swap_build_device)");
+    const bool merge = !(c->cfg.flags & SWAP_FLAG_KMER_GRAPH);
+    if (merge && c->cfg.k < swapk::kMergeMinK)
+        return set_err(c, SWAP_E_INVAL, "merge needs k >= 19 (the reference's range, sequence.cpp:44)");
+    void *base = swapk::static_scratch(c->cfg.device);
+    if (!base) return set_err(c, SWAP_E_DEVICE, "the static scratch's address could not be looked up on this device");
+    swapk::Arena arena;
+    arena.base = static_cast<uint8_t *>(base);
+    arena.size = swapk::kStaticScratchBytes;
+    arena.epoch = ++epochs;
+    int rc;
+    {
+        swapk::ArenaScope scope(&arena);
+        rc = swap_build_device(c, d_fasta, n);
+        c->from_static = true;
+        if (rc == SWAP_OK && merge) rc = swap_merge(c);
+    }
+    if (rc != SWAP_OK && arena.wanted > arena.size)
+        return set_err(c, SWAP_E_TOOBIG,
+                       "the build needs more device memory than the static scratch holds (" +
+                           std::to_string(arena.wanted) + " bytes asked for, " +
+                           std::to_string(swapk::kStaticScratchBytes) + " held)");
+    return rc;
 }
 
 int swap_get_stats(const swap_ctx *c, swap_stats *s) {

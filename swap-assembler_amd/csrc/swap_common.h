@@ -21,6 +21,14 @@ constexpr int kPartBitsMax = 20;
 // 19..31 (sequence.cpp:44).  Shorter k-mer graphs are built (-s) but not merged.
 constexpr int kMergeMinK = 19;
 
+// The static device scratch of swap_assemble_static (one per GPU, held until
+// the process ends) and the largest FASTA input that entry point accepts: the
+// buffers of a build and merge come to about 130 device bytes per input byte
+// on the densest inputs (k = 19, 250-base reads at 2% errors, c = 1: most
+// k-mers survive and are merged), plus ~40 MB that do not scale.
+constexpr size_t kStaticScratchBytes = 512ull << 20;
+constexpr size_t kStaticMaxInput = 3ull << 20;
+
 SWAP_HD uint64_t kmask(int k) { return (k >= 32) ? ~0ull : ((1ull << (2 * k)) - 1); }
 
 // Reverse complement of a 2k-bit k-mer in O(1): complement every 2-bit group,

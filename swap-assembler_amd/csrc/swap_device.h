@@ -10,12 +10,40 @@
 
 namespace swapk {
 
-// Grow-only device buffer.
+// A bump allocator over a fixed device region (the static scratch of
+// swap_assemble_static): while an ArenaScope is alive on a host thread, every
+// DevBuf::reserve on that thread takes fresh space from it instead of calling
+// hipMalloc; running out of space is reported, nothing is allocated.
+struct Arena {
+    uint8_t *base = nullptr;
+    size_t size = 0;
+    size_t used = 0;
+    size_t wanted = 0;   // bytes asked for in all (more than size after an overflow)
+    uint64_t epoch = 0;  // distinguishes the space of one call from the next
+};
+Arena *current_arena();
+struct ArenaScope {
+    explicit ArenaScope(Arena *a);
+    ~ArenaScope();
+    Arena *prev;
+};
+
+// Grow-only device buffer.  Outside an arena its space comes from hipMalloc
+// (kept in heap); inside one, from the arena.  A buffer keeps pointing at
+// arena space after the scope ends (results of a static call stay readable
+// until the next reserve that needs more room) until leave_arena().
 struct DevBuf {
     void *p = nullptr;
     size_t cap = 0;
+    void *heap = nullptr;
+    size_t heap_cap = 0;
+    uint64_t arena_epoch = 0;  // nonzero: p is arena space of that call
     ~DevBuf();
     hipError_t reserve(size_t bytes);
+    // back to the buffer's own device memory (nothing is freed or allocated)
+    void leave_arena() {
+        if (arena_epoch) { p = heap; cap = heap_cap; arena_epoch = 0; }
+    }
     template <typename T> T *as() const { return static_cast<T *>(p); }
 };
 
@@ -134,6 +162,8 @@ public:
 
     int k() const { return k_; }
     int c() const { return c_; }
+    // Every device buffer back to its own memory (after calls that used an arena).
+    void leave_arena();
     // Device time of the build phases of the last build, from events recorded
     // on the pipeline stream around the launches: 0 record scan, 1 pack,
     // 2 partition histogram, 3 level-1 partition, 4 LDS count + cutoff,
@@ -188,5 +218,10 @@ private:
 };
 
 const char *error_string(int code);
+
+// The static scratch (swap_static.hip): looked up once per device with that
+// device current; null until then or if the lookup failed.
+hipError_t static_scratch_setup(int device);
+void *static_scratch(int device);
 
 }  // namespace swapk

@@ -28,7 +28,8 @@ SWAP_FILE_JUNG_MUL = 6
 
 # exported entry points and their signatures (mirrors include/swap_hip.h)
 EXPORTS = [
-    "swap_create", "swap_destroy", "swap_build", "swap_build_device", "swap_merge", "swap_get_stats",
+    "swap_create", "swap_destroy", "swap_build", "swap_build_device", "swap_merge", "swap_assemble_static",
+    "swap_get_stats",
     "swap_export_kmer_graph", "swap_format_file", "swap_write_file", "swap_strerror", "swap_last_error",
     "swap_table_order", "swap_visit_order", "swap_table_epochs", "swap_reconfigure", "swap_export_compacted",
     "swap_server_create", "swap_server_submit", "swap_server_submit_batch", "swap_server_wait",
@@ -80,6 +81,8 @@ def lib():
         L.swap_build_device.restype = ctypes.c_int
         L.swap_merge.argtypes = [ctypes.c_void_p]
         L.swap_merge.restype = ctypes.c_int
+        L.swap_assemble_static.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t]
+        L.swap_assemble_static.restype = ctypes.c_int
         L.swap_get_stats.argtypes = [ctypes.c_void_p, ctypes.POINTER(SwapStats)]
         L.swap_get_stats.restype = ctypes.c_int
         L.swap_export_kmer_graph.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t]
@@ -183,6 +186,11 @@ class Assembler:
 
     def merge(self):
         self._check(lib().swap_merge(self._h))
+
+    def assemble_static(self, ptr: int, n: int):
+        """Build (and merge, unless kmer_graph_only) device-resident FASTA bytes
+        in the library's static scratch: no device allocation in the call."""
+        self._check(lib().swap_assemble_static(self._h, ctypes.c_void_p(ptr), n))
 
     def stats(self):
         s = SwapStats()
