@@ -1,0 +1,75 @@
+"""Writes tests/golden/np1_batch.json: digests of what the unmodified reference,
+run as one rank (oracle/_ref/ref_swap1, see make_golden_np1.py), writes for every
+sample of the many-sample workload (256 genomes of 20 kbp - 2 Mbp, 150-base
+reads at 20-80x, k=31, c=4) and for the first requests of the request-stream
+workload (genomes of 50 kbp - 5 Mbp, 100/150/250-base reads, k in {21,25,31},
+c=3), all from the seeded generator (swap-assembler_amd/bin/swap_gen).
+
+Run in the build container (needs /root/reference built by `make -C oracle ref`):
+
+    python tests/golden/make_golden_batch.py [--jobs 6] [--requests 64]
+
+Each input runs once with -d: contigGraph.txt, noCEcontigs.fasta and
+CEContig.fasta are recorded as size + sha256.  Inputs are generated into a
+temporary directory outside the repository and deleted after use.
+"""
+import argparse
+import concurrent.futures as cf
+import hashlib
+import json
+import os
+import shutil
+import subprocess
+import sys
+import tempfile
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+REPO = os.path.dirname(os.path.dirname(HERE))
+REF1 = os.path.join(REPO, "oracle", "_ref", "ref_swap1")
+GEN = os.path.join(REPO, "swap-assembler_amd", "bin", "swap_gen")
+MPIRUN = "/opt/conda/bin/mpirun"
+FILES = [("contigGraph.txt", "contigGraph"), ("noCEcontigs.fasta", "noce"), ("CEContig.fasta", "cecontig")]
+
+
+def one(workload, index, scratch):
+    d = tempfile.mkdtemp(prefix="%s%d_" % (workload, index), dir=scratch)
+    fa = os.path.join(d, "in.fa")
+    g = subprocess.run([GEN, "--workload", workload, "--index", str(index), "-o", fa], check=True,
+                       capture_output=True, text=True)
+    k, c = (int(x) for x in g.stdout.split()[:2])
+    out = os.path.join(d, "out")
+    subprocess.run([MPIRUN, "-np", "1", REF1, "-k", str(k), "-c", str(c), "-i", fa, "-o", out, "-d"], check=True,
+                   stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL, timeout=3600)
+    rec = {"k": k, "c": c, "bytes": os.path.getsize(fa), "files": {}}
+    for fname, suffix in FILES:
+        data = open(os.path.join(out, fname), "rb").read()
+        rec["files"][suffix] = {"size": len(data), "sha256": hashlib.sha256(data).hexdigest()}
+    shutil.rmtree(d)
+    return workload, index, rec
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--jobs", type=int, default=6)
+    ap.add_argument("--samples", type=int, default=256)
+    ap.add_argument("--requests", type=int, default=64)
+    args = ap.parse_args()
+    if not os.path.exists(REF1):
+        sys.exit("build the reference first: make -C oracle ref")
+    scratch = tempfile.mkdtemp(prefix="swap_batch_")  # outside the repository
+    out = {"batch": {}, "request": {}}
+    work = [("batch", i) for i in range(args.samples)] + [("request", i) for i in range(args.requests)]
+    try:
+        with cf.ThreadPoolExecutor(args.jobs) as ex:
+            for n, (wl, i, rec) in enumerate(ex.map(lambda w: one(w[0], w[1], scratch), work)):
+                out[wl][str(i)] = rec
+                print("%s %d: %d bytes k=%d c=%d (%d/%d)" % (wl, i, rec["bytes"], rec["k"], rec["c"], n + 1,
+                                                            len(work)), flush=True)
+    finally:
+        shutil.rmtree(scratch)
+    with open(os.path.join(HERE, "np1_batch.json"), "w") as f:
+        json.dump(out, f, indent=0, sort_keys=True)
+
+
+if __name__ == "__main__":
+    main()
