@@ -77,6 +77,25 @@ typedef struct swap_cfg {
     unsigned flags; /* SWAP_FLAG_* */
 } swap_cfg;
 
+/* Performance parameters of a build and merge (no counterpart in the reference,
+ * whose container and hash are fixed, graph.cpp:20-34, 353-354).  None of them
+ * changes a result, only the time it takes:
+ *   part_recs     records per count partition the partition count aims at,
+ *                 which sets the load of the on-chip counting table (1024..65536)
+ *   fill_recs     records a shared fill of the on-chip table aims at on small
+ *                 inputs (512..8192)
+ *   index_load    largest load factor of a survivor-index region, percent (50..85)
+ *   ruler_stride  one ruler per this many oriented nodes in the list ranking (4..256)
+ *   occ_groups    workgroups per compute unit of the k-mer passes (1..4)
+ * Values outside the ranges are clamped. */
+typedef struct swap_tuning {
+    uint32_t part_recs;
+    uint32_t fill_recs;
+    uint32_t index_load;
+    uint32_t ruler_stride;
+    uint32_t occ_groups;
+} swap_tuning;
+
 typedef struct swap_stats {
     uint64_t reads;        /* records with at least k bases */
     uint64_t kmers;        /* input k-mer occurrences (N) */
@@ -126,6 +145,17 @@ int swap_merge(swap_ctx *ctx);
 int swap_assemble_static(swap_ctx *ctx, const void *d_fasta, size_t n);
 
 int swap_get_stats(const swap_ctx *ctx, swap_stats *out);
+
+/* The tuning a build of input_bytes of FASTA takes from the autotuned table
+ * (swap-assembler_amd/tune/swap_tune.txt beside the library's directory, or the
+ * file named by the SWAP_TUNE_FILE environment variable; built-in defaults
+ * without one).  Needs no device.  No counterpart in the reference. */
+int swap_tuning_for(uint64_t input_bytes, swap_tuning *out);
+/* Fixes the tuning of the context's next builds (NULL: back to the table).
+ * Used by the autotuner; no counterpart in the reference. */
+int swap_set_tuning(swap_ctx *ctx, const swap_tuning *t);
+/* The tuning the context's last build used. */
+int swap_get_tuning(const swap_ctx *ctx, swap_tuning *out);
 
 /* Survivors of the cutoff sorted by key: keys[i] and 8 multiplicities
  * mult[8*i .. 8*i+7] (slot t = "X followed by base t" for t<4, "rc(X) followed

@@ -910,17 +910,13 @@ __device__ void table_emit(const uint64_t *tkey, CNT cnt_of, const uint32_t *tfo
 constexpr int kPre = 6;
 // Table fills per workgroup whose bounds are staged in LDS (grid >= fills / this).
 constexpr uint32_t kMaxFills = 512;
-// Partitions that may share one table fill (2^kMaxFillBits), and the records a
-// shared fill aims at: at most one record per sub-table slot on average.
-// Reads at 20x coverage or more carry one distinct key per 3-5 records, so
-// sub-tables run under a third full (2048, half a slot per record, measured
-// 4% fewer samples/s on the many-sample workload); an input of nearly all
-// distinct keys overflows shared fills to the device-memory table instead.
+// Partitions that may share one table fill (2^kMaxFillBits).  The records a
+// shared fill aims at are tuned (Tuning::fill_recs, 4096 by default: one record
+// per sub-table slot on average; reads at 20x coverage or more carry one
+// distinct key per 3-5 records, so sub-tables run under a third full); an
+// input of nearly all distinct keys overflows shared fills to the
+// device-memory table instead.
 constexpr int kMaxFillBits = 3;
-constexpr uint64_t kFillRecs = 4096;
-// Records per partition the partition count aims at (distinct keys are a
-// fraction of them; the LDS table holds 4096).
-constexpr uint64_t kTargetPartRecs = 8192;
 
 // Persistent and software-pipelined: one workgroup per CU walks the table
 // fills blockIdx.x, blockIdx.x + gridDim.x, ...; fill f counts partitions
@@ -1581,12 +1577,13 @@ int Pipeline::build(const uint8_t *d_in, uint64_t n) {
     oc.mx = mx_;
     int ncu = 0;
     SWAP_CHECK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev_));
-    const unsigned occ_grid = (unsigned)std::max(8, ncu);
+    const unsigned occ_grid = (unsigned)std::max(8, ncu * (int)tune_.occ_groups);
     const uint64_t n1 = (uint64_t)kL1 * occ_grid;
-    // partition bits: about kTargetPartRecs records per partition, so that its
-    // distinct keys fit the 4096-slot LDS table; at most 2k (the key bits)
+    // partition bits: about part_recs records per partition (8192 by default),
+    // so that its distinct keys fit the 4096-slot LDS table; at most 2k (the key
+    // bits)
     int pbits = kPartBitsMin;
-    while (pbits < kPartBitsMax && pbits < 2 * k_ && (N >> pbits) > kTargetPartRecs) pbits++;
+    while (pbits < kPartBitsMax && pbits < 2 * k_ && (N >> pbits) > tune_.part_recs) pbits++;
     if (wide_parts_) pbits = std::max(pbits, std::min(kPartBitsMax - 2, 2 * k_));
     pbits_ = pbits;
     const int l2bits = pbits - kL1Bits;
@@ -1700,9 +1697,9 @@ int Pipeline::build(const uint8_t *d_in, uint64_t n) {
     ca.spill = table_spill_ ? 1 : 0;
     ca.dfo = ref_order_ ? dfo_.as<uint32_t>() : nullptr;
     // small partitions share table fills: 2^gbits of them per fill, while a
-    // fill of average partitions stays within kFillRecs records
+    // fill of average partitions stays within fill_recs records (4096 by default)
     int gbits = 0;
-    while (!single_fill_ && gbits < kMaxFillBits && gbits < l2bits && ((N >> pbits) << (gbits + 1)) <= kFillRecs)
+    while (!single_fill_ && gbits < kMaxFillBits && gbits < l2bits && ((N >> pbits) << (gbits + 1)) <= tune_.fill_recs)
         gbits++;
     ca.gbits = gbits;
     const uint64_t nfills = nparts >> gbits;

@@ -91,6 +91,7 @@ struct swap_ctx {
     int ce_rc = 0;
     swapk::CeFiles ce;
     bool from_static = false;  // the last build ran in the static scratch (swap_assemble_static)
+    bool tune_fixed = false;   // swap_set_tuning: the table is not consulted
 };
 
 namespace {
@@ -225,6 +226,7 @@ int swap_build_device(swap_ctx *c, const void *d_fasta, size_t n) {
     c->built = c->merged = c->have_graph = c->have_compacted = c->have_ce = false;
     c->from_static = false;
     if (n == 0) return set_err(c, SWAP_E_EMPTY, "empty input");
+    if (!c->tune_fixed) c->pipe->set_tuning(swapk::tuning_for(n));
     auto t0 = clk::now();
     int rc = c->pipe->build(static_cast<const uint8_t *>(d_fasta), n);
     c->build_ms = std::chrono::duration<double, std::milli>(clk::now() - t0).count();
@@ -296,6 +298,43 @@ int swap_assemble_static(swap_ctx *c, const void *d_fasta, size_t n) {
                            std::to_string(arena.wanted) + " bytes asked for, " +
                            std::to_string(swapk::kStaticScratchBytes) + " held)");
     return rc;
+}
+
+static swap_tuning to_c(const swapk::Tuning &t) {
+    swap_tuning o;
+    o.part_recs = t.part_recs;
+    o.fill_recs = t.fill_recs;
+    o.index_load = t.index_load;
+    o.ruler_stride = t.ruler_stride;
+    o.occ_groups = t.occ_groups;
+    return o;
+}
+
+int swap_tuning_for(uint64_t input_bytes, swap_tuning *out) {
+    if (!out) return SWAP_E_INVAL;
+    *out = to_c(swapk::tuning_for(input_bytes));
+    return SWAP_OK;
+}
+
+int swap_set_tuning(swap_ctx *c, const swap_tuning *t) {
+    if (!c) return SWAP_E_INVAL;
+    c->tune_fixed = t != nullptr;
+    if (t) {
+        swapk::Tuning x;
+        x.part_recs = t->part_recs;
+        x.fill_recs = t->fill_recs;
+        x.index_load = t->index_load;
+        x.ruler_stride = t->ruler_stride;
+        x.occ_groups = t->occ_groups;
+        c->pipe->set_tuning(x);
+    }
+    return SWAP_OK;
+}
+
+int swap_get_tuning(const swap_ctx *c, swap_tuning *out) {
+    if (!c || !out) return SWAP_E_INVAL;
+    *out = to_c(c->pipe->tuning());
+    return SWAP_OK;
 }
 
 int swap_get_stats(const swap_ctx *c, swap_stats *s) {

@@ -28,6 +28,26 @@ struct ArenaScope {
     Arena *prev;
 };
 
+// Performance parameters of a build and merge; none of them changes a result.
+//   part_recs     records per count partition the partition count aims at
+//                 (sets the load of the on-chip counting table)
+//   fill_recs     records a shared fill of the on-chip table aims at (small inputs)
+//   index_load    largest load of a survivor-index region, in percent (50..85)
+//   ruler_stride  one ruler per this many oriented nodes in the list ranking
+//   occ_groups    workgroups per CU of the k-mer passes (histogram, level-1 partition)
+struct Tuning {
+    uint32_t part_recs = 8192;
+    uint32_t fill_recs = 4096;
+    uint32_t index_load = 67;
+    uint32_t ruler_stride = 32;
+    uint32_t occ_groups = 1;
+};
+// Clamped to the ranges the kernels support.
+Tuning clamp_tuning(const Tuning &t);
+// The table entry for an input of n bytes (the table file found next to the
+// library, or SWAP_TUNE_FILE; built-in defaults without one).
+Tuning tuning_for(uint64_t n_bytes);
+
 // Grow-only device buffer.  Outside an arena its space comes from hipMalloc
 // (kept in heap); inside one, from the arena.  A buffer keeps pointing at
 // arena space after the scope ends (results of a static call stay readable
@@ -130,6 +150,8 @@ public:
     void set_wide_parts(bool on) { wide_parts_ = on; }
     // one partition per on-chip table fill on any input (a testing aid)
     void set_single_fill(bool on) { single_fill_ = on; }
+    void set_tuning(const Tuning &t) { tune_ = clamp_tuning(t); }
+    const Tuning &tuning() const { return tune_; }
     int device() const { return dev_; }
     hipStream_t stream() const { return st_; }
 
@@ -183,6 +205,7 @@ private:
     bool table_spill_ = false;
     bool wide_parts_ = false;
     bool single_fill_ = false;
+    Tuning tune_;
     int pbits_ = kPartBitsMin;  // partition bits of the last build
     hipStream_t st_ = nullptr;
     hipError_t init_err_ = hipSuccess;

@@ -47,9 +47,10 @@ constexpr uint64_t kIdxEmpty = ~0ull;
 
 // The survivors of count partition p (p = top pbits bits of the mixed key,
 // the partition the count kernel took them from) own one open-addressing
-// region of entries [base[p], base[p+1]); its size is a power of two >= 1.5x
-// their number (0 for a partition without survivors).  An entry is 16 bytes
-// (key, survivor id), read by one load.
+// region of entries [base[p], base[p+1]); its size is the power of two that
+// holds them at no more than the tuned load (67% by default; 0 for a partition
+// without survivors).  An entry is 16 bytes (key, survivor id), read by one
+// load.
 
 struct SurvIndex {
     const uint64_t *ent;   // 2 words per entry: key (kIdxEmpty if free), survivor id
@@ -58,15 +59,19 @@ struct SurvIndex {
     int shift;             // 2k - pbits
 };
 
-__device__ __forceinline__ uint32_t region_size(uint32_t ns) {
+// Region of ns survivors: the smallest power of two holding ns at the tuned
+// load (at least ns + 1 entries: a probe always meets a free entry).
+// mul256 = 256 / load.
+__device__ __forceinline__ uint32_t region_size(uint32_t ns, uint32_t mul256) {
     if (!ns) return 0;
-    const uint32_t want = ns + (ns >> 1) + 1;
+    const uint32_t want = (uint32_t)(((uint64_t)ns * mul256) >> 8) + 1;
     return 1u << (32 - __clz(want - 1));
 }
 
-__global__ void k_index_sizes(const uint32_t *__restrict__ part_surv, uint32_t nparts, uint32_t *__restrict__ size) {
+__global__ void k_index_sizes(const uint32_t *__restrict__ part_surv, uint32_t nparts, uint32_t mul256,
+                              uint32_t *__restrict__ size) {
     const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p < nparts) size[p] = region_size(part_surv[p]);
+    if (p < nparts) size[p] = region_size(part_surv[p], mul256);
 }
 
 // Regions built in LDS, one workgroup per partition whose survivors form one
@@ -224,11 +229,11 @@ struct Walk {
 };
 
 // Ruling-set list ranking.  Rulers: the first oriented node of every walk
-// that starts at a kept node, plus a hashed 1/kRuleStride sample of the rest
-// (so cycles and long chains are cut into short pieces).  Each ruler walks to
-// the next ruler or to a kept node, labelling the nodes it passes with
-// (ruler, offset); pointer jumping then runs over the rulers only.
-constexpr uint32_t kRuleStride = 32;
+// that starts at a kept node, plus a hashed 1/stride sample of the rest (so
+// cycles and long chains are cut into short pieces; the stride is tuned, 32 by
+// default).  Each ruler walks to the next ruler or to a kept node, labelling
+// the nodes it passes with (ruler, offset); pointer jumping then runs over the
+// rulers only.
 
 // props[v]: bit 0 (1,1) node, bit 1 ruler, bit 2 self-adjacent, so that
 // k_rs_prep reads one byte of each successor (and the ruler numbering one byte
@@ -770,7 +775,9 @@ int Pipeline::merge_device(bool reference_order, MergeDev &out) {
     unsigned long long *cnt = (unsigned long long *)(d_small + 16);
     // --- survivor index: one region per count partition
     const uint64_t nparts = 1ull << pbits_;
-    const uint64_t ent_bound = 3 * S + 2 * nparts;  // region_size(n) <= 3n + 2
+    // 256 / load, rounded up so that the load is never exceeded
+    const uint32_t mul256 = (25600u + tune_.index_load - 1) / tune_.index_load;
+    const uint64_t ent_bound = 2 * ((S * mul256) >> 8) + 2 * nparts + 2;  // region_size(n) <= 2 (n mul + 1)
     MCHECK(mb_[M_IBASE].reserve((2 * nparts + 1) * 8));
     MCHECK(mb_[M_IENT].reserve(ent_bound * 16));
     MCHECK(scan_scratch_.reserve((scan_scratch_words(nparts) + 16) * 8));
@@ -783,7 +790,7 @@ int Pipeline::merge_device(bool reference_order, MergeDev &out) {
     ix.shift = 2 * k_ - pbits_;
     phase_begin(8);
     hipLaunchKernelGGL(k_index_sizes, dim3((unsigned)(nparts / 256)), dim3(256), 0, st_, part_surv_.as<uint32_t>(),
-                       (uint32_t)nparts, isize);
+                       (uint32_t)nparts, mul256, isize);
     MCHECK(scan_excl<uint32_t>(isize, ibase, nparts, scan_scratch_.as<uint64_t>(), ibase + nparts, st_));
     hipLaunchKernelGGL(k_index_part, dim3((unsigned)nparts), dim3(kIdxThreads), 0, st_, surv_key_.as<uint64_t>(),
                        part_surv_.as<uint32_t>(), part_first_.as<uint32_t>(), ix, mb_[M_IENT].as<uint64_t>());
@@ -837,7 +844,7 @@ int Pipeline::merge_device(bool reference_order, MergeDev &out) {
     MCHECK(mb_[M_PROPS].reserve(2 * S));
     hipLaunchKernelGGL(k_rs_mark, dim3(nblk(2 * S)), dim3(256), 0, st_, mb_[M_IS11].as<uint8_t>(),
                        mb_[M_NXT].as<uint32_t>(), mb_[M_SELF].as<uint8_t>(), S, mb_[M_PROPS].as<uint8_t>(),
-                       kRuleStride, label);
+                       tune_.ruler_stride, label);
     const RulerPred ruler{mb_[M_PROPS].as<uint8_t>()};
     MCHECK(compact_count_scan(2 * S, ruler, mb_[M_BCNT].as<uint32_t>(), mb_[M_BOFF].as<uint64_t>(),
                               scan_scratch_.as<uint64_t>(), d_small + 26, st_));

@@ -29,7 +29,7 @@ SWAP_FILE_JUNG_MUL = 6
 # exported entry points and their signatures (mirrors include/swap_hip.h)
 EXPORTS = [
     "swap_create", "swap_destroy", "swap_build", "swap_build_device", "swap_merge", "swap_assemble_static",
-    "swap_get_stats",
+    "swap_get_stats", "swap_tuning_for", "swap_set_tuning", "swap_get_tuning",
     "swap_export_kmer_graph", "swap_format_file", "swap_write_file", "swap_strerror", "swap_last_error",
     "swap_table_order", "swap_visit_order", "swap_table_epochs", "swap_reconfigure", "swap_export_compacted",
     "swap_server_create", "swap_server_submit", "swap_server_submit_batch", "swap_server_wait",
@@ -56,6 +56,14 @@ class SwapStats(ctypes.Structure):
 class SwapJob(ctypes.Structure):
     _fields_ = [("fasta", ctypes.c_void_p), ("n", ctypes.c_size_t), ("k", ctypes.c_int), ("c", ctypes.c_int),
                 ("flags", ctypes.c_uint), ("outputs", ctypes.c_uint), ("job_flags", ctypes.c_uint)]
+
+
+class SwapTuning(ctypes.Structure):
+    _fields_ = [("part_recs", ctypes.c_uint32), ("fill_recs", ctypes.c_uint32), ("index_load", ctypes.c_uint32),
+                ("ruler_stride", ctypes.c_uint32), ("occ_groups", ctypes.c_uint32)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
 
 
 class SwapJobInfo(ctypes.Structure):
@@ -85,6 +93,12 @@ def lib():
         L.swap_assemble_static.restype = ctypes.c_int
         L.swap_get_stats.argtypes = [ctypes.c_void_p, ctypes.POINTER(SwapStats)]
         L.swap_get_stats.restype = ctypes.c_int
+        L.swap_tuning_for.argtypes = [ctypes.c_uint64, ctypes.POINTER(SwapTuning)]
+        L.swap_tuning_for.restype = ctypes.c_int
+        L.swap_set_tuning.argtypes = [ctypes.c_void_p, ctypes.POINTER(SwapTuning)]
+        L.swap_set_tuning.restype = ctypes.c_int
+        L.swap_get_tuning.argtypes = [ctypes.c_void_p, ctypes.POINTER(SwapTuning)]
+        L.swap_get_tuning.restype = ctypes.c_int
         L.swap_export_kmer_graph.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t]
         L.swap_export_kmer_graph.restype = ctypes.c_int64
         L.swap_format_file.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_size_t]
@@ -191,6 +205,20 @@ class Assembler:
         """Build (and merge, unless kmer_graph_only) device-resident FASTA bytes
         in the library's static scratch: no device allocation in the call."""
         self._check(lib().swap_assemble_static(self._h, ctypes.c_void_p(ptr), n))
+
+    def set_tuning(self, tuning=None):
+        """Fixes the performance parameters of the next builds (a dict with the
+        SwapTuning fields); None goes back to the autotuned table."""
+        if tuning is None:
+            self._check(lib().swap_set_tuning(self._h, None))
+        else:
+            t = SwapTuning(**tuning)
+            self._check(lib().swap_set_tuning(self._h, ctypes.byref(t)))
+
+    def tuning(self):
+        t = SwapTuning()
+        self._check(lib().swap_get_tuning(self._h, ctypes.byref(t)))
+        return t.as_dict()
 
     def stats(self):
         s = SwapStats()
@@ -330,6 +358,15 @@ class Server:
     def release(self, jid):
         lib().swap_server_release(self._h, jid)
         self._keep.pop(jid, None)
+
+
+def tuning_for(input_bytes):
+    """The tuning a build of that many FASTA bytes takes from the autotuned table."""
+    t = SwapTuning()
+    rc = lib().swap_tuning_for(input_bytes, ctypes.byref(t))
+    if rc:
+        raise SwapError(rc)
+    return t.as_dict()
 
 
 def contig_extension(k, survivors, contig_graph: bytes, which):

@@ -10,7 +10,8 @@ Run in the build container (needs /root/reference built by `make -C oracle ref`)
     python tests/golden/make_golden_batch.py [--jobs 6] [--requests 64]
 
 Each input runs once with -d: contigGraph.txt, noCEcontigs.fasta and
-CEContig.fasta are recorded as size + sha256.  Inputs are generated into a
+CEContig.fasta are recorded as size + sha256, beside k, c, the input's size
+and its read length.  Inputs are generated into a
 temporary directory outside the repository and deleted after use.
 """
 import argparse
@@ -40,7 +41,10 @@ def one(workload, index, scratch):
     out = os.path.join(d, "out")
     subprocess.run([MPIRUN, "-np", "1", REF1, "-k", str(k), "-c", str(c), "-i", fa, "-o", out, "-d"], check=True,
                    stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL, timeout=3600)
-    rec = {"k": k, "c": c, "bytes": os.path.getsize(fa), "files": {}}
+    with open(fa, "rb") as f:
+        f.readline()
+        read_len = len(f.readline().rstrip(b"\n"))
+    rec = {"k": k, "c": c, "bytes": os.path.getsize(fa), "read_len": read_len, "files": {}}
     for fname, suffix in FILES:
         data = open(os.path.join(out, fname), "rb").read()
         rec["files"][suffix] = {"size": len(data), "sha256": hashlib.sha256(data).hexdigest()}

@@ -219,3 +219,26 @@ def check_single_rank_reference(name, fa):
 @pytest.mark.parametrize("name", NP1_SMALL)
 def test_single_rank_reference(name):
     check_single_rank_reference(name, np1_input(name))
+
+
+@pytest.mark.parametrize("tuning", [
+    {"part_recs": 1024, "fill_recs": 512, "index_load": 50, "ruler_stride": 4, "occ_groups": 4},
+    {"part_recs": 65536, "fill_recs": 8192, "index_load": 85, "ruler_stride": 256, "occ_groups": 1},
+    {"part_recs": 4096, "fill_recs": 2048, "index_load": 75, "ruler_stride": 8, "occ_groups": 2},
+])
+def test_tuning_never_changes_results(tuning):
+    """Every performance parameter at the ends of its range gives the same files."""
+    case = CASES["repeats_k25_c3"]
+    fa = gold_fa("repeats_k25_c3")
+    want = check_merge(fa, case["k"], case["c"])  # default tuning, checked against the oracle
+    o = Oracle(fa, case["k"], case["c"])
+    a = swapamd.Assembler(case["k"], case["c"])
+    a.set_tuning(tuning)
+    a.build(fa)
+    a.merge()
+    assert a.tuning() == tuning
+    assert a.stats()["kept"] == want["kept"]
+    assert a.file_bytes(swapamd.SWAP_FILE_CONTIG_GRAPH) == o.file(1)
+    assert a.file_bytes(swapamd.SWAP_FILE_NOCE) == o.file(2)
+    a.set_tuning(None)
+    a.close()
