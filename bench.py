@@ -12,8 +12,10 @@ the project's generator.  Prints one JSON line.
 roofline: the dominant device kernel of the step, its algorithmic bytes per
 launch (the words it must read and write once) over its device time measured
 with HIP events recorded on the library's own stream around the launch.
-cpu_baseline: the single-threaded CPU oracle (a restatement of the
-reference's np=1 path) on a bounded prefix of the same reads.
+cpu_baseline: the reference itself (built from its sources into oracle/_ref,
+where it was built) under mpirun on the host's cores, wall clock on the first
+quarter of the reads; without it, the single-threaded CPU oracle on a bounded
+prefix.
 """
 import argparse
 import json
@@ -81,6 +83,16 @@ def algorithmic_bytes(phase, st, words, ref_order):
     return 0
 
 
+BUILD_PHASES = (0, 1, 2, 3, 4, 5, 6, 7, 15)
+
+
+def build_bytes(st, fasta_bytes, read_len, k):
+    """Table-build algorithmic bytes F + 8N + 8E + 48D + 16S, E = 2(L-k) per read."""
+    N, S, D, R = st["kmers"], st["survivors"], st["distinct"], st["reads"]
+    E = 2 * R * max(read_len - k, 0)
+    return fasta_bytes + 8 * N + 8 * E + 48 * D + 16 * S
+
+
 def path_bytes(st, fasta_bytes, read_len, k, edge_bases):
     """Whole-path algorithmic bytes of the build+merge (word-granular unique traffic):
     F + 8N + 8E + 48D + 16S (build) + 16S + edge bases (merge), E = 2(L-k) per read."""
@@ -112,26 +124,41 @@ def find_mpirun():
     return None
 
 
-def reference_baseline(fa: bytes, k: int, c: int, frac=0.25, nprocs=8):
-    """The reference itself (built from its sources into oracle/_ref) under mpirun on the host cores.
+def host_cores():
+    """CPU cores this process may run on (the GPU box gives a share of its host)."""
+    try:
+        return len(os.sched_getaffinity(0))
+    except AttributeError:
+        return os.cpu_count() or 1
 
-    Timed like the reference reports itself: the sum of its build+merge phase
-    times from logtime.txt (Constructing Graph + cutoffGraph + Constructing
-    DistGraph + Simplify DistGraph, rank-0 clocks after barriers), on the
-    first `frac` of the reads.  Returns None if the binary or mpirun is absent.
-    """
+
+def reference_baseline(fa: bytes, k: int, c: int, frac=0.25):
+    """The reference itself (built from its sources into oracle/_ref) under mpirun
+    on the host cores, on the first `frac` of the reads, in its -p mode (the
+    process ends after the merge; graph.cpp:3591-3611).
+
+    value: input k-mers per second of wall-clock time around the whole mpirun
+    (process start, MPI start-up and file reading included).  Also reported: the
+    sum of the reference's own build+merge phase clocks from logtime.txt
+    (Constructing Graph + cutoffGraph + Constructing DistGraph + Simplify
+    DistGraph: rank-0 CPU time after barriers).  One rank per core, at least
+    4 (at 2 and 3 ranks the reference's merge overruns an MPI_Status array,
+    graph.cpp:1215/1230).  Returns None if the binary or mpirun is absent."""
     exe = os.path.join(REPO, "oracle", "_ref", "swap")
     mpirun = find_mpirun()
     if not os.path.exists(exe) or not mpirun:
         return None
+    nprocs = max(4, min(16, host_cores()))
     tmpdir = tempfile.mkdtemp(prefix="ref_base_")
     sample = sample_prefix(fa, frac)
     path = os.path.join(tmpdir, "s.fa")
     with open(path, "wb") as f:
         f.write(sample)
     out = os.path.join(tmpdir, "out")
+    t0 = time.perf_counter()
     p = subprocess.run([mpirun, "-np", str(nprocs), exe, "-k", str(k), "-c", str(c), "-i", path, "-o", out, "-p"],
                        capture_output=True, text=True, timeout=600)
+    wall = time.perf_counter() - t0
     log = os.path.join(out, "logtime.txt")
     if p.returncode != 0 or not os.path.exists(log):
         print("reference baseline failed (rc %d): %s" % (p.returncode, p.stderr[-400:]), file=sys.stderr)
@@ -144,12 +171,12 @@ def reference_baseline(fa: bytes, k: int, c: int, frac=0.25, nprocs=8):
             secs += float(m.group(2))
     kmers = count_kmers(sample, k)
     shutil.rmtree(tmpdir, ignore_errors=True)
-    if secs <= 0:
-        return None
-    return {"value": kmers / secs, "unit": "input k-mers/s", "cores": nprocs, "kind": "reference",
+    return {"value": kmers / wall, "unit": "input k-mers/s", "cores": nprocs, "kind": "reference",
+            "host_cores": host_cores(), "wall_s": wall, "phase_clock_s": secs,
+            "phase_clock_value": kmers / secs if secs > 0 else None,
             "sample": "first %.0f%% of the reads (%d input k-mers); reference swap -p under mpirun -np %d, "
-                      "build+merge phase times from its logtime.txt, %.1f s" % (
-                          100 * len(sample) / len(fa), kmers, nprocs, secs)}
+                      "wall clock around mpirun %.1f s (its own build+merge phase clocks: %.1f s)" % (
+                          100 * len(sample) / len(fa), kmers, nprocs, wall, secs)}
 
 
 def cpu_baseline(fa: bytes, k: int, c: int, target_s=15.0):
@@ -249,11 +276,24 @@ def batched_vs_per_sample(n_samples, slots, warmup=True):
     srv.close()
     for b in bufs:
         b.close()
+    # every sample's noCEcontigs.fasta against what the reference wrote for it
+    # running alone as one rank (digests from tests/golden/make_golden_batch.py)
+    ref_ok, ref_n = 0, 0
+    dpath = os.path.join(REPO, "tests", "golden", "np1_batch.json")
+    if os.path.exists(dpath):
+        import hashlib
+        with open(dpath) as f:
+            ref = json.load(f)["batch"]
+        for i in range(n_samples):
+            r = ref.get(str(i))
+            if r:
+                ref_n += 1
+                ref_ok += hashlib.sha256(plain[i]).hexdigest() == r["files"]["noce"]["sha256"]
     return {"workload": "batch", "samples": n_samples, "input_bytes": sum(len(f) for f in fas),
             "input_kmers": kmers, "k": k, "c": c, "slots": slots,
             "per_sample_samples_per_s": n_samples / t_plain, "batched_samples_per_s": n_samples / t_batch,
             "factor": t_plain / t_batch, "per_sample_s": t_plain, "batched_s": t_batch,
-            "identical_outputs": same}
+            "identical_outputs": same, "reference_checked": ref_n, "reference_identical": ref_ok}
 
 
 def serving(n_requests, rate, slots, seed=5):
@@ -366,6 +406,7 @@ def main():
     swapamd.hip().hipDeviceSynchronize()
     dt = time.perf_counter() - t0
     st = a.stats()
+    tuning = a.tuning()  # from the autotuned table (swap-assembler_amd/tune/swap_tune.txt)
     ms = 1000.0 * dt / args.steps
     N = st["kmers"]
     value = N / (dt / args.steps)
@@ -402,6 +443,12 @@ def main():
     pb = path_bytes(st, len(fa), read_len, k, edge_bases)
     roof["path"] = {"algorithmic_bytes": pb, "achieved": pb / (dt / args.steps) / 1e9, "unit": "GB/s",
                     "frac": pb / (dt / args.steps) / HBM_PEAK}
+    # the table build alone: F + 8N + 8E + 48D + 16S over the device time of the
+    # build phases (record scan through survivor keys)
+    bb = build_bytes(st, len(fa), read_len, k)
+    build_ms = sum(phases.get(i, 0.0) for i in BUILD_PHASES)
+    roof["build"] = {"algorithmic_bytes": bb, "ms": build_ms, "achieved": bb / (build_ms * 1e-3) / 1e9,
+                     "unit": "GB/s", "frac": bb / (build_ms * 1e-3) / HBM_PEAK}
     cpu = None
     if not args.no_cpu_baseline:
         cpu = reference_baseline(fa, k, c) or cpu_baseline(fa, k, c)
@@ -434,6 +481,7 @@ def main():
         "roofline": roof,
         "cpu_baseline": cpu,
         "stats": {kk: st[kk] for kk in ("kmers", "distinct", "survivors", "kept", "overflow", "special")},
+        "tuning": tuning,
         "phase_ms": {PHASES[i]: round(v, 4) for i, v in phases.items()},
         "host_input": host_input,
         "batched": batched,
