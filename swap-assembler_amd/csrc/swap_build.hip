@@ -1032,6 +1032,10 @@ __global__ void __launch_bounds__(kCountThreads) k_count_lds(CountArgs a) {
             }
             if (!ok) overflow[it & 1] = 1;
         }
+        // the next fill's records are requested as soon as this fill's are
+        // inserted: their loads are in flight through the output stores of the
+        // previous fill, this fill's table read-out and both barriers
+        if (fill + gridDim.x < nfills) prefetch(it + 1);
         if (w == 0 && have_prev && !prev_ovf) {
             // the reservation returns (distinct << 32) | survivors before this fill
             const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)pend);
@@ -1092,7 +1096,6 @@ __global__ void __launch_bounds__(kCountThreads) k_count_lds(CountArgs a) {
         pv = ((uint64_t)nd << 32) | ns;
         pinc = wave_incl_sum(pv);
         if (lane == 63) wsum[w] = pinc;
-        if (fill + gridDim.x < nfills) prefetch(it + 1);
         __syncthreads();  // wave sums complete; the table is empty
         if (w == 0) {
             // wave offsets: exclusive scan of the per-wave totals by wave 0

@@ -48,3 +48,33 @@ def test_cartesian_model_matches_sequential_merge(name):
         assert mults[s] == m, (X, s, mults[s], m)
         checked += 1
     assert checked > 0
+
+
+@pytest.mark.parametrize("k", [9, 13, 19, 31])
+def test_chain_bookkeeping_is_consistent(k):
+    """The device merge's bookkeeping, restated on the CPU, on a random genome:
+    every chain edge agrees with its other end and every chain array position
+    is written exactly once.  k = 9 is the case whose GPU run once stalled (in
+    an earlier version of the kernels): short k gives many branching nodes and
+    cycle / fold-back components."""
+    import swapamd
+    fa = swapamd.generate_params(15000, 100, 25.0, 0.01, 300 + k)
+    o = Oracle(fa, k, 2)
+    keys, mult = o.survivors()
+    problems, stats = merge_model.chain_bookkeeping(keys, mult, k)
+    assert problems == []
+    assert stats["chains"] > 0
+    if k == 9:
+        assert stats["special_edges"] > 0
+
+
+def test_chain_bookkeeping_sees_a_missing_node():
+    import swapamd
+    fa = swapamd.generate_params(5000, 100, 25.0, 0.0, 77)
+    o = Oracle(fa, 21, 2)
+    keys, mult = o.survivors()
+    one = [i for i in range(len(keys)) if (mult[i][:4] > 0).sum() == 1 and (mult[i][4:] > 0).sum() == 1]
+    drop = np.ones(len(keys), dtype=bool)
+    drop[one[len(one) // 2] + 1 if one[len(one) // 2] + 1 < len(keys) else 0] = False
+    problems, _ = merge_model.chain_bookkeeping(keys[drop], mult[drop], 21)
+    assert problems
