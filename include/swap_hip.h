@@ -87,6 +87,9 @@ typedef struct swap_cfg {
  *   index_load    largest load factor of a survivor-index region, percent (50..85)
  *   ruler_stride  one ruler per this many oriented nodes in the list ranking (4..256)
  *   occ_groups    workgroups per compute unit of the k-mer passes (1..4)
+ *   order_bin_keys  keys per bin the visit-order sort aims at on average; a
+ *                 bin is ordered in LDS, or in device memory above 6144 keys
+ *                 (256..2^30)
  * Values outside the ranges are clamped. */
 typedef struct swap_tuning {
     uint32_t part_recs;
@@ -94,6 +97,7 @@ typedef struct swap_tuning {
     uint32_t index_load;
     uint32_t ruler_stride;
     uint32_t occ_groups;
+    uint32_t order_bin_keys;
 } swap_tuning;
 
 typedef struct swap_stats {

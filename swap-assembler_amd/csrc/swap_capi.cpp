@@ -307,6 +307,7 @@ static swap_tuning to_c(const swapk::Tuning &t) {
     o.index_load = t.index_load;
     o.ruler_stride = t.ruler_stride;
     o.occ_groups = t.occ_groups;
+    o.order_bin_keys = t.order_bin_keys;
     return o;
 }
 
@@ -326,6 +327,7 @@ int swap_set_tuning(swap_ctx *c, const swap_tuning *t) {
         x.index_load = t->index_load;
         x.ruler_stride = t->ruler_stride;
         x.occ_groups = t->occ_groups;
+        x.order_bin_keys = t->order_bin_keys;
         c->pipe->set_tuning(x);
     }
     return SWAP_OK;

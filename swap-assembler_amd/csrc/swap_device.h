@@ -35,12 +35,15 @@ struct ArenaScope {
 //   index_load    largest load of a survivor-index region, in percent (50..85)
 //   ruler_stride  one ruler per this many oriented nodes in the list ranking
 //   occ_groups    workgroups per CU of the k-mer passes (histogram, level-1 partition)
+//   order_bin_keys  average keys per bin of the visit-order sort (a bin is
+//                 ordered in LDS up to 6144 keys, in device memory above)
 struct Tuning {
     uint32_t part_recs = 8192;
     uint32_t fill_recs = 4096;
     uint32_t index_load = 67;
     uint32_t ruler_stride = 32;
     uint32_t occ_groups = 1;
+    uint32_t order_bin_keys = 2048;
 };
 // Clamped to the ranges the kernels support.
 Tuning clamp_tuning(const Tuning &t);
@@ -164,8 +167,6 @@ public:
     int merge_device(bool reference_order, MergeDev &out);
     int fetch_merge(MergeDev &out);
     int visit_priorities(uint64_t *d_prio);
-    int table_order_device(const uint64_t *d_rank, const EpochTabHost &tab, const void *d_et, uint32_t *d_order);
-    int table_reorder_device(const uint64_t *d_rank, const void *d_et, uint32_t *d_order);
 
     // Counters of the last build / merge.
     uint64_t n_reads = 0, n_occ = 0, n_distinct = 0, n_surv = 0, n_kept = 0, n_overflow = 0;
@@ -222,7 +223,7 @@ private:
         M_CMUL, M_CEDGE, M_LEV, M_LEVOFF, M_PAR, M_NEED, M_NL, M_NR, M_INL, M_INR, M_CNT, M_SPEC, M_PROPS, M_COUNT
     };
     DevBuf mb_[M_COUNT];
-    enum { O_KEY, O_BKT, O_BKT2, O_IDX, O_TMP, O_RANK, O_ORD, O_BCNT, O_BOFF, O_BSTART, O_SK, O_SV, O_COUNT };
+    enum { O_ENT, O_R2, O_GSORT, O_BCNT, O_BOFF, O_BSTART, O_COUNT };
     DevBuf ob_[O_COUNT];
     // host mirrors of small results
     uint64_t *h_small_ = nullptr;

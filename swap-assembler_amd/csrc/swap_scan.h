@@ -202,4 +202,29 @@ static void compact_scatter(uint64_t n, Pred pred, const uint64_t *blk_off, uint
                                n, pred, blk_off, pos_of, list);
 }
 
+// Exclusive scan of cnt[0..n) into start[] by the whole workgroup (n <= items
+// per thread x blockDim.x; blockDim.x a multiple of 64, at most 1024).  Each
+// thread reads its items before any is written, so start may be cnt.
+template <int kItems>
+__device__ __forceinline__ void block_scan_excl(const uint32_t *cnt, uint32_t *start, uint32_t n) {
+    __shared__ uint32_t wtot[16];
+    const uint32_t t = threadIdx.x, base = t * kItems;
+    uint32_t v[kItems], sum = 0;
+#pragma unroll
+    for (int j = 0; j < kItems; j++) {
+        v[j] = base + j < n ? cnt[base + j] : 0;
+        sum += v[j];
+    }
+    const uint32_t inc = wave_incl_sum(sum);
+    if ((t & 63) == 63) wtot[t >> 6] = inc;
+    __syncthreads();
+    uint32_t off = inc - sum;
+    for (uint32_t w = 0; w < (t >> 6); w++) off += wtot[w];
+#pragma unroll
+    for (int j = 0; j < kItems; j++) {
+        if (base + j < n) start[base + j] = off;
+        off += v[j];
+    }
+}
+
 }  // namespace swapk

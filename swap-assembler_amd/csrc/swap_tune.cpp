@@ -4,8 +4,8 @@
 // beside the library's directory, or from the file SWAP_TUNE_FILE names.
 //
 // One row per size class, ascending:
-//   <max input bytes> <part_recs> <fill_recs> <index_load> <ruler_stride> <occ_groups>
-// '#' starts a comment.  An input takes the first row whose bound it does not
+//   <max input bytes> <part_recs> <fill_recs> <index_load> <ruler_stride> <occ_groups> [<order_bin_keys>]
+// '#' starts a comment; a row without the last column takes its default.  An input takes the first row whose bound it does not
 // exceed (the last row above every bound); without a table, the built-in
 // defaults.  No parameter changes a result, only the time it takes.
 #include <dlfcn.h>
@@ -29,6 +29,7 @@ Tuning clamp_tuning(const Tuning &t) {
     c.index_load = std::min<uint32_t>(std::max<uint32_t>(c.index_load, 50), 85);
     c.ruler_stride = std::min<uint32_t>(std::max<uint32_t>(c.ruler_stride, 4), 256);
     c.occ_groups = std::min<uint32_t>(std::max<uint32_t>(c.occ_groups, 1), 4);
+    c.order_bin_keys = std::min<uint32_t>(std::max<uint32_t>(c.order_bin_keys, 256), 1u << 30);
     return c;
 }
 
@@ -57,8 +58,9 @@ std::vector<Row> read_table(const std::string &path) {
     while (std::fgets(line, sizeof line, f)) {
         if (char *h = std::strchr(line, '#')) *h = 0;
         unsigned long long mb;
-        unsigned pr, fr, il, rs, og;
-        if (std::sscanf(line, "%llu %u %u %u %u %u", &mb, &pr, &fr, &il, &rs, &og) != 6) continue;
+        unsigned pr, fr, il, rs, og, ob = Tuning().order_bin_keys;
+        const int got = std::sscanf(line, "%llu %u %u %u %u %u %u", &mb, &pr, &fr, &il, &rs, &og, &ob);
+        if (got != 6 && got != 7) continue;
         Row r;
         r.max_bytes = mb;
         r.t.part_recs = pr;
@@ -66,6 +68,7 @@ std::vector<Row> read_table(const std::string &path) {
         r.t.index_load = il;
         r.t.ruler_stride = rs;
         r.t.occ_groups = og;
+        r.t.order_bin_keys = ob;
         r.t = clamp_tuning(r.t);
         rows.push_back(r);
     }

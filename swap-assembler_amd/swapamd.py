@@ -60,7 +60,8 @@ class SwapJob(ctypes.Structure):
 
 class SwapTuning(ctypes.Structure):
     _fields_ = [("part_recs", ctypes.c_uint32), ("fill_recs", ctypes.c_uint32), ("index_load", ctypes.c_uint32),
-                ("ruler_stride", ctypes.c_uint32), ("occ_groups", ctypes.c_uint32)]
+                ("ruler_stride", ctypes.c_uint32), ("occ_groups", ctypes.c_uint32),
+                ("order_bin_keys", ctypes.c_uint32)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}

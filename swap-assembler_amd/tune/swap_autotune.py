@@ -8,16 +8,16 @@ For each input-size class it takes representative seeded inputs and tunes,
 by coordinate descent from the library's defaults, the parameters of
 swap_tuning (include/swap_hip.h): the records per count partition (the load of
 the on-chip counting table), the records of a shared table fill, the load
-factor of the survivor index (0.50-0.85), the ruler stride of the list ranking
-and the workgroups per CU of the k-mer passes.  A candidate is timed as the
-median of --reps builds + merges of the input already in device memory (the
+factor of the survivor index (0.50-0.85), the ruler stride of the list ranking,
+the workgroups per CU of the k-mer passes and the keys per bin of the
+visit-order sort.  A candidate is timed as the median of --reps builds + merges of the input already in device memory (the
 two small classes: as the time per sample of a server batch, in a child
 process that reads the candidate from a one-row table), and is only accepted
 if every output file is byte-identical to the defaults' (the parameters change
 timings, never results).
 
 Writes <out>/swap_tune.txt, the table the library reads (one row per class:
-the largest input in bytes it covers, then the five parameters), and
+the largest input in bytes it covers, then the six parameters), and
 <out>/swap_tune_report.json with every timing.
 """
 import argparse
@@ -33,13 +33,15 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.dirname(HERE))
 import swapamd  # noqa: E402
 
-DEFAULTS = {"part_recs": 8192, "fill_recs": 4096, "index_load": 67, "ruler_stride": 32, "occ_groups": 1}
+DEFAULTS = {"part_recs": 8192, "fill_recs": 4096, "index_load": 67, "ruler_stride": 32, "occ_groups": 1,
+            "order_bin_keys": 2048}
 CANDIDATES = [
     ("part_recs", [4096, 6144, 8192, 12288]),
     ("fill_recs", [2048, 4096, 8192]),
     ("index_load", [50, 60, 67, 75, 85]),
     ("ruler_stride", [8, 16, 32, 64]),
     ("occ_groups", [1, 2]),
+    ("order_bin_keys", [1024, 2048, 4096, 8192]),
 ]
 # size classes: (name, largest input in bytes, input generator arguments, k, c,
 # objective).  Inputs of the two small classes come in batches (the many-sample
@@ -77,8 +79,8 @@ def batch_run(bound, gen, k, c, params, reps):
     table once per process)."""
     table = os.path.join(os.environ.get("TMPDIR", "/tmp"), "swap_tune_candidate.txt")
     with open(table, "w") as f:
-        f.write("%d %d %d %d %d %d\n" % (bound, params["part_recs"], params["fill_recs"], params["index_load"],
-                                          params["ruler_stride"], params["occ_groups"]))
+        f.write("%d %d %d %d %d %d %d\n" % (bound, params["part_recs"], params["fill_recs"], params["index_load"],
+                                             params["ruler_stride"], params["occ_groups"], params["order_bin_keys"]))
     env = dict(os.environ, SWAP_TUNE_FILE=table)
     arg = json.dumps({"gen": gen, "k": k, "c": c, "reps": reps})
     r = subprocess.run([sys.executable, os.path.abspath(__file__), "--batch-child", arg], env=env,
@@ -169,11 +171,11 @@ def main():
     os.makedirs(args.out, exist_ok=True)
     with open(os.path.join(args.out, "swap_tune.txt"), "w") as f:
         f.write("# Written by swap_autotune.py on %s: per input-size class, the largest input in\n"
-                "# bytes, then part_recs fill_recs index_load ruler_stride occ_groups.\n" %
+                "# bytes, then part_recs fill_recs index_load ruler_stride occ_groups order_bin_keys.\n" %
                 time.strftime("%Y-%m-%d"))
         for bound, t in rows:
-            f.write("%d %d %d %d %d %d\n" % (bound, t["part_recs"], t["fill_recs"], t["index_load"],
-                                              t["ruler_stride"], t["occ_groups"]))
+            f.write("%d %d %d %d %d %d %d\n" % (bound, t["part_recs"], t["fill_recs"], t["index_load"],
+                                                 t["ruler_stride"], t["occ_groups"], t["order_bin_keys"]))
     with open(os.path.join(args.out, "swap_tune_report.json"), "w") as f:
         json.dump(report, f, indent=1, sort_keys=True)
 

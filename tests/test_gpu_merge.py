@@ -222,9 +222,12 @@ def test_single_rank_reference(name):
 
 
 @pytest.mark.parametrize("tuning", [
-    {"part_recs": 1024, "fill_recs": 512, "index_load": 50, "ruler_stride": 4, "occ_groups": 4},
-    {"part_recs": 65536, "fill_recs": 8192, "index_load": 85, "ruler_stride": 256, "occ_groups": 1},
-    {"part_recs": 4096, "fill_recs": 2048, "index_load": 75, "ruler_stride": 8, "occ_groups": 2},
+    {"part_recs": 1024, "fill_recs": 512, "index_load": 50, "ruler_stride": 4, "occ_groups": 4,
+     "order_bin_keys": 256},
+    {"part_recs": 65536, "fill_recs": 8192, "index_load": 85, "ruler_stride": 256, "occ_groups": 1,
+     "order_bin_keys": 1 << 30},
+    {"part_recs": 4096, "fill_recs": 2048, "index_load": 75, "ruler_stride": 8, "occ_groups": 2,
+     "order_bin_keys": 2048},
 ])
 def test_tuning_never_changes_results(tuning):
     """Every performance parameter at the ends of its range gives the same files."""
@@ -241,4 +244,27 @@ def test_tuning_never_changes_results(tuning):
     assert a.file_bytes(swapamd.SWAP_FILE_CONTIG_GRAPH) == o.file(1)
     assert a.file_bytes(swapamd.SWAP_FILE_NOCE) == o.file(2)
     a.set_tuning(None)
+    a.close()
+
+
+@pytest.mark.parametrize("bin_keys", [256, 1 << 30])
+def test_visit_order_bins(bin_keys):
+    """The visit-order sort with bins of ~256 keys (thousands of bins per table)
+    and with one bin per table: the latter holds every survivor of a 60 kbp
+    genome, above the 6144 keys a bin is ordered with in LDS, so it is ordered
+    in device memory.  Files equal the oracle's either way."""
+    fa = swapamd.generate_params(60000, 100, 20.0, 0.005, 77)
+    print("visit order bins: order_bin_keys=%d, %d bytes" % (bin_keys, len(fa)), flush=True)
+    o = Oracle(fa, 31, 2)
+    a = swapamd.Assembler(31, 2)
+    tuning = dict(a.tuning(), order_bin_keys=bin_keys)
+    a.set_tuning(tuning)
+    a.build(fa)
+    a.merge()
+    st = a.stats()
+    assert st["survivors"] > 6144
+    assert st["kept"] == o.stats()["kept"]
+    assert a.file_bytes(swapamd.SWAP_FILE_CONTIG_GRAPH) == o.file(1)
+    assert a.file_bytes(swapamd.SWAP_FILE_NOCE) == o.file(2)
+    assert a.file_bytes(swapamd.SWAP_FILE_PERF_CONTIGS) == o.file(3)
     a.close()

@@ -13,7 +13,8 @@ import swapamd
 
 REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 PKG = os.path.join(REPO, "swap-assembler_amd")
-DEFAULT = {"part_recs": 8192, "fill_recs": 4096, "index_load": 67, "ruler_stride": 32, "occ_groups": 1}
+DEFAULT = {"part_recs": 8192, "fill_recs": 4096, "index_load": 67, "ruler_stride": 32, "occ_groups": 1,
+           "order_bin_keys": 2048}
 
 
 def lookup(table_text, sizes):
@@ -35,17 +36,22 @@ def lookup(table_text, sizes):
 
 def test_rows_are_chosen_by_input_size():
     table = ("# comment line\n"
-             "1000 4096 2048 50 8 2   # first class\n"
-             "100000 12288 8192 85 64 1\n")
+             "1000 4096 2048 50 8 2 1024   # first class\n"
+             "100000 12288 8192 85 64 1\n")  # no order_bin_keys column: its default
     got = lookup(table, [1, 1000, 1001, 100000, 10 ** 9])
-    first = {"part_recs": 4096, "fill_recs": 2048, "index_load": 50, "ruler_stride": 8, "occ_groups": 2}
-    second = {"part_recs": 12288, "fill_recs": 8192, "index_load": 85, "ruler_stride": 64, "occ_groups": 1}
+    first = {"part_recs": 4096, "fill_recs": 2048, "index_load": 50, "ruler_stride": 8, "occ_groups": 2,
+             "order_bin_keys": 1024}
+    second = {"part_recs": 12288, "fill_recs": 8192, "index_load": 85, "ruler_stride": 64, "occ_groups": 1,
+              "order_bin_keys": 2048}
     assert got == [first, first, second, second, second]  # above every bound: the last row
 
 
 def test_values_are_clamped_and_bad_lines_skipped():
-    got = lookup("not a row\n5000 1 1 99 1000 9\n", [10])
-    assert got == [{"part_recs": 1024, "fill_recs": 512, "index_load": 85, "ruler_stride": 256, "occ_groups": 4}]
+    got = lookup("not a row\n5000 1 1 99 1000 9 1\n6000 1 1 1 1 1 4294967295\n", [10, 5500])
+    assert got == [{"part_recs": 1024, "fill_recs": 512, "index_load": 85, "ruler_stride": 256, "occ_groups": 4,
+                    "order_bin_keys": 256},
+                   {"part_recs": 1024, "fill_recs": 512, "index_load": 50, "ruler_stride": 4, "occ_groups": 1,
+                    "order_bin_keys": 1 << 30}]
 
 
 def test_defaults_without_a_table():
@@ -57,11 +63,12 @@ def test_committed_table_is_read():
     with open(os.path.join(PKG, "tune", "swap_tune.txt")) as f:
         for line in f:
             line = line.split("#")[0].split()
-            if len(line) == 6:
+            if len(line) == 7:
                 rows.append([int(x) for x in line])
     assert len(rows) >= 3 and rows == sorted(rows)
-    for bound, pr, fr, il, rs, og in rows:
-        want = {"part_recs": pr, "fill_recs": fr, "index_load": il, "ruler_stride": rs, "occ_groups": og}
+    for bound, pr, fr, il, rs, og, ob in rows:
+        want = {"part_recs": pr, "fill_recs": fr, "index_load": il, "ruler_stride": rs, "occ_groups": og,
+                "order_bin_keys": ob}
         assert swapamd.tuning_for(min(bound, 2 ** 63)) == want
     report = json.load(open(os.path.join(PKG, "tune", "swap_tune_report.json")))
     assert set(report) == {"tiny", "small", "medium", "large"}
