@@ -113,7 +113,8 @@ typedef struct swap_stats {
     /* device time of each phase (events on the context's stream): 0 record scan,
      * 1 2-bit pack, 2 partition histogram, 3 level-1 partition, 4 LDS count +
      * cutoff, 5 global-table count (overflow), 6 first-occurrence ranks,
-     * 7 level-2 partition, 8 survivor index, 9 list ranking, 10 kept edges,
+     * 7 level-2 partition, 8 survivor renumbering and index, 9 list ranking,
+     * 10 kept edges,
      * 11 edge strings, 12 Cartesian trees, 13 visit order, 14 classification,
      * 15 survivor keys (un-mixing) */
     double phase_ms[16];

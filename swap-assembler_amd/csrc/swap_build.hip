@@ -1389,6 +1389,7 @@ void Pipeline::leave_arena() {
     for (DevBuf *b : bufs) b->leave_arena();
     for (DevBuf &b : mb_) b.leave_arena();
     for (DevBuf &b : ob_) b.leave_arena();
+    for (DevBuf &b : rn_) b.leave_arena();
 }
 
 static uint64_t inv_odd(uint64_t a) {
@@ -1468,6 +1469,7 @@ int Pipeline::build(const uint8_t *d_in, uint64_t n) {
     SWAP_CHECK(hipSetDevice(dev_));
     if (!current_arena()) leave_arena();  // an ordinary build never writes a static call's space
     n_reads = n_occ = n_distinct = n_surv = n_kept = n_overflow = n_special = 0;
+    surv_renumbered_ = false;
     if (n == 0) return fail(-2, "empty input");
     const uint8_t *buf = d_in;
     if ((reinterpret_cast<uintptr_t>(d_in) & 15) != 0) {

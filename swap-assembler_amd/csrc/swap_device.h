@@ -167,6 +167,7 @@ public:
     int merge_device(bool reference_order, MergeDev &out);
     int fetch_merge(MergeDev &out);
     int visit_priorities(uint64_t *d_prio);
+    int renumber_survivors(bool *renumbered);
 
     // Counters of the last build / merge.
     uint64_t n_reads = 0, n_occ = 0, n_distinct = 0, n_surv = 0, n_kept = 0, n_overflow = 0;
@@ -225,6 +226,12 @@ private:
     DevBuf mb_[M_COUNT];
     enum { O_ENT, O_R2, O_GSORT, O_BCNT, O_BOFF, O_BSTART, O_COUNT };
     DevBuf ob_[O_COUNT];
+    // survivor renumbering
+    enum { R_CHIST, R_CPRE, R_HEAD, R_HPRE, R_BINMAP, R_CNT, R_OFF, R_BSTART, R_KV, R_FI, R_GSRT, R_KEY, R_MULT, R_FO, R_PERM, R_COUNT };
+    DevBuf rn_[R_COUNT];
+    // the survivor arrays are in first-occurrence order (renumbered by a merge
+    // of this build): they no longer follow the count's partition runs
+    bool surv_renumbered_ = false;
     // host mirrors of small results
     uint64_t *h_small_ = nullptr;
     uint64_t last_first_ = 0;

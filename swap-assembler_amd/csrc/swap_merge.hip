@@ -78,20 +78,26 @@ __global__ void k_index_sizes(const uint32_t *__restrict__ part_surv, uint32_t n
 // run of the survivor array and whose region fits kIdxLds entries: the region
 // is written whole (free entries included) with coalesced stores.  Other
 // regions are only cleared here and filled by k_index_loose.
+//
+// key: the survivors in the count's order.  Entries hold their numbers after
+// renumbering, perm[i] for survivor i (i itself when perm is null).  With runs
+// false (the survivor arrays were renumbered by an earlier merge of the build,
+// and key is in that order) every region is filled by k_index_loose.
 constexpr uint32_t kIdxLds = 1024;
 constexpr uint32_t kIdxThreads = 128;
 
 __global__ void __launch_bounds__(kIdxThreads) k_index_part(const uint64_t *__restrict__ key,
                                                     const uint32_t *__restrict__ part_surv,
                                                     const uint32_t *__restrict__ part_first, SurvIndex ix,
-                                                    uint64_t *__restrict__ ent) {
+                                                    uint64_t *__restrict__ ent, const uint32_t *__restrict__ perm,
+                                                    bool runs) {
     __shared__ unsigned long long lkey[kIdxLds];
     __shared__ uint32_t lid[kIdxLds];
     const uint32_t p = blockIdx.x;
     const uint64_t b = ix.base[p];
     const uint32_t size = (uint32_t)(ix.base[p + 1] - b);
     if (!size) return;
-    const uint32_t first = part_first[p];
+    const uint32_t first = runs ? part_first[p] : ~0u;
     if (first == ~0u || size > kIdxLds) {
         for (uint32_t s = threadIdx.x; s < 2 * size; s += blockDim.x) ent[2 * b + s] = kIdxEmpty;
         return;
@@ -105,7 +111,7 @@ __global__ void __launch_bounds__(kIdxThreads) k_index_part(const uint64_t *__re
         for (uint32_t probe = 0; probe <= mask; probe++) {  // the region is at most 2/3 full
             const unsigned long long prev = atomicCAS(&lkey[s], (unsigned long long)kIdxEmpty, (unsigned long long)k);
             if (prev == kIdxEmpty) {
-                lid[s] = first + i;
+                lid[s] = perm ? perm[first + i] : first + i;
                 break;
             }
             s = (s + 1) & mask;
@@ -120,7 +126,7 @@ __global__ void __launch_bounds__(kIdxThreads) k_index_part(const uint64_t *__re
 
 // Survivors of the regions k_index_part left empty: device-memory inserts.
 __global__ void k_index_loose(const uint64_t *__restrict__ key, uint64_t n, const uint32_t *__restrict__ part_first,
-                              SurvIndex ix, uint64_t *ent) {
+                              SurvIndex ix, uint64_t *ent, const uint32_t *__restrict__ perm, bool runs) {
     const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const uint64_t k = key[i];
@@ -128,14 +134,14 @@ __global__ void k_index_loose(const uint64_t *__restrict__ key, uint64_t n, cons
     const uint64_t p = h >> ix.shift;
     const uint64_t b = ix.base[p];
     const uint64_t size = ix.base[p + 1] - b;  // the region holds key's partition: size >= 2
-    if (part_first[p] != ~0u && size <= kIdxLds) return;
+    if (runs && part_first[p] != ~0u && size <= kIdxLds) return;
     const uint64_t mask = size - 1;
     uint64_t s = h & mask;
     for (uint64_t probe = 0; probe <= mask; probe++) {  // the region is at most 2/3 full
         const uint64_t prev = atomicCAS((unsigned long long *)&ent[2 * (b + s)], (unsigned long long)kIdxEmpty,
                                         (unsigned long long)k);
         if (prev == kIdxEmpty) {
-            ent[2 * (b + s) + 1] = i;
+            ent[2 * (b + s) + 1] = perm ? perm[i] : i;
             return;
         }
         s = (s + 1) & mask;
@@ -168,6 +174,7 @@ struct MergeArgs {
     uint64_t S;
     int k;
     SurvIndex ix;
+    bool near;  // survivors numbered in first-occurrence order
 };
 
 __device__ __forceinline__ uint64_t orient_kmer(uint64_t X, int slot, int k) { return slot >= 4 ? revcomp(X, k) : X; }
@@ -179,6 +186,27 @@ __device__ __forceinline__ uint32_t hop(const MergeArgs &a, uint64_t X, int s) {
     const uint64_t r = revcomp(t, a.k);
     const uint64_t Y = t > r ? t : r;
     const uint32_t j = index_find(a.ix, Y);
+    if (j == kNil) return kNil;
+    return (j << 1) | (t == Y ? 0u : 1u);
+}
+
+// hop() from survivor i: with survivors numbered in first-occurrence order,
+// the k-mer that follows a survivor in the read that first contains it is
+// nearly always the next or the previous survivor, so those two keys are
+// compared before the index is probed.
+__device__ __forceinline__ uint32_t hop_near(const MergeArgs &a, uint64_t X, int s, uint64_t i) {
+    const uint64_t km = kmask(a.k);
+    const uint64_t t = ((orient_kmer(X, s, a.k) << 2) | (uint64_t)(s & 3)) & km;
+    const uint64_t r = revcomp(t, a.k);
+    const uint64_t Y = t > r ? t : r;
+    uint32_t j = kNil;
+    if (a.near) {
+        if (i + 1 < a.S && a.key[i + 1] == Y)
+            j = (uint32_t)(i + 1);
+        else if (i > 0 && a.key[i - 1] == Y)
+            j = (uint32_t)(i - 1);
+    }
+    if (j == kNil) j = index_find(a.ix, Y);
     if (j == kNil) return kNil;
     return (j << 1) | (t == Y ? 0u : 1u);
 }
@@ -211,7 +239,7 @@ __global__ void k_classify(MergeArgs a, uint8_t *__restrict__ is11, uint32_t *__
     if (!one) return;
     const uint64_t X = a.key[i];
     const int r = nz_slot(m, 0), l = nz_slot(m, 4);
-    const uint32_t nr = hop(a, X, r), nl = hop(a, X, l);
+    const uint32_t nr = hop_near(a, X, r, i), nl = hop_near(a, X, l, i);
     nxt[2 * i] = nr;
     nxt[2 * i + 1] = nl;
     ex[2 * i] = (uint8_t)(m >> (8 * r));
@@ -789,13 +817,30 @@ int Pipeline::merge_device(bool reference_order, MergeDev &out) {
     ix.mx = mx_;
     ix.shift = 2 * k_ - pbits_;
     phase_begin(8);
+    // survivors renumbered in first-occurrence order (reference order: the
+    // first occurrences are known), once per build; the index is built from
+    // the survivors in the count's partition runs, with their new numbers
+    const bool runs = !surv_renumbered_;
+    bool renumbered = false;
+    if (reference_order && !surv_renumbered_) {
+        const int rc = renumber_survivors(&renumbered);
+        if (rc) return rc;
+    }
+    const uint32_t *perm = renumbered ? rn_[R_PERM].as<uint32_t>() : nullptr;
     hipLaunchKernelGGL(k_index_sizes, dim3((unsigned)(nparts / 256)), dim3(256), 0, st_, part_surv_.as<uint32_t>(),
                        (uint32_t)nparts, mul256, isize);
     MCHECK(scan_excl<uint32_t>(isize, ibase, nparts, scan_scratch_.as<uint64_t>(), ibase + nparts, st_));
     hipLaunchKernelGGL(k_index_part, dim3((unsigned)nparts), dim3(kIdxThreads), 0, st_, surv_key_.as<uint64_t>(),
-                       part_surv_.as<uint32_t>(), part_first_.as<uint32_t>(), ix, mb_[M_IENT].as<uint64_t>());
+                       part_surv_.as<uint32_t>(), part_first_.as<uint32_t>(), ix, mb_[M_IENT].as<uint64_t>(), perm,
+                       runs);
     hipLaunchKernelGGL(k_index_loose, dim3(nblk(S)), dim3(256), 0, st_, surv_key_.as<uint64_t>(), S,
-                       part_first_.as<uint32_t>(), ix, mb_[M_IENT].as<uint64_t>());
+                       part_first_.as<uint32_t>(), ix, mb_[M_IENT].as<uint64_t>(), perm, runs);
+    if (renumbered) {  // the new order in place of the count's
+        MCHECK(hipMemcpyAsync(surv_key_.p, rn_[R_KEY].p, S * 8, hipMemcpyDeviceToDevice, st_));
+        MCHECK(hipMemcpyAsync(surv_mult_.p, rn_[R_MULT].p, S * 8, hipMemcpyDeviceToDevice, st_));
+        MCHECK(hipMemcpyAsync(surv_first_.p, rn_[R_FO].p, S * 4, hipMemcpyDeviceToDevice, st_));
+        surv_renumbered_ = true;
+    }
     // --- priorities (visit time)
     MCHECK(mb_[M_PRIO].reserve(S * 8));
     phase_end(8);
@@ -816,6 +861,7 @@ int Pipeline::merge_device(bool reference_order, MergeDev &out) {
     ma.S = S;
     ma.k = k_;
     ma.ix = ix;
+    ma.near = surv_renumbered_;
     MCHECK(mb_[M_IS11].reserve(S));
     MCHECK(mb_[M_NXT].reserve(2 * S * 4));
     MCHECK(mb_[M_EX].reserve(2 * S));

@@ -91,6 +91,7 @@ struct OrdEntry {
 };
 
 constexpr int kBinThreads = 1024;
+constexpr int kBinBatch = 4;  // keys a thread loads at once in the count and scatter passes
 // bins per table: with more, a workgroup's keys of a bin become runs too short
 // for whole-line stores in the scatter
 constexpr int kMaxBinBits = 12;
@@ -123,12 +124,22 @@ __global__ void __launch_bounds__(kBinThreads) k_bin_count(const uint64_t *__res
     __syncthreads();
     const uint32_t w = xcd_range(blockIdx.x, gridDim.x);
     const uint64_t b = (uint64_t)w * per, e = min(n, b + per);
-    for (uint64_t i = b + threadIdx.x; i < e; i += blockDim.x) {
-        uint64_t h;
-        uint32_t b1, b2;
-        bins_of(key[i], bs, h, b1, b2);
-        atomicAdd(&hist[b1], 1u);
-        atomicAdd(&hist[b2], 1u);
+    for (uint64_t i0 = b + threadIdx.x; i0 < e; i0 += kBinBatch * blockDim.x) {
+        uint64_t k[kBinBatch];
+#pragma unroll
+        for (int u = 0; u < kBinBatch; u++) {
+            const uint64_t i = i0 + (uint64_t)u * blockDim.x;
+            if (i < e) k[u] = key[i];
+        }
+#pragma unroll
+        for (int u = 0; u < kBinBatch; u++) {
+            if (i0 + (uint64_t)u * blockDim.x >= e) continue;
+            uint64_t h;
+            uint32_t b1, b2;
+            bins_of(k[u], bs, h, b1, b2);
+            atomicAdd(&hist[b1], 1u);
+            atomicAdd(&hist[b2], 1u);
+        }
     }
     __syncthreads();
     for (uint32_t i = threadIdx.x; i < nb; i += blockDim.x) cnt[(uint64_t)i * gridDim.x + w] = hist[i];
@@ -153,13 +164,28 @@ __global__ void __launch_bounds__(kBinThreads) k_bin_scatter(const uint64_t *__r
     if (w == 0 && threadIdx.x == 0) bin_start[nb] = (uint32_t)(2 * n);
     __syncthreads();
     const uint64_t b = (uint64_t)w * per, e = min(n, b + per);
-    for (uint64_t i = b + threadIdx.x; i < e; i += blockDim.x) {
-        uint64_t h;
-        uint32_t b1, b2;
-        bins_of(key[i], bs, h, b1, b2);
-        const uint32_t s1 = atomicAdd(&cur[b1], 1u), s2 = atomicAdd(&cur[b2], 1u);
-        ent[s1] = OrdEntry{h, first[i], s2 - (uint32_t)n};
-        ent[s2] = OrdEntry{h, (uint32_t)i, 0u};
+    for (uint64_t i0 = b + threadIdx.x; i0 < e; i0 += kBinBatch * blockDim.x) {
+        uint64_t k[kBinBatch];
+        uint32_t fo[kBinBatch];
+#pragma unroll
+        for (int u = 0; u < kBinBatch; u++) {
+            const uint64_t i = i0 + (uint64_t)u * blockDim.x;
+            if (i < e) {
+                k[u] = key[i];
+                fo[u] = first[i];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < kBinBatch; u++) {
+            const uint64_t i = i0 + (uint64_t)u * blockDim.x;
+            if (i >= e) continue;
+            uint64_t h;
+            uint32_t b1, b2;
+            bins_of(k[u], bs, h, b1, b2);
+            const uint32_t s1 = atomicAdd(&cur[b1], 1u), s2 = atomicAdd(&cur[b2], 1u);
+            ent[s1] = OrdEntry{h, fo[u], s2 - (uint32_t)n};
+            ent[s2] = OrdEntry{h, (uint32_t)i, 0u};
+        }
     }
 }
 
