@@ -617,7 +617,7 @@ __global__ void __launch_bounds__(kMaxThreads) k_max_levels(const uint64_t *__re
 }
 
 struct TreeArgs {
-    const uint64_t *lev;      // concatenated levels
+    const uint64_t *lev;      // concatenated levels (level 0: the chain priorities)
     const uint64_t *lev_off;  // offset of each level
     int nlev;
     uint64_t M;               // chain array length
@@ -625,14 +625,20 @@ struct TreeArgs {
     const uint32_t *cedge;
     uint32_t *par;            // parent position, kNil for roots / sentinels
     uint8_t *need;            // children to wait for (0..2); 255 for sentinels
-    uint32_t *nl;
+    uint32_t *nl;             // nearest larger position on each side
     uint32_t *nr;
+    uint32_t *far;            // positions with a nearest larger outside their tile
+    unsigned int *nfar;
     unsigned int *err;        // bit 0: a search left the array (a chain without its sentinel); bit 2: a
                               // climb passed more parents than positions
 };
 
 __device__ __forceinline__ uint64_t levv(const TreeArgs &t, int l, uint64_t i) { return t.lev[t.lev_off[l] + i]; }
 
+// Nearest position left / right of x whose priority exceeds v, by the global
+// max hierarchy: skip aligned blocks whose max is at most v, then descend into
+// the first block that holds a larger one.  A chain array position always has
+// a sentinel (kInf) on both sides, so leaving the array is an error.
 __device__ uint64_t nearest_left(const TreeArgs &t, uint64_t x, uint64_t v) {
     int64_t p = (int64_t)x - 1;
     int l = 0;
@@ -665,27 +671,124 @@ __device__ uint64_t nearest_right(const TreeArgs &t, uint64_t x, uint64_t v) {
     return p;
 }
 
-__global__ void k_tree_links(TreeArgs t) {
-    const uint64_t q = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= t.M) return;
-    const uint64_t v = t.lev[q];
-    if (v == kInf) {
-        t.par[q] = kNil;
-        t.need[q] = 255;
-        return;
-    }
-    const uint64_t L = nearest_left(t, q, v), R = nearest_right(t, q, v);
-    if (L >= t.M || R >= t.M) {  // reported through err; keep the position out of the evaluation
-        t.par[q] = kNil;
-        t.need[q] = 255;
-        return;
-    }
-    const uint64_t vl = t.lev[L], vr = t.lev[R];
+// parent and children to wait for, from the nearest larger neighbours
+__device__ __forceinline__ void tree_node(const TreeArgs &t, uint64_t q, uint64_t L, uint64_t R, uint64_t vl,
+                                          uint64_t vr) {
     t.nl[q] = (uint32_t)L;
     t.nr[q] = (uint32_t)R;
-    if (vl == kInf && vr == kInf) t.par[q] = kNil;
-    else t.par[q] = (uint32_t)(vl < vr ? L : R);
+    t.par[q] = (vl == kInf && vr == kInf) ? kNil : (uint32_t)(vl < vr ? L : R);
     t.need[q] = (uint8_t)((L != q - 1) + (R != q + 1));
+}
+
+// Links of the Cartesian trees, one tile of kTreeTile positions per
+// workgroup: each position's nearest larger neighbours are searched in an LDS
+// max hierarchy of its tile (12 levels).  A position with one outside the
+// tile (a few per tile: the maxima seen from the tile's edges) goes to a list
+// that k_tree_far finishes with the global hierarchy.
+constexpr int kTreeBits = 11;
+constexpr uint32_t kTreeTile = 1u << kTreeBits;
+constexpr int kTreeLinkThreads = 512;
+constexpr int kTreeEvalThreads = 512;
+constexpr uint32_t kTreeLevTotal = 2 * kTreeTile - 1;  // entries of levels 0..kTreeBits
+
+__device__ __forceinline__ uint32_t tree_loff(int l) { return 2 * kTreeTile - (2 * kTreeTile >> l); }
+
+__global__ void __launch_bounds__(kTreeLinkThreads) k_tree_links(TreeArgs t) {
+    __shared__ uint64_t lv[kTreeLevTotal];
+    __shared__ uint32_t sfar[kTreeTile];
+    __shared__ uint32_t nfar, far_base;
+    const uint64_t base = (uint64_t)blockIdx.x * kTreeTile;
+    if (threadIdx.x == 0) nfar = 0;
+    for (uint32_t j = threadIdx.x; j < kTreeTile; j += kTreeLinkThreads) {
+        const uint64_t q = base + j;
+        lv[j] = q < t.M ? t.lev[q] : 0;  // 0 past the end: never larger than a priority
+    }
+    __syncthreads();
+    for (int l = 1; l <= kTreeBits; l++) {
+        const uint32_t n = kTreeTile >> l, o = tree_loff(l), po = tree_loff(l - 1);
+        for (uint32_t j = threadIdx.x; j < n; j += kTreeLinkThreads) {
+            const uint64_t a = lv[po + 2 * j], b = lv[po + 2 * j + 1];
+            lv[o + j] = a > b ? a : b;
+        }
+        __syncthreads();
+    }
+    for (uint32_t x = threadIdx.x; x < kTreeTile; x += kTreeLinkThreads) {
+        const uint64_t q = base + x;
+        if (q >= t.M) break;
+        const uint64_t v = lv[x];
+        if (v == kInf) {  // sentinel
+            t.par[q] = kNil;
+            t.need[q] = 255;
+            continue;
+        }
+        uint32_t L = kNil, R = kNil;
+        // most nearest larger neighbours are a few positions away: four direct
+        // probes per side (independent reads) before the hierarchy search
+        for (int d = 4; d >= 1; d--) {
+            if ((int)x - d >= 0 && lv[x - d] > v) L = x - d;
+            if (x + d < kTreeTile && lv[x + d] > v) R = x + d;
+        }
+        if (L == kNil) {
+            int p = (int)x - 5, l = 0;
+            while (p >= 0) {
+                while (l < kTreeBits && ((p + 1) & ((2 << l) - 1)) == 0) l++;
+                if (lv[tree_loff(l) + (((uint32_t)p + 1) >> l) - 1] > v) break;
+                p -= 1 << l;
+            }
+            if (p >= 0) {
+                while (l > 0) {
+                    l--;
+                    if (lv[tree_loff(l) + (((uint32_t)p + 1) >> l) - 1] <= v) p -= 1 << l;
+                }
+                L = (uint32_t)p;
+            }
+        }
+        if (R == kNil) {
+            uint32_t p = x + 5;
+            int l = 0;
+            while (p < kTreeTile) {
+                while (l < kTreeBits && (p & ((2u << l) - 1)) == 0) l++;
+                if (lv[tree_loff(l) + (p >> l)] > v) break;
+                p += 1u << l;
+            }
+            if (p < kTreeTile) {
+                while (l > 0) {
+                    l--;
+                    if (lv[tree_loff(l) + (p >> l)] <= v) p += 1u << l;
+                }
+                R = p;
+            }
+        }
+        if (L != kNil && R != kNil) {
+            tree_node(t, q, base + L, base + R, lv[L], lv[R]);
+        } else {
+            // finished by k_tree_far; the side found here travels in nl / nr
+            t.nl[q] = L == kNil ? kNil : (uint32_t)(base + L);
+            t.nr[q] = R == kNil ? kNil : (uint32_t)(base + R);
+            sfar[atomicAdd(&nfar, 1u)] = (uint32_t)q;
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0 && nfar) far_base = atomicAdd(t.nfar, nfar);
+    __syncthreads();
+    for (uint32_t j = threadIdx.x; j < nfar; j += kTreeLinkThreads) t.far[far_base + j] = sfar[j];
+}
+
+__global__ void k_tree_far(TreeArgs t) {
+    const uint32_t n = *t.nfar;
+    for (uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; j < n; j += (uint64_t)gridDim.x * blockDim.x) {
+        const uint64_t q = t.far[j];
+        const uint64_t v = t.lev[q];
+        const uint32_t l0 = t.nl[q], r0 = t.nr[q];
+        const uint64_t L = l0 != kNil ? l0 : nearest_left(t, q, v);
+        const uint64_t R = r0 != kNil ? r0 : nearest_right(t, q, v);
+        if (L >= t.M || R >= t.M) {  // reported through err; keep the position out of the evaluation
+            t.par[q] = kNil;
+            t.need[q] = 255;
+            continue;
+        }
+        tree_node(t, q, L, R, t.lev[L], t.lev[R]);
+    }
 }
 
 // merged (multiplicity, length) of two spliced edges, with the reference's
@@ -695,46 +798,105 @@ __device__ __forceinline__ uint64_t splice(uint64_t a, uint64_t b) {
     const uint64_t ma = a & 0xFF, la = a >> 8, mb = b & 0xFF, lb = b >> 8;
     const uint64_t s = ma * la + mb * lb;
     const int32_t pre = (int32_t)(uint32_t)s;
-    const uint64_t q = (uint64_t)(int64_t)pre / (la + lb);
+    // a non-negative 32-bit numerator over a 32-bit length divides in 32 bits
+    const uint64_t q = (pre >= 0 && la + lb <= 0xFFFFFFFFull)
+                           ? (uint64_t)((uint32_t)pre / (uint32_t)(la + lb))
+                           : (uint64_t)(int64_t)pre / (la + lb);
     return ((la + lb) << 8) | (q & 0xFF);
 }
 
-// Bottom-up evaluation.  The length of a subtree's merged edge is fixed by
-// positions (a node spans the unit edges between its nearest larger
-// neighbours), so only the 8-bit multiplicity travels up.  Each parent has one
-// 64-bit word: left multiplicity (bits 0-7), right (8-15), arrival flags (16,
-// 17).  A child publishes with a single atomicOr and learns from the returned
-// word whether its sibling already arrived -- exactly one of them sees both
-// and goes on with the parent, with no fences between separate locations.
-__global__ void k_tree_eval(TreeArgs t, unsigned long long *word, uint8_t *emult_out) {
-    const uint64_t q0 = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (q0 >= t.M) return;
-    if (t.need[q0] != 0) return;
-    uint64_t q = q0;
-    // multiplicity of q's merged edge from its two inputs
-    auto eval = [&](uint64_t p, uint32_t mL, uint32_t mR) -> uint32_t {
-        const uint64_t lenL = p - t.nl[p], lenR = t.nr[p] - p;
-        return (uint32_t)(splice(((uint64_t)lenL << 8) | mL, ((uint64_t)lenR << 8) | mR) & 0xFF);
-    };
-    uint32_t m = eval(q, t.cmul[q - 1], t.cmul[q]);
-    for (uint64_t steps = 0; steps <= t.M; steps++) {  // parents have strictly larger priority
-        const uint32_t p = t.par[q];
-        if (p == kNil) {
-            emult_out[t.cedge[t.nl[q]]] = (uint8_t)m;
-            emult_out[t.cedge[t.nr[q]]] = (uint8_t)m;
-            return;
-        }
-        const int side = (q < p) ? 0 : 1;
-        const unsigned long long mine = ((unsigned long long)m << (8 * side)) | (1ull << (16 + side));
-        const unsigned long long w = atomicOr(&word[p], mine) | mine;
-        const bool needL = t.nl[p] != p - 1, needR = t.nr[p] != p + 1;
-        if ((needL && !(w & (1ull << 16))) || (needR && !(w & (1ull << 17)))) return;  // sibling finishes p
-        const uint32_t mL = needL ? (uint32_t)(w & 0xFF) : t.cmul[p - 1];
-        const uint32_t mR = needR ? (uint32_t)((w >> 8) & 0xFF) : t.cmul[p];
-        m = eval(p, mL, mR);
-        q = p;
+// multiplicity of node p's merged edge from its left and right inputs: the
+// lengths are fixed by positions (p spans the unit edges between its nearest
+// larger neighbours L and R)
+__device__ __forceinline__ uint32_t tree_eval(uint64_t p, uint64_t L, uint64_t R, uint32_t mL, uint32_t mR) {
+    return (uint32_t)(splice(((p - L) << 8) | mL, ((R - p) << 8) | mR) & 0xFF);
+}
+
+// Bottom-up evaluation, one tile per workgroup: one thread climbs from each
+// leaf, the last-arriving child of a parent finishing it.  A node whose span
+// (L, R) lies within [tile - 1, tile end] has all its descendants in the tile:
+// its children arrive through a 32-bit LDS word (left multiplicity bits 0-7,
+// right 8-15, arrival flags 16, 17) and it is finished in LDS.  The others --
+// those whose span crosses a tile edge, a few per tile, and all their
+// ancestors -- take their arrivals in a 64-bit word in device memory (same
+// layout; the word carries the value, so one atomicOr publishes it with no
+// fence).
+__global__ void __launch_bounds__(kTreeEvalThreads) k_tree_eval(TreeArgs t, unsigned long long *__restrict__ word,
+                                                               uint8_t *__restrict__ emult_out) {
+    __shared__ uint32_t snl[kTreeTile], snr[kTreeTile], spar[kTreeTile], sword[kTreeTile];
+    __shared__ uint8_t sneed[kTreeTile];
+    __shared__ uint8_t smul[kTreeTile + 1];  // cmul of positions base - 1 .. base + kTreeTile - 1
+    const uint64_t base = (uint64_t)blockIdx.x * kTreeTile;
+    for (uint32_t j = threadIdx.x; j < kTreeTile; j += kTreeEvalThreads) {
+        const uint64_t q = base + j;
+        const bool in = q < t.M;
+        sneed[j] = in ? t.need[q] : 255;
+        const bool node = in && sneed[j] != 255;
+        snl[j] = node ? t.nl[q] : kNil;
+        snr[j] = node ? t.nr[q] : kNil;
+        spar[j] = node ? t.par[q] : kNil;
+        smul[j + 1] = in ? t.cmul[q] : 0;
+        sword[j] = 0;
     }
-    atomicOr(t.err, 4u);  // more than M parents: the parent links hold a cycle
+    if (threadIdx.x == 0) smul[0] = base ? t.cmul[base - 1] : 0;
+    __syncthreads();
+    for (uint32_t x = threadIdx.x; x < kTreeTile; x += kTreeEvalThreads) {
+        if (sneed[x] != 0) continue;
+        uint64_t q = base + x;
+        uint64_t L = snl[x], R = snr[x], p = spar[x];
+        uint32_t m = tree_eval(q, L, R, smul[x], smul[x + 1]);
+        // in the tile while the parent's span lies within it
+        bool local = true;
+        while (p != kNil) {
+            const uint64_t pl = p - base;
+            if (p < base || pl >= kTreeTile || (uint64_t)snl[pl] + 1 < base || snr[pl] > base + kTreeTile) {
+                local = false;
+                break;
+            }
+            const uint32_t side = q < p ? 0u : 1u;
+            const uint32_t mine = (m << (8 * side)) | (1u << (16 + side));
+            const uint32_t w = atomicOr(&sword[pl], mine) | mine;
+            const uint64_t PL = snl[pl], PR = snr[pl];
+            const bool needL = PL != p - 1, needR = PR != p + 1;
+            if ((needL && !(w & (1u << 16))) || (needR && !(w & (1u << 17)))) break;  // the sibling finishes p
+            const uint32_t mL = needL ? (w & 0xFF) : smul[pl];
+            const uint32_t mR = needR ? ((w >> 8) & 0xFF) : smul[pl + 1];
+            m = tree_eval(p, PL, PR, mL, mR);
+            q = p;
+            L = PL;
+            R = PR;
+            p = spar[pl];
+        }
+        if (local && p != kNil) continue;  // stopped at a parent its sibling finishes
+        // through device memory from here (or q is a root)
+        bool done = false;
+        for (uint64_t steps = 0; steps <= t.M; steps++) {
+            if (p == kNil) {  // the root: its merged edge is both ends' kept edge
+                if (L < t.M && R < t.M) {  // (else q was excluded by a reported error)
+                    emult_out[t.cedge[L]] = (uint8_t)m;
+                    emult_out[t.cedge[R]] = (uint8_t)m;
+                }
+                done = true;
+                break;
+            }
+            // the parent's links load while the arrival is published
+            const uint64_t PL = t.nl[p], PR = t.nr[p], PP = t.par[p];
+            const uint32_t side = q < p ? 0u : 1u;
+            const unsigned long long mine = ((unsigned long long)m << (8 * side)) | (1ull << (16 + side));
+            const unsigned long long w = atomicOr(&word[p], mine) | mine;
+            if (t.need[p] == 255) { done = true; break; }  // excluded by a reported error
+            const bool needL = PL != p - 1, needR = PR != p + 1;
+            if ((needL && !(w & (1ull << 16))) || (needR && !(w & (1ull << 17)))) { done = true; break; }
+            const uint32_t mL = needL ? (uint32_t)(w & 0xFF) : t.cmul[p - 1];
+            const uint32_t mR = needR ? (uint32_t)((w >> 8) & 0xFF) : t.cmul[p];
+            m = tree_eval(p, PL, PR, mL, mR);
+            q = p;
+            L = PL;
+            R = PR;
+            p = PP;
+        }
+        if (!done) atomicOr(t.err, 4u);  // more than M parents: the parent links hold a cycle
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -1042,8 +1204,10 @@ int Pipeline::merge_device(bool reference_order, MergeDev &out) {
         MCHECK(mb_[M_NEED].reserve(M));
         MCHECK(mb_[M_NL].reserve(M * 4));
         MCHECK(mb_[M_NR].reserve(M * 4));
+        MCHECK(mb_[M_INL].reserve(M * 4));
         MCHECK(mb_[M_INR].reserve(M * 8));
         MCHECK(hipMemsetAsync(mb_[M_INR].p, 0, M * 8, st_));
+        MCHECK(hipMemsetAsync(d_small + 27, 0, 8, st_));
         TreeArgs ta;
         ta.lev = lev;
         ta.lev_off = mb_[M_LEVOFF].as<uint64_t>();
@@ -1055,9 +1219,13 @@ int Pipeline::merge_device(bool reference_order, MergeDev &out) {
         ta.need = mb_[M_NEED].as<uint8_t>();
         ta.nl = mb_[M_NL].as<uint32_t>();
         ta.nr = mb_[M_NR].as<uint32_t>();
+        ta.far = mb_[M_INL].as<uint32_t>();
+        ta.nfar = (unsigned int *)(d_small + 27);
         ta.err = (unsigned int *)(d_small + 25);
-        hipLaunchKernelGGL(k_tree_links, dim3(nblk(M)), dim3(256), 0, st_, ta);
-        hipLaunchKernelGGL(k_tree_eval, dim3(nblk(M)), dim3(256), 0, st_, ta,
+        const unsigned tiles = (unsigned)((M + kTreeTile - 1) / kTreeTile);
+        hipLaunchKernelGGL(k_tree_links, dim3(tiles), dim3(kTreeLinkThreads), 0, st_, ta);
+        hipLaunchKernelGGL(k_tree_far, dim3(std::min(tiles, 1024u)), dim3(256), 0, st_, ta);
+        hipLaunchKernelGGL(k_tree_eval, dim3(tiles), dim3(kTreeEvalThreads), 0, st_, ta,
                            (unsigned long long *)mb_[M_INR].p, mb_[M_EMULT].as<uint8_t>());
     }
     phase_end(12);
