@@ -130,6 +130,19 @@ int swap_build(swap_ctx *ctx, const char *fasta, size_t n);
 /* Build from FASTA bytes already in device memory on the context's device. */
 int swap_build_device(swap_ctx *ctx, const void *d_fasta, size_t n);
 
+/* Read-ahead of a later swap_build's input, the device-side counterpart of the
+ * reference's windowed reads (kmerGraph.cpp:100-156 reads the next window while
+ * the previous one is processed): starts copying the n host bytes at fasta to
+ * a device buffer of the context on a copy stream of its own and returns; the
+ * next swap_build with the same pointer and size takes the copied bytes
+ * instead of copying them, so the transfer of input i+1 overlaps the build and
+ * merge of input i.  Builds consume prefetches in the order they were made; a
+ * swap_build of any other input drops them.  At most two inputs may be
+ * prefetched and not yet built (SWAP_E_STATE beyond).  The bytes must stay
+ * valid and unchanged until the swap_build that consumes them returns (or the
+ * context is destroyed). */
+int swap_prefetch(swap_ctx *ctx, const char *fasta, size_t n);
+
 /* Merge the k-mer graph of the last build into unitig edges (k >= 19, else
  * SWAP_E_INVAL). */
 int swap_merge(swap_ctx *ctx);

@@ -7,6 +7,7 @@
 #include <algorithm>
 #include <atomic>
 #include <chrono>
+#include <deque>
 #include <cstdio>
 #include <cstring>
 #include <memory>
@@ -73,11 +74,31 @@ struct Staging {
     }
 };
 
+// Read-ahead of the next input: swap_prefetch copies host bytes to one of two
+// device buffers on a copy stream of its own, driven by a host thread (the
+// staging loop above), while the caller's build of the previous input runs
+// on the context's stream.  Builds consume prefetches in the order they were
+// made (same pointer and size), so a stream of inputs costs the longer of the
+// transfer and the device path per input instead of their sum.
+struct Prefetch {
+    const char *src = nullptr;
+    size_t n = 0;
+    int buf = 0;        // which of the context's two read-ahead buffers
+    std::thread th;     // the staging loop
+    hipError_t rc = hipSuccess;
+};
+
 struct swap_ctx {
     swap_cfg cfg;
     std::unique_ptr<swapk::Pipeline> pipe;
     swapk::DevBuf input;
     Staging staging;
+    // read-ahead (at most two inputs prefetched and not yet built)
+    swapk::DevBuf ahead[2];
+    hipStream_t copy_st = nullptr;
+    hipEvent_t ahead_done[2] = {nullptr, nullptr};
+    std::deque<std::unique_ptr<Prefetch>> fifo;
+    int next_buf = 0;
     bool built = false, merged = false;
     double build_ms = 0, merge_ms = 0;
     std::string err;
@@ -92,6 +113,25 @@ struct swap_ctx {
     swapk::CeFiles ce;
     bool from_static = false;  // the last build ran in the static scratch (swap_assemble_static)
     bool tune_fixed = false;   // swap_set_tuning: the table is not consulted
+
+    // waits for every staging loop still running (the staging buffers and the
+    // copy stream are shared by them); returns the first error
+    hipError_t join_prefetches() {
+        hipError_t e = hipSuccess;
+        for (auto &p : fifo)
+            if (p->th.joinable()) {
+                p->th.join();
+                if (p->rc != hipSuccess && e == hipSuccess) e = p->rc;
+            }
+        return e;
+    }
+    ~swap_ctx() {
+        (void)join_prefetches();
+        if (copy_st) (void)hipStreamSynchronize(copy_st);
+        for (auto &e : ahead_done)
+            if (e) (void)hipEventDestroy(e);
+        if (copy_st) (void)hipStreamDestroy(copy_st);
+    }
 };
 
 namespace {
@@ -235,8 +275,78 @@ int swap_build_device(swap_ctx *c, const void *d_fasta, size_t n) {
     return SWAP_OK;
 }
 
+int swap_prefetch(swap_ctx *c, const char *fasta, size_t n) {
+    if (!c || !fasta || n == 0) return SWAP_E_INVAL;
+    if (c->fifo.size() >= 2) return set_err(c, SWAP_E_STATE, "two inputs are prefetched and not yet built");
+    if (hipSetDevice(c->cfg.device) != hipSuccess) return set_err(c, SWAP_E_DEVICE, "hipSetDevice");
+    // one staging loop at a time: the previous one has (nearly always) ended
+    // during the build that followed it
+    if (c->join_prefetches() != hipSuccess) {
+        c->fifo.clear();
+        return set_err(c, SWAP_E_DEVICE, "copy of a prefetched input to the device");
+    }
+    if (!c->copy_st) {
+        if (hipStreamCreateWithFlags(&c->copy_st, hipStreamNonBlocking) != hipSuccess)
+            return set_err(c, SWAP_E_DEVICE, "copy stream");
+        for (auto &e : c->ahead_done)
+            if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess)
+                return set_err(c, SWAP_E_DEVICE, "copy events");
+    }
+    // the buffer no queued prefetch holds (builds are synchronous, so the
+    // buffer of an input already built is free)
+    int b = c->next_buf;
+    if (!c->fifo.empty()) b = 1 - c->fifo.back()->buf;
+    if (c->ahead[b].reserve(n + 16) != hipSuccess)
+        return set_err(c, SWAP_E_DEVICE, "device allocation of the prefetched input");
+    std::unique_ptr<Prefetch> p(new Prefetch());
+    p->src = fasta;
+    p->n = n;
+    p->buf = b;
+    Prefetch *q = p.get();
+    void *dst = c->ahead[b].p;
+    hipEvent_t done = c->ahead_done[b];
+    Staging *stg = &c->staging;
+    hipStream_t st = c->copy_st;
+    // the copy starts after the context's queued device work (a build that
+    // failed part way may still read the buffer)
+    if (hipEventRecord(c->ahead_done[b], c->pipe->stream()) != hipSuccess ||
+        hipStreamWaitEvent(st, c->ahead_done[b], 0) != hipSuccess)
+        return set_err(c, SWAP_E_DEVICE, "ordering of the prefetch after the context's work");
+    const int dev = c->cfg.device;
+    q->th = std::thread([q, dst, done, stg, st, dev] {
+        hipError_t e = hipSetDevice(dev);
+        if (e == hipSuccess) e = stg->copy(dst, q->src, q->n, st);
+        if (e == hipSuccess) e = hipEventRecord(done, st);
+        q->rc = e;
+    });
+    c->fifo.push_back(std::move(p));
+    c->next_buf = 1 - b;
+    return SWAP_OK;
+}
+
 int swap_build(swap_ctx *c, const char *fasta, size_t n) {
-    if (!c || (!fasta && n)) return SWAP_E_INVAL;
+    if (!c) return SWAP_E_INVAL;
+    if (!c->fifo.empty()) {
+        Prefetch &p = *c->fifo.front();
+        if (p.src == fasta && p.n == n && fasta && n) {
+            if (hipSetDevice(c->cfg.device) != hipSuccess) return set_err(c, SWAP_E_DEVICE, "hipSetDevice");
+            // the oldest prefetch is this input: its copy (and only its copy)
+            // is waited for on the device
+            if (p.th.joinable()) p.th.join();
+            const hipError_t e = p.rc;
+            const int b = p.buf;
+            c->fifo.pop_front();
+            if (e != hipSuccess) return set_err(c, SWAP_E_DEVICE, "copy of a prefetched input to the device");
+            if (hipStreamWaitEvent(c->pipe->stream(), c->ahead_done[b], 0) != hipSuccess)
+                return set_err(c, SWAP_E_DEVICE, "wait for a prefetched input");
+            return swap_build_device(c, c->ahead[b].p, n);
+        }
+        // not the input that was prefetched: the read-ahead is dropped
+        const hipError_t e = c->join_prefetches();
+        c->fifo.clear();
+        if (e != hipSuccess) return set_err(c, SWAP_E_DEVICE, "copy of a prefetched input to the device");
+    }
+    if (!fasta && n) return SWAP_E_INVAL;
     if (n == 0) return set_err(c, SWAP_E_EMPTY, "empty input");
     if (hipSetDevice(c->cfg.device) != hipSuccess) return set_err(c, SWAP_E_DEVICE, "hipSetDevice");
     if (c->input.reserve(n + 16) != hipSuccess) return set_err(c, SWAP_E_DEVICE, "device allocation of the input");

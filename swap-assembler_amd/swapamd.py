@@ -18,6 +18,7 @@ SWAP_FLAG_HASH_ORDER = 2
 SWAP_FLAG_TABLE_SPILL = 4
 SWAP_FLAG_WIDE_PARTITIONS = 8
 SWAP_FLAG_SINGLE_FILL = 16
+SWAP_E_STATE = -8  # call out of order (include/swap_hip.h)
 SWAP_FILE_KMER_GRAPH = 0
 SWAP_FILE_CONTIG_GRAPH = 1
 SWAP_FILE_NOCE = 2
@@ -28,7 +29,7 @@ SWAP_FILE_JUNG_MUL = 6
 
 # exported entry points and their signatures (mirrors include/swap_hip.h)
 EXPORTS = [
-    "swap_create", "swap_destroy", "swap_build", "swap_build_device", "swap_merge", "swap_assemble_static",
+    "swap_create", "swap_destroy", "swap_build", "swap_build_device", "swap_prefetch", "swap_merge", "swap_assemble_static",
     "swap_get_stats", "swap_tuning_for", "swap_set_tuning", "swap_get_tuning",
     "swap_export_kmer_graph", "swap_format_file", "swap_write_file", "swap_strerror", "swap_last_error",
     "swap_table_order", "swap_visit_order", "swap_table_epochs", "swap_reconfigure", "swap_export_compacted",
@@ -88,6 +89,8 @@ def lib():
         L.swap_build.restype = ctypes.c_int
         L.swap_build_device.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t]
         L.swap_build_device.restype = ctypes.c_int
+        L.swap_prefetch.argtypes = [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_size_t]
+        L.swap_prefetch.restype = ctypes.c_int
         L.swap_merge.argtypes = [ctypes.c_void_p]
         L.swap_merge.restype = ctypes.c_int
         L.swap_assemble_static.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t]
@@ -176,11 +179,13 @@ class Assembler:
         if rc:
             raise SwapError(rc)
         self._h = h
+        self._ahead = []  # prefetched inputs, kept alive until their build
 
     def close(self):
         if self._h:
             lib().swap_destroy(self._h)
             self._h = None
+        self._ahead = []
 
     def __del__(self):
         try:
@@ -194,7 +199,21 @@ class Assembler:
         return rc
 
     def build(self, fasta: bytes):
-        self._check(lib().swap_build(self._h, fasta, len(fasta)))
+        try:
+            self._check(lib().swap_build(self._h, fasta, len(fasta)))
+        finally:
+            # the library consumed the oldest prefetch if it was this input,
+            # else dropped them all
+            if self._ahead and self._ahead[0] is fasta:
+                self._ahead.pop(0)
+            else:
+                self._ahead = []
+
+    def prefetch(self, fasta: bytes):
+        """Starts the host-to-device copy of a later build's input (swap_prefetch):
+        the next build() of the same bytes object uses the copied bytes."""
+        self._check(lib().swap_prefetch(self._h, fasta, len(fasta)))
+        self._ahead.append(fasta)
 
     def build_device(self, ptr: int, n: int):
         self._check(lib().swap_build_device(self._h, ctypes.c_void_p(ptr), n))
