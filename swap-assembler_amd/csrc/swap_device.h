@@ -221,7 +221,8 @@ private:
     enum {
         M_IENT, M_IBASE, M_PRIO, M_IS11, M_NXT, M_EX, M_SELF, M_WALKA, M_WALKB, M_SPA, M_SPB, M_BCNT, M_BOFF,
         M_KEPTOF, M_KEPTSRC, M_ELEN, M_EMULT, M_EOTHER, M_CSIZE, M_ESPEC, M_EOFF, M_COFF, M_POOL, M_CPRI,
-        M_CMUL, M_CEDGE, M_LEV, M_LEVOFF, M_PAR, M_NEED, M_NL, M_NR, M_INL, M_INR, M_CNT, M_SPEC, M_PROPS, M_COUNT
+        M_CMUL, M_CEDGE, M_LEV, M_LEVOFF, M_PAR, M_NEED, M_NL, M_NR, M_INL, M_INR, M_CNT, M_SPEC, M_PROPS, M_SEG,
+        M_COUNT
     };
     DevBuf mb_[M_COUNT];
     enum { O_ENT, O_R2, O_GSORT, O_BCNT, O_BOFF, O_BSTART, O_COUNT };

@@ -312,12 +312,80 @@ __global__ void k_rs_prep(const uint8_t *__restrict__ props, const uint32_t *__r
     nx[v] = (uint64_t)t | (kind << 32) | (sa << 34);
 }
 
-// one thread per ruler: walk to the next ruler / kept node, labelling the
-// nodes passed with (ruler, offset)
+// Runs: a run is a maximal sequence of oriented (1,1) nodes of one tile of
+// kRunTile consecutive ids, each linked to the next by a pass-through step
+// (the successor lies in the tile and is not a ruler).  Survivors numbered in
+// first-occurrence order put chain neighbours next to each other, so runs are
+// long and the rulers' walks take one run per step.  Per node, by pointer
+// doubling in LDS: the run's last node (its exit), the steps from the node to
+// it, and the OR of the self-adjacency flags of the nodes after it up to the
+// exit.  seg = exit | steps << 32 | flag << 48 | kSegNoExit (a cycle inside
+// the tile: no exit).
+constexpr int kRunBits = 11;
+constexpr uint32_t kRunTile = 1u << kRunBits;
+constexpr int kRunThreads = 1024;
+constexpr uint64_t kSegNoExit = 1ull << 49;
+
+__global__ void __launch_bounds__(kRunThreads) k_rs_runs(const uint8_t *__restrict__ props,
+                                                         const uint64_t *__restrict__ nx, uint64_t n2,
+                                                         uint64_t *__restrict__ seg) {
+    __shared__ uint16_t snx[2][kRunTile];
+    __shared__ uint16_t scnt[2][kRunTile];
+    __shared__ uint8_t sor[2][kRunTile];  // self-adjacency seen
+    const uint64_t base = (uint64_t)blockIdx.x * kRunTile;
+    for (uint32_t x = threadIdx.x; x < kRunTile; x += kRunThreads) {
+        const uint64_t v = base + x;
+        uint32_t nxt = x, c = 0, o = 0;
+        if (v < n2 && (props[v] & 1)) {
+            const uint64_t e = nx[v];
+            const uint32_t t = (uint32_t)e, kind = (uint32_t)(e >> 32) & 3;
+            // (a node that is its own successor ends its run: only its own walk,
+            // if it is a ruler, reaches it)
+            if (kind == 0 && t != (uint32_t)v && t >= base && t - base < kRunTile) {
+                nxt = t - (uint32_t)base;
+                c = 1;
+                o = (uint32_t)(e >> 34) & 1;
+            }
+        }
+        snx[0][x] = (uint16_t)nxt;
+        scnt[0][x] = (uint16_t)c;
+        sor[0][x] = (uint8_t)o;
+    }
+    __syncthreads();
+    int cur = 0;
+    // after round r every pointer is 2^(r+1) steps ahead or at its exit; the
+    // rounds stop when no pointer moved (runs are mostly far shorter than a tile)
+    for (int r = 0; r < kRunBits; r++) {
+        int moved = 0;
+        for (uint32_t x = threadIdx.x; x < kRunTile; x += kRunThreads) {
+            const uint32_t y = snx[cur][x];
+            const uint32_t z = snx[cur][y];
+            moved |= z != y;
+            snx[cur ^ 1][x] = (uint16_t)z;
+            scnt[cur ^ 1][x] = (uint16_t)(scnt[cur][x] + scnt[cur][y]);  // at most 2^11 in a cycle
+            sor[cur ^ 1][x] = (uint8_t)(sor[cur][x] | sor[cur][y]);
+        }
+        cur ^= 1;
+        if (!__syncthreads_or(moved)) break;
+    }
+    for (uint32_t x = threadIdx.x; x < kRunTile; x += kRunThreads) {
+        const uint64_t v = base + x;
+        if (v >= n2) break;
+        if (!(props[v] & 1)) continue;
+        const uint32_t xe = snx[cur][x];
+        const bool cyc = snx[cur][xe] != xe;
+        seg[v] = (base + xe) | ((uint64_t)scnt[cur][x] << 32) | ((uint64_t)sor[cur][x] << 48) |
+                 (cyc ? kSegNoExit : 0);
+    }
+}
+
+// one thread per ruler: walk to the next ruler / kept node a run per step,
+// labelling each run's exit with (ruler, offset); a node's offset is its
+// run exit's minus its steps to the exit
 __global__ void k_rs_walk(const uint32_t *__restrict__ rlist, uint64_t nr, const uint64_t *__restrict__ nx,
-                          const uint8_t *__restrict__ selfadj, const uint32_t *__restrict__ rid, uint64_t S,
-                          uint2 *__restrict__ label, Walk *__restrict__ rw, uint8_t *__restrict__ rsp,
-                          unsigned int *__restrict__ err) {
+                          const uint64_t *__restrict__ seg, const uint8_t *__restrict__ selfadj,
+                          const uint32_t *__restrict__ rid, uint64_t S, uint2 *__restrict__ label,
+                          Walk *__restrict__ rw, uint8_t *__restrict__ rsp, unsigned int *__restrict__ err) {
     const uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= nr) return;
     uint32_t u = rlist[r];
@@ -326,32 +394,37 @@ __global__ void k_rs_walk(const uint32_t *__restrict__ rlist, uint64_t nr, const
     x.succ = kNil;
     x.term = kNil;
     x.last = u;
-    uint32_t steps = 1;
+    uint32_t off = 0;  // nodes before the current one in this walk
     bool ended = false;
     for (uint64_t guard = 0; guard <= 2 * S; guard++) {
-        const uint64_t e = nx[u];
+        const uint64_t sg = seg[u];
+        if (sg & kSegNoExit) break;  // (a run entered from outside always has an exit)
+        const uint32_t xe = (uint32_t)sg;
+        off += (uint32_t)(sg >> 32) & 0xFFFFu;
+        sp |= (uint32_t)(sg >> 48) & 1;
+        label[xe] = make_uint2((uint32_t)r, off);
+        const uint64_t e = nx[xe];
         const uint32_t t = (uint32_t)e;
         const uint32_t kind = (uint32_t)(e >> 32) & 3;
         if (kind == 2) {  // arrives at a kept node
             x.term = t;
-            x.last = u;
+            x.last = xe;
             ended = true;
             break;
         }
         if (kind == 1) {  // next ruler
             x.succ = rid[t];
-            x.last = u;
+            x.last = xe;
             ended = true;
             break;
         }
-        label[t] = make_uint2((uint32_t)r, steps);
-        sp |= (uint32_t)(e >> 34) & 1;
+        sp |= (uint32_t)(e >> 34) & 1;  // the next run's first node
+        off++;
         u = t;
-        steps++;
     }
-    // 2S + 1 steps pass every oriented node: a walk still going revisits one
+    // every run of a walk is a different one: a walk still going revisits one
     if (!ended) atomicOr(err, 2u);
-    x.dist = steps;
+    x.dist = off + 1;
     rw[r] = x;
     rsp[r] = (uint8_t)sp;
 }
@@ -383,7 +456,7 @@ __global__ void k_rs_jump(const Walk *__restrict__ win, const uint8_t *__restric
 
 // per oriented node: walk state from its ruler
 __global__ void k_rs_final(const uint8_t *__restrict__ is11, const uint32_t *__restrict__ rid,
-                           const uint2 *__restrict__ label,
+                           const uint64_t *__restrict__ seg, const uint2 *__restrict__ label,
                            const Walk *__restrict__ rw, const uint8_t *__restrict__ rsp, uint64_t S,
                            Walk *__restrict__ w, uint8_t *__restrict__ sp) {
     const uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -394,9 +467,12 @@ __global__ void k_rs_final(const uint8_t *__restrict__ is11, const uint32_t *__r
     if (is11[v >> 1]) {
         uint32_t r = rid[v], off = 0;
         if (r == kNil) {
-            const uint2 lb = label[v];
-            r = lb.x;
-            off = lb.y;
+            const uint64_t sg = seg[v];
+            if (!(sg & kSegNoExit)) {  // (else a cycle inside one tile)
+                const uint2 lb = label[(uint32_t)sg];
+                r = lb.x;
+                off = lb.y - ((uint32_t)(sg >> 32) & 0xFFFFu);
+            }
         }
         if (r == kNil) {  // a cycle no ruler reached: never terminates
             x.succ = (uint32_t)v;
@@ -1067,10 +1143,14 @@ int Pipeline::merge_device(bool reference_order, MergeDev &out) {
     uint8_t *rspa = mb_[M_CMUL].as<uint8_t>(), *rspb = rspa + nr + 1;
     hipLaunchKernelGGL(k_rs_prep, dim3(nblk(2 * S)), dim3(256), 0, st_, mb_[M_PROPS].as<uint8_t>(),
                        mb_[M_NXT].as<uint32_t>(), S, nx);
+    MCHECK(mb_[M_SEG].reserve(2 * S * 8));
+    uint64_t *seg = mb_[M_SEG].as<uint64_t>();
+    hipLaunchKernelGGL(k_rs_runs, dim3((unsigned)((2 * S + kRunTile - 1) / kRunTile)), dim3(kRunThreads), 0, st_,
+                       mb_[M_PROPS].as<uint8_t>(), nx, 2 * S, seg);
     compact_scatter<true>(2 * S, ruler, mb_[M_BOFF].as<uint64_t>(), rid, rlist, st_);
     if (nr) {
-        hipLaunchKernelGGL(k_rs_walk, dim3(nblk(nr)), dim3(256), 0, st_, rlist, nr, nx, mb_[M_SELF].as<uint8_t>(),
-                           rid, S, label, rwa, rspa, (unsigned int *)(d_small + 25));
+        hipLaunchKernelGGL(k_rs_walk, dim3(nblk(nr)), dim3(256), 0, st_, rlist, nr, nx, seg,
+                           mb_[M_SELF].as<uint8_t>(), rid, S, label, rwa, rspa, (unsigned int *)(d_small + 25));
         int rounds = 1;
         for (uint64_t span = 3; span < nr + 2; span *= 3) rounds++;
         rounds++;
@@ -1080,7 +1160,7 @@ int Pipeline::merge_device(bool reference_order, MergeDev &out) {
             std::swap(rspa, rspb);
         }
     }
-    hipLaunchKernelGGL(k_rs_final, dim3(nblk(2 * S)), dim3(256), 0, st_, mb_[M_IS11].as<uint8_t>(), rid, label,
+    hipLaunchKernelGGL(k_rs_final, dim3(nblk(2 * S)), dim3(256), 0, st_, mb_[M_IS11].as<uint8_t>(), rid, seg, label,
                        rwa, rspa, S, wa, spa);
     phase_end(9);
     // --- kept nodes

@@ -39,7 +39,7 @@ CANDIDATES = [
     ("part_recs", [4096, 6144, 8192, 12288]),
     ("fill_recs", [2048, 4096, 8192]),
     ("index_load", [50, 60, 67, 75, 85]),
-    ("ruler_stride", [8, 16, 32, 64]),
+    ("ruler_stride", [8, 16, 32, 64, 128, 256]),
     ("occ_groups", [1, 2]),
     ("order_bin_keys", [1024, 2048, 4096, 8192]),
 ]
