@@ -96,6 +96,10 @@ __global__ void k_tile_count(const uint8_t *__restrict__ buf, uint64_t n, uint32
     }
 }
 
+// The build's small counters: [0] (first '>' position, taken by atomicMin)
+// starts at ~0, the other 1023 words at 0.
+__global__ void k_small_init(uint64_t *__restrict__ small) { small[threadIdx.x] = threadIdx.x ? 0 : ~0ull; }
+
 // Writes the position of every '\n' in file order.  A thread's 16 bytes give
 // a 16-bit mask of newline bytes; it stores one position per set bit.
 __global__ void k_tile_emit(const uint8_t *__restrict__ buf, uint64_t n, const uint64_t *__restrict__ tile_off,
@@ -1208,7 +1212,10 @@ struct SelInfo {
 };
 
 __global__ void __launch_bounds__(1024) k_sel_targets(const uint32_t *__restrict__ hist, const uint64_t *__restrict__ starts,
-                                                      int n_starts, int *__restrict__ target, SelInfo *__restrict__ si) {
+                                                      int n_starts, int *__restrict__ target, SelInfo *__restrict__ si,
+                                                      uint32_t *__restrict__ sel_out, uint32_t sel_out_words) {
+    // the marking pass's bitmaps start empty
+    for (uint32_t i = threadIdx.x; i < sel_out_words; i += 1024) sel_out[i] = 0;
     __shared__ uint32_t cum[kSelBins + 1];
     __shared__ uint32_t wsum[16];
     constexpr int per = kSelBins / 1024;
@@ -1485,8 +1492,7 @@ int Pipeline::build(const uint8_t *d_in, uint64_t n) {
     const uint64_t scan_words = scan_scratch_words(std::max<uint64_t>(tiles, 1)) + 16;
     SWAP_CHECK(small_.reserve(1024 * 8));
     uint64_t *d_small = small_.as<uint64_t>();  // [0]=first_gt [1]=nnl [2]=j0 [3]=R [4]=N [5]=W [8..]=counters
-    SWAP_CHECK(hipMemsetAsync(d_small, 0xFF, 8, st_));
-    SWAP_CHECK(hipMemsetAsync(d_small + 1, 0, 1023 * 8, st_));
+    hipLaunchKernelGGL(k_small_init, dim3(1), dim3(1024), 0, st_, d_small);
     for (int i = 0; i < kPhases; i++) ev_used_[i] = false;
     phase_begin(0);
     hipLaunchKernelGGL(k_tile_count, dim3((unsigned)tiles), dim3(kTileThreads), 0, st_, buf, n,
@@ -1764,11 +1770,10 @@ int Pipeline::build(const uint8_t *d_in, uint64_t n) {
         std::memcpy(h_starts_, starts.data(), starts.size() * 8);
         SWAP_CHECK(hipMemcpyAsync(d_starts, h_starts_, starts.size() * 8, hipMemcpyHostToDevice, st_));
         SWAP_CHECK(hipMemsetAsync(d_hist, 0, kSelBins * 4, st_));
-        SWAP_CHECK(hipMemsetAsync(d_small + 12, 0, 8, st_));
+        // (d_small[12], the last first occurrence, was zeroed at the build's start)
         SWAP_CHECK(sel_target_.reserve(kSelBins * 4 + 64));
         const uint32_t sel_words = (1u << shift) >= 32 ? (1u << shift) / 32 : 1;  // bitmap words per bin
         SWAP_CHECK(sel_out_.reserve((uint64_t)32 * sel_words * 4 + 16));
-        SWAP_CHECK(hipMemsetAsync(sel_out_.p, 0, (uint64_t)32 * sel_words * 4, st_));
         // two histogram workgroups per CU (64 KB of LDS each): twice the waves
         // reading the first occurrences
         const unsigned sel_rows = 2 * (unsigned)ncu;
@@ -1786,7 +1791,7 @@ int Pipeline::build(const uint8_t *d_in, uint64_t n) {
         hipLaunchKernelGGL(k_col_sum, dim3(kSelBins / 256, kHistRowGroups), dim3(256), 0, st_, rows_.as<uint32_t>(),
                            sel_rows, kSelBins, d_hist);
         hipLaunchKernelGGL(k_sel_targets, dim3(1), dim3(1024), 0, st_, d_hist, d_starts, (int)starts.size(),
-                           sel_target_.as<int>(), d_si);
+                           sel_target_.as<int>(), d_si, sel_out_.as<uint32_t>(), 32 * sel_words);
         hipLaunchKernelGGL(k_sel_mark, dim3(ncu * 4), dim3(1024), 0, st_, dfo_.as<uint32_t>(),
                            (const unsigned long long *)(d_small + 9), shift, sel_target_.as<int>(),
                            sel_out_.as<uint32_t>());

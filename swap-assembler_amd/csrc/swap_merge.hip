@@ -32,6 +32,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cstdio>
+#include <cstring>
 #include <map>
 #include <numeric>
 #include <unordered_map>
@@ -454,6 +455,44 @@ __global__ void k_rs_jump(const Walk *__restrict__ win, const uint8_t *__restric
     spout[v] = f;
 }
 
+// All jump rounds in one workgroup when the rulers are few (small samples):
+// one launch instead of one per round.  Round r reads buffer r & 1 and writes
+// the other; the fences around each barrier make a round's device-memory
+// writes visible to every wave of the workgroup.
+constexpr int kJumpOneThreads = 1024;
+constexpr uint64_t kJumpOneMax = 8192;
+
+__global__ void __launch_bounds__(kJumpOneThreads) k_rs_jump_one(Walk *__restrict__ wa, uint8_t *__restrict__ spa,
+                                                                Walk *__restrict__ wb, uint8_t *__restrict__ spb,
+                                                                uint64_t nr, int rounds) {
+    for (int r = 0; r < rounds; r++) {
+        const Walk *win = (r & 1) ? wb : wa;
+        const uint8_t *spin = (r & 1) ? spb : spa;
+        Walk *wout = (r & 1) ? wa : wb;
+        uint8_t *spout = (r & 1) ? spa : spb;
+        for (uint64_t v = threadIdx.x; v < nr; v += kJumpOneThreads) {
+            Walk x = win[v];
+            uint8_t f = spin[v];
+#pragma unroll
+            for (int hop = 0; hop < 2; hop++) {
+                if (x.succ == kNil) break;
+                const uint32_t s = x.succ;
+                const Walk y = win[s];
+                f |= spin[s];
+                x.dist += y.dist;
+                x.succ = y.succ;
+                x.term = y.term;
+                x.last = y.last;
+            }
+            wout[v] = x;
+            spout[v] = f;
+        }
+        __threadfence();
+        __syncthreads();
+        __threadfence();
+    }
+}
+
 // per oriented node: walk state from its ruler
 __global__ void k_rs_final(const uint8_t *__restrict__ is11, const uint32_t *__restrict__ rid,
                            const uint64_t *__restrict__ seg, const uint2 *__restrict__ label,
@@ -694,7 +733,7 @@ __global__ void __launch_bounds__(kMaxThreads) k_max_levels(const uint64_t *__re
 
 struct TreeArgs {
     const uint64_t *lev;      // concatenated levels (level 0: the chain priorities)
-    const uint64_t *lev_off;  // offset of each level
+    uint64_t lev_off[64];     // offset of each level (passed by value: no copy to the device)
     int nlev;
     uint64_t M;               // chain array length
     const uint8_t *cmul;
@@ -1074,6 +1113,8 @@ int Pipeline::merge_device(bool reference_order, MergeDev &out) {
     hipLaunchKernelGGL(k_index_loose, dim3(nblk(S)), dim3(256), 0, st_, surv_key_.as<uint64_t>(), S,
                        part_first_.as<uint32_t>(), ix, mb_[M_IENT].as<uint64_t>(), perm, runs);
     if (renumbered) {  // the new order in place of the count's
+        // (copied, not swapped: the survivor arrays keep the capacity the
+        // count reserves, the renumbering's stay at S)
         MCHECK(hipMemcpyAsync(surv_key_.p, rn_[R_KEY].p, S * 8, hipMemcpyDeviceToDevice, st_));
         MCHECK(hipMemcpyAsync(surv_mult_.p, rn_[R_MULT].p, S * 8, hipMemcpyDeviceToDevice, st_));
         MCHECK(hipMemcpyAsync(surv_first_.p, rn_[R_FO].p, S * 4, hipMemcpyDeviceToDevice, st_));
@@ -1154,10 +1195,19 @@ int Pipeline::merge_device(bool reference_order, MergeDev &out) {
         int rounds = 1;
         for (uint64_t span = 3; span < nr + 2; span *= 3) rounds++;
         rounds++;
-        for (int r = 0; r < rounds; r++) {
-            hipLaunchKernelGGL(k_rs_jump, dim3(nblk(nr)), dim3(256), 0, st_, rwa, rspa, nr, rwb, rspb);
-            std::swap(rwa, rwb);
-            std::swap(rspa, rspb);
+        if (nr <= kJumpOneMax) {
+            hipLaunchKernelGGL(k_rs_jump_one, dim3(1), dim3(kJumpOneThreads), 0, st_, rwa, rspa, rwb, rspb, nr,
+                               rounds);
+            if (rounds & 1) {  // the last round wrote the second buffer
+                std::swap(rwa, rwb);
+                std::swap(rspa, rspb);
+            }
+        } else {
+            for (int r = 0; r < rounds; r++) {
+                hipLaunchKernelGGL(k_rs_jump, dim3(nblk(nr)), dim3(256), 0, st_, rwa, rspa, nr, rwb, rspb);
+                std::swap(rwa, rwb);
+                std::swap(rspa, rspb);
+            }
         }
     }
     hipLaunchKernelGGL(k_rs_final, dim3(nblk(2 * S)), dim3(256), 0, st_, mb_[M_IS11].as<uint8_t>(), rid, seg, label,
@@ -1262,9 +1312,7 @@ int Pipeline::merge_device(bool reference_order, MergeDev &out) {
             len = (len + 1) / 2;
         }
         MCHECK(mb_[M_LEV].reserve(tot * 8 + 16));
-        MCHECK(mb_[M_LEVOFF].reserve(loff.size() * 8 + 16));
         uint64_t *lev = mb_[M_LEV].as<uint64_t>();
-        MCHECK(hipMemcpyAsync(mb_[M_LEVOFF].p, loff.data(), loff.size() * 8, hipMemcpyHostToDevice, st_));
         if (nlev > 64) return fail(-4, "chain array too long for the max hierarchy");
         LevTab lt;
         len = M;
@@ -1287,10 +1335,9 @@ int Pipeline::merge_device(bool reference_order, MergeDev &out) {
         MCHECK(mb_[M_INL].reserve(M * 4));
         MCHECK(mb_[M_INR].reserve(M * 8));
         MCHECK(hipMemsetAsync(mb_[M_INR].p, 0, M * 8, st_));
-        MCHECK(hipMemsetAsync(d_small + 27, 0, 8, st_));
         TreeArgs ta;
         ta.lev = lev;
-        ta.lev_off = mb_[M_LEVOFF].as<uint64_t>();
+        for (int l = 0; l < 64; l++) ta.lev_off[l] = l < nlev ? loff[l] : 0;
         ta.nlev = nlev;
         ta.M = M;
         ta.cmul = fa.cmul;
@@ -1300,7 +1347,7 @@ int Pipeline::merge_device(bool reference_order, MergeDev &out) {
         ta.nl = mb_[M_NL].as<uint32_t>();
         ta.nr = mb_[M_NR].as<uint32_t>();
         ta.far = mb_[M_INL].as<uint32_t>();
-        ta.nfar = (unsigned int *)(d_small + 27);
+        ta.nfar = (unsigned int *)(d_small + 27);  // zeroed with d_small[16..31] at the merge's start
         ta.err = (unsigned int *)(d_small + 25);
         const unsigned tiles = (unsigned)((M + kTreeTile - 1) / kTreeTile);
         hipLaunchKernelGGL(k_tree_links, dim3(tiles), dim3(kTreeLinkThreads), 0, st_, ta);
@@ -1326,6 +1373,33 @@ int Pipeline::merge_device(bool reference_order, MergeDev &out) {
 }
 
 // Copies the merge results of the last merge_device to the host.
+// The merge's per-edge and per-kept-node results gathered into one buffer, so
+// the host reads them with one copy: segment y of the grid, 8-byte aligned
+// offsets in a.off.
+struct PackArgs {
+    const uint64_t *eoff, *prio, *key;
+    const uint32_t *kept_src, *elen;
+    const uint8_t *emult, *espec;
+    uint64_t ne, nk;
+    uint8_t *dst;
+    uint64_t off[7];
+};
+
+__global__ void k_pack_merge(PackArgs a) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    uint8_t *d = a.dst + a.off[blockIdx.y];
+    switch (blockIdx.y) {
+        case 0: if (i < a.ne) reinterpret_cast<uint64_t *>(d)[i] = a.eoff[i]; break;
+        case 1: if (i < a.nk) reinterpret_cast<uint64_t *>(d)[i] = a.prio[a.kept_src[i]]; break;
+        case 2: if (i < a.nk) reinterpret_cast<uint64_t *>(d)[i] = a.key[a.kept_src[i]]; break;
+        case 3: if (i < a.nk) reinterpret_cast<uint32_t *>(d)[i] = a.kept_src[i]; break;
+        case 4: if (i < a.ne) reinterpret_cast<uint32_t *>(d)[i] = a.elen[i]; break;
+        case 5: if (i < a.ne) d[i] = a.emult[i]; break;
+        case 6: if (i < a.ne) d[i] = a.espec[i]; break;
+        default: break;
+    }
+}
+
 int Pipeline::fetch_merge(MergeDev &out) {
     MCHECK(hipSetDevice(dev_));
     out = MergeDev();
@@ -1337,25 +1411,32 @@ int Pipeline::fetch_merge(MergeDev &out) {
     out.emult.resize(ne);
     out.especial.resize(ne);
     out.pool.resize(pool_n);
-    if (nk) MCHECK(hipMemcpyAsync(out.kept_src.data(), mb_[M_KEPTSRC].p, nk * 4, hipMemcpyDeviceToHost, st_));
-    if (ne) {
-        MCHECK(hipMemcpyAsync(out.elen.data(), mb_[M_ELEN].p, ne * 4, hipMemcpyDeviceToHost, st_));
-        MCHECK(hipMemcpyAsync(out.eoff.data(), mb_[M_EOFF].p, ne * 8, hipMemcpyDeviceToHost, st_));
-        MCHECK(hipMemcpyAsync(out.emult.data(), mb_[M_EMULT].p, ne, hipMemcpyDeviceToHost, st_));
-        MCHECK(hipMemcpyAsync(out.especial.data(), mb_[M_ESPEC].p, ne, hipMemcpyDeviceToHost, st_));
-    }
     if (pool_n) MCHECK(hipMemcpyAsync(out.pool.data(), mb_[M_POOL].p, pool_n, hipMemcpyDeviceToHost, st_));
     out.kept_prio.resize(nk);
     out.kept_key.resize(nk);
-    if (nk) {
-        MCHECK(mb_[M_INL].reserve(2 * nk * 8 + 16));
-        uint64_t *tmp = mb_[M_INL].as<uint64_t>();
-        hipLaunchKernelGGL(k_gather_u64, dim3(nblk(nk)), dim3(256), 0, st_, mb_[M_PRIO].as<uint64_t>(),
-                           mb_[M_KEPTSRC].as<uint32_t>(), nk, tmp);
-        hipLaunchKernelGGL(k_gather_u64, dim3(nblk(nk)), dim3(256), 0, st_, surv_key_.as<uint64_t>(),
-                           mb_[M_KEPTSRC].as<uint32_t>(), nk, tmp + nk);
-        MCHECK(hipMemcpyAsync(out.kept_prio.data(), tmp, nk * 8, hipMemcpyDeviceToHost, st_));
-        MCHECK(hipMemcpyAsync(out.kept_key.data(), tmp + nk, nk * 8, hipMemcpyDeviceToHost, st_));
+    std::vector<uint8_t> packed;
+    PackArgs pa;
+    if (nk || ne) {
+        const uint64_t sz[7] = {ne * 8, nk * 8, nk * 8, nk * 4, ne * 4, ne, ne};
+        uint64_t tot = 0;
+        for (int j = 0; j < 7; j++) {
+            pa.off[j] = tot;
+            tot += (sz[j] + 7) & ~7ull;
+        }
+        MCHECK(mb_[M_INL].reserve(tot + 16));
+        pa.eoff = mb_[M_EOFF].as<uint64_t>();
+        pa.prio = mb_[M_PRIO].as<uint64_t>();
+        pa.key = surv_key_.as<uint64_t>();
+        pa.kept_src = mb_[M_KEPTSRC].as<uint32_t>();
+        pa.elen = mb_[M_ELEN].as<uint32_t>();
+        pa.emult = mb_[M_EMULT].as<uint8_t>();
+        pa.espec = mb_[M_ESPEC].as<uint8_t>();
+        pa.ne = ne;
+        pa.nk = nk;
+        pa.dst = mb_[M_INL].as<uint8_t>();
+        hipLaunchKernelGGL(k_pack_merge, dim3(nblk(std::max(ne, nk)), 7), dim3(256), 0, st_, pa);
+        packed.resize(tot);
+        MCHECK(hipMemcpyAsync(packed.data(), pa.dst, tot, hipMemcpyDeviceToHost, st_));
     }
     if (nspec) {
         out.special.resize(nspec);
@@ -1368,6 +1449,20 @@ int Pipeline::fetch_merge(MergeDev &out) {
         MCHECK(hipMemcpyAsync(out.all_prio.data(), mb_[M_PRIO].p, S * 8, hipMemcpyDeviceToHost, st_));
     }
     MCHECK(hipStreamSynchronize(st_));
+    if (!packed.empty()) {
+        const uint8_t *b = packed.data();
+        if (ne) {
+            std::memcpy(out.eoff.data(), b + pa.off[0], ne * 8);
+            std::memcpy(out.elen.data(), b + pa.off[4], ne * 4);
+            std::memcpy(out.emult.data(), b + pa.off[5], ne);
+            std::memcpy(out.especial.data(), b + pa.off[6], ne);
+        }
+        if (nk) {
+            std::memcpy(out.kept_prio.data(), b + pa.off[1], nk * 8);
+            std::memcpy(out.kept_key.data(), b + pa.off[2], nk * 8);
+            std::memcpy(out.kept_src.data(), b + pa.off[3], nk * 4);
+        }
+    }
     return 0;
 }
 

@@ -93,6 +93,50 @@ static __global__ void k_u64_to_u32(const uint64_t *__restrict__ in, uint64_t n,
     if (out2) out2[i] = v;
 }
 
+// Arrays up to kScanOneMax elements: one workgroup scans them in chunks of
+// kScanOneThreads x kScanItems, carrying the running total (one launch instead
+// of three; many samples' arrays are this small, and per-sample launches
+// bound the many-sample throughput).
+constexpr int kScanOneThreads = 1024;
+constexpr uint64_t kScanOneMax = 1u << 14;
+
+template <typename TIn>
+static __global__ void __launch_bounds__(kScanOneThreads) k_scan_one(const TIn *__restrict__ in,
+                                                                    uint64_t *__restrict__ out, uint64_t n,
+                                                                    uint64_t *__restrict__ total) {
+    __shared__ uint64_t wsum[kScanOneThreads / 64];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    uint64_t carry = 0;
+    for (uint64_t c0 = 0; c0 < n; c0 += (uint64_t)kScanOneThreads * kScanItems) {
+        const uint64_t base = c0 + (uint64_t)threadIdx.x * kScanItems;
+        uint64_t v[kScanItems];
+        uint64_t s = 0;
+#pragma unroll
+        for (int i = 0; i < kScanItems; i++) {
+            v[i] = (base + i < n) ? (uint64_t)in[base + i] : 0;
+            s += v[i];
+        }
+        const uint64_t inc = wave_incl_sum(s);
+        if (lane == 63) wsum[w] = inc;
+        __syncthreads();
+        uint64_t woff = 0, ctot = 0;
+        for (int i = 0; i < kScanOneThreads / 64; i++) {
+            const uint64_t t = wsum[i];
+            if (i < w) woff += t;
+            ctot += t;
+        }
+        uint64_t run = carry + inc - s + woff;
+#pragma unroll
+        for (int i = 0; i < kScanItems; i++) {
+            if (base + i < n) out[base + i] = run;
+            run += v[i];
+        }
+        carry += ctot;
+        __syncthreads();  // wsum is rewritten by the next chunk
+    }
+    if (threadIdx.x == 0 && total) *total = carry;
+}
+
 // total <- sum, out <- exclusive prefix.  scratch must hold scan_scratch_words(n).
 static inline uint64_t scan_scratch_words(uint64_t n) {
     // each level holds its block sums and their exclusive scan
@@ -111,6 +155,10 @@ static hipError_t scan_excl(const TIn *in, uint64_t *out, uint64_t n, uint64_t *
     if (n == 0) return hipMemsetAsync(d_total, 0, 8, st);
     const uint64_t nb = (n + kScanTile - 1) / kScanTile;
     uint64_t *sums = scratch;
+    if (nb > 1 && n <= kScanOneMax) {
+        hipLaunchKernelGGL(k_scan_one<TIn>, dim3(1), dim3(kScanOneThreads), 0, st, in, out, n, d_total);
+        return hipGetLastError();
+    }
     hipLaunchKernelGGL(k_scan_local<TIn>, dim3((unsigned)nb), dim3(kScanThreads), 0, st, in, out, n, sums,
                        nb == 1 ? d_total : (uint64_t *)nullptr);
     if (nb == 1) return hipGetLastError();
