@@ -296,6 +296,56 @@ def batched_vs_per_sample(n_samples, slots, warmup=True):
             "identical_outputs": same, "reference_checked": ref_n, "reference_identical": ref_ok}
 
 
+def plasmid_reads(genome_bp=100000, read_len=100, coverage=50, error_rate=0.01, seed=7):
+    """FASTA reads of a seeded circular genome (a plasmid): reads wrap around
+    its end, so its unitig closes into a cycle; both strands, uniform
+    substitution errors."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    g = rng.integers(0, 4, genome_bp, dtype=np.uint8)
+    n = genome_bp * coverage // read_len
+    starts = rng.integers(0, genome_bp, n)
+    idx = (starts[:, None] + np.arange(read_len)[None, :]) % genome_bp
+    r = g[idx]
+    rev = rng.random(n) < 0.5
+    r[rev] = 3 - r[rev, ::-1]
+    err = rng.random(r.shape) < error_rate
+    r = np.where(err, (r + rng.integers(1, 4, r.shape, dtype=np.uint8)) % 4, r).astype(np.uint8)
+    seqs = np.frombuffer(b"ACGT", dtype=np.uint8)[r]
+    out = bytearray()
+    for i in range(n):
+        out += b">p%d\n" % i
+        out += seqs[i].tobytes()
+        out += b"\n"
+    return bytes(out)
+
+
+def with_cycle(fa, k, c, steps, base_ms):
+    """The bacterial reads plus a 100 kbp circular plasmid's at 50x: the plasmid's
+    unitig is a cycle, merged by the host replay of cycles and fold-backs
+    (reference order).  Step time of build + merge with inputs in HBM."""
+    import swapamd
+    fa2 = fa + plasmid_reads()
+    dev = swapamd.DeviceBuffer(fa2)
+    a = swapamd.Assembler(k, c)
+    for _ in range(2):
+        a.build_device(dev.ptr.value, dev.n)
+        a.merge()
+    swapamd.hip().hipDeviceSynchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        a.build_device(dev.ptr.value, dev.n)
+        a.merge()
+    swapamd.hip().hipDeviceSynchronize()
+    ms = 1000.0 * (time.perf_counter() - t0) / steps
+    st = a.stats()
+    a.close()
+    dev.close()
+    return {"workload": "bacterial + 100 kbp circular plasmid at 50x", "ms_per_step": ms,
+            "bacterial_ms_per_step": base_ms, "special": st["special"], "kept": st["kept"],
+            "survivors": st["survivors"]}
+
+
 def serving(n_requests, rate, slots, seed=5):
     """The request-stream workload: requests arrive as a Poisson stream (mean
     `rate` per second), each a seeded genome log-uniform in [50 kbp, 5 Mbp] with
@@ -478,6 +528,9 @@ def main():
     build_ms = sum(phases.get(i, 0.0) for i in BUILD_PHASES)
     roof["build"] = {"algorithmic_bytes": bb, "ms": build_ms, "achieved": bb / (build_ms * 1e-3) / 1e9,
                      "unit": "GB/s", "frac": bb / (build_ms * 1e-3) / HBM_PEAK}
+    cycle = None
+    if ref_order and has_merge and args.workload == "bacterial":
+        cycle = with_cycle(fa, k, c, max(3, args.steps // 2), ms)
     cpu = None
     if not args.no_cpu_baseline:
         cpu = reference_baseline(fa, k, c) or cpu_baseline(fa, k, c)
@@ -513,6 +566,7 @@ def main():
         "tuning": tuning,
         "phase_ms": {PHASES[i]: round(v, 4) for i, v in phases.items()},
         "host_input": host_input,
+        "with_cycle": cycle,
         "batched": batched,
         "serving": served,
     }

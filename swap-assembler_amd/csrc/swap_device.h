@@ -136,7 +136,12 @@ struct MergeDev {
     std::vector<uint32_t> special;    // (1,1) survivors of cycle / fold-back components
     std::vector<uint64_t> kept_prio;  // visit priority of each kept node
     std::vector<uint64_t> kept_key;   // key of each kept node
-    // all survivors (device order), copied only when special components exist
+    // when special components exist, reference order: the special nodes' key,
+    // multiplicities and priority, the kept nodes' multiplicities, and the
+    // (1,1) nodes as a bitmap over visit ranks (priorities are ranks 0..S-1)
+    std::vector<uint64_t> spec_key, spec_mult, spec_prio, kept_mult;
+    std::vector<uint32_t> one_bits;
+    // hash order (priorities are key hashes, not ranks): all survivors
     std::vector<uint64_t> all_key, all_mult, all_prio;
     uint64_t n11 = 0;
 };
@@ -237,6 +242,7 @@ private:
     uint64_t *h_small_ = nullptr;
     uint64_t last_first_ = 0;
     uint64_t merged_ne_ = 0, merged_pool_ = 0, merged_M_ = 0;
+    bool merged_ref_ = true;  // the last merge's visit priorities are reference-order ranks
     bool epoch_first_valid_ = false;
     EpochTabHost epoch1_;           // growth epochs of the k-mer table (starts on the device)
     DevBuf etab_, etabS_;           // device epoch tables: k-mer table, S-sized tables

@@ -364,34 +364,119 @@ namespace {
 
 // A node of a cycle / fold-back component (or a kept node next to one), with
 // the reference's string edges (graph.cpp:42-49).
+constexpr uint32_t kNoNode = 0xFFFFFFFFu;
+
+// A replayed node.  Each slot's edge string is a rope (the splices only ever
+// append one string to another), with its length and its last 31 bases
+// packed two bits each: all the SWAP pass reads of a string are its length
+// (the splice arithmetic) and its terminal k-mer (node::getNodeID), so the
+// characters are only laid out for the output.
 struct HNode {
     uint64_t id = 0;
-    std::string arcs[8];
+    uint32_t rope[8];
+    uint64_t len[8] = {0};
+    uint64_t tail[8] = {0};
     uint8_t m[8] = {0};
     bool live = true;
     bool one = false;  // exactly one left and one right edge
     uint64_t prio = 0;
     int64_t kept = -1;  // kept index, or -1 for a (1,1) node of the component
+    // the node at the end of each slot's string and the direction it is
+    // reached in (node::getNodeID), kept current as the strings grow
+    uint64_t nb[8] = {0};
+    int8_t nbs[8] = {0};
+    uint32_t nbi[8];  // index of that node among the replay's nodes (kNoNode: none)
+};
+
+struct Rope {
+    uint32_t left, right;  // concatenation of two ropes, or a leaf (left == kNoNode)
+    char base;
 };
 
 struct Replay {
     int k;
     uint64_t km;
-    std::unordered_map<uint64_t, HNode> nodes;
+    std::vector<HNode> nodes;
+    std::unordered_map<uint64_t, uint32_t> index;  // key -> position in nodes
+    std::vector<Rope> ropes;
 
     uint64_t orient(uint64_t X, int s) const { return s >= 4 ? revcomp(X, k) : X; }
-    // node::getNodeID (graph.cpp:164-196)
-    int neighbour(const HNode &n, int s, uint64_t *id) const {
-        return terminal(n.id, s, n.arcs[s].data(), n.arcs[s].size(), k, id);
-    }
-    // node::getSrcNodeEdge, direction 2 (graph.cpp:263-324)
-    int src_slot(const HNode &d, uint64_t src, int side) const {
-        for (int i = 0; i < 8; i++) {
-            uint64_t id;
-            const int r = neighbour(d, i, &id);
-            if (id == src && (r == 1 ? 1 : 0) == side) return i;
+    // node::getNodeID (graph.cpp:164-196) of slot s, after its string changed:
+    // the last k bases of orient(X) followed by the string
+    void refresh(HNode &n, int s) const {
+        const uint64_t L = n.len[s];
+        uint64_t t;
+        if (L >= (uint64_t)k) t = n.tail[s] & km;
+        else t = ((orient(n.id, s) << (2 * L)) | n.tail[s]) & km;
+        const uint64_t r = revcomp(t, k);
+        if (t < r) {
+            n.nb[s] = r;
+            n.nbs[s] = -1;
+        } else {
+            n.nb[s] = t;
+            n.nbs[s] = 1;
         }
+    }
+    void refresh_all(HNode &n) const {
+        for (int s = 0; s < 8; s++) refresh(n, s);
+    }
+    // the position of slot s's node (after refresh)
+    void link(HNode &n, int s) const {
+        auto it = index.find(n.nb[s]);
+        n.nbi[s] = it == index.end() ? kNoNode : it->second;
+    }
+    void link_all(HNode &n, uint32_t self) const {
+        for (int s = 0; s < 8; s++) {
+            if (n.len[s]) link(n, s);
+            else n.nbi[s] = self;  // an empty slot leads back to the node itself
+        }
+    }
+    int neighbour(const HNode &n, int s, uint64_t *id) const {
+        *id = n.nb[s];
+        return n.nbs[s];
+    }
+    // node::getSrcNodeEdge, direction 2 (graph.cpp:263-324): every slot is
+    // tested, an empty one included (it leads back to the node itself)
+    int src_slot(const HNode &d, uint64_t src, int side) const {
+        for (int i = 0; i < 8; i++)
+            if (d.nb[i] == src && (d.nbs[i] == 1 ? 1 : 0) == side) return i;
         return -1;
+    }
+    uint32_t leaf(char base) {
+        ropes.push_back(Rope{kNoNode, kNoNode, base});
+        return (uint32_t)(ropes.size() - 1);
+    }
+    // string s of d += the string (rope, length, tail) of another slot
+    void append(HNode &d, int s, uint32_t rope, uint64_t len, uint64_t tail) {
+        if (!len) return;
+        if (!d.len[s]) {
+            d.rope[s] = rope;
+        } else {
+            ropes.push_back(Rope{d.rope[s], rope, 0});
+            d.rope[s] = (uint32_t)(ropes.size() - 1);
+        }
+        d.tail[s] = len >= 31 ? tail : ((d.tail[s] << (2 * len)) | tail);
+        d.tail[s] &= (1ull << 62) - 1;
+        d.len[s] += len;
+    }
+    // the characters of a rope
+    std::string text(uint32_t rope, uint64_t len) const {
+        std::string out;
+        out.reserve(len);
+        if (!len) return out;
+        std::vector<uint32_t> stack{rope};
+        while (!stack.empty()) {
+            const uint32_t r = stack.back();
+            stack.pop_back();
+            const Rope &x = ropes[r];
+            if (x.left == kNoNode) {
+                out.push_back(x.base);
+            } else {
+                stack.push_back(x.right);
+                stack.push_back(x.left);
+            }
+        }
+        return out;
     }
 };
 
@@ -422,13 +507,20 @@ int host_merge(Pipeline &p, const swap_cfg &cfg, bool *materialized, HostCompact
 }
 
 int materialize_merge(Pipeline &p, const swap_cfg &cfg, HostCompacted &out) {
-    const bool ref = !(cfg.flags & SWAP_FLAG_HASH_ORDER);
-    const int k = cfg.k;
-    out = HostCompacted();
     MergeDev md;
     int rc = p.fetch_merge(md);
     if (rc) return rc;
-    const size_t S = p.n_surv;
+    uint64_t kept = 0;
+    rc = replay_merge(md, p.n_surv, cfg, out, &kept);
+    if (rc) return rc;
+    p.n_kept = kept;
+    return 0;
+}
+
+int replay_merge(const MergeDev &md, size_t S, const swap_cfg &cfg, HostCompacted &out, uint64_t *n_kept) {
+    const bool ref = !(cfg.flags & SWAP_FLAG_HASH_ORDER);
+    const int k = cfg.k;
+    out = HostCompacted();
     const uint64_t P3 = ref ? table_buckets(S, false) : 0;
     const size_t nk = md.n_kept;
     // per kept node: 8 (string, multiplicity) taken from the device results
@@ -459,47 +551,97 @@ int materialize_merge(Pipeline &p, const swap_cfg &cfg, HostCompacted &out) {
         Replay R;
         R.k = k;
         R.km = kmask(k);
+        // reference order: the device sent the special nodes, the kept nodes'
+        // multiplicities and a bitmap of the (1,1) visit ranks; hash order: all
+        // survivors (its priorities are key hashes, not ranks)
+        const bool ranks = md.all_key.empty();
         const std::vector<uint64_t> &gkey = md.all_key, &gmult = md.all_mult, &prio = md.all_prio;
         std::unordered_map<uint64_t, uint32_t> sidx_of;
-        sidx_of.reserve(S * 2);
-        for (size_t i = 0; i < S; i++) sidx_of[gkey[i]] = (uint32_t)i;
-        auto make = [&](uint32_t sidx, int64_t keptIdx) {
+        if (!ranks) {
+            sidx_of.reserve(S * 2);
+            for (size_t i = 0; i < S; i++) sidx_of[gkey[i]] = (uint32_t)i;
+        }
+        // the component's nodes: (1,1) nodes first, sorted by visit priority so
+        // the SWAP pass walks them in memory order, then their kept neighbours;
+        // a key's node is found through idx_of
+        std::vector<HNode> &nodes = R.nodes;
+        std::unordered_map<uint64_t, uint32_t> &idx_of = R.index;
+        auto make = [&](uint64_t key, uint64_t mult, uint64_t pr, int64_t keptIdx) {
             HNode n;
-            n.id = gkey[sidx];
-            n.prio = prio[sidx];
+            n.id = key;
+            n.prio = pr;
             n.kept = keptIdx;
             int nr = 0, nl = 0;
             for (int s = 0; s < 8; s++) {
-                n.m[s] = (uint8_t)(gmult[sidx] >> (8 * s));
+                n.m[s] = (uint8_t)(mult >> (8 * s));
+                n.rope[s] = kNoNode;
                 if (n.m[s]) {
-                    n.arcs[s] = std::string(1, "ACGT"[s & 3]);
+                    n.rope[s] = R.leaf("ACGT"[s & 3]);
+                    n.len[s] = 1;
+                    n.tail[s] = (uint64_t)(s & 3);
                     (s < 4 ? nr : nl)++;
                 }
             }
             n.one = nr == 1 && nl == 1;
+            R.refresh_all(n);
             return n;
         };
+        const size_t nspec = md.special.size();
+        std::vector<std::pair<uint64_t, uint32_t>> pj(nspec);  // (priority, special index)
+        for (size_t j = 0; j < nspec; j++) pj[j] = {ranks ? md.spec_prio[j] : prio[md.special[j]], (uint32_t)j};
+        std::sort(pj.begin(), pj.end());
+        std::vector<uint32_t> by_prio(nspec);
+        for (size_t j = 0; j < nspec; j++) by_prio[j] = pj[j].second;
+        nodes.reserve(nspec + 64);  // (kept neighbours: a few per component)
+        idx_of.reserve(2 * nspec + 128);
+        R.ropes.reserve(6 * nspec + 64);
+        for (uint32_t j : by_prio) {
+            HNode n = ranks ? make(md.spec_key[j], md.spec_mult[j], md.spec_prio[j], -1)
+                            : make(gkey[md.special[j]], gmult[md.special[j]], prio[md.special[j]], -1);
+            if (idx_of.emplace(n.id, (uint32_t)nodes.size()).second) nodes.push_back(std::move(n));
+        }
         std::unordered_map<uint64_t, int64_t> kept_index;
         for (size_t i = 0; i < nk; i++) kept_index[md.kept_key[i]] = (int64_t)i;
-        for (uint32_t sidx : md.special) R.nodes[gkey[sidx]] = make(sidx, -1);
         // kept neighbours of the component
-        std::vector<uint64_t> spec_keys;
-        for (auto &kv : R.nodes) spec_keys.push_back(kv.first);
-        for (uint64_t key : spec_keys) {
-            const HNode &n = R.nodes[key];
+        const size_t n_one = nodes.size();
+        for (size_t q = 0; q < n_one; q++) {
             for (int s = 0; s < 8; s++) {
-                if (n.arcs[s].empty()) continue;
+                if (!nodes[q].len[s]) continue;
                 uint64_t id;
-                R.neighbour(n, s, &id);
-                if (R.nodes.count(id)) continue;
+                R.neighbour(nodes[q], s, &id);
+                if (idx_of.count(id)) continue;
                 auto it = kept_index.find(id);
                 if (it == kept_index.end()) continue;  // cannot happen: neighbours are survivors
-                R.nodes[id] = make(sidx_of[id], it->second);
+                const size_t ki = (size_t)it->second;
+                HNode n;
+                if (ranks) {
+                    n = make(id, md.kept_mult[ki], md.kept_prio[ki], it->second);
+                } else {
+                    const uint32_t sidx = sidx_of[id];
+                    n = make(id, gmult[sidx], prio[sidx], it->second);
+                }
+                idx_of.emplace(id, (uint32_t)nodes.size());
+                nodes.push_back(std::move(n));
             }
         }
+        for (size_t q = 0; q < nodes.size(); q++) R.link_all(nodes[q], (uint32_t)q);
         // all (1,1) priorities, for the deferred unlock of a half-locked neighbour
         std::vector<uint64_t> one_prio;
         auto next_one_after = [&](uint64_t after, uint64_t excluding_prio) -> uint64_t {
+            if (ranks) {  // the next set bit above `after` (ranks are unique: one may be excluded)
+                const std::vector<uint32_t> &b = md.one_bits;
+                for (uint64_t r = after + 1; (r >> 5) < b.size();) {
+                    const uint32_t w = b[r >> 5] >> (r & 31);
+                    if (!w) {
+                        r = (r | 31) + 1;
+                        continue;
+                    }
+                    r += (uint64_t)__builtin_ctz(w);
+                    if (r != excluding_prio) return r;
+                    r++;
+                }
+                return ~0ull;
+            }
             if (one_prio.empty()) {
                 for (size_t i = 0; i < S; i++) {
                     const uint64_t m = gmult[i];
@@ -514,21 +656,18 @@ int materialize_merge(Pipeline &p, const swap_cfg &cfg, HostCompacted &out) {
             while (it != one_prio.end() && *it == excluding_prio) ++it;
             return it == one_prio.end() ? ~0ull : *it;
         };
-        // SWAP pass (graph.cpp:1392-1566) over the component's (1,1) nodes in visit order
-        std::vector<HNode *> order;
-        for (auto &kv : R.nodes)
-            if (kv.second.kept < 0) order.push_back(&kv.second);
-        std::sort(order.begin(), order.end(), [](const HNode *a, const HNode *b) { return a->prio < b->prio; });
+        // SWAP pass (graph.cpp:1392-1566) over the component's (1,1) nodes in
+        // visit order: nodes[0 .. n_one)
         bool pending = false;
         uint64_t pendingId = 0, pendingAt = 0, pendingPrio = 0;
-        for (HNode *Mp : order) {
-            HNode &M = *Mp;
+        for (size_t q = 0; q < n_one; q++) {
+            HNode &M = nodes[q];
             if (!M.live) continue;
             if (pending && next_one_after(pendingAt, pendingPrio) < M.prio) pending = false;
             if (pending && pendingId == M.id) continue;  // still locked: its own trylock fails
             int li = -1, ri = -1, nl = 0, nr = 0;
-            for (int s = 4; s < 8; s++) if (!M.arcs[s].empty()) { nl++; li = s; }
-            for (int s = 0; s < 4; s++) if (!M.arcs[s].empty()) { nr++; ri = s; }
+            for (int s = 4; s < 8; s++) if (M.len[s]) { nl++; li = s; }
+            for (int s = 0; s < 4; s++) if (M.len[s]) { nr++; ri = s; }
             if (nl != 1 || nr != 1) continue;
             uint64_t L, Rr;
             R.neighbour(M, li, &L);
@@ -541,40 +680,50 @@ int materialize_merge(Pipeline &p, const swap_cfg &cfg, HostCompacted &out) {
                     pending = true;
                     pendingId = lockL ? L : Rr;
                     pendingAt = M.prio;
-                    auto it = R.nodes.find(pendingId);
-                    pendingPrio = (it != R.nodes.end() && it->second.kept < 0) ? it->second.prio : ~0ull;
+                    const uint32_t pi = lockL ? M.nbi[li] : M.nbi[ri];
+                    pendingPrio = (pi != kNoNode && nodes[pi].kept < 0) ? nodes[pi].prio : ~0ull;
                 }
                 continue;
             }
-            const std::string leftEdge = M.arcs[li], rightEdge = M.arcs[ri];
-            const uint8_t leftM = M.m[li], rightM = M.m[ri];
-            const uint64_t dsts[2] = {Rr, L};
+            // the right neighbour's back edge takes M's left edge and the left
+            // neighbour's M's right edge (M's two strings, taken before either
+            // splice)
+            const uint32_t srope[2] = {M.rope[li], M.rope[ri]};
+            const uint64_t slen[2] = {M.len[li], M.len[ri]}, stail[2] = {M.tail[li], M.tail[ri]};
+            const uint8_t ms[2] = {M.m[li], M.m[ri]};
+            const uint32_t dsts[2] = {M.nbi[ri], M.nbi[li]};
             const int sides[2] = {0, 1};
-            const std::string *strs[2] = {&leftEdge, &rightEdge};
-            const uint8_t ms[2] = {leftM, rightM};
-            for (int q = 0; q < 2; q++) {
-                HNode &D = R.nodes[dsts[q]];
-                const int idx = R.src_slot(D, M.id, sides[q]);
+            for (int q2 = 0; q2 < 2; q2++) {
+                if (dsts[q2] == kNoNode) return SWAP_E_DEVICE;  // a neighbour outside the component
+                HNode &D = nodes[dsts[q2]];
+                const int idx = R.src_slot(D, M.id, sides[q2]);
                 if (idx < 0) return SWAP_E_DEVICE;
-                D.m[idx] = splice_service(D.m[idx], D.arcs[idx].size(), ms[q], strs[q]->size());
-                D.arcs[idx] += *strs[q];
+                D.m[idx] = splice_service(D.m[idx], D.len[idx], ms[q2], slen[q2]);
+                R.append(D, idx, srope[q2], slen[q2], stail[q2]);
+                R.refresh(D, idx);
+                R.link(D, idx);
             }
             M.live = false;
         }
         // The table as the merge leaves it: contigGraph.txt, the -p file and the
         // contig extension's input are written from this state (graph.cpp:3591-3622).
         auto collect = [&](std::vector<Out> &dst) {
-            for (auto &kv : R.nodes) {
-                HNode &n = kv.second;
+            for (HNode &n : nodes) {
                 if (n.kept >= 0) {
                     Out &o = dst[(size_t)n.kept];
                     for (int s = 0; s < 8; s++)
-                        if (md.especial[8 * (size_t)n.kept + s]) { o.arcs[s] = n.arcs[s]; o.m[s] = n.m[s]; }
+                        if (md.especial[8 * (size_t)n.kept + s]) {
+                            o.arcs[s] = R.text(n.rope[s], n.len[s]);
+                            o.m[s] = n.m[s];
+                        }
                 } else if (n.live) {
                     Out o;
                     o.id = n.id;
                     o.prio = n.prio;
-                    for (int s = 0; s < 8; s++) { o.arcs[s] = n.arcs[s]; o.m[s] = n.arcs[s].empty() ? 0 : n.m[s]; }
+                    for (int s = 0; s < 8; s++) {
+                        o.arcs[s] = R.text(n.rope[s], n.len[s]);
+                        o.m[s] = n.len[s] ? n.m[s] : 0;
+                    }
                     dst.push_back(std::move(o));
                 }
             }
@@ -583,41 +732,45 @@ int materialize_merge(Pipeline &p, const swap_cfg &cfg, HostCompacted &out) {
         collect(after_swap);
         // masterSimplifyNodeGraph (graph.cpp:2089-2192) over the live (1,1) nodes
         // left, in the order of the table the gather re-fills (graph.cpp:3614-3615)
-        std::vector<HNode *> rest;
-        for (auto &kv : R.nodes)
-            if (kv.second.kept < 0 && kv.second.live) rest.push_back(&kv.second);
+        std::vector<uint32_t> rest;
+        for (size_t q = 0; q < n_one; q++)
+            if (nodes[q].live) rest.push_back((uint32_t)q);
         if (ref && P3) {
-            std::sort(rest.begin(), rest.end(), [&](const HNode *a, const HNode *b) {
-                const uint64_t ba = ref_hash(a->id) % P3, bb = ref_hash(b->id) % P3;
+            std::sort(rest.begin(), rest.end(), [&](uint32_t a, uint32_t b) {
+                const uint64_t ba = ref_hash(nodes[a].id) % P3, bb = ref_hash(nodes[b].id) % P3;
                 if (ba != bb) return ba < bb;
-                return a->prio > b->prio;
+                return nodes[a].prio > nodes[b].prio;
             });
         } else {
-            std::sort(rest.begin(), rest.end(), [](const HNode *a, const HNode *b) { return a->prio < b->prio; });
+            std::sort(rest.begin(), rest.end(), [&](uint32_t a, uint32_t b) { return nodes[a].prio < nodes[b].prio; });
         }
         bool finisher_merged = false;
-        for (HNode *Mp : rest) {
-            HNode &M = *Mp;
+        for (uint32_t q : rest) {
+            HNode &M = nodes[q];
             if (!M.live) continue;
             int left = -1, right = -1, nl = 0, nr = 0;
-            for (int s = 0; s < 4; s++) if (!M.arcs[s].empty()) { right = s; nr++; }
-            for (int s = 4; s < 8; s++) if (!M.arcs[s].empty()) { left = s; nl++; }
+            for (int s = 0; s < 4; s++) if (M.len[s]) { right = s; nr++; }
+            for (int s = 4; s < 8; s++) if (M.len[s]) { left = s; nl++; }
             if (nr != 1 || nl != 1) continue;
             uint64_t rightNode, leftNode;
             R.neighbour(M, right, &rightNode);
             R.neighbour(M, left, &leftNode);
-            auto itR = R.nodes.find(rightNode), itL = R.nodes.find(leftNode);
-            if (itR == R.nodes.end() || itL == R.nodes.end()) continue;
-            const int rr = R.src_slot(itR->second, M.id, 0);
-            const int lr = R.src_slot(itL->second, M.id, 1);
+            const uint32_t ir = M.nbi[right], il = M.nbi[left];
+            if (ir == kNoNode || il == kNoNode) continue;
+            const int rr = R.src_slot(nodes[ir], M.id, 0);
+            const int lr = R.src_slot(nodes[il], M.id, 1);
             if (rr == -1 || lr == -1) continue;
             if (leftNode == M.id || rightNode == M.id) continue;
-            HNode &Rn = itR->second;
-            Rn.m[rr] = splice_master(Rn.m[rr], Rn.arcs[rr].size(), M.m[left], M.arcs[left].size());
-            Rn.arcs[rr] += M.arcs[left];
-            HNode &Ln = itL->second;
-            Ln.m[lr] = splice_master(Ln.m[lr], Ln.arcs[lr].size(), M.m[right], M.arcs[right].size());
-            Ln.arcs[lr] += M.arcs[right];
+            HNode &Rn = nodes[ir];
+            Rn.m[rr] = splice_master(Rn.m[rr], Rn.len[rr], M.m[left], M.len[left]);
+            R.append(Rn, rr, M.rope[left], M.len[left], M.tail[left]);
+            R.refresh(Rn, rr);
+            R.link(Rn, rr);
+            HNode &Ln = nodes[il];
+            Ln.m[lr] = splice_master(Ln.m[lr], Ln.len[lr], M.m[right], M.len[right]);
+            R.append(Ln, lr, M.rope[right], M.len[right], M.tail[right]);
+            R.refresh(Ln, lr);
+            R.link(Ln, lr);
             M.live = false;
             finisher_merged = true;
         }
@@ -696,7 +849,7 @@ int materialize_merge(Pipeline &p, const swap_cfg &cfg, HostCompacted &out) {
         fin->node_buckets = out.node_buckets;
         out.finished = std::move(fin);
     }
-    p.n_kept = live.size();
+    *n_kept = live.size();
     return 0;
 }
 

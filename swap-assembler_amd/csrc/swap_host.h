@@ -59,5 +59,9 @@ int contig_extension(int k, const HostCompacted &g, CeFiles &out);
 // *materialized is false; materialize_merge() copies and assembles them.
 int host_merge(Pipeline &p, const swap_cfg &cfg, bool *materialized, HostCompacted &out);
 int materialize_merge(Pipeline &p, const swap_cfg &cfg, HostCompacted &out);
+// The host half of materialize_merge: the replay of cycles and fold-backs over
+// the merge results fetched from the device (S survivors), and the compacted
+// graph in file order; *n_kept receives its node count.
+int replay_merge(const MergeDev &md, size_t S, const swap_cfg &cfg, HostCompacted &out, uint64_t *n_kept);
 
 }  // namespace swapk

@@ -1366,6 +1366,7 @@ int Pipeline::merge_device(bool reference_order, MergeDev &out) {
     collect_phases();
     n_kept = nk;
     n_special = nspec;
+    merged_ref_ = reference_order;
     merged_ne_ = ne;
     merged_pool_ = pool_n;
     merged_M_ = M;
@@ -1397,6 +1398,32 @@ __global__ void k_pack_merge(PackArgs a) {
         case 5: if (i < a.ne) d[i] = a.emult[i]; break;
         case 6: if (i < a.ne) d[i] = a.espec[i]; break;
         default: break;
+    }
+}
+
+// What the host replay of cycles / fold-backs needs beyond the kept nodes:
+// the special nodes (key, multiplicities, priority), the kept nodes'
+// multiplicities, and a bitmap of the (1,1) nodes by visit rank.
+__global__ void k_special_gather(const uint32_t *__restrict__ special, uint64_t nspec,
+                                 const uint32_t *__restrict__ kept_src, uint64_t nk,
+                                 const uint64_t *__restrict__ key, const uint64_t *__restrict__ mult,
+                                 const uint64_t *__restrict__ prio, uint64_t *__restrict__ out) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < nspec) {
+        const uint32_t j = special[i];
+        out[i] = key[j];
+        out[nspec + i] = mult[j];
+        out[2 * nspec + i] = prio[j];
+    }
+    if (i < nk) out[3 * nspec + i] = mult[kept_src[i]];
+}
+
+__global__ void k_one_bits(const uint8_t *__restrict__ is11, const uint64_t *__restrict__ prio, uint64_t S,
+                           uint32_t *__restrict__ bits) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < S && is11[i]) {
+        const uint64_t r = prio[i];  // a rank < S
+        atomicOr(&bits[r >> 5], 1u << (r & 31));
     }
 }
 
@@ -1438,17 +1465,43 @@ int Pipeline::fetch_merge(MergeDev &out) {
         packed.resize(tot);
         MCHECK(hipMemcpyAsync(packed.data(), pa.dst, tot, hipMemcpyDeviceToHost, st_));
     }
+    std::vector<uint64_t> sg;
     if (nspec) {
         out.special.resize(nspec);
         MCHECK(hipMemcpyAsync(out.special.data(), mb_[M_SPEC].p, nspec * 4, hipMemcpyDeviceToHost, st_));
-        out.all_key.resize(S);
-        out.all_mult.resize(S);
-        out.all_prio.resize(S);
-        MCHECK(hipMemcpyAsync(out.all_key.data(), surv_key_.p, S * 8, hipMemcpyDeviceToHost, st_));
-        MCHECK(hipMemcpyAsync(out.all_mult.data(), surv_mult_.p, S * 8, hipMemcpyDeviceToHost, st_));
-        MCHECK(hipMemcpyAsync(out.all_prio.data(), mb_[M_PRIO].p, S * 8, hipMemcpyDeviceToHost, st_));
+        if (merged_ref_) {
+            // O(special + kept) values and S bits instead of three S-entry arrays
+            const uint64_t words = (S + 31) / 32;
+            MCHECK(mb_[M_PAR].reserve((3 * nspec + nk) * 8 + 16));
+            MCHECK(mb_[M_NEED].reserve(words * 4 + 16));
+            uint64_t *g = mb_[M_PAR].as<uint64_t>();
+            uint32_t *bits = mb_[M_NEED].as<uint32_t>();
+            MCHECK(hipMemsetAsync(bits, 0, words * 4, st_));
+            hipLaunchKernelGGL(k_special_gather, dim3(nblk(std::max(nspec, nk))), dim3(256), 0, st_,
+                               mb_[M_SPEC].as<uint32_t>(), nspec, mb_[M_KEPTSRC].as<uint32_t>(), nk,
+                               surv_key_.as<uint64_t>(), surv_mult_.as<uint64_t>(), mb_[M_PRIO].as<uint64_t>(), g);
+            hipLaunchKernelGGL(k_one_bits, dim3(nblk(S)), dim3(256), 0, st_, mb_[M_IS11].as<uint8_t>(),
+                               mb_[M_PRIO].as<uint64_t>(), S, bits);
+            sg.resize(3 * nspec + nk);
+            out.one_bits.resize(words);
+            MCHECK(hipMemcpyAsync(sg.data(), g, sg.size() * 8, hipMemcpyDeviceToHost, st_));
+            MCHECK(hipMemcpyAsync(out.one_bits.data(), bits, words * 4, hipMemcpyDeviceToHost, st_));
+        } else {
+            out.all_key.resize(S);
+            out.all_mult.resize(S);
+            out.all_prio.resize(S);
+            MCHECK(hipMemcpyAsync(out.all_key.data(), surv_key_.p, S * 8, hipMemcpyDeviceToHost, st_));
+            MCHECK(hipMemcpyAsync(out.all_mult.data(), surv_mult_.p, S * 8, hipMemcpyDeviceToHost, st_));
+            MCHECK(hipMemcpyAsync(out.all_prio.data(), mb_[M_PRIO].p, S * 8, hipMemcpyDeviceToHost, st_));
+        }
     }
     MCHECK(hipStreamSynchronize(st_));
+    if (!sg.empty()) {
+        out.spec_key.assign(sg.begin(), sg.begin() + nspec);
+        out.spec_mult.assign(sg.begin() + nspec, sg.begin() + 2 * nspec);
+        out.spec_prio.assign(sg.begin() + 2 * nspec, sg.begin() + 3 * nspec);
+        out.kept_mult.assign(sg.begin() + 3 * nspec, sg.end());
+    }
     if (!packed.empty()) {
         const uint8_t *b = packed.data();
         if (ne) {
