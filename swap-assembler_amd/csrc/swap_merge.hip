@@ -1068,6 +1068,65 @@ __global__ void k_special_flags(const uint8_t *__restrict__ is11, const Walk *__
         if (e_ != hipSuccess) return fail(-5, hipGetErrorString(e_));        \
     } while (0)
 
+// Simple cycles on the device.  An isolated cycle of (1,1) nodes, none of
+// them its own neighbour and the cycle not its own reverse complement, merges
+// in the reference's visit order with every lock granted (each node's two
+// neighbours are other nodes while two or more are left), so every node but the
+// last-visited one is absorbed: the result is the chain from the cycle's
+// highest-priority node around to itself.  Making that node a kept node turns
+// the cycle into such a chain for the list ranking, the strings and the
+// Cartesian trees.  Candidates: oriented (1,1) nodes whose walk never reaches a
+// kept node; each learns its cycle's maximum (priority, oriented id) and
+// whether a self-adjacent node lies on it by pointer doubling.
+struct CyclePred {
+    const uint8_t *is11;
+    const Walk *walk;
+    __device__ bool operator()(uint64_t v) const { return is11[v >> 1] && walk[v].succ != kNil; }
+};
+
+__global__ void k_cyc_init(const uint32_t *__restrict__ cl, uint64_t nc, const uint32_t *__restrict__ cpos,
+                           const uint32_t *__restrict__ nxt, const uint64_t *__restrict__ prio,
+                           const uint8_t *__restrict__ selfadj, uint64_t *__restrict__ key, uint32_t *__restrict__ nn,
+                           uint8_t *__restrict__ bad) {
+    const uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= nc) return;
+    const uint32_t v = cl[j];
+    key[j] = (prio[v >> 1] << 32) | v;
+    const uint32_t t = nxt[v];
+    const uint32_t q = t == kNil ? kNil : cpos[t];
+    nn[j] = q == kNil ? (uint32_t)j : q;  // (a successor off the candidates: not a cycle, never converted)
+    bad[j] = (uint8_t)((selfadj[v >> 1] ? 1 : 0) | (q == kNil ? 1 : 0));
+}
+
+__global__ void k_cyc_jump(const uint64_t *__restrict__ kin, const uint32_t *__restrict__ nin,
+                           const uint8_t *__restrict__ bin, uint64_t nc, uint64_t *__restrict__ kout,
+                           uint32_t *__restrict__ nout, uint8_t *__restrict__ bout) {
+    const uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= nc) return;
+    const uint32_t q = nin[j];
+    const uint64_t a = kin[j], b = kin[q];
+    kout[j] = a > b ? a : b;
+    bout[j] = bin[j] | bin[q];
+    nout[j] = nin[q];
+}
+
+// the cycle's maximum node becomes a kept node where the cycle and its reverse
+// complement are two cycles (their maxima are the node's two orientations)
+__global__ void k_cyc_mark(const uint32_t *__restrict__ cl, uint64_t nc, const uint32_t *__restrict__ cpos,
+                           const uint64_t *__restrict__ key, const uint8_t *__restrict__ bad,
+                           uint8_t *__restrict__ is11, unsigned long long *__restrict__ nconv) {
+    const uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= nc) return;
+    const uint32_t v = cl[j];
+    const uint64_t m = key[j];
+    if ((uint32_t)m != v) return;  // one thread per cycle: its maximum's own candidate
+    const uint32_t r = cpos[v ^ 1u];
+    if (r == kNil || bad[j] || bad[r]) return;
+    if (key[r] != (m ^ 1ull)) return;  // the cycle is its own reverse complement
+    is11[v >> 1] = 0;  // (both orientations' cycles decide alike)
+    if (!(v & 1u)) atomicAdd(nconv, 1ull);
+}
+
 int Pipeline::merge_device(bool reference_order, MergeDev &out) {
     MCHECK(hipSetDevice(dev_));
     out = MergeDev();
@@ -1148,7 +1207,14 @@ int Pipeline::merge_device(bool reference_order, MergeDev &out) {
     hipLaunchKernelGGL(k_classify, dim3(nblk(S)), dim3(256), 0, st_, ma, mb_[M_IS11].as<uint8_t>(),
                        mb_[M_NXT].as<uint32_t>(), mb_[M_EX].as<uint8_t>(), mb_[M_SELF].as<uint8_t>(), cnt);
     phase_end(14);
-    // --- list ranking: ruling set, then pointer jumping over the rulers
+    // --- list ranking: ruling set, then pointer jumping over the rulers; a
+    // second pass when simple cycles became chains (their maxima kept)
+    uint64_t nconv = 0;
+    Walk *wa = nullptr;
+    uint8_t *spa = nullptr;
+    uint64_t n11 = 0, missing = 0, nk = 0, nspec = 0;
+    const KeptPred kept{mb_[M_IS11].as<uint8_t>()};
+    for (int pass = 0;; pass++) {
     MCHECK(mb_[M_WALKA].reserve(2 * S * sizeof(Walk)));
     MCHECK(mb_[M_WALKB].reserve(2 * S * sizeof(Walk)));
     MCHECK(mb_[M_SPA].reserve(2 * S));
@@ -1160,8 +1226,10 @@ int Pipeline::merge_device(bool reference_order, MergeDev &out) {
     MCHECK(mb_[M_NR].reserve(2 * S * 8));      // label: (ruler, offset)
     MCHECK(mb_[M_PAR].reserve(2 * S * 8));     // packed successor
     MCHECK(scan_scratch_.reserve((scan_scratch_words(nbc + 1) + 16) * 8));
-    Walk *wa = mb_[M_WALKA].as<Walk>(), *wb = mb_[M_WALKB].as<Walk>();
-    uint8_t *spa = mb_[M_SPA].as<uint8_t>(), *spb = mb_[M_SPB].as<uint8_t>();
+    wa = mb_[M_WALKA].as<Walk>();
+    Walk *wb = mb_[M_WALKB].as<Walk>();
+    spa = mb_[M_SPA].as<uint8_t>();
+    uint8_t *spb = mb_[M_SPB].as<uint8_t>();
     uint32_t *rid = mb_[M_NL].as<uint32_t>();
     uint2 *label = mb_[M_NR].as<uint2>();
     uint64_t *nx = mb_[M_PAR].as<uint64_t>();
@@ -1215,7 +1283,6 @@ int Pipeline::merge_device(bool reference_order, MergeDev &out) {
     phase_end(9);
     // --- kept nodes
     phase_begin(10);
-    const KeptPred kept{mb_[M_IS11].as<uint8_t>()};
     MCHECK(compact_count_scan(S, kept, mb_[M_BCNT].as<uint32_t>(), mb_[M_BOFF].as<uint64_t>(),
                               scan_scratch_.as<uint64_t>(), d_small + 20, st_));
     MCHECK(mb_[M_SPEC].reserve(S * 4));
@@ -1223,8 +1290,52 @@ int Pipeline::merge_device(bool reference_order, MergeDev &out) {
                        mb_[M_SPEC].as<uint32_t>(), (unsigned long long *)(d_small + 21));
     MCHECK(hipMemcpyAsync(hs + 16, d_small + 16, 8 * 8, hipMemcpyDeviceToHost, st_));
     MCHECK(hipStreamSynchronize(st_));
-    const uint64_t n11 = hs[16], missing = hs[17], nk = hs[20], nspec = hs[21];
+    n11 = hs[16] - nconv;
+    missing = hs[17];
+    nk = hs[20];
+    nspec = hs[21];
     if (missing) return fail(-5, "edge to a k-mer missing from the graph (edge symmetry broken)");
+    if (pass > 0 || nspec == 0) break;
+    // simple cycles among the special components become chains
+    // (block counts of their own: M_BOFF holds the kept nodes' for the scatter below)
+    const CyclePred cyc{mb_[M_IS11].as<uint8_t>(), wa};
+    MCHECK(mb_[M_LEV].reserve((nbc + 1) * 16 + 64));
+    uint64_t *cboff = mb_[M_LEV].as<uint64_t>();
+    uint32_t *cbcnt = reinterpret_cast<uint32_t *>(cboff + nbc + 1);
+    MCHECK(compact_count_scan(2 * S, cyc, cbcnt, cboff, scan_scratch_.as<uint64_t>(), d_small + 28, st_));
+    MCHECK(hipMemcpyAsync(hs + 28, d_small + 28, 8, hipMemcpyDeviceToHost, st_));
+    MCHECK(hipStreamSynchronize(st_));
+    const uint64_t nc = hs[28];
+    if (nc == 0) break;
+    MCHECK(mb_[M_INL].reserve(2 * S * 4 + 16));             // candidate position of each oriented node
+    MCHECK(mb_[M_INR].reserve(nc * (4 + 2 * 8 + 2 * 4 + 2) + 64));
+    uint32_t *cpos = mb_[M_INL].as<uint32_t>();
+    uint32_t *cl = mb_[M_INR].as<uint32_t>();
+    uint64_t *ka = reinterpret_cast<uint64_t *>(mb_[M_INR].as<uint8_t>() + ((nc * 4 + 15) & ~15ull));
+    uint64_t *kb = ka + nc;
+    uint32_t *na = reinterpret_cast<uint32_t *>(kb + nc), *nb2 = na + nc;
+    uint8_t *ba = reinterpret_cast<uint8_t *>(nb2 + nc), *bb = ba + nc;
+    compact_scatter<true>(2 * S, cyc, cboff, cpos, cl, st_);
+    hipLaunchKernelGGL(k_cyc_init, dim3(nblk(nc)), dim3(256), 0, st_, cl, nc, cpos, mb_[M_NXT].as<uint32_t>(),
+                       mb_[M_PRIO].as<uint64_t>(), mb_[M_SELF].as<uint8_t>(), ka, na, ba);
+    int crounds = 1;
+    while ((1ull << crounds) < nc) crounds++;
+    for (int r = 0; r <= crounds; r++) {  // 2^(crounds+1) >= 2 nc successors: every cycle covered
+        hipLaunchKernelGGL(k_cyc_jump, dim3(nblk(nc)), dim3(256), 0, st_, ka, na, ba, nc, kb, nb2, bb);
+        std::swap(ka, kb);
+        std::swap(na, nb2);
+        std::swap(ba, bb);
+    }
+    MCHECK(hipMemsetAsync(d_small + 29, 0, 8, st_));
+    hipLaunchKernelGGL(k_cyc_mark, dim3(nblk(nc)), dim3(256), 0, st_, cl, nc, cpos, ka, ba,
+                       mb_[M_IS11].as<uint8_t>(), (unsigned long long *)(d_small + 29));
+    MCHECK(hipMemcpyAsync(hs + 29, d_small + 29, 8, hipMemcpyDeviceToHost, st_));
+    MCHECK(hipStreamSynchronize(st_));
+    nconv = hs[29];
+    if (nconv == 0) break;
+    // the list ranking again, the counters it adds to cleared
+    MCHECK(hipMemsetAsync(d_small + 20, 0, 2 * 8, st_));
+    }
     out.n11 = n11;
     out.n_kept = nk;
     MCHECK(mb_[M_KEPTOF].reserve(S * 4));
