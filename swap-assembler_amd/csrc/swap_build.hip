@@ -454,22 +454,166 @@ __global__ void k_col_sum(const uint32_t *__restrict__ rows, uint32_t nrows, uin
     if (s) atomicAdd(&out[col], s);
 }
 
-// Level 1: every occurrence of the workgroup's words goes to the workgroup's run
-// of its level-1 bucket (run starts from the histogram pass).
+// The occurrences of one word as a resumable generator (the loop of
+// for_each_occ_word with its state in registers), so that a pass can take a
+// word's k-mers a few at a time.
+struct OccGen {
+    uint64_t x, rcx, st, km;
+    uint32_t before, i, nh, occ0, k;
+    bool has_prev, ends;
+};
+
+__device__ __forceinline__ void occgen_init(const OccCtx &c, const OccWord &o, OccGen &g) {
+    const uint64_t meta = o.meta;
+    const int k = c.k;
+    g.nh = (uint32_t)(meta >> 32) & 63;
+    g.occ0 = (uint32_t)meta;
+    g.has_prev = (meta >> 40) & 1;
+    g.ends = (meta >> 41) & 1;
+    g.k = (uint32_t)k;
+    g.km = kmask(k);
+    g.x = o.w0 >> (64 - 2 * k);
+    g.rcx = revcomp(g.x, k);
+    g.st = (o.w0 << (2 * k)) | (k < 32 ? o.w1 >> (64 - 2 * k) : 0);
+    g.before = g.has_prev ? o.prev_base : 0;
+    g.i = 0;
+}
+
+// Next occurrence (g.i < g.nh): mixed canonical k-mer, edge mask, input index.
+__device__ __forceinline__ void occgen_next(const OccCtx &c, OccGen &g, uint64_t &h, uint32_t &m, uint32_t &occ) {
+    const uint32_t k = g.k;
+    const uint32_t next = (uint32_t)(g.st >> 62);
+    g.st <<= 2;
+    const bool fwd = g.x > g.rcx;
+    const uint64_t X = fwd ? g.x : g.rcx;
+    m = 0;
+    if (!(g.ends && g.i + 1 == g.nh)) m |= 1u << (fwd ? next : 4 + next);
+    if (g.i > 0 || g.has_prev) {
+        const uint32_t a = 3 - g.before;
+        m |= 1u << (fwd ? 4 + a : a);
+    }
+    h = mix_fwd(c.mx, X);
+    occ = g.occ0 + g.i;
+    g.before = (uint32_t)(g.x >> (2 * k - 2));
+    g.x = ((g.x << 2) | next) & g.km;
+    g.rcx = (g.rcx >> 2) | ((uint64_t)(3 - next) << (2 * k - 2));
+    g.i++;
+}
+
+// A record as staged in LDS: the level-1 record with its input index
+// (reference order) or alone.
 template <bool WITH_OCC>
+struct Staged {
+    typedef Rec12 type;
+};
+template <>
+struct Staged<false> {
+    typedef uint64_t type;
+};
+
+// Level 1: every occurrence of the workgroup's words goes to the workgroup's
+// run of its level-1 bucket (run starts from the histogram pass).  The
+// occurrences are taken in rounds of up to R per thread (a word's k-mers over
+// ceil(32 / R) rounds); each round is counted by bucket in LDS, ordered by
+// bucket in LDS and stored in that order, so that consecutive lanes write
+// consecutive records of one run (scattered 12-byte stores into 256 runs per
+// workgroup left partial lines and cost 1.25x the time).  Three barriers per
+// round: counts complete, offsets known, round staged.  Counters, offsets and
+// staging of a round alternate with the next one's where they would race.
+template <bool WITH_OCC, int R>
 __global__ void __launch_bounds__(kOccThreads) k_part1(OccCtx c, const uint64_t *__restrict__ base1,
                                                         uint32_t *__restrict__ rec_out) {
-    __shared__ uint32_t fill[kL1];
-    for (uint32_t b = threadIdx.x; b < kL1; b += blockDim.x) fill[b] = (uint32_t)base1[(uint64_t)b * gridDim.x + blockIdx.x];
+    typedef typename Staged<WITH_OCC>::type S;
+    constexpr uint32_t kStage = (uint32_t)R * kOccThreads;
+    constexpr int kRounds = (32 + R - 1) / R;
+    static_assert(kStage < (1u << 16), "round ranks are 16-bit");
+    __shared__ S stage[kStage];
+    __shared__ uint8_t sb[kStage];
+    __shared__ __align__(16) uint32_t cnt[2][kL1];
+    __shared__ __align__(16) uint32_t start[kL1];
+    __shared__ __align__(16) uint32_t delta[kL1];
+    __shared__ __align__(16) uint32_t fill[kL1];
+    __shared__ uint32_t ntot;
+    const uint32_t t = threadIdx.x;
+    for (uint32_t b = t; b < kL1; b += blockDim.x) {
+        fill[b] = (uint32_t)base1[(uint64_t)b * gridDim.x + blockIdx.x];
+        cnt[0][b] = 0;
+        cnt[1][b] = 0;
+    }
     __syncthreads();
     uint64_t wb, we;
     word_range(c.W, wb, we);
+    const uint64_t niter = (we - wb + blockDim.x - 1) / blockDim.x;  // uniform over the workgroup
     const int sh1 = 2 * c.k - kL1Bits;
     const uint64_t low1 = (1ull << sh1) - 1;
-    for_each_occ_range(c, wb, we, [&](uint64_t h, uint32_t m, uint64_t occ) {
-        const uint32_t g = atomicAdd(&fill[h >> sh1], 1u);
-        rec_store<WITH_OCC>(rec_out, g, ((h & low1) << 8) | m, (uint32_t)occ);
-    });
+    uint64_t w = wb + t;
+    OccWord nxt{};
+    if (w < we) nxt = load_word(c, w);
+    int p = 0;
+    for (uint64_t it = 0; it < niter; it++, w += blockDim.x) {
+        OccGen g;
+        g.nh = 0;
+        g.i = 0;
+        if (w < we) {
+            occgen_init(c, nxt, g);
+            if (w + blockDim.x < we) nxt = load_word(c, w + blockDim.x);
+        }
+        for (int r = 0; r < kRounds; r++) {
+            S rec[R];
+            uint32_t pos[R];
+#pragma unroll
+            for (int j = 0; j < R; j++) {
+                pos[j] = ~0u;
+                if (g.i < g.nh) {
+                    uint64_t h;
+                    uint32_t m, o;
+                    occgen_next(c, g, h, m, o);
+                    const uint32_t b = (uint32_t)(h >> sh1);
+                    const uint64_t v = ((h & low1) << 8) | m;
+                    if constexpr (WITH_OCC) {
+                        rec[j].lo = (uint32_t)v;
+                        rec[j].hi = (uint32_t)(v >> 32);
+                        rec[j].occ = o;
+                    } else {
+                        rec[j] = v;
+                    }
+                    pos[j] = (b << 16) | atomicAdd(&cnt[p][b], 1u);
+                }
+            }
+            __syncthreads();  // the round's counts are complete
+            if (t < 64) {
+                // wave 0: bucket offsets of the round (4 buckets per lane), the
+                // runs' next slots, the other parity's counters cleared
+                const uint4 v = *reinterpret_cast<const uint4 *>(&cnt[p][4 * t]);
+                const uint32_t sum = v.x + v.y + v.z + v.w;
+                const uint32_t inc = wave_incl_sum(sum);
+                const uint32_t e0 = inc - sum, e1 = e0 + v.x, e2 = e1 + v.y, e3 = e2 + v.z;
+                *reinterpret_cast<uint4 *>(&start[4 * t]) = make_uint4(e0, e1, e2, e3);
+                const uint4 f = *reinterpret_cast<const uint4 *>(&fill[4 * t]);
+                *reinterpret_cast<uint4 *>(&delta[4 * t]) = make_uint4(f.x - e0, f.y - e1, f.z - e2, f.w - e3);
+                *reinterpret_cast<uint4 *>(&fill[4 * t]) = make_uint4(f.x + v.x, f.y + v.y, f.z + v.z, f.w + v.w);
+                *reinterpret_cast<uint4 *>(&cnt[p ^ 1][4 * t]) = make_uint4(0, 0, 0, 0);
+                if (t == 63) ntot = inc;
+            }
+            __syncthreads();  // offsets known
+#pragma unroll
+            for (int j = 0; j < R; j++) {
+                if (pos[j] != ~0u) {
+                    const uint32_t b = pos[j] >> 16;
+                    const uint32_t s = start[b] + (pos[j] & 0xFFFFu);
+                    stage[s] = rec[j];
+                    sb[s] = (uint8_t)b;
+                }
+            }
+            __syncthreads();  // the round is staged in bucket order
+            const uint32_t n = ntot;
+            for (uint32_t i = t; i < n; i += blockDim.x) {
+                const uint32_t gi = delta[sb[i]] + i;  // < 2^32: a build holds under 2^32 records
+                reinterpret_cast<S *>(rec_out)[gi] = stage[i];
+            }
+            p ^= 1;
+        }
+    }
 }
 
 // Level-1 bucket b and its first record of the chunk `chunk` (chunk_start[b] =
@@ -1611,12 +1755,14 @@ int Pipeline::build(const uint8_t *d_in, uint64_t n) {
     SWAP_CHECK(rec1_.reserve(rec_bytes));
     SWAP_CHECK(rec_.reserve(rec_bytes));
     phase_begin(3);
+    // rounds of 11 records per thread (12-byte records) or 16 (8-byte), the
+    // most the 160 KB of LDS stages with one workgroup per CU
     if (ref_order_)
-        hipLaunchKernelGGL(k_part1<true>, dim3(occ_grid), dim3(kOccThreads), 0, st_, oc, base1_.as<uint64_t>(),
+        hipLaunchKernelGGL((k_part1<true, 11>), dim3(occ_grid), dim3(kOccThreads), 0, st_, oc, base1_.as<uint64_t>(),
                            rec1_.as<uint32_t>());
     else
-        hipLaunchKernelGGL(k_part1<false>, dim3(occ_grid), dim3(kOccThreads), 0, st_, oc, base1_.as<uint64_t>(),
-                           rec1_.as<uint32_t>());
+        hipLaunchKernelGGL((k_part1<false, 16>), dim3(occ_grid), dim3(kOccThreads), 0, st_, oc,
+                           base1_.as<uint64_t>(), rec1_.as<uint32_t>());
     phase_end(3);
     if (pbits > kPartBitsMin) {
         // finer partitions: their sizes from the level-1 output
