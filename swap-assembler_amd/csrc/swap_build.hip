@@ -928,6 +928,75 @@ __device__ __forceinline__ bool table_add16(uint64_t *tkey, uint32_t *tc, uint32
     return true;
 }
 
+// Records whose key is not in its home bucket and cannot enter it (the
+// bucket is full of other keys, or the CAS lost to another key) are queued in
+// LDS during a fill's inserts and probed after them by all threads together:
+// probing inline made nearly every wave run the probe loop for its few such
+// lanes.  Counts and first-occurrence minima commute, so the order of the
+// updates does not matter; a key enters the first empty slot of its probe
+// sequence whichever path inserts it (buckets never lose keys during a fill).
+constexpr uint32_t kQueueCap = 2048;
+
+// The home-bucket part of table_add16: true when the record was counted,
+// false when it needs the probe.
+template <bool WITH_OCC>
+__device__ __forceinline__ bool table_add16_home(uint64_t *tkey, uint32_t *tc, uint32_t *tfo, uint64_t rec,
+                                                 uint32_t occ, uint32_t q, int g) {
+    const uint64_t key = rec >> 8;
+    const uint32_t l0 = home_bucket(key) & ((kBuckets >> g) - 1);
+    const uint32_t s0 = sub_bucket(l0, q, g) * kBucketSlots;
+    uint64_t v[kBucketSlots];
+    load_bucket(tkey, s0, v);
+    uint32_t slot = ~0u;
+    int cq = -1;
+#pragma unroll
+    for (int j = kBucketSlots - 1; j >= 0; j--) {
+        if (v[j] == key) {
+            slot = s0 + j;
+            cq = -1;
+        } else if (v[j] == kEmpty) {
+            slot = ~0u;
+            cq = j;
+        }
+    }
+    if (cq >= 0) {
+        const uint64_t prev = atomicCAS((unsigned long long *)&tkey[s0 + cq], (unsigned long long)kEmpty,
+                                        (unsigned long long)key);
+        if (prev == kEmpty || prev == key) slot = s0 + cq;
+    }
+    if (slot == ~0u) return false;
+    if (WITH_OCC) atomicMin(&tfo[slot], occ);
+    add_edges16(tc, slot, (uint32_t)(rec & 0xFF));
+    return true;
+}
+
+// The probe part: returns false when the sub-table is full.
+template <bool WITH_OCC>
+__device__ __forceinline__ bool table_add16_probe(uint64_t *tkey, uint32_t *tc, uint32_t *tfo, uint64_t rec,
+                                                  uint32_t occ, uint32_t q, int g) {
+    const uint64_t key = rec >> 8;
+    const uint32_t slot = bucket_probe(tkey, key, home_bucket(key) & ((kBuckets >> g) - 1), q, g);
+    if (slot == ~0u) return false;
+    if (WITH_OCC) atomicMin(&tfo[slot], occ);
+    add_edges16(tc, slot, (uint32_t)(rec & 0xFF));
+    return true;
+}
+
+// table_add16 with the probe deferred to the queue (probed inline when the
+// queue is full).  Returns false when the sub-table is full.
+template <bool WITH_OCC>
+__device__ __forceinline__ bool table_add16_q(uint64_t *tkey, uint32_t *tc, uint32_t *tfo, uint64_t rec, uint32_t occ,
+                                              uint32_t q, int g, uint64_t *qrec, uint32_t *qocc, uint32_t *qn) {
+    if (table_add16_home<WITH_OCC>(tkey, tc, tfo, rec, occ, q, g)) return true;
+    const uint32_t qi = atomicAdd(qn, 1u);
+    if (qi < kQueueCap) {
+        qrec[qi] = rec;
+        if (WITH_OCC) qocc[qi] = occ;
+        return true;
+    }
+    return table_add16_probe<WITH_OCC>(tkey, tc, tfo, rec, occ, q, g);
+}
+
 // min(count, 255) and the cutoff of graph.cpp:1100-1112 on the 16-bit counters
 // of one slot, two counters per packed 16-bit operation: a counter survives
 // when min(count, 255) + (0x8000 - c) has its top bit set, i.e. when it is >= c.
@@ -1066,6 +1135,9 @@ __global__ void __launch_bounds__(kCountThreads) k_count_lds(CountArgs a) {
     __shared__ uint64_t wsum[kCountThreads / 64];
     __shared__ unsigned long long base_s, base_d;
     __shared__ int overflow[2];  // set during the insert of iterations of each parity
+    __shared__ uint64_t qrec[kQueueCap];  // records left for the probe (16-bit tables)
+    __shared__ uint32_t qocc[WITH_OCC ? kQueueCap : 1];
+    __shared__ uint32_t qn;
     __shared__ uint32_t bnd[kMaxFills + 1][2];  // [i] = record bounds of this workgroup's i-th fill
     const uint32_t t = threadIdx.x;
     const int lane = t & 63, w = t >> 6;
@@ -1086,6 +1158,7 @@ __global__ void __launch_bounds__(kCountThreads) k_count_lds(CountArgs a) {
     }
     for (uint32_t i = t; i < 4 * kTab; i += kCountThreads) tc[i] = 0;
     if (t < 2) overflow[t] = 0;
+    if (t == 0) qn = 0;
     for (uint32_t i = t; i <= kMaxFills; i += kCountThreads) {
         const uint64_t f = blockIdx.x + (uint64_t)i * gridDim.x;
         bnd[i][0] = f < nfills ? a.bucket[f << g] : 0;
@@ -1133,12 +1206,13 @@ __global__ void __launch_bounds__(kCountThreads) k_count_lds(CountArgs a) {
                 for (int j = 0; j < kPre; j++) {
                     const uint64_t i = (uint64_t)b + t + (uint64_t)j * kCountThreads;
                     if (i < e && ok)
-                        ok = table_add16<WITH_OCC>(tkey, tc, tfo, pr[j], po[j], (uint32_t)(pr[j] >> qsh) & qmask, g);
+                        ok = table_add16_q<WITH_OCC>(tkey, tc, tfo, pr[j], po[j], (uint32_t)(pr[j] >> qsh) & qmask, g,
+                                                      qrec, qocc, &qn);
                 }
                 for (uint64_t i = (uint64_t)b + t + (uint64_t)kPre * kCountThreads; i < e && ok; i += kCountThreads) {
                     uint32_t o;
                     const uint64_t r = rec_load<WITH_OCC>(a.rec, i, o);
-                    ok = table_add16<WITH_OCC>(tkey, tc, tfo, r, o, (uint32_t)(r >> qsh) & qmask, g);
+                    ok = table_add16_q<WITH_OCC>(tkey, tc, tfo, r, o, (uint32_t)(r >> qsh) & qmask, g, qrec, qocc, &qn);
                 }
             } else if (g > 0) {
                 ok = false;  // a shared fill of 2^16 records: its partitions go to the device-memory table
@@ -1170,7 +1244,18 @@ __global__ void __launch_bounds__(kCountThreads) k_count_lds(CountArgs a) {
             }
             if (lane <= qmask) a.part_first[prev_p0 + lane] = lo + prev_off;
         }
-        __syncthreads();  // table complete; previous output bases and wave offsets visible
+        __syncthreads();  // inserts done, queue complete; previous output bases and wave offsets visible
+        if (have && small) {
+            // the queued records, probed by all threads
+            const uint32_t nq = min(qn, kQueueCap);
+            bool qok = true;
+            for (uint32_t i = t; i < nq && qok; i += kCountThreads) {
+                const uint64_t r = qrec[i];
+                qok = table_add16_probe<WITH_OCC>(tkey, tc, tfo, r, WITH_OCC ? qocc[i] : 0u,
+                                                  (uint32_t)(r >> qsh) & qmask, g);
+            }
+            if (!qok) overflow[it & 1] = 1;
+        }
         if (have_prev && !prev_ovf) {
             const uint64_t ex = pinc - pv + wsum[w];
             uint64_t os = base_s + (uint32_t)ex, od = base_d + (ex >> 32);
@@ -1187,6 +1272,8 @@ __global__ void __launch_bounds__(kCountThreads) k_count_lds(CountArgs a) {
             }
         }
         if (!have) break;
+        __syncthreads();  // table complete
+        if (t == 0) qn = 0;  // read by every thread before the barrier above
         const bool ovf = overflow[it & 1] != 0 || a.spill;
         uint32_t ns = 0, nd = 0;
 #pragma unroll
