@@ -340,19 +340,68 @@ __device__ __forceinline__ void for_each_occ_word(const OccCtx &c, const OccWord
     }
 }
 
-// f over the occurrences of words wb + threadIdx.x, wb + threadIdx.x +
-// blockDim.x, ... below we, each word's loads issued one word early.
+// A workgroup's words taken in groups of blockDim.x, each group reordered by
+// k-mer count (descending) before its words go to the threads: a read's words
+// start 32, ..., 32 k-mers, then fewer, then none, so consecutive words mix
+// full and short ones in every wave, whose loop over a word's k-mers then runs
+// as long as its fullest lane; grouped by count, the full words fill whole
+// waves and the short ones finish early.  The next group is sorted and its
+// words loaded while the current one is processed.
+struct WordSort {
+    uint32_t ccnt[33];  // words of each class (32 - k-mers) of the group being sorted
+    uint32_t coff[33];
+    uint16_t perm[1024];  // group position of the word the thread takes
+};
+
+__device__ __forceinline__ uint32_t word_nh(uint64_t meta) { return (uint32_t)(meta >> 32) & 63; }
+
+// This thread's word (offset in the group) after sorting the group by the
+// k-mer counts nh of its words (nh of thread t's unsorted word).  Three
+// barriers; counters cleared for the next group.
+__device__ __forceinline__ uint32_t sort_group(WordSort &ws, uint32_t nh) {
+    const uint32_t cls = 32 - min(nh, 32u);
+    const uint32_t rank = atomicAdd(&ws.ccnt[cls], 1u);
+    __syncthreads();
+    if (threadIdx.x < 64) {
+        const uint32_t v = threadIdx.x < 33 ? ws.ccnt[threadIdx.x] : 0;
+        const uint32_t inc = wave_incl_sum(v);
+        if (threadIdx.x < 33) {
+            ws.coff[threadIdx.x] = inc - v;
+            ws.ccnt[threadIdx.x] = 0;
+        }
+    }
+    __syncthreads();
+    ws.perm[ws.coff[cls] + rank] = (uint16_t)threadIdx.x;
+    __syncthreads();
+    return ws.perm[threadIdx.x];
+}
+
+// f(word, valid) once per group for every thread (uniform: f may hold
+// barriers), words of the range [wb, we) in sorted groups; invalid words
+// (past we) come last in their group.  ws.ccnt must be zero on entry.
 template <typename F>
-__device__ __forceinline__ void for_each_occ_range(const OccCtx &c, uint64_t wb, uint64_t we, F &&f) {
-    uint64_t w = wb + threadIdx.x;
-    if (w >= we) return;
-    OccWord cur = load_word(c, w);
-    for (; w < we; w += blockDim.x) {
-        const uint64_t wn = w + blockDim.x;
-        OccWord nxt = cur;
-        if (wn < we) nxt = load_word(c, wn);
-        for_each_occ_word(c, cur, f);
+__device__ __forceinline__ void for_each_word_sorted(const OccCtx &c, WordSort &ws, uint64_t wb, uint64_t we, F &&f) {
+    const uint32_t T = blockDim.x, t = threadIdx.x;
+    const uint64_t ngroups = (we - wb + T - 1) / T;
+    if (ngroups == 0) return;
+    uint64_t m = wb + t;
+    uint64_t w = wb + sort_group(ws, m < we ? word_nh(c.wmeta[m]) : 0);
+    OccWord cur{};
+    if (w < we) cur = load_word(c, w);
+    m = wb + T + t;
+    uint64_t mnext = m < we ? c.wmeta[m] : 0;
+    for (uint64_t gi = 0; gi < ngroups; gi++) {
+        OccWord nxt{};
+        uint64_t wn = we;
+        if (gi + 1 < ngroups) {
+            wn = wb + (gi + 1) * T + sort_group(ws, word_nh(mnext));
+            if (wn < we) nxt = load_word(c, wn);
+            m = wb + (gi + 2) * T + t;
+            mnext = m < we ? c.wmeta[m] : 0;
+        }
+        f(cur, w < we);
         cur = nxt;
+        w = wn;
     }
 }
 
@@ -426,12 +475,16 @@ __device__ __forceinline__ void word_range(uint64_t W, uint64_t &wb, uint64_t &w
 __global__ void __launch_bounds__(kOccThreads) k_part_hist(OccCtx c, uint32_t *__restrict__ rows,
                                                             uint32_t *__restrict__ hist1) {
     __shared__ uint32_t lh[kHistBins];
+    __shared__ WordSort ws;
     for (uint32_t i = threadIdx.x; i < kHistBins; i += blockDim.x) lh[i] = 0;
+    if (threadIdx.x < 33) ws.ccnt[threadIdx.x] = 0;
     __syncthreads();
     uint64_t wb, we;
     word_range(c.W, wb, we);
     const int sh = 2 * c.k - kPartBitsMin;
-    for_each_occ_range(c, wb, we, [&](uint64_t h, uint32_t, uint64_t) { atomicAdd(&lh[h >> sh], 1u); });
+    for_each_word_sorted(c, ws, wb, we, [&](const OccWord &o, bool valid) {
+        if (valid) for_each_occ_word(c, o, [&](uint64_t h, uint32_t, uint64_t) { atomicAdd(&lh[h >> sh], 1u); });
+    });
     __syncthreads();
     uint32_t *row = rows + (uint64_t)blockIdx.x * kHistBins;
     for (uint32_t i = threadIdx.x; i < kHistBins; i += blockDim.x) row[i] = lh[i];
@@ -535,29 +588,24 @@ __global__ void __launch_bounds__(kOccThreads) k_part1(OccCtx c, const uint64_t 
     __shared__ __align__(16) uint32_t fill[kL1];
     __shared__ uint32_t ntot;
     const uint32_t t = threadIdx.x;
+    __shared__ WordSort ws;
     for (uint32_t b = t; b < kL1; b += blockDim.x) {
         fill[b] = (uint32_t)base1[(uint64_t)b * gridDim.x + blockIdx.x];
         cnt[0][b] = 0;
         cnt[1][b] = 0;
     }
+    if (t < 33) ws.ccnt[t] = 0;
     __syncthreads();
     uint64_t wb, we;
     word_range(c.W, wb, we);
-    const uint64_t niter = (we - wb + blockDim.x - 1) / blockDim.x;  // uniform over the workgroup
     const int sh1 = 2 * c.k - kL1Bits;
     const uint64_t low1 = (1ull << sh1) - 1;
-    uint64_t w = wb + t;
-    OccWord nxt{};
-    if (w < we) nxt = load_word(c, w);
     int p = 0;
-    for (uint64_t it = 0; it < niter; it++, w += blockDim.x) {
+    for_each_word_sorted(c, ws, wb, we, [&](const OccWord &o, bool valid) {
         OccGen g;
         g.nh = 0;
         g.i = 0;
-        if (w < we) {
-            occgen_init(c, nxt, g);
-            if (w + blockDim.x < we) nxt = load_word(c, w + blockDim.x);
-        }
+        if (valid) occgen_init(c, o, g);
         for (int r = 0; r < kRounds; r++) {
             S rec[R];
             uint32_t pos[R];
@@ -613,7 +661,7 @@ __global__ void __launch_bounds__(kOccThreads) k_part1(OccCtx c, const uint64_t 
             }
             p ^= 1;
         }
-    }
+    });
 }
 
 // Level-1 bucket b and its first record of the chunk `chunk` (chunk_start[b] =
