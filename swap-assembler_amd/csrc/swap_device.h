@@ -233,7 +233,7 @@ private:
     enum { O_ENT, O_R2, O_GSORT, O_BCNT, O_BOFF, O_BSTART, O_COUNT };
     DevBuf ob_[O_COUNT];
     // survivor renumbering
-    enum { R_CHIST, R_CPRE, R_HEAD, R_HPRE, R_BINMAP, R_CNT, R_OFF, R_BSTART, R_KV, R_FI, R_GSRT, R_KEY, R_MULT, R_FO, R_PERM, R_COUNT };
+    enum { R_CHIST, R_CPRE, R_HEAD, R_HPRE, R_BINMAP, R_CNT, R_OFF, R_BSTART, R_KV, R_FI, R_GSRT, R_KEY, R_PERM, R_COUNT };
     DevBuf rn_[R_COUNT];
     // the survivor arrays are in first-occurrence order (renumbered by a merge
     // of this build): they no longer follow the count's partition runs

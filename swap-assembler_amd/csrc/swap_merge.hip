@@ -1171,12 +1171,11 @@ int Pipeline::merge_device(bool reference_order, MergeDev &out) {
                        runs);
     hipLaunchKernelGGL(k_index_loose, dim3(nblk(S)), dim3(256), 0, st_, surv_key_.as<uint64_t>(), S,
                        part_first_.as<uint32_t>(), ix, mb_[M_IENT].as<uint64_t>(), perm, runs);
-    if (renumbered) {  // the new order in place of the count's
-        // (copied, not swapped: the survivor arrays keep the capacity the
-        // count reserves, the renumbering's stay at S)
+    if (renumbered) {  // the new key order in place of the count's (multiplicities
+        // and first occurrences were written in place); copied, not swapped: the
+        // survivor arrays keep the capacity the count reserves, the
+        // renumbering's stay at S
         MCHECK(hipMemcpyAsync(surv_key_.p, rn_[R_KEY].p, S * 8, hipMemcpyDeviceToDevice, st_));
-        MCHECK(hipMemcpyAsync(surv_mult_.p, rn_[R_MULT].p, S * 8, hipMemcpyDeviceToDevice, st_));
-        MCHECK(hipMemcpyAsync(surv_first_.p, rn_[R_FO].p, S * 4, hipMemcpyDeviceToDevice, st_));
         surv_renumbered_ = true;
     }
     // --- priorities (visit time)

@@ -331,8 +331,10 @@ __global__ void __launch_bounds__(kRnThreads) k_rn_order(RnOrderArgs a, uint32_t
         if (e_ != hipSuccess) return fail(-5, hipGetErrorString(e_));        \
     } while (0)
 
-// New order of the survivors into rn_[R_KEY], rn_[R_MULT], rn_[R_FO], and the
-// new number of every old one into rn_[R_PERM]; *renumbered false (nothing
+// New order of the survivors: keys into rn_[R_KEY] (the survivor index is
+// built from the old order next), multiplicities and first occurrences in
+// place (their old order is dead once the survivors are scattered into bins),
+// and the new number of every old one into rn_[R_PERM]; *renumbered false (nothing
 // launched) above 2^kRnMaxBinBits bins or 2^(kRnCoarseBitsMax + 18) input
 // k-mers.
 int Pipeline::renumber_survivors(bool *renumbered) {
@@ -371,8 +373,6 @@ int Pipeline::renumber_survivors(bool *renumbered) {
     RCHECK(rn_[R_FI].reserve(S * 8));
     RCHECK(rn_[R_GSRT].reserve(S * 8));
     RCHECK(rn_[R_KEY].reserve(S * 8));
-    RCHECK(rn_[R_MULT].reserve(S * 8));
-    RCHECK(rn_[R_FO].reserve(S * 4));
     RCHECK(rn_[R_PERM].reserve(S * 4));
     RCHECK(scan_scratch_.reserve((scan_scratch_words(std::max<uint64_t>(ncnt, nc)) + 16) * 8));
     const uint32_t *fo = surv_first_.as<uint32_t>();
@@ -400,8 +400,8 @@ int Pipeline::renumber_survivors(bool *renumbered) {
     a.bin_start = rn_[R_BSTART].as<uint32_t>();
     a.gsrt = rn_[R_GSRT].as<uint64_t>();
     a.key = rn_[R_KEY].as<uint64_t>();
-    a.mult = rn_[R_MULT].as<uint64_t>();
-    a.first = rn_[R_FO].as<uint32_t>();
+    a.mult = surv_mult_.as<uint64_t>();  // read only by the scatter above
+    a.first = surv_first_.as<uint32_t>();
     a.perm = rn_[R_PERM].as<uint32_t>();
     const uint32_t words = 1u << (cshift + kRnBlockBits - 5);
     hipLaunchKernelGGL(k_rn_order, dim3((unsigned)nb), dim3(kRnThreads), 2 * words * 4, st_, a, words);
