@@ -24,6 +24,10 @@
  *   swap_export_kmer_graph       the kmolecules table after cutoffGraph
  *                                (graph.cpp:353, 1112)
  *   swap_export_compacted        the nodes table after the merge (graph.cpp:354)
+ *   swap_build_batch /           constructDistNodeGraph + cutoffGraph of many
+ *   swap_batch_take              samples in one build (the reference builds one
+ *                                sample per run); a taken sample is as after
+ *                                swap_build_device
  *   swap_reconfigure, swap_server_*  no counterpart (many samples per process)
  *   swap_format_file /           printKmerGraph (graph.cpp:3352-3440),
  *   swap_write_file              gatherDistNodeGraph + the contigGraph.txt write
@@ -142,6 +146,26 @@ int swap_build_device(swap_ctx *ctx, const void *d_fasta, size_t n);
  * valid and unchanged until the swap_build that consumes them returns (or the
  * context is destroyed). */
 int swap_prefetch(swap_ctx *ctx, const char *fasta, size_t n);
+
+/* Many samples in one build: the k-mer graphs of nsamples (1..512) samples
+ * whose FASTA bytes are in device memory (d_fasta[i], n[i] bytes) are built by
+ * one pass of the build kernels over all of them: one record scan, one packing,
+ * two partition passes and one persistent count whose table keys carry the
+ * sample, then the survivors of all samples are ordered by first occurrence
+ * and every sample's growth ranks selected in one pass.  The reference builds
+ * one sample per run (graph.cpp:3443-3698, constructDistNodeGraph at
+ * graph.cpp:1049-1090); a sample's results equal those of its own build.
+ * Needs reference order and no SWAP_FLAG_KMER_GRAPH on ctx; every sample must
+ * start with '>', end with a newline and, except the last, hold whole line
+ * pairs (SWAP_E_INVAL otherwise: build those samples alone); at most 2^32-16
+ * k-mers in all.  The samples are then taken out one by one with
+ * swap_batch_take. */
+int swap_build_batch(swap_ctx *ctx, const void *const *d_fasta, const size_t *n, size_t nsamples);
+/* Makes dst's state that of a build of sample i of batch's last
+ * swap_build_batch alone (dst: same k, c and device, reference order): merge,
+ * exports and output files follow as after swap_build_device.  Copies the
+ * sample's survivors; batch keeps its results until its next build. */
+int swap_batch_take(swap_ctx *batch, size_t i, swap_ctx *dst);
 
 /* Merge the k-mer graph of the last build into unitig edges (k >= 19, else
  * SWAP_E_INVAL). */
@@ -286,6 +310,18 @@ int swap_server_create(int device, int slots, size_t hbm_budget, swap_server **o
 int64_t swap_server_submit(swap_server *s, const swap_job *job);
 /* Queues n jobs, largest input first; ids[i] receives the id of jobs[i]. */
 int swap_server_submit_batch(swap_server *s, const swap_job *jobs, size_t n, int64_t *ids);
+/* Queues n merge jobs of one k, c and flags (reference order, no
+ * SWAP_FLAG_KMER_GRAPH) whose FASTA is in device memory
+ * (SWAP_JOB_DEVICE_INPUT), built together: largest input first, the jobs are
+ * cut into groups of at most group_bytes of FASTA (0: 512 MiB) and 512
+ * samples; a builder thread of the server builds each group with one
+ * swap_build_batch (the next group while the slots merge the previous one),
+ * and the slot that runs a job takes its sample (swap_batch_take) instead of
+ * building it.  A group whose batch build fails runs as ordinary jobs.
+ * Outputs, statuses and the wait calls are those of ordinary jobs; each
+ * job's files equal those of its own build.  No counterpart in the reference
+ * (one sample per run, graph.cpp:3443-3698). */
+int swap_server_submit_fused(swap_server *s, const swap_job *jobs, size_t n, size_t group_bytes, int64_t *ids);
 /* Blocks until the job has ended; returns its status (0 or a negative code). */
 int swap_server_wait(swap_server *s, int64_t id);
 /* Blocks until every queued job has ended; returns 0 or the first job error. */

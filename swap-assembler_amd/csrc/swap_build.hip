@@ -686,21 +686,44 @@ __device__ __forceinline__ bool chunk_range(uint32_t chunk, const uint32_t *__re
 // per sub-bucket, orders the chunk by sub-bucket in LDS and writes it out in
 // that order, so consecutive lanes store consecutive records of one run.
 // L2MAX bounds 2^l2bits (the LDS counters are sized by it).
-template <bool WITH_OCC, int L2MAX>
+//
+// BATCH (many samples in one build): the output record's key bits above the
+// partition's (implied by the partition from here on) are replaced by the
+// record's sample, found from its input index among the samples' first input
+// indices sbase[0..nsamp] (sbase[0] = 0, sbase[nsamp] = all k-mers): bits
+// [8, 8 + 2k - pbits) keep the key's low bits, bits 55-63 hold the sample.
+constexpr int kBatchSampleShift = 55;
+constexpr uint32_t kMaxBatch = (uint32_t)kMaxBatchSamples;  // 9 sample bits
+
+__device__ __forceinline__ uint32_t sample_of(const uint32_t *sb, uint32_t nsamp, uint32_t occ) {
+    uint32_t lo = 0, hi = nsamp;  // last s with sb[s] <= occ (empty samples share their start)
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (sb[mid] <= occ) lo = mid; else hi = mid;
+    }
+    return lo;
+}
+
+template <bool WITH_OCC, int L2MAX, bool BATCH>
 __global__ void __launch_bounds__(kL2Threads) k_part2(const uint32_t *__restrict__ rec_in,
                                                        const uint32_t *__restrict__ l1_start,
                                                        const uint32_t *__restrict__ chunk_start, int k, int l2bits,
-                                                       uint32_t *__restrict__ cursor, uint32_t *__restrict__ rec_out) {
+                                                       uint32_t *__restrict__ cursor, uint32_t *__restrict__ rec_out,
+                                                       const uint32_t *__restrict__ sbase, uint32_t nsamp) {
+    static_assert(!BATCH || WITH_OCC, "a batch build keeps the input index of every record");
     __shared__ uint32_t lcnt[L2MAX];
     __shared__ uint32_t lstart[L2MAX];
     __shared__ uint32_t gbase[L2MAX];
     __shared__ uint32_t slo[kChunkRecs], shi[kChunkRecs];  // the chunk in sub-bucket order
     __shared__ uint32_t socc[WITH_OCC ? kChunkRecs : 1];
+    __shared__ uint32_t lsb[BATCH ? kMaxBatch + 1 : 1];
     const uint32_t nl2 = 1u << l2bits, m2 = nl2 - 1;
     uint32_t b1;
     uint64_t a, e;
     if (!chunk_range(blockIdx.x, l1_start, chunk_start, kChunkRecs, b1, a, e)) return;
     for (uint32_t b = threadIdx.x; b < nl2; b += blockDim.x) lcnt[b] = 0;
+    if (BATCH)
+        for (uint32_t s = threadIdx.x; s <= nsamp; s += blockDim.x) lsb[s] = sbase[s];
     __syncthreads();
     const uint32_t n = (uint32_t)(e - a);
     const int sh2 = 8 + 2 * k - (kL1Bits + l2bits);
@@ -732,10 +755,13 @@ __global__ void __launch_bounds__(kL2Threads) k_part2(const uint32_t *__restrict
         }
     }
     __syncthreads();
+    const uint64_t low = (1ull << sh2) - 1;  // mask and key bits below the partition's
     for (uint32_t i = threadIdx.x; i < n; i += kL2Threads) {
-        const uint64_t rec = (uint64_t)slo[i] | ((uint64_t)shi[i] << 32);
+        uint64_t rec = (uint64_t)slo[i] | ((uint64_t)shi[i] << 32);
         const uint32_t b = (uint32_t)(rec >> sh2) & m2;
-        rec_store<WITH_OCC>(rec_out, (uint64_t)gbase[b] + (i - lstart[b]), rec, WITH_OCC ? socc[i] : 0u);
+        const uint32_t oc = WITH_OCC ? socc[i] : 0u;
+        if (BATCH) rec = (rec & low) | ((uint64_t)sample_of(lsb, nsamp, oc) << kBatchSampleShift);
+        rec_store<WITH_OCC>(rec_out, (uint64_t)gbase[b] + (i - lstart[b]), rec, oc);
     }
 }
 
@@ -823,7 +849,28 @@ struct CountArgs {
     int spill;                     // send every partition to the device-memory table
     uint32_t *dfo;                 // first-occurrence index of every distinct key (reference order), or null
     int gbits;                     // 2^gbits consecutive partitions share one fill of the LDS table
+    // batch builds: record keys carry their sample above the partition's key
+    // bits; per sample, the distinct keys and the largest first occurrence
+    int batch;
+    int pbits;
+    uint32_t nsamp;
+    uint32_t *samp_distinct;
+    uint32_t *samp_last;
 };
+
+// The mixed key h of a counted key of partition `part`.  Ordinary builds keep
+// the 2k - 8 bits below the level-1 bucket; batch builds the 2k - pbits bits
+// below the partition, with the sample above them.
+__device__ __forceinline__ uint64_t counted_key(const CountArgs &a, bool batch, uint32_t part, uint64_t key) {
+    if (batch) {
+        const int lb = 2 * a.k - a.pbits;
+        return ((uint64_t)part << lb) | (key & ((1ull << lb) - 1));
+    }
+    return ((uint64_t)(part >> a.l2bits) << (2 * a.k - kL1Bits)) | key;
+}
+__device__ __forceinline__ uint32_t counted_sample(uint64_t key) {
+    return (uint32_t)(key >> (kBatchSampleShift - 8));
+}
 
 __device__ __forceinline__ uint32_t slot_hash(uint64_t key, uint32_t mask) {
     return (uint32_t)((key * 0x9E3779B97F4A7C15ull) >> 40) & mask;
@@ -1091,7 +1138,6 @@ __device__ void table_emit(const uint64_t *tkey, CNT cnt_of, const uint32_t *tfo
                            uint32_t part, const CountArgs &a) {
     __shared__ uint64_t wsum[kCountThreads / 64];
     __shared__ unsigned long long base_s, base_d;
-    const int hb = 2 * a.k - kL1Bits;
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
     constexpr int kPer = 4;  // one round over the 4096-slot table
     for (uint32_t r0 = 0; r0 < n_slots; r0 += blockDim.x * kPer) {
@@ -1134,8 +1180,13 @@ __device__ void table_emit(const uint64_t *tkey, CNT cnt_of, const uint32_t *tfo
         for (int j = 0; j < kPer; j++) {
             if (keys[j] == kEmpty) continue;
             if (WITH_OCC && a.dfo) a.dfo[od++] = fos[j];
+            if (a.batch) {
+                const uint32_t sm = counted_sample(keys[j]);
+                atomicAdd(&a.samp_distinct[sm], 1u);
+                atomicMax(&a.samp_last[sm], fos[j]);
+            }
             if (mults[j]) {
-                a.surv_key[os] = ((uint64_t)(part >> a.l2bits) << hb) | keys[j];  // mixed; k_unmix inverts
+                a.surv_key[os] = counted_key(a, a.batch, part, keys[j]);  // mixed; k_unmix inverts
                 a.surv_mult[os] = mults[j];
                 if (WITH_OCC) a.surv_first[os] = fos[j];
                 os++;
@@ -1174,9 +1225,14 @@ constexpr int kMaxFillBits = 3;
 // across a wave.  Sharing a fill among the small partitions of a small input
 // divides the per-fill costs (barriers, the reservation on one device-wide
 // counter, which every workgroup's fills serialise on) by 2^gbits.
-template <bool WITH_OCC, bool SHARED>
+//
+// BATCH: the keys are (sample, key bits below the partition) and each
+// workgroup tallies every sample's distinct keys and largest first occurrence
+// in LDS, added to the device totals once at its end.
+template <bool WITH_OCC, bool SHARED, bool BATCH>
 __global__ void __launch_bounds__(kCountThreads) k_count_lds(CountArgs a) {
     static_assert(kTab == 4 * kCountThreads, "the emit takes 4 slots per thread");
+    static_assert(!BATCH || (WITH_OCC && !SHARED), "batch builds count one partition per fill");
     __shared__ __align__(16) uint64_t tkey[kTab];
     __shared__ uint32_t tc[4 * kTab];  // 8 x u16 counters per slot, or 8 x u8 in the first 2*kTab words
     __shared__ uint32_t tfo[WITH_OCC ? kTab : 1];
@@ -1187,9 +1243,9 @@ __global__ void __launch_bounds__(kCountThreads) k_count_lds(CountArgs a) {
     __shared__ uint32_t qocc[WITH_OCC ? kQueueCap : 1];
     __shared__ uint32_t qn;
     __shared__ uint32_t bnd[kMaxFills + 1][2];  // [i] = record bounds of this workgroup's i-th fill
+    __shared__ uint32_t sdist[BATCH ? kMaxBatch : 1], slast[BATCH ? kMaxBatch : 1];
     const uint32_t t = threadIdx.x;
     const int lane = t & 63, w = t >> 6;
-    const int hb = 2 * a.k - kL1Bits;
     const uint32_t nparts = a.nparts;
     const int g = SHARED ? a.gbits : 0;  // single fills: the sub-table arithmetic folds away
     const uint32_t nfills = nparts >> g, qmask = (1u << g) - 1;
@@ -1207,6 +1263,8 @@ __global__ void __launch_bounds__(kCountThreads) k_count_lds(CountArgs a) {
     for (uint32_t i = t; i < 4 * kTab; i += kCountThreads) tc[i] = 0;
     if (t < 2) overflow[t] = 0;
     if (t == 0) qn = 0;
+    if (BATCH)
+        for (uint32_t i = t; i < a.nsamp; i += kCountThreads) sdist[i] = slast[i] = 0;
     for (uint32_t i = t; i <= kMaxFills; i += kCountThreads) {
         const uint64_t f = blockIdx.x + (uint64_t)i * gridDim.x;
         bnd[i][0] = f < nfills ? a.bucket[f << g] : 0;
@@ -1311,8 +1369,13 @@ __global__ void __launch_bounds__(kCountThreads) k_count_lds(CountArgs a) {
             for (int j = 0; j < 4; j++) {
                 if (keys[j] == kEmpty) continue;
                 if (WITH_OCC && a.dfo) a.dfo[od++] = fos[j];
+                if (BATCH) {
+                    const uint32_t sm = counted_sample(keys[j]);
+                    atomicAdd(&sdist[sm], 1u);
+                    atomicMax(&slast[sm], fos[j]);
+                }
                 if (mults[j]) {
-                    a.surv_key[os] = ((uint64_t)(prev_p0 >> a.l2bits) << hb) | keys[j];  // mixed; k_unmix inverts
+                    a.surv_key[os] = counted_key(a, BATCH, prev_p0, keys[j]);  // mixed; k_unmix inverts
                     a.surv_mult[os] = mults[j];
                     if (WITH_OCC) a.surv_first[os] = fos[j];
                     os++;
@@ -1396,6 +1459,14 @@ __global__ void __launch_bounds__(kCountThreads) k_count_lds(CountArgs a) {
         have_prev = true;
         prev_ovf = ovf;
         prev_p0 = p0;
+    }
+    if (BATCH) {
+        __syncthreads();  // every thread left the loop after the same fill
+        for (uint32_t i = t; i < a.nsamp; i += kCountThreads)
+            if (sdist[i]) {
+                atomicAdd(&a.samp_distinct[i], sdist[i]);
+                atomicMax(&a.samp_last[i], slast[i]);
+            }
     }
     // the last workgroup to finish publishes the totals of the reservations as
     // the survivor and distinct counters (the device-memory pass appends to them)
@@ -1619,6 +1690,207 @@ __global__ void __launch_bounds__(1024) k_sel_pick(const SelInfo *__restrict__ s
 }
 
 // ---------------------------------------------------------------------------
+// Batch builds: many samples' FASTA bytes concatenated and built together.
+// A sample's records are its own when every sample starts with '>' and ends
+// with a newline and an even number of lines precedes every sample (records
+// are line pairs from the first '>'): then no record spans two samples, and
+// the records of sample s are those whose sequence starts in its bytes.
+
+// One thread per sample boundary s = 0..nsamp: rfirst[s] = first record of
+// sample s (R at the end), kbase[s] = its first input k-mer index (all k-mers
+// at the end).  bad[0] is set when a sample breaks the rule above.
+__global__ void k_batch_bounds(const uint8_t *__restrict__ buf, const uint64_t *__restrict__ off, uint32_t nsamp,
+                               const uint64_t *__restrict__ nl_pos, const uint64_t *__restrict__ d_small,
+                               const uint64_t *__restrict__ rec_start, const uint64_t *__restrict__ occ_base,
+                               uint64_t *__restrict__ rfirst, uint32_t *__restrict__ kbase, uint32_t *__restrict__ bad) {
+    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s > nsamp) return;
+    const uint64_t nnl = d_small[1], R = d_small[3];
+    const uint64_t b = off[s];
+    if (s < nsamp) {
+        const uint64_t e = off[s + 1];
+        uint64_t lo = 0, hi = nnl;  // newlines before byte b
+        while (lo < hi) {
+            const uint64_t mid = (lo + hi) >> 1;
+            if (nl_pos[mid] < b) lo = mid + 1; else hi = mid;
+        }
+        if (e <= b || buf[b] != '>' || buf[e - 1] != '\n' || (lo & 1)) atomicOr(bad, 1u);
+    }
+    uint64_t lo = 0, hi = R;  // records whose sequence starts before byte b
+    while (lo < hi) {
+        const uint64_t mid = (lo + hi) >> 1;
+        if (rec_start[mid] < b) lo = mid + 1; else hi = mid;
+    }
+    rfirst[s] = lo;
+    kbase[s] = (uint32_t)(lo < R ? occ_base[lo] : d_small[4]);  // the scan leaves its total in d_small[4]
+}
+
+// Reads of each sample that hold at least one k-mer (the count a build of the
+// sample alone reports).
+__global__ void __launch_bounds__(256) k_batch_reads(const uint32_t *__restrict__ occ_count, uint64_t R,
+                                                     const uint64_t *__restrict__ rfirst, uint32_t nsamp,
+                                                     unsigned long long *__restrict__ reads) {
+    __shared__ uint64_t lrf[kMaxBatch + 1];
+    __shared__ uint32_t lcnt[kMaxBatch];
+    for (uint32_t i = threadIdx.x; i <= nsamp; i += blockDim.x) lrf[i] = rfirst[i];
+    for (uint32_t i = threadIdx.x; i < nsamp; i += blockDim.x) lcnt[i] = 0;
+    __syncthreads();
+    for (uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; r < R; r += (uint64_t)gridDim.x * blockDim.x) {
+        if (!occ_count[r]) continue;
+        uint32_t lo = 0, hi = nsamp;  // last s with rfirst[s] <= r
+        while (hi - lo > 1) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (lrf[mid] <= r) lo = mid; else hi = mid;
+        }
+        atomicAdd(&lcnt[lo], 1u);
+    }
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < nsamp; i += blockDim.x)
+        if (lcnt[i]) atomicAdd(&reads[i], (unsigned long long)lcnt[i]);
+}
+
+// Growth ranks of many samples' k-mer tables at once.  Sample s's distinct
+// keys hold the global first-occurrence ranks [D_<s, D_<s + D_s) (its input
+// indices all lie above the earlier samples'), so its growth rank r is the
+// global order statistic D_<s + r.  One workgroup: bin of each wanted rank
+// (ranks ascending) and its position in the bin, the target bins numbered in
+// bin order (target[bin] = its bitmap, -1 elsewhere).
+__global__ void __launch_bounds__(1024) k_bsel_targets(const uint32_t *__restrict__ hist,
+                                                       const uint64_t *__restrict__ rank, uint32_t T,
+                                                       int *__restrict__ target, uint32_t *__restrict__ t_slot,
+                                                       uint32_t *__restrict__ t_pos, uint32_t *__restrict__ slot_bin) {
+    __shared__ uint32_t cum[kSelBins + 1];
+    __shared__ uint8_t flag[kSelBins];
+    __shared__ uint32_t wsum[16];
+    constexpr int per = kSelBins / 1024;
+    const int t = threadIdx.x;
+    uint32_t loc[per];
+    uint32_t sum = 0;
+#pragma unroll
+    for (int j = 0; j < per; j++) { loc[j] = hist[t * per + j]; sum += loc[j]; flag[t * per + j] = 0; }
+    uint32_t inc = wave_incl_sum(sum);
+    if ((t & 63) == 63) wsum[t >> 6] = inc;
+    __syncthreads();
+    uint32_t off = inc - sum;
+    for (int w = 0; w < (t >> 6); w++) off += wsum[w];
+#pragma unroll
+    for (int j = 0; j < per; j++) { cum[t * per + j] = off; off += loc[j]; }
+    if (t == 1023) cum[kSelBins] = off;
+    __syncthreads();
+    for (uint32_t i = t; i < T; i += 1024) {
+        const uint64_t r = rank[i];
+        uint32_t lo = 0, hi = kSelBins;  // last bin with cum <= r
+        while (hi - lo > 1) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (cum[mid] <= r) lo = mid; else hi = mid;
+        }
+        t_pos[i] = (uint32_t)(r - cum[lo]);
+        t_slot[i] = lo;  // the bin for now
+        flag[lo] = 1;
+    }
+    __syncthreads();
+    // number the target bins: exclusive scan of the flags
+    sum = 0;
+#pragma unroll
+    for (int j = 0; j < per; j++) sum += flag[t * per + j];
+    inc = wave_incl_sum(sum);
+    __syncthreads();  // wsum reused
+    if ((t & 63) == 63) wsum[t >> 6] = inc;
+    __syncthreads();
+    off = inc - sum;
+    for (int w = 0; w < (t >> 6); w++) off += wsum[w];
+#pragma unroll
+    for (int j = 0; j < per; j++) {
+        const uint32_t b = t * per + j;
+        if (flag[b]) {
+            target[b] = (int)off;
+            slot_bin[off] = b;
+            cum[b] = off;  // the bin's slot (the cumulative counts are no longer read)
+            off++;
+        } else {
+            target[b] = -1;
+        }
+    }
+    __syncthreads();
+    for (uint32_t i = t; i < T; i += 1024) t_slot[i] = cum[t_slot[i]];
+}
+
+// Workgroup per wanted rank: its value by prefix popcount over its bin's
+// bitmap (marked by k_sel_mark).
+__global__ void __launch_bounds__(1024) k_bsel_pick(const uint32_t *__restrict__ t_slot,
+                                                    const uint32_t *__restrict__ t_pos,
+                                                    const uint32_t *__restrict__ slot_bin,
+                                                    const uint32_t *__restrict__ bitmap, int shift,
+                                                    uint64_t *__restrict__ value) {
+    __shared__ uint32_t part[1024];
+    __shared__ uint32_t found;
+    const uint32_t tg = blockIdx.x;
+    const uint32_t slot = t_slot[tg], want = t_pos[tg];
+    const uint32_t nw = (1u << shift) >= 32 ? (1u << shift) / 32 : 1;
+    const uint32_t *bm = bitmap + (uint64_t)slot * nw;
+    const uint32_t per = (nw + blockDim.x - 1) / blockDim.x;
+    uint32_t c = 0;
+    for (uint32_t j = 0; j < per; j++) {
+        const uint32_t wi = threadIdx.x * per + j;
+        if (wi < nw) c += __popc(bm[wi]);
+    }
+    const uint32_t inc = wave_incl_sum(c);
+    __shared__ uint32_t wsum[16];
+    if ((threadIdx.x & 63) == 63) wsum[threadIdx.x >> 6] = inc;
+    if (threadIdx.x == 0) found = 0xFFFFFFFFu;
+    __syncthreads();
+    uint32_t lo = inc - c;
+    for (uint32_t w = 0; w < (threadIdx.x >> 6); w++) lo += wsum[w];
+    part[threadIdx.x] = lo;
+    if (want >= lo && want < lo + c) {
+        uint32_t r = want - lo;
+        for (uint32_t q = 0; q < per; q++) {
+            const uint32_t wi = threadIdx.x * per + q;
+            if (wi >= nw) break;
+            uint32_t w = bm[wi];
+            const uint32_t pc = __popc(w);
+            if (r < pc) {
+                for (uint32_t z = 0; z < r; z++) w &= w - 1;  // drop the r lowest set bits
+                found = wi * 32 + (__ffs(w) - 1);
+                break;
+            }
+            r -= pc;
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) value[tg] = ((uint64_t)slot_bin[slot] << shift) + found;
+}
+
+// First survivor of each sample once the survivors are in first-occurrence
+// order: surv0[s] = survivors whose first occurrence lies below kbase[s].
+__global__ void k_batch_slices(const uint32_t *__restrict__ first, uint64_t S, const uint32_t *__restrict__ kbase,
+                               uint32_t nsamp, uint64_t *__restrict__ surv0) {
+    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s > nsamp) return;
+    const uint32_t b = kbase[s];
+    uint64_t lo = 0, hi = S;
+    while (lo < hi) {
+        const uint64_t mid = (lo + hi) >> 1;
+        if (first[mid] < b) lo = mid + 1; else hi = mid;
+    }
+    surv0[s] = s == nsamp ? S : lo;
+}
+
+// A sample's survivors taken from a batch build: first occurrences relative to
+// its first input k-mer, and its survivors per count partition of its own.
+__global__ void k_rebase_first(const uint32_t *__restrict__ src, uint32_t *__restrict__ dst, uint64_t n,
+                               uint32_t base) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = src[i] - base;
+}
+
+__global__ void k_part_surv_count(const uint64_t *__restrict__ key, uint64_t n, Mixer mx, int shift,
+                                  uint32_t *__restrict__ part_surv) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) atomicAdd(&part_surv[mix_fwd(mx, key[i]) >> shift], 1u);
+}
+
+// ---------------------------------------------------------------------------
 // Pipeline::build
 
 static thread_local Arena *t_arena = nullptr;
@@ -1751,6 +2023,12 @@ int Pipeline::fail(int code, const char *msg) {
         if (e_ != hipSuccess) return fail(-5, hipGetErrorString(e_));        \
     } while (0)
 
+// Layout of bsel_ in a batch build: per-sample distinct counts and largest
+// first occurrences (u32), the samples' first survivors (u64), and per wanted
+// growth rank (at most 32 per sample): rank (u64), slot, position in its bin,
+// bin of each slot (u32), value (u64).
+static uint64_t batch_sel_bytes(uint64_t ns) { return ns * 8 + (ns + 1) * 8 + 32 * ns * (8 + 4 + 4 + 4 + 8) + 256; }
+
 int Pipeline::build(const uint8_t *d_in, uint64_t n) {
     SWAP_CHECK(hipSetDevice(dev_));
     if (!current_arena()) leave_arena();  // an ordinary build never writes a static call's space
@@ -1818,6 +2096,37 @@ int Pipeline::build(const uint8_t *d_in, uint64_t n) {
     for (int i = 0; i < 32; i++) n_reads += hs[512 + 16 * i];
     n_occ = N;
     if (N >= 0xFFFFFFF0ull) return fail(-4, "more than 2^32 k-mers in one build");
+    // batch build: every sample's first record and first input k-mer, its reads
+    const uint32_t nsamp = batch_off_ ? (uint32_t)(batch_off_->size() - 1) : 0;
+    uint32_t *d_kbase = nullptr;
+    if (batch_off_) {
+        SWAP_CHECK(bdev_.reserve((uint64_t)(nsamp + 1) * 40 + 64));
+        uint64_t *d_off = bdev_.as<uint64_t>();
+        uint64_t *d_rfirst = d_off + (nsamp + 1);
+        unsigned long long *d_reads = (unsigned long long *)(d_rfirst + (nsamp + 1));
+        d_kbase = reinterpret_cast<uint32_t *>(d_reads + (nsamp + 1));
+        uint32_t *d_bad = d_kbase + (nsamp + 1);
+        SWAP_CHECK(hipMemcpyAsync(d_off, batch_off_->data(), (nsamp + 1) * 8, hipMemcpyHostToDevice, st_));
+        SWAP_CHECK(hipMemsetAsync(d_reads, 0, (nsamp + 1) * 8, st_));
+        SWAP_CHECK(hipMemsetAsync(d_bad, 0, 4, st_));
+        hipLaunchKernelGGL(k_batch_bounds, dim3((nsamp + 256) / 256), dim3(256), 0, st_, buf, d_off, nsamp,
+                           nl_pos_.as<uint64_t>(), d_small, rec_start_.as<uint64_t>(), occ_base_.as<uint64_t>(),
+                           d_rfirst, d_kbase, d_bad);
+        hipLaunchKernelGGL(k_batch_reads, dim3((unsigned)std::min<uint64_t>(1024, (R + 255) / 256)), dim3(256), 0, st_,
+                           occ_count_.as<uint32_t>(), R, d_rfirst, nsamp, d_reads);
+        std::vector<uint64_t> reads(nsamp);
+        std::vector<uint32_t> kb(nsamp + 2);
+        SWAP_CHECK(hipMemcpyAsync(reads.data(), d_reads, nsamp * 8, hipMemcpyDeviceToHost, st_));
+        SWAP_CHECK(hipMemcpyAsync(kb.data(), d_kbase, (nsamp + 2) * 4, hipMemcpyDeviceToHost, st_));
+        SWAP_CHECK(hipStreamSynchronize(st_));
+        if (kb[nsamp + 1]) return fail(-1, "a sample does not start with '>', end with a newline and hold line pairs");
+        batch_.assign(nsamp, BatchSample());
+        for (uint32_t i = 0; i < nsamp; i++) {
+            batch_[i].kbase = kb[i];
+            batch_[i].kmers = (uint64_t)kb[i + 1] - kb[i];
+            batch_[i].reads = reads[i];
+        }
+    }
     if (N == 0) return 0;
     // --- pack
     SWAP_CHECK(packed_.reserve((W + 1) * 8));
@@ -1918,23 +2227,19 @@ int Pipeline::build(const uint8_t *d_in, uint64_t n) {
     // by the conversion above)
     const unsigned grid2 = (unsigned)(N / kChunkRecs + kL1 + 1);
     phase_begin(7);
+#define SWAP_L2(OCC, MAX, B)                                                                                   \
+    hipLaunchKernelGGL((k_part2<OCC, MAX, B>), dim3(grid2), dim3(kL2Threads), 0, st_, rec1_.as<uint32_t>(), l1_start, \
+                       chunk_start, k_, l2bits, cursor_.as<uint32_t>(), rec_.as<uint32_t>(), d_kbase, nsamp)
     if (pbits == kPartBitsMin) {
-        if (ref_order_)
-            hipLaunchKernelGGL((k_part2<true, kHistL2>), dim3(grid2), dim3(kL2Threads), 0, st_, rec1_.as<uint32_t>(),
-                               l1_start, chunk_start, k_, l2bits, cursor_.as<uint32_t>(), rec_.as<uint32_t>());
-        else
-            hipLaunchKernelGGL((k_part2<false, kHistL2>), dim3(grid2), dim3(kL2Threads), 0, st_, rec1_.as<uint32_t>(),
-                               l1_start, chunk_start, k_, l2bits, cursor_.as<uint32_t>(), rec_.as<uint32_t>());
+        if (batch_off_) SWAP_L2(true, kHistL2, true);
+        else if (ref_order_) SWAP_L2(true, kHistL2, false);
+        else SWAP_L2(false, kHistL2, false);
     } else {
-        if (ref_order_)
-            hipLaunchKernelGGL((k_part2<true, (1 << kL2MaxBits)>), dim3(grid2), dim3(kL2Threads), 0, st_,
-                               rec1_.as<uint32_t>(), l1_start, chunk_start, k_, l2bits, cursor_.as<uint32_t>(),
-                               rec_.as<uint32_t>());
-        else
-            hipLaunchKernelGGL((k_part2<false, (1 << kL2MaxBits)>), dim3(grid2), dim3(kL2Threads), 0, st_,
-                               rec1_.as<uint32_t>(), l1_start, chunk_start, k_, l2bits, cursor_.as<uint32_t>(),
-                               rec_.as<uint32_t>());
+        if (batch_off_) SWAP_L2(true, (1 << kL2MaxBits), true);
+        else if (ref_order_) SWAP_L2(true, (1 << kL2MaxBits), false);
+        else SWAP_L2(false, (1 << kL2MaxBits), false);
     }
+#undef SWAP_L2
     phase_end(7);
     // --- count + cutoff
     // survivors <= distinct <= N; reserve N to avoid a sync (grow-only buffers)
@@ -1964,10 +2269,21 @@ int Pipeline::build(const uint8_t *d_in, uint64_t n) {
     ca.part_first = part_first_.as<uint32_t>();
     ca.spill = table_spill_ ? 1 : 0;
     ca.dfo = ref_order_ ? dfo_.as<uint32_t>() : nullptr;
+    ca.batch = batch_off_ ? 1 : 0;
+    ca.pbits = pbits;
+    ca.nsamp = nsamp;
+    ca.samp_distinct = nullptr;
+    ca.samp_last = nullptr;
+    if (batch_off_) {
+        SWAP_CHECK(bsel_.reserve(batch_sel_bytes(nsamp)));
+        ca.samp_distinct = bsel_.as<uint32_t>();
+        ca.samp_last = ca.samp_distinct + nsamp;
+        SWAP_CHECK(hipMemsetAsync(ca.samp_distinct, 0, (uint64_t)nsamp * 8, st_));
+    }
     // small partitions share table fills: 2^gbits of them per fill, while a
     // fill of average partitions stays within fill_recs records (4096 by default)
     int gbits = 0;
-    while (!single_fill_ && gbits < kMaxFillBits && gbits < l2bits && ((N >> pbits) << (gbits + 1)) <= tune_.fill_recs)
+    while (!single_fill_ && !batch_off_ && gbits < kMaxFillBits && gbits < l2bits && ((N >> pbits) << (gbits + 1)) <= tune_.fill_recs)
         gbits++;
     ca.gbits = gbits;
     const uint64_t nfills = nparts >> gbits;
@@ -1976,14 +2292,16 @@ int Pipeline::build(const uint8_t *d_in, uint64_t n) {
     // and at least the number whose fill bounds fit the kernel's LDS list
     const unsigned count_grid = (unsigned)std::min<int>(
         (int)nfills, std::max(ncu, (int)((nfills + kMaxFills - 1) / kMaxFills)));
-    if (ref_order_ && gbits)
-        hipLaunchKernelGGL((k_count_lds<true, true>), dim3(count_grid), dim3(kCountThreads), 0, st_, ca);
+    if (batch_off_)
+        hipLaunchKernelGGL((k_count_lds<true, false, true>), dim3(count_grid), dim3(kCountThreads), 0, st_, ca);
+    else if (ref_order_ && gbits)
+        hipLaunchKernelGGL((k_count_lds<true, true, false>), dim3(count_grid), dim3(kCountThreads), 0, st_, ca);
     else if (ref_order_)
-        hipLaunchKernelGGL((k_count_lds<true, false>), dim3(count_grid), dim3(kCountThreads), 0, st_, ca);
+        hipLaunchKernelGGL((k_count_lds<true, false, false>), dim3(count_grid), dim3(kCountThreads), 0, st_, ca);
     else if (gbits)
-        hipLaunchKernelGGL((k_count_lds<false, true>), dim3(count_grid), dim3(kCountThreads), 0, st_, ca);
+        hipLaunchKernelGGL((k_count_lds<false, true, false>), dim3(count_grid), dim3(kCountThreads), 0, st_, ca);
     else
-        hipLaunchKernelGGL((k_count_lds<false, false>), dim3(count_grid), dim3(kCountThreads), 0, st_, ca);
+        hipLaunchKernelGGL((k_count_lds<false, false, false>), dim3(count_grid), dim3(kCountThreads), 0, st_, ca);
     phase_end(4);
     SWAP_CHECK(hipGetLastError());
     SWAP_CHECK(hipMemcpyAsync(hs + 8, d_small + 8, 3 * 8, hipMemcpyDeviceToHost, st_));
@@ -2037,7 +2355,7 @@ int Pipeline::build(const uint8_t *d_in, uint64_t n) {
         phase_end(15);
         SWAP_CHECK(hipGetLastError());
     }
-    if (ref_order_ && n_distinct) {
+    if (ref_order_ && n_distinct && !batch_off_) {
         // growth ranks of the reference's k-mer table: the first occurrence of
         // the key of each growth rank, selected on the device
         std::vector<uint64_t> starts = table_epoch_starts(n_distinct, false);
@@ -2083,6 +2401,168 @@ int Pipeline::build(const uint8_t *d_in, uint64_t n) {
         epoch_first_valid_ = false;
     }
     collect_phases();
+    return 0;
+}
+
+int Pipeline::build_batch(const uint8_t *d_in, const std::vector<uint64_t> &off) {
+    if (!ref_order_) return fail(-1, "a batch build replays the reference's table order");
+    const size_t ns = off.empty() ? 0 : off.size() - 1;
+    if (ns == 0 || ns > kMaxBatch) return fail(-1, "a batch build holds 1 to 512 samples");
+    for (size_t i = 0; i < ns; i++)
+        if (off[i + 1] <= off[i]) return fail(-2, "an empty sample");
+    if (off[0] != 0) return fail(-1, "sample offsets start at 0");
+    batch_.clear();
+    batch_off_ = &off;
+    int rc = build(d_in, off[ns]);
+    batch_off_ = nullptr;
+    if (rc) {
+        batch_.clear();
+        return rc;
+    }
+    // no complete record anywhere: the samples were never checked
+    if (batch_.size() != ns) return fail(-1, "a batch without a complete record");
+    const uint32_t nsamp = (uint32_t)ns;
+    const uint64_t S = n_surv, N = n_occ;
+    uint32_t *d_dist = bsel_.as<uint32_t>();
+    uint32_t *d_last = d_dist + nsamp;
+    uint64_t *d_surv0 = reinterpret_cast<uint64_t *>(d_last + nsamp);
+    uint64_t *d_rank = d_surv0 + (nsamp + 1);
+    uint32_t *d_tslot = reinterpret_cast<uint32_t *>(d_rank + 32 * ns);
+    uint32_t *d_tpos = d_tslot + 32 * ns;
+    uint32_t *d_sbin = d_tpos + 32 * ns;
+    uint64_t *d_val = reinterpret_cast<uint64_t *>(d_sbin + 32 * ns);
+    const uint32_t *d_kbase = reinterpret_cast<const uint32_t *>(bdev_.as<uint64_t>() + 3 * (nsamp + 1));
+    if (N == 0) return 0;  // no k-mer: every sample is empty (kmers = 0 from the bounds)
+    // survivors in first-occurrence order: each sample's are one run
+    if (S) {
+        bool renumbered = false;
+        rc = renumber_survivors(&renumbered);
+        if (rc) return rc;
+        if (!renumbered) return fail(-4, "too many input k-mers in one batch to order its survivors");
+        SWAP_CHECK(hipMemcpyAsync(surv_key_.p, rn_[R_KEY].p, S * 8, hipMemcpyDeviceToDevice, st_));
+        surv_renumbered_ = true;
+    }
+    hipLaunchKernelGGL(k_batch_slices, dim3((nsamp + 256) / 256), dim3(256), 0, st_, surv_first_.as<uint32_t>(), S,
+                       d_kbase, nsamp, d_surv0);
+    std::vector<uint32_t> dl(2 * ns);
+    std::vector<uint64_t> s0(ns + 1);
+    SWAP_CHECK(hipMemcpyAsync(dl.data(), d_dist, 2 * ns * 4, hipMemcpyDeviceToHost, st_));
+    SWAP_CHECK(hipMemcpyAsync(s0.data(), d_surv0, (ns + 1) * 8, hipMemcpyDeviceToHost, st_));
+    SWAP_CHECK(hipStreamSynchronize(st_));
+    // growth ranks of every sample's k-mer table as global order statistics
+    std::vector<uint64_t> rank;
+    std::vector<std::pair<uint32_t, uint32_t>> owner;  // (sample, growth) of each wanted rank
+    uint64_t dpre = 0;
+    for (uint32_t i = 0; i < nsamp; i++) {
+        BatchSample &b = batch_[i];
+        b.distinct = dl[i];
+        b.last_first = dl[i] ? dl[ns + i] - b.kbase : 0;
+        b.surv0 = s0[i];
+        b.survivors = s0[i + 1] - s0[i];
+        b.epoch_first.clear();
+        if (b.distinct) {
+            std::vector<uint64_t> st = table_epoch_starts(b.distinct, false);
+            if (st.size() > 32) st.resize(32);
+            b.epoch_first.assign(st.size(), 0);
+            for (size_t j = 0; j < st.size(); j++) {
+                rank.push_back(dpre + st[j]);
+                owner.emplace_back(i, (uint32_t)j);
+            }
+        }
+        dpre += b.distinct;
+    }
+    if (dpre != n_distinct) return fail(-5, "per-sample distinct counts do not add up");
+    const uint32_t T = (uint32_t)rank.size();
+    if (T) {
+        int ncu = 0;
+        SWAP_CHECK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev_));
+        int shift = 0;
+        while ((N >> shift) >= kSelBins) shift++;
+        const unsigned sel_rows = 2 * (unsigned)ncu;
+        SWAP_CHECK(sel_.reserve((kSelBins + 64) * 4));
+        SWAP_CHECK(sel_target_.reserve(kSelBins * 4 + 64));
+        SWAP_CHECK(rows_.reserve((uint64_t)sel_rows * kSelBins * 4));
+        const uint64_t nw = (1u << shift) >= 32 ? (1u << shift) / 32 : 1;
+        SWAP_CHECK(bbits_.reserve((uint64_t)T * nw * 4 + 64));
+        uint32_t *d_hist = sel_.as<uint32_t>();
+        SWAP_CHECK(hipMemsetAsync(d_hist, 0, kSelBins * 4, st_));
+        SWAP_CHECK(hipMemsetAsync(bbits_.p, 0, (uint64_t)T * nw * 4, st_));
+        SWAP_CHECK(hipMemcpyAsync(d_rank, rank.data(), (uint64_t)T * 8, hipMemcpyHostToDevice, st_));
+        hipLaunchKernelGGL(k_sel_hist, dim3(sel_rows), dim3(1024), 0, st_, dfo_.as<uint32_t>(),
+                           (const unsigned long long *)(small_.as<uint64_t>() + 9), shift, rows_.as<uint32_t>(),
+                           (unsigned long long *)(small_.as<uint64_t>() + 12));
+        hipLaunchKernelGGL(k_col_sum, dim3(kSelBins / 256, kHistRowGroups), dim3(256), 0, st_, rows_.as<uint32_t>(),
+                           sel_rows, kSelBins, d_hist);
+        hipLaunchKernelGGL(k_bsel_targets, dim3(1), dim3(1024), 0, st_, d_hist, d_rank, T, sel_target_.as<int>(),
+                           d_tslot, d_tpos, d_sbin);
+        hipLaunchKernelGGL(k_sel_mark, dim3(ncu * 4), dim3(1024), 0, st_, dfo_.as<uint32_t>(),
+                           (const unsigned long long *)(small_.as<uint64_t>() + 9), shift, sel_target_.as<int>(),
+                           bbits_.as<uint32_t>());
+        hipLaunchKernelGGL(k_bsel_pick, dim3(T), dim3(1024), 0, st_, d_tslot, d_tpos, d_sbin, bbits_.as<uint32_t>(),
+                           shift, d_val);
+        SWAP_CHECK(hipGetLastError());
+        std::vector<uint64_t> val(T);
+        SWAP_CHECK(hipMemcpyAsync(val.data(), d_val, (uint64_t)T * 8, hipMemcpyDeviceToHost, st_));
+        SWAP_CHECK(hipStreamSynchronize(st_));
+        for (uint32_t t = 0; t < T; t++) {
+            BatchSample &b = batch_[owner[t].first];
+            b.epoch_first[owner[t].second] = val[t] - b.kbase;
+        }
+    }
+    return 0;
+}
+
+int Pipeline::take_sample(const Pipeline &src, size_t i) {
+    SWAP_CHECK(hipSetDevice(dev_));
+    if (!current_arena()) leave_arena();
+    if (i >= src.batch_.size()) return fail(-1, "no such sample in the last batch build");
+    if (src.k_ != k_ || src.c_ != c_ || !ref_order_ || !src.ref_order_)
+        return fail(-1, "a sample is taken by a context of the batch's k, c and order");
+    const BatchSample &b = src.batch_[i];
+    n_reads = b.reads;
+    n_occ = b.kmers;
+    n_distinct = b.distinct;
+    n_surv = b.survivors;
+    n_kept = n_overflow = n_special = 0;
+    surv_renumbered_ = true;  // the survivors come in first-occurrence order
+    epoch_first_.clear();
+    epoch_first_valid_ = false;
+    for (int p = 0; p < kPhases; p++) {
+        ev_used_[p] = false;
+        phase_ms[p] = 0;
+    }
+    int pbits = kPartBitsMin;
+    while (pbits < kPartBitsMax && pbits < 2 * k_ && (n_occ >> pbits) > tune_.part_recs) pbits++;
+    pbits_ = pbits;
+    const uint64_t nparts = 1ull << pbits, S = n_surv;
+    SWAP_CHECK(small_.reserve(1024 * 8));
+    SWAP_CHECK(part_surv_.reserve(nparts * 4));
+    SWAP_CHECK(part_first_.reserve(nparts * 4));
+    SWAP_CHECK(hipMemsetAsync(part_surv_.p, 0, nparts * 4, st_));
+    if (S) {
+        SWAP_CHECK(surv_key_.reserve(S * 8));
+        SWAP_CHECK(surv_mult_.reserve(S * 8));
+        SWAP_CHECK(surv_first_.reserve(S * 4));
+        SWAP_CHECK(hipMemcpyAsync(surv_key_.p, src.surv_key_.as<uint64_t>() + b.surv0, S * 8, hipMemcpyDeviceToDevice,
+                                  st_));
+        SWAP_CHECK(hipMemcpyAsync(surv_mult_.p, src.surv_mult_.as<uint64_t>() + b.surv0, S * 8,
+                                  hipMemcpyDeviceToDevice, st_));
+        hipLaunchKernelGGL(k_rebase_first, dim3((unsigned)((S + 255) / 256)), dim3(256), 0, st_,
+                           src.surv_first_.as<uint32_t>() + b.surv0, surv_first_.as<uint32_t>(), S, (uint32_t)b.kbase);
+        hipLaunchKernelGGL(k_part_surv_count, dim3((unsigned)((S + 255) / 256)), dim3(256), 0, st_,
+                           surv_key_.as<uint64_t>(), S, mx_, 2 * k_ - pbits, part_surv_.as<uint32_t>());
+    }
+    if (n_distinct) {
+        epoch1_ = epoch_table(n_distinct, false, &b.epoch_first);
+        SWAP_CHECK(etab_.reserve(sizeof(EpochTabHost) + 64));
+        std::memcpy(h_etab_, &epoch1_, sizeof(EpochTabHost));
+        SWAP_CHECK(hipMemcpyAsync(etab_.p, h_etab_, sizeof(EpochTabHost), hipMemcpyHostToDevice, st_));
+    }
+    // the last first occurrence, where a build leaves it for the exports
+    h_small_[41] = b.last_first;
+    SWAP_CHECK(hipMemcpyAsync(small_.as<uint64_t>() + 12, h_small_ + 41, 8, hipMemcpyHostToDevice, st_));
+    SWAP_CHECK(hipGetLastError());
+    SWAP_CHECK(hipStreamSynchronize(st_));
     return 0;
 }
 

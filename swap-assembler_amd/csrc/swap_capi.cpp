@@ -112,6 +112,7 @@ struct swap_ctx {
     int ce_rc = 0;
     swapk::CeFiles ce;
     bool from_static = false;  // the last build ran in the static scratch (swap_assemble_static)
+    std::vector<uint64_t> batch_off;  // the samples' byte offsets of the last batch build, empty if none
     bool tune_fixed = false;   // swap_set_tuning: the table is not consulted
 
     // waits for every staging loop still running (the staging buffers and the
@@ -264,6 +265,7 @@ void swap_destroy(swap_ctx *c) { delete c; }
 int swap_build_device(swap_ctx *c, const void *d_fasta, size_t n) {
     if (!c || (!d_fasta && n)) return SWAP_E_INVAL;
     c->built = c->merged = c->have_graph = c->have_compacted = c->have_ce = false;
+    c->batch_off.clear();
     c->from_static = false;
     if (n == 0) return set_err(c, SWAP_E_EMPTY, "empty input");
     if (!c->tune_fixed) c->pipe->set_tuning(swapk::tuning_for(n));
@@ -354,6 +356,55 @@ int swap_build(swap_ctx *c, const char *fasta, size_t n) {
     if (c->staging.copy(c->input.p, fasta, n, c->pipe->stream()) != hipSuccess)
         return set_err(c, SWAP_E_DEVICE, "copy of the input to the device");
     return swap_build_device(c, c->input.p, n);
+}
+
+int swap_build_batch(swap_ctx *c, const void *const *d_fasta, const size_t *n, size_t nsamples) {
+    if (!c || !d_fasta || !n || nsamples == 0) return SWAP_E_INVAL;
+    c->built = c->merged = c->have_graph = c->have_compacted = c->have_ce = false;
+    c->from_static = false;
+    c->batch_off.clear();
+    if (c->cfg.flags & (SWAP_FLAG_HASH_ORDER | SWAP_FLAG_KMER_GRAPH))
+        return set_err(c, SWAP_E_INVAL, "a batch build is for merges in the reference's order");
+    if (nsamples > swapk::kMaxBatchSamples) return set_err(c, SWAP_E_INVAL, "more than 512 samples in one batch");
+    std::vector<uint64_t> off(nsamples + 1, 0);
+    for (size_t i = 0; i < nsamples; i++) {
+        if (!d_fasta[i] || n[i] == 0) return set_err(c, SWAP_E_EMPTY, "an empty sample");
+        off[i + 1] = off[i] + n[i];
+    }
+    if (hipSetDevice(c->cfg.device) != hipSuccess) return set_err(c, SWAP_E_DEVICE, "hipSetDevice");
+    auto t0 = clk::now();
+    // the samples back to back in the context's input buffer
+    if (c->input.reserve(off[nsamples] + 16) != hipSuccess)
+        return set_err(c, SWAP_E_DEVICE, "device allocation of the batch input");
+    for (size_t i = 0; i < nsamples; i++)
+        if (hipMemcpyAsync(static_cast<char *>(c->input.p) + off[i], d_fasta[i], n[i], hipMemcpyDeviceToDevice,
+                           c->pipe->stream()) != hipSuccess)
+            return set_err(c, SWAP_E_DEVICE, "copy of a sample into the batch input");
+    if (!c->tune_fixed) c->pipe->set_tuning(swapk::tuning_for(off[nsamples]));
+    int rc = c->pipe->build_batch(static_cast<const uint8_t *>(c->input.p), off);
+    c->build_ms = std::chrono::duration<double, std::milli>(clk::now() - t0).count();
+    if (rc) return set_err(c, rc, c->pipe->last_error());
+    c->batch_off = off;
+    return SWAP_OK;
+}
+
+int swap_batch_take(swap_ctx *batch, size_t i, swap_ctx *dst) {
+    if (!batch || !dst || batch == dst) return SWAP_E_INVAL;
+    dst->built = dst->merged = dst->have_graph = dst->have_compacted = dst->have_ce = false;
+    dst->from_static = false;
+    if (batch->batch_off.empty()) return set_err(dst, SWAP_E_STATE, "no batch build to take a sample from");
+    if (i + 1 >= batch->batch_off.size()) return set_err(dst, SWAP_E_INVAL, "no such sample in the batch");
+    if (dst->cfg.k != batch->cfg.k || dst->cfg.c != batch->cfg.c || dst->cfg.device != batch->cfg.device ||
+        (dst->cfg.flags & (SWAP_FLAG_HASH_ORDER | SWAP_FLAG_KMER_GRAPH)))
+        return set_err(dst, SWAP_E_INVAL, "the taking context needs the batch's k, c, device and order");
+    if (hipSetDevice(dst->cfg.device) != hipSuccess) return set_err(dst, SWAP_E_DEVICE, "hipSetDevice");
+    if (!dst->tune_fixed) dst->pipe->set_tuning(swapk::tuning_for(batch->batch_off[i + 1] - batch->batch_off[i]));
+    auto t0 = clk::now();
+    const int rc = dst->pipe->take_sample(*batch->pipe, i);
+    dst->build_ms = std::chrono::duration<double, std::milli>(clk::now() - t0).count();
+    if (rc) return set_err(dst, rc, dst->pipe->last_error());
+    dst->built = true;
+    return SWAP_OK;
 }
 
 int swap_merge(swap_ctx *c) {

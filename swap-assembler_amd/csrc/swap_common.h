@@ -20,6 +20,9 @@ constexpr int kPartBitsMax = 20;
 // Smallest k the merge accepts: the reference's command line allows odd k in
 // 19..31 (sequence.cpp:44).  Shorter k-mer graphs are built (-s) but not merged.
 constexpr int kMergeMinK = 19;
+// Samples of one batch build (swap_build_batch): the sample of a record rides
+// in 9 key bits of the records between the partition passes and the count.
+constexpr size_t kMaxBatchSamples = 512;
 
 // The static device scratch of swap_assemble_static (one per GPU, held until
 // the process ends) and the largest FASTA input that entry point accepts: the

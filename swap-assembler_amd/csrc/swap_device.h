@@ -146,6 +146,16 @@ struct MergeDev {
     uint64_t n11 = 0;
 };
 
+// One sample of a batch build (Pipeline::build_batch): what a build of the
+// sample alone leaves for the merge and the exports.
+struct BatchSample {
+    uint64_t kbase = 0;       // its first input k-mer index in the batch
+    uint64_t reads = 0, kmers = 0, distinct = 0, survivors = 0;
+    uint64_t surv0 = 0;       // its survivors: [surv0, surv0 + survivors) of the batch's, in first-occurrence order
+    uint64_t last_first = 0;  // largest first occurrence of its distinct keys (its own input indices)
+    std::vector<uint64_t> epoch_first;  // first occurrence of the key of each growth rank of its k-mer table
+};
+
 class Pipeline {
 public:
     Pipeline(int device, int k, int c, bool reference_order);
@@ -166,6 +176,17 @@ public:
     // Builds the k-mer graph from FASTA bytes already in device memory.
     // Returns 0 or a negative SWAP_E* code.
     int build(const uint8_t *d_fasta, uint64_t n);
+    // Builds the k-mer graphs of several samples in one pass of the build
+    // kernels (reference order only): the samples' FASTA bytes lie back to back
+    // in d_fasta, sample i at [off[i], off[i + 1]).  Every sample must start
+    // with '>', end with a newline and hold an even number of lines before the
+    // last sample (SWAP_E_INVAL otherwise).  Results per sample in batch().
+    int build_batch(const uint8_t *d_fasta, const std::vector<uint64_t> &off);
+    const std::vector<BatchSample> &batch() const { return batch_; }
+    // This context's state becomes that of a build of sample i of src's last
+    // batch build alone (same k, c and order mode): ready for merge_device and
+    // the exports.  Returns when the copies are done.
+    int take_sample(const Pipeline &src, size_t i);
     // Compacts unitigs (k-molecule merge).  prio: per-survivor visit rank in
     // survivor order (device or host array, may be null for the built-in
     // hash order).
@@ -214,6 +235,11 @@ private:
     bool single_fill_ = false;
     Tuning tune_;
     int pbits_ = kPartBitsMin;  // partition bits of the last build
+    // batch builds: the samples' byte offsets while build() runs for
+    // build_batch, and the per-sample results
+    const std::vector<uint64_t> *batch_off_ = nullptr;
+    std::vector<BatchSample> batch_;
+    DevBuf bdev_, bsel_, bbits_;
     hipStream_t st_ = nullptr;
     hipError_t init_err_ = hipSuccess;
     Mixer mx_;

@@ -18,7 +18,8 @@ SWAP_FLAG_HASH_ORDER = 2
 SWAP_FLAG_TABLE_SPILL = 4
 SWAP_FLAG_WIDE_PARTITIONS = 8
 SWAP_FLAG_SINGLE_FILL = 16
-SWAP_E_STATE = -8  # call out of order (include/swap_hip.h)
+SWAP_E_INVAL = -1  # bad argument (include/swap_hip.h)
+SWAP_E_STATE = -8  # call out of order
 SWAP_FILE_KMER_GRAPH = 0
 SWAP_FILE_CONTIG_GRAPH = 1
 SWAP_FILE_NOCE = 2
@@ -30,6 +31,7 @@ SWAP_FILE_JUNG_MUL = 6
 # exported entry points and their signatures (mirrors include/swap_hip.h)
 EXPORTS = [
     "swap_create", "swap_destroy", "swap_build", "swap_build_device", "swap_prefetch", "swap_merge", "swap_assemble_static",
+    "swap_build_batch", "swap_batch_take", "swap_server_submit_fused",
     "swap_get_stats", "swap_tuning_for", "swap_set_tuning", "swap_get_tuning",
     "swap_export_kmer_graph", "swap_format_file", "swap_write_file", "swap_strerror", "swap_last_error",
     "swap_table_order", "swap_visit_order", "swap_table_epochs", "swap_reconfigure", "swap_export_compacted",
@@ -93,6 +95,11 @@ def lib():
         L.swap_prefetch.restype = ctypes.c_int
         L.swap_merge.argtypes = [ctypes.c_void_p]
         L.swap_merge.restype = ctypes.c_int
+        L.swap_build_batch.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_void_p), ctypes.POINTER(ctypes.c_size_t),
+                                       ctypes.c_size_t]
+        L.swap_build_batch.restype = ctypes.c_int
+        L.swap_batch_take.argtypes = [ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p]
+        L.swap_batch_take.restype = ctypes.c_int
         L.swap_assemble_static.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t]
         L.swap_assemble_static.restype = ctypes.c_int
         L.swap_get_stats.argtypes = [ctypes.c_void_p, ctypes.POINTER(SwapStats)]
@@ -134,6 +141,9 @@ def lib():
         L.swap_server_submit_batch.argtypes = [ctypes.c_void_p, ctypes.POINTER(SwapJob), ctypes.c_size_t,
                                                ctypes.c_void_p]
         L.swap_server_submit_batch.restype = ctypes.c_int
+        L.swap_server_submit_fused.argtypes = [ctypes.c_void_p, ctypes.POINTER(SwapJob), ctypes.c_size_t,
+                                               ctypes.c_size_t, ctypes.c_void_p]
+        L.swap_server_submit_fused.restype = ctypes.c_int
         L.swap_server_wait.argtypes = [ctypes.c_void_p, ctypes.c_int64]
         L.swap_server_wait.restype = ctypes.c_int
         L.swap_server_wait_all.argtypes = [ctypes.c_void_p]
@@ -220,6 +230,19 @@ class Assembler:
 
     def merge(self):
         self._check(lib().swap_merge(self._h))
+
+    def build_batch(self, bufs):
+        """One build of many samples (swap_build_batch): bufs is a list of
+        device-resident FASTA inputs, each a DeviceBuffer or a (pointer, size) pair."""
+        pairs = [(b.ptr.value, b.n) if isinstance(b, DeviceBuffer) else (int(b[0]), int(b[1])) for b in bufs]
+        n = len(pairs)
+        ptrs = (ctypes.c_void_p * n)(*[p for p, _ in pairs])
+        sizes = (ctypes.c_size_t * n)(*[m for _, m in pairs])
+        self._check(lib().swap_build_batch(self._h, ptrs, sizes, n))
+
+    def take(self, batch, i):
+        """This context takes sample i of batch's last build_batch (swap_batch_take)."""
+        self._check(lib().swap_batch_take(batch._h, i, self._h))
 
     def assemble_static(self, ptr: int, n: int):
         """Build (and merge, unless kmer_graph_only) device-resident FASTA bytes
@@ -346,6 +369,18 @@ class Server:
             self._keep[int(jid)] = (jobs[i], keep[i] if keep else None)
         return [int(x) for x in ids]
 
+    def submit_fused(self, jobs, group_bytes=0, keep=None):
+        """Merge jobs of one k and c with device-resident inputs, built in
+        groups by one batch build each (swap_server_submit_fused)."""
+        arr = (SwapJob * len(jobs))(*jobs)
+        ids = np.zeros(len(jobs), dtype=np.int64)
+        rc = lib().swap_server_submit_fused(self._h, arr, len(jobs), group_bytes, ids.ctypes.data)
+        if rc:
+            raise SwapError(rc)
+        for i, jid in enumerate(ids):
+            self._keep[int(jid)] = (jobs[i], keep[i] if keep else None)
+        return [int(x) for x in ids]
+
     def wait(self, jid):
         rc = lib().swap_server_wait(self._h, jid)
         if rc < 0:
@@ -374,6 +409,10 @@ class Server:
         if n < 0:
             raise SwapError(int(n))
         return buf.raw[:n]
+
+    def error(self, jid):
+        """Error text of an ended job ("" if none)."""
+        return lib().swap_server_job_error(self._h, jid).decode()
 
     def release(self, jid):
         lib().swap_server_release(self._h, jid)
