@@ -42,7 +42,7 @@ PHASES = {0: "record_scan", 1: "pack_2bit", 2: "partition_hist", 3: "partition_l
 # phase index -> device kernel of that phase whose hardware byte counters are kept
 # (the bacterial workload: one partition per count table fill, 2^7 level-2 sub-buckets)
 # in profiles/traffic.json (written from rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE)
-PHASE_KERNEL = {4: "swapk::k_count_lds<%s, false>", 3: "swapk::k_part1<%s>", 7: "swapk::k_part2<%s, 128>",
+PHASE_KERNEL = {4: "swapk::k_count_lds<%s, false, false>", 3: "swapk::k_part1<%s>", 7: "swapk::k_part2<%s, 128, false>",
                 2: "swapk::k_part_hist", 1: "swapk::k_pack"}
 
 
@@ -57,7 +57,8 @@ def measured_traffic(phase, ref_order, workload):
         return None
     name = PHASE_KERNEL[phase]
     if "%s" in name:
-        name = name % ("true" if ref_order else "false")
+        # level 1 stages rounds of 11 records (reference order) or 16
+        name = name % (("true, 11" if ref_order else "false, 16") if phase == 3 else ("true" if ref_order else "false"))
     k = t.get("kernels", {}).get(name)
     return k["bytes"] if k else None
 
@@ -225,7 +226,7 @@ def copy_probe(nbytes):
     H.hipFree(b)
 
 
-def batched_vs_per_sample(n_samples, slots, group_mib=512, warmup=True):
+def batched_vs_per_sample(n_samples, slots, group_mib=256, fused_slots=8, warmup=True):
     """Metric 2: the many-sample workload (seeded genomes log-uniform in
     [20 kbp, 2 Mbp], 150 bp reads at 20-80x, 0.5% errors, k=31, c=4), inputs
     resident in HBM, noCEcontigs.fasta produced for every sample.  Plain path:
@@ -279,7 +280,7 @@ def batched_vs_per_sample(n_samples, slots, group_mib=512, warmup=True):
     srv.close()
 
     # fused: the same jobs built in groups by batch builds
-    srv = swapamd.Server(slots=slots)
+    srv = swapamd.Server(slots=fused_slots)
     gb = group_mib << 20
     if warmup:  # grow the batch contexts and the slots once
         ids = srv.submit_fused(jobs, group_bytes=gb)
@@ -314,7 +315,7 @@ def batched_vs_per_sample(n_samples, slots, group_mib=512, warmup=True):
             "per_sample_samples_per_s": n_samples / t_plain, "batched_samples_per_s": n_samples / t_fused,
             "factor": t_plain / t_fused, "per_sample_s": t_plain, "batched_s": t_fused,
             "batched_path": "fused: groups of up to %d MiB of FASTA, one batch build each, merged by %d slots"
-                            % (group_mib, slots),
+                            % (group_mib, fused_slots),
             "identical_outputs": same_fused and same,
             "concurrent_slots": {"samples_per_s": n_samples / t_batch, "factor": t_plain / t_batch, "s": t_batch,
                                  "identical_outputs": same},
@@ -435,8 +436,11 @@ def main():
     ap.add_argument("--batch-slots", type=int, default=12,
                     help="concurrent contexts of the batched and serving figures (12: the best of 4/8/12/16 "
                          "on the batch workload)")
-    ap.add_argument("--fuse-group-mib", type=int, default=512,
-                    help="FASTA MiB per batch build of the batched figure's fused submission")
+    ap.add_argument("--fuse-group-mib", type=int, default=256,
+                    help="FASTA MiB per batch build of the batched figure's fused submission (256: the best of "
+                         "128/256/512)")
+    ap.add_argument("--fuse-slots", type=int, default=8,
+                    help="merge slots of the fused submission (8: the best of 8/12/16 with 256 MiB groups)")
     ap.add_argument("--serve-requests", type=int, default=60,
                     help="requests of the serving figure (Poisson arrivals at --serve-rate; 0 skips it)")
     ap.add_argument("--serve-rate", type=float, default=20.0)
@@ -565,7 +569,7 @@ def main():
     if args.batch_samples > 0:
         a.close()
         dev.close()
-        batched = batched_vs_per_sample(args.batch_samples, args.batch_slots, args.fuse_group_mib)
+        batched = batched_vs_per_sample(args.batch_samples, args.batch_slots, args.fuse_group_mib, args.fuse_slots)
     served = None
     if args.serve_requests > 0:
         a.close()

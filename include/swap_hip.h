@@ -313,7 +313,7 @@ int swap_server_submit_batch(swap_server *s, const swap_job *jobs, size_t n, int
 /* Queues n merge jobs of one k, c and flags (reference order, no
  * SWAP_FLAG_KMER_GRAPH) whose FASTA is in device memory
  * (SWAP_JOB_DEVICE_INPUT), built together: largest input first, the jobs are
- * cut into groups of at most group_bytes of FASTA (0: 512 MiB) and 512
+ * cut into groups of at most group_bytes of FASTA (0: 256 MiB; at most 512 MiB) and 512
  * samples; a builder thread of the server builds each group with one
  * swap_build_batch (the next group while the slots merge the previous one),
  * and the slot that runs a job takes its sample (swap_batch_take) instead of
