@@ -1943,7 +1943,7 @@ void Pipeline::leave_arena() {
                       &occ_count_, &word_count_, &occ_base_, &word_base_, &block_rec_, &packed_, &wmeta_, &rows_,
                       &hist_, &bucket_, &cursor_, &hist1_, &base1_, &l1info_, &rec1_, &rec_, &surv_key_, &surv_mult_,
                       &surv_first_, &dfo_, &sel_, &sel_target_, &sel_out_, &overflow_, &part_surv_, &part_first_,
-                      &gtab_key_, &gtab_cnt_, &gtab_fo_, &etab_, &etabS_};
+                      &gtab_key_, &gtab_cnt_, &gtab_fo_, &etab_, &etabS_, &scan_scratch2_, &bdev_, &bsel_, &bbits_};
     for (DevBuf *b : bufs) b->leave_arena();
     for (DevBuf &b : mb_) b.leave_arena();
     for (DevBuf &b : ob_) b.leave_arena();
@@ -1963,6 +1963,10 @@ Pipeline::Pipeline(int device, int k, int c, bool reference_order) : dev_(device
     };
     keep(hipSetDevice(dev_));
     keep(hipStreamCreateWithFlags(&st_, hipStreamNonBlocking));
+    keep(hipStreamCreateWithFlags(&st2_, hipStreamNonBlocking));
+    keep(hipEventCreateWithFlags(&sel_done_, hipEventDisableTiming));
+    keep(hipEventCreateWithFlags(&prio_done_, hipEventDisableTiming));
+    keep(hipEventCreateWithFlags(&fork_, hipEventDisableTiming));
     keep(hipHostMalloc((void **)&h_small_, 1024 * sizeof(uint64_t), hipHostMallocDefault));
     keep(hipHostMalloc((void **)&h_starts_, 64 * sizeof(uint64_t), hipHostMallocDefault));
     keep(hipHostMalloc((void **)&h_etab_, sizeof(EpochTabHost), hipHostMallocDefault));
@@ -1985,14 +1989,15 @@ void Pipeline::reconfigure(int k, int c, bool reference_order) {
 }
 
 // Phase timing is diagnostic: a failed event call leaves that phase at 0 ms.
-void Pipeline::phase_begin(int i) {
-    ev_used_[i] = hipEventRecord(ev_[2 * i], st_) == hipSuccess;
+void Pipeline::phase_begin(int i, hipStream_t s) {
+    ev_used_[i] = hipEventRecord(ev_[2 * i], s ? s : st_) == hipSuccess;
 }
-void Pipeline::phase_end(int i) {
-    if (hipEventRecord(ev_[2 * i + 1], st_) != hipSuccess) ev_used_[i] = false;
+void Pipeline::phase_end(int i, hipStream_t s) {
+    if (hipEventRecord(ev_[2 * i + 1], s ? s : st_) != hipSuccess) ev_used_[i] = false;
 }
-void Pipeline::collect_phases() {
-    const bool synced = hipStreamSynchronize(st_) == hipSuccess;
+// (a phase recorded on the side stream and not yet ended reads 0 ms)
+void Pipeline::collect_phases(bool side) {
+    const bool synced = hipStreamSynchronize(st_) == hipSuccess && (!side || hipStreamSynchronize(st2_) == hipSuccess);
     for (int i = 0; i < kPhases; i++) {
         phase_ms[i] = 0;
         if (synced && ev_used_[i] && hipEventElapsedTime(&phase_ms[i], ev_[2 * i], ev_[2 * i + 1]) != hipSuccess)
@@ -2003,8 +2008,12 @@ void Pipeline::collect_phases() {
 Pipeline::~Pipeline() {
     // teardown: nothing useful can be done about a failing release here
     (void)hipSetDevice(dev_);
+    if (st2_) (void)hipStreamSynchronize(st2_);
     for (auto &e : ev_)
         if (e) (void)hipEventDestroy(e);
+    for (hipEvent_t e : {sel_done_, prio_done_, fork_})
+        if (e) (void)hipEventDestroy(e);
+    if (st2_) (void)hipStreamDestroy(st2_);
     if (h_small_) (void)hipHostFree(h_small_);
     if (h_starts_) (void)hipHostFree(h_starts_);
     if (h_etab_) (void)hipHostFree(h_etab_);
@@ -2031,6 +2040,9 @@ static uint64_t batch_sel_bytes(uint64_t ns) { return ns * 8 + (ns + 1) * 8 + 32
 
 int Pipeline::build(const uint8_t *d_in, uint64_t n) {
     SWAP_CHECK(hipSetDevice(dev_));
+    // the previous build's selection or merge's replay may still read buffers
+    // this build rewrites or grows
+    SWAP_CHECK(hipStreamSynchronize(st2_));
     if (!current_arena()) leave_arena();  // an ordinary build never writes a static call's space
     n_reads = n_occ = n_distinct = n_surv = n_kept = n_overflow = n_special = 0;
     surv_renumbered_ = false;
@@ -2366,9 +2378,13 @@ int Pipeline::build(const uint8_t *d_in, uint64_t n) {
         uint32_t *d_hist = sel_.as<uint32_t>();
         uint64_t *d_starts = reinterpret_cast<uint64_t *>(d_hist + kSelBins + 64);
         SelInfo *d_si = reinterpret_cast<SelInfo *>(d_starts + 32);
+        // on the side stream: the merge's renumbering and index (main stream)
+        // overlap it; the visit-order replay, queued behind it on the side
+        // stream, and the k-mer graph export (sel_done_) read its epoch table
+        hipStream_t ss = st2_;
         std::memcpy(h_starts_, starts.data(), starts.size() * 8);
-        SWAP_CHECK(hipMemcpyAsync(d_starts, h_starts_, starts.size() * 8, hipMemcpyHostToDevice, st_));
-        SWAP_CHECK(hipMemsetAsync(d_hist, 0, kSelBins * 4, st_));
+        SWAP_CHECK(hipMemcpyAsync(d_starts, h_starts_, starts.size() * 8, hipMemcpyHostToDevice, ss));
+        SWAP_CHECK(hipMemsetAsync(d_hist, 0, kSelBins * 4, ss));
         // (d_small[12], the last first occurrence, was zeroed at the build's start)
         SWAP_CHECK(sel_target_.reserve(kSelBins * 4 + 64));
         const uint32_t sel_words = (1u << shift) >= 32 ? (1u << shift) / 32 : 1;  // bitmap words per bin
@@ -2381,26 +2397,27 @@ int Pipeline::build(const uint8_t *d_in, uint64_t n) {
         epoch1_ = epoch_table(n_distinct, false, nullptr);
         SWAP_CHECK(etab_.reserve(sizeof(EpochTabHost) + 64));
         std::memcpy(h_etab_, &epoch1_, sizeof(EpochTabHost));
-        SWAP_CHECK(hipMemcpyAsync(etab_.p, h_etab_, sizeof(EpochTabHost), hipMemcpyHostToDevice, st_));
+        SWAP_CHECK(hipMemcpyAsync(etab_.p, h_etab_, sizeof(EpochTabHost), hipMemcpyHostToDevice, ss));
         EpochTabHost *d_et = etab_.as<EpochTabHost>();
-        phase_begin(6);
-        hipLaunchKernelGGL(k_sel_hist, dim3(sel_rows), dim3(1024), 0, st_, dfo_.as<uint32_t>(),
+        phase_begin(6, ss);
+        hipLaunchKernelGGL(k_sel_hist, dim3(sel_rows), dim3(1024), 0, ss, dfo_.as<uint32_t>(),
                            (const unsigned long long *)(d_small + 9), shift, rows_.as<uint32_t>(),
                            (unsigned long long *)(d_small + 12));
-        hipLaunchKernelGGL(k_col_sum, dim3(kSelBins / 256, kHistRowGroups), dim3(256), 0, st_, rows_.as<uint32_t>(),
+        hipLaunchKernelGGL(k_col_sum, dim3(kSelBins / 256, kHistRowGroups), dim3(256), 0, ss, rows_.as<uint32_t>(),
                            sel_rows, kSelBins, d_hist);
-        hipLaunchKernelGGL(k_sel_targets, dim3(1), dim3(1024), 0, st_, d_hist, d_starts, (int)starts.size(),
+        hipLaunchKernelGGL(k_sel_targets, dim3(1), dim3(1024), 0, ss, d_hist, d_starts, (int)starts.size(),
                            sel_target_.as<int>(), d_si, sel_out_.as<uint32_t>(), 32 * sel_words);
-        hipLaunchKernelGGL(k_sel_mark, dim3(ncu * 4), dim3(1024), 0, st_, dfo_.as<uint32_t>(),
+        hipLaunchKernelGGL(k_sel_mark, dim3(ncu * 4), dim3(1024), 0, ss, dfo_.as<uint32_t>(),
                            (const unsigned long long *)(d_small + 9), shift, sel_target_.as<int>(),
                            sel_out_.as<uint32_t>());
-        hipLaunchKernelGGL(k_sel_pick, dim3(32), dim3(1024), 0, st_, d_si, sel_out_.as<uint32_t>(), shift,
+        hipLaunchKernelGGL(k_sel_pick, dim3(32), dim3(1024), 0, ss, d_si, sel_out_.as<uint32_t>(), shift,
                            &d_et->start[0]);
-        phase_end(6);
+        phase_end(6, ss);
+        SWAP_CHECK(hipEventRecord(sel_done_, ss));
         epoch_first_.clear();
         epoch_first_valid_ = false;
     }
-    collect_phases();
+    collect_phases(false);  // the selection's phase is read after the merge
     return 0;
 }
 
@@ -2514,6 +2531,7 @@ int Pipeline::build_batch(const uint8_t *d_in, const std::vector<uint64_t> &off)
 
 int Pipeline::take_sample(const Pipeline &src, size_t i) {
     SWAP_CHECK(hipSetDevice(dev_));
+    SWAP_CHECK(hipStreamSynchronize(st2_));  // a selection or replay of an earlier build may still run
     if (!current_arena()) leave_arena();
     if (i >= src.batch_.size()) return fail(-1, "no such sample in the last batch build");
     if (src.k_ != k_ || src.c_ != c_ || !ref_order_ || !src.ref_order_)
@@ -2572,6 +2590,7 @@ int Pipeline::export_kmer_graph(HostKmerGraph &out) {
     out.mult.resize(n_surv);
     out.first.clear();
     if (ref_order_ && n_distinct && !epoch_first_valid_) {
+        SWAP_CHECK(hipStreamWaitEvent(st_, sel_done_, 0));
         EpochTabHost et;
         SWAP_CHECK(hipMemcpyAsync(&et, etab_.p, sizeof et, hipMemcpyDeviceToHost, st_));
         SWAP_CHECK(hipMemcpyAsync(hs_tail(), d_small_ptr() + 12, 8, hipMemcpyDeviceToHost, st_));

@@ -192,7 +192,8 @@ public:
     // hash order).
     int merge_device(bool reference_order, MergeDev &out);
     int fetch_merge(MergeDev &out);
-    int visit_priorities(uint64_t *d_prio);
+    // on stream s (the side stream in a merge)
+    int visit_priorities(uint64_t *d_prio, hipStream_t s);
     int renumber_survivors(bool *renumbered);
 
     // Counters of the last build / merge.
@@ -221,9 +222,10 @@ public:
     // phases from 8.
     static constexpr int kPhases = 16;
     float phase_ms[kPhases] = {0};
-    void phase_begin(int i);
-    void phase_end(int i);
-    void collect_phases();
+    void phase_begin(int i, hipStream_t s = nullptr);
+    void phase_end(int i, hipStream_t s = nullptr);
+    // side: also wait for the side stream (phases recorded there)
+    void collect_phases(bool side = true);
 
 private:
     int fail(int code, const char *msg);
@@ -241,6 +243,13 @@ private:
     std::vector<BatchSample> batch_;
     DevBuf bdev_, bsel_, bbits_;
     hipStream_t st_ = nullptr;
+    // Side stream (default priority): the build's growth-rank selection and
+    // the merge's visit-order replay run there, overlapping the main stream's
+    // work that does not need them.  sel_done_: the last selection has ended
+    // (the epoch table is final); prio_done_: the visit priorities are written.
+    hipStream_t st2_ = nullptr;
+    hipEvent_t sel_done_ = nullptr, prio_done_ = nullptr, fork_ = nullptr;
+    DevBuf scan_scratch2_;  // the side stream's scans
     hipError_t init_err_ = hipSuccess;
     Mixer mx_;
     std::string err_;

@@ -1181,15 +1181,30 @@ int Pipeline::merge_device(bool reference_order, MergeDev &out) {
     // --- priorities (visit time)
     MCHECK(mb_[M_PRIO].reserve(S * 8));
     phase_end(8);
-    phase_begin(13);
+    // Reference order: the visit-order replay runs on the side stream (behind
+    // the build's growth-rank selection there), overlapping classification,
+    // list ranking and the kept edges; the main stream waits for it (prio_done_)
+    // before the first reader of the priorities.  Every return from here on
+    // leaves the side stream idle.
+    struct SideIdle {
+        hipStream_t s;
+        ~SideIdle() { (void)hipStreamSynchronize(s); }
+    } side_idle{st2_};
     if (reference_order) {
-        const int rc = visit_priorities(mb_[M_PRIO].as<uint64_t>());
+        MCHECK(hipEventRecord(fork_, st_));
+        MCHECK(hipStreamWaitEvent(st2_, fork_, 0));
+        phase_begin(13, st2_);
+        const int rc = visit_priorities(mb_[M_PRIO].as<uint64_t>(), st2_);
         if (rc) return rc;
+        phase_end(13, st2_);
+        MCHECK(hipEventRecord(prio_done_, st2_));
     } else {
+        phase_begin(13);
         hipLaunchKernelGGL(k_prio_hash, dim3(nblk(S)), dim3(256), 0, st_, surv_key_.as<uint64_t>(), S, mx_,
                            mb_[M_PRIO].as<uint64_t>());
+        phase_end(13);
+        MCHECK(hipEventRecord(prio_done_, st_));
     }
-    phase_end(13);
     phase_begin(14);
     // --- classification
     MergeArgs ma;
@@ -1315,6 +1330,7 @@ int Pipeline::merge_device(bool reference_order, MergeDev &out) {
     uint32_t *na = reinterpret_cast<uint32_t *>(kb + nc), *nb2 = na + nc;
     uint8_t *ba = reinterpret_cast<uint8_t *>(nb2 + nc), *bb = ba + nc;
     compact_scatter<true>(2 * S, cyc, cboff, cpos, cl, st_);
+    MCHECK(hipStreamWaitEvent(st_, prio_done_, 0));  // the cycles' maxima read the priorities
     hipLaunchKernelGGL(k_cyc_init, dim3(nblk(nc)), dim3(256), 0, st_, cl, nc, cpos, mb_[M_NXT].as<uint32_t>(),
                        mb_[M_PRIO].as<uint64_t>(), mb_[M_SELF].as<uint8_t>(), ka, na, ba);
     int crounds = 1;
@@ -1399,6 +1415,7 @@ int Pipeline::merge_device(bool reference_order, MergeDev &out) {
     fa.cpri = mb_[M_CPRI].as<uint64_t>();
     fa.cmul = mb_[M_CMUL].as<uint8_t>();
     fa.cedge = mb_[M_CEDGE].as<uint32_t>();
+    MCHECK(hipStreamWaitEvent(st_, prio_done_, 0));  // the fills read the visit priorities
     phase_begin(11);
     if (ne) {
         hipLaunchKernelGGL(k_fill_edges, dim3(nblk(ne)), dim3(256), 0, st_, fa, mb_[M_EMULT].as<uint8_t>(), ne);

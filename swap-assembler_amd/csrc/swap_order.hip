@@ -459,13 +459,13 @@ EpochTabHost epoch_table(uint64_t n_total, bool extra, const std::vector<uint64_
 }
 
 // Visit priorities (position in the nodes table order) of all survivors.
-int Pipeline::visit_priorities(uint64_t *d_prio) {
+int Pipeline::visit_priorities(uint64_t *d_prio, hipStream_t s) {
     const uint64_t S = n_surv;
     if (!S) return 0;
     // the cutoff copy and the nodes table: S inserts each
     *h_etabS_ = epoch_table(S, false, nullptr);
     OCHECK(etabS_.reserve(sizeof(EpochTabHost) + 64));
-    OCHECK(hipMemcpyAsync(etabS_.p, h_etabS_, sizeof(EpochTabHost), hipMemcpyHostToDevice, st_));
+    OCHECK(hipMemcpyAsync(etabS_.p, h_etabS_, sizeof(EpochTabHost), hipMemcpyHostToDevice, s));
     OCHECK(ob_[O_ENT].reserve(2 * S * sizeof(OrdEntry)));
     OCHECK(ob_[O_R2].reserve(S * 4));
     const uint64_t keys = std::max<uint32_t>(tune_.order_bin_keys, 1);
@@ -485,18 +485,18 @@ int Pipeline::visit_priorities(uint64_t *d_prio) {
     OCHECK(ob_[O_BCNT].reserve(ncnt * 4 + 4));
     OCHECK(ob_[O_BOFF].reserve(ncnt * 8 + 8));
     OCHECK(ob_[O_BSTART].reserve((uint64_t)(nb + 1) * 4));
-    OCHECK(scan_scratch_.reserve((scan_scratch_words(ncnt) + 16) * 8));
+    OCHECK(scan_scratch2_.reserve((scan_scratch_words(ncnt) + 16) * 8));
     // room to order any bin in device memory (a bin above kSortCap keys: far
     // above the average, which stays below 2 x order_bin_keys by default)
     OCHECK(ob_[O_GSORT].reserve(S * 8));
     const uint64_t *key = surv_key_.as<uint64_t>();
-    hipLaunchKernelGGL(k_bin_count, dim3((unsigned)nblk), dim3(kBinThreads), 0, st_, key, S, per, bs,
+    hipLaunchKernelGGL(k_bin_count, dim3((unsigned)nblk), dim3(kBinThreads), 0, s, key, S, per, bs,
                        ob_[O_BCNT].as<uint32_t>());
     OCHECK(scan_excl<uint32_t>(ob_[O_BCNT].as<uint32_t>(), ob_[O_BOFF].as<uint64_t>(), ncnt,
-                               scan_scratch_.as<uint64_t>(), ob_[O_BOFF].as<uint64_t>() + ncnt, st_));
+                               scan_scratch2_.as<uint64_t>(), ob_[O_BOFF].as<uint64_t>() + ncnt, s));
     // table 1's ranks are first occurrences; its epoch table's starts are the
     // first occurrences of the growth ranks, selected on the device by the build
-    hipLaunchKernelGGL(k_bin_scatter, dim3((unsigned)nblk), dim3(kBinThreads), 0, st_, key,
+    hipLaunchKernelGGL(k_bin_scatter, dim3((unsigned)nblk), dim3(kBinThreads), 0, s, key,
                        surv_first_.as<uint32_t>(), S, per, bs, ob_[O_BOFF].as<uint64_t>(), ob_[O_ENT].as<OrdEntry>(),
                        ob_[O_BSTART].as<uint32_t>());
     BinOrderArgs a;
@@ -510,14 +510,14 @@ int Pipeline::visit_priorities(uint64_t *d_prio) {
     a.M = bs.M1;
     a.shift = bs.shift1;
     a.et = static_cast<const EpochTab *>(etab_.p);
-    hipLaunchKernelGGL(k_bin_order<false>, dim3(bs.nb1), dim3(kSortThreads), 0, st_, a);
+    hipLaunchKernelGGL(k_bin_order<false>, dim3(bs.nb1), dim3(kSortThreads), 0, s, a);
     a.bin_start += bs.nb1;
     a.base = (uint32_t)S;
     a.P = bs.P2;
     a.M = bs.M2;
     a.shift = bs.shift2;
     a.et = static_cast<const EpochTab *>(etabS_.p);
-    hipLaunchKernelGGL(k_bin_order<true>, dim3(bs.nb2), dim3(kSortThreads), 0, st_, a);
+    hipLaunchKernelGGL(k_bin_order<true>, dim3(bs.nb2), dim3(kSortThreads), 0, s, a);
     return 0;
 }
 

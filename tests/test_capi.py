@@ -52,6 +52,17 @@ def test_server_rejects_bad_slot_count():
     assert swapamd.lib().swap_server_create(0, 65, 0, ctypes.byref(h)) == -1
 
 
+def test_batch_entry_points_reject_bad_arguments():
+    L = swapamd.lib()
+    ptrs = (ctypes.c_void_p * 1)(None)
+    sizes = (ctypes.c_size_t * 1)(10)
+    assert L.swap_build_batch(None, ptrs, sizes, 1) == swapamd.SWAP_E_INVAL
+    assert L.swap_batch_take(None, 0, None) == swapamd.SWAP_E_INVAL
+    jobs = (swapamd.SwapJob * 1)()
+    ids = (ctypes.c_int64 * 1)()
+    assert L.swap_server_submit_fused(None, jobs, 1, 0, ids) == swapamd.SWAP_E_INVAL
+
+
 def test_create_rejects_bad_k():
     cfg = swapamd.SwapCfg(20, 3, 0, 0)
     h = ctypes.c_void_p()
