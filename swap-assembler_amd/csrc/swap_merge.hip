@@ -458,9 +458,13 @@ __global__ void k_rs_jump(const Walk *__restrict__ win, const uint8_t *__restric
 // All jump rounds in one workgroup when the rulers are few (small samples):
 // one launch instead of one per round.  Round r reads buffer r & 1 and writes
 // the other; the fences around each barrier make a round's device-memory
-// writes visible to every wave of the workgroup.
+// writes visible to every wave of the workgroup.  A thread holds its
+// kJumpOnePer rulers in registers and issues each hop's loads for all of them
+// together (three waves of independent loads per round instead of a chain of
+// dependent ones per ruler).
 constexpr int kJumpOneThreads = 1024;
 constexpr uint64_t kJumpOneMax = 8192;
+constexpr int kJumpOnePer = (int)(kJumpOneMax / kJumpOneThreads);
 
 __global__ void __launch_bounds__(kJumpOneThreads) k_rs_jump_one(Walk *__restrict__ wa, uint8_t *__restrict__ spa,
                                                                 Walk *__restrict__ wb, uint8_t *__restrict__ spb,
@@ -470,22 +474,45 @@ __global__ void __launch_bounds__(kJumpOneThreads) k_rs_jump_one(Walk *__restric
         const uint8_t *spin = (r & 1) ? spb : spa;
         Walk *wout = (r & 1) ? wa : wb;
         uint8_t *spout = (r & 1) ? spa : spb;
-        for (uint64_t v = threadIdx.x; v < nr; v += kJumpOneThreads) {
-            Walk x = win[v];
-            uint8_t f = spin[v];
+        Walk x[kJumpOnePer];
+        uint8_t f[kJumpOnePer];
 #pragma unroll
-            for (int hop = 0; hop < 2; hop++) {
-                if (x.succ == kNil) break;
-                const uint32_t s = x.succ;
-                const Walk y = win[s];
-                f |= spin[s];
-                x.dist += y.dist;
-                x.succ = y.succ;
-                x.term = y.term;
-                x.last = y.last;
+        for (int j = 0; j < kJumpOnePer; j++) {
+            const uint64_t v = threadIdx.x + (uint64_t)j * kJumpOneThreads;
+            if (v < nr) {
+                x[j] = win[v];
+                f[j] = spin[v];
+            } else {
+                x[j].succ = kNil;
             }
-            wout[v] = x;
-            spout[v] = f;
+        }
+#pragma unroll
+        for (int hop = 0; hop < 2; hop++) {
+            Walk y[kJumpOnePer];
+            uint8_t g[kJumpOnePer];
+#pragma unroll
+            for (int j = 0; j < kJumpOnePer; j++)
+                if (x[j].succ != kNil) {
+                    y[j] = win[x[j].succ];
+                    g[j] = spin[x[j].succ];
+                }
+#pragma unroll
+            for (int j = 0; j < kJumpOnePer; j++)
+                if (x[j].succ != kNil) {
+                    f[j] |= g[j];
+                    x[j].dist += y[j].dist;
+                    x[j].succ = y[j].succ;
+                    x[j].term = y[j].term;
+                    x[j].last = y[j].last;
+                }
+        }
+#pragma unroll
+        for (int j = 0; j < kJumpOnePer; j++) {
+            const uint64_t v = threadIdx.x + (uint64_t)j * kJumpOneThreads;
+            if (v < nr) {
+                wout[v] = x[j];
+                spout[v] = f[j];
+            }
         }
         __threadfence();
         __syncthreads();
