@@ -42,7 +42,7 @@ PHASES = {0: "record_scan", 1: "pack_2bit", 2: "partition_hist", 3: "partition_l
 # phase index -> device kernel of that phase whose hardware byte counters are kept
 # (the bacterial workload: one partition per count table fill, 2^7 level-2 sub-buckets)
 # in profiles/traffic.json (written from rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE)
-PHASE_KERNEL = {4: "swapk::k_count_lds<%s, false, false>", 3: "swapk::k_part1<%s>", 7: "swapk::k_part2<%s, 128, false>",
+PHASE_KERNEL = {4: "swapk::k_count_lds<%s, false, false>", 3: "swapk::k_part1<%s>", 7: "swapk::k_part2<%s, 128, false, 4096, 512>",
                 2: "swapk::k_part_hist", 1: "swapk::k_pack"}
 
 

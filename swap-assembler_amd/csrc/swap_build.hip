@@ -427,7 +427,12 @@ constexpr int kOccThreads = 1024;
 constexpr int kHistRowGroups = 8;     // row groups of the column sum
 constexpr int kChunkRecs = 4096;      // level-2 records per workgroup
 constexpr int kL2Threads = 512;
-constexpr int kL2Per = kChunkRecs / kL2Threads;
+// with 2^10 or more sub-buckets per level-1 bucket (inputs of more than
+// ~2^28 k-mers) a 4096-record chunk leaves runs of 2-4 records per
+// sub-bucket: chunks of 8192 records (one 1024-thread workgroup per CU) double them
+constexpr int kChunkRecsWide = 8192;
+constexpr int kL2ThreadsWide = 1024;
+constexpr int kL2WideBits = 10;
 
 // Occurrence records between the passes.  Reference order carries the input
 // index with each record: 12 bytes {rec lo, rec hi, occ}, moved as one
@@ -704,49 +709,50 @@ __device__ __forceinline__ uint32_t sample_of(const uint32_t *sb, uint32_t nsamp
     return lo;
 }
 
-template <bool WITH_OCC, int L2MAX, bool BATCH>
-__global__ void __launch_bounds__(kL2Threads) k_part2(const uint32_t *__restrict__ rec_in,
+template <bool WITH_OCC, int L2MAX, bool BATCH, int CHUNK, int THREADS>
+__global__ void __launch_bounds__(THREADS) k_part2(const uint32_t *__restrict__ rec_in,
                                                        const uint32_t *__restrict__ l1_start,
                                                        const uint32_t *__restrict__ chunk_start, int k, int l2bits,
                                                        uint32_t *__restrict__ cursor, uint32_t *__restrict__ rec_out,
                                                        const uint32_t *__restrict__ sbase, uint32_t nsamp) {
     static_assert(!BATCH || WITH_OCC, "a batch build keeps the input index of every record");
+    constexpr int kPer = CHUNK / THREADS;
     __shared__ uint32_t lcnt[L2MAX];
     __shared__ uint32_t lstart[L2MAX];
     __shared__ uint32_t gbase[L2MAX];
-    __shared__ uint32_t slo[kChunkRecs], shi[kChunkRecs];  // the chunk in sub-bucket order
-    __shared__ uint32_t socc[WITH_OCC ? kChunkRecs : 1];
+    __shared__ uint32_t slo[CHUNK], shi[CHUNK];  // the chunk in sub-bucket order
+    __shared__ uint32_t socc[WITH_OCC ? CHUNK : 1];
     __shared__ uint32_t lsb[BATCH ? kMaxBatch + 1 : 1];
     const uint32_t nl2 = 1u << l2bits, m2 = nl2 - 1;
     uint32_t b1;
     uint64_t a, e;
-    if (!chunk_range(blockIdx.x, l1_start, chunk_start, kChunkRecs, b1, a, e)) return;
+    if (!chunk_range(blockIdx.x, l1_start, chunk_start, CHUNK, b1, a, e)) return;
     for (uint32_t b = threadIdx.x; b < nl2; b += blockDim.x) lcnt[b] = 0;
     if (BATCH)
         for (uint32_t s = threadIdx.x; s <= nsamp; s += blockDim.x) lsb[s] = sbase[s];
     __syncthreads();
     const uint32_t n = (uint32_t)(e - a);
     const int sh2 = 8 + 2 * k - (kL1Bits + l2bits);
-    uint64_t r[kL2Per];
-    uint32_t o[kL2Per], rank[kL2Per];
+    uint64_t r[kPer];
+    uint32_t o[kPer], rank[kPer];
 #pragma unroll
-    for (int j = 0; j < kL2Per; j++) {
-        const uint32_t i = j * kL2Threads + threadIdx.x;
+    for (int j = 0; j < kPer; j++) {
+        const uint32_t i = j * THREADS + threadIdx.x;
         if (i < n) {
             r[j] = rec_load<WITH_OCC>(rec_in, a + i, o[j]);
             rank[j] = atomicAdd(&lcnt[(r[j] >> sh2) & m2], 1u);
         }
     }
     __syncthreads();
-    block_scan_excl<(L2MAX + kL2Threads - 1) / kL2Threads>(lcnt, lstart, nl2);
+    block_scan_excl<(L2MAX + THREADS - 1) / THREADS>(lcnt, lstart, nl2);
     for (uint32_t b = threadIdx.x; b < nl2; b += blockDim.x) {
         const uint32_t c = lcnt[b];
         gbase[b] = c ? atomicAdd(&cursor[((uint64_t)b1 << l2bits) + b], c) : 0;
     }
     __syncthreads();
 #pragma unroll
-    for (int j = 0; j < kL2Per; j++) {
-        const uint32_t i = j * kL2Threads + threadIdx.x;
+    for (int j = 0; j < kPer; j++) {
+        const uint32_t i = j * THREADS + threadIdx.x;
         if (i < n) {
             const uint32_t pos = lstart[(r[j] >> sh2) & m2] + rank[j];
             slo[pos] = (uint32_t)r[j];
@@ -756,7 +762,7 @@ __global__ void __launch_bounds__(kL2Threads) k_part2(const uint32_t *__restrict
     }
     __syncthreads();
     const uint64_t low = (1ull << sh2) - 1;  // mask and key bits below the partition's
-    for (uint32_t i = threadIdx.x; i < n; i += kL2Threads) {
+    for (uint32_t i = threadIdx.x; i < n; i += THREADS) {
         uint64_t rec = (uint64_t)slo[i] | ((uint64_t)shi[i] << 32);
         const uint32_t b = (uint32_t)(rec >> sh2) & m2;
         const uint32_t oc = WITH_OCC ? socc[i] : 0u;
@@ -1822,7 +1828,6 @@ __global__ void __launch_bounds__(1024) k_bsel_pick(const uint32_t *__restrict__
                                                     const uint32_t *__restrict__ slot_bin,
                                                     const uint32_t *__restrict__ bitmap, int shift,
                                                     uint64_t *__restrict__ value) {
-    __shared__ uint32_t part[1024];
     __shared__ uint32_t found;
     const uint32_t tg = blockIdx.x;
     const uint32_t slot = t_slot[tg], want = t_pos[tg];
@@ -1839,9 +1844,8 @@ __global__ void __launch_bounds__(1024) k_bsel_pick(const uint32_t *__restrict__
     if ((threadIdx.x & 63) == 63) wsum[threadIdx.x >> 6] = inc;
     if (threadIdx.x == 0) found = 0xFFFFFFFFu;
     __syncthreads();
-    uint32_t lo = inc - c;
+    uint32_t lo = inc - c;  // set bits of the bitmap before this thread's words
     for (uint32_t w = 0; w < (threadIdx.x >> 6); w++) lo += wsum[w];
-    part[threadIdx.x] = lo;
     if (want >= lo && want < lo + c) {
         uint32_t r = want - lo;
         for (uint32_t q = 0; q < per; q++) {
@@ -2202,7 +2206,9 @@ int Pipeline::build(const uint8_t *d_in, uint64_t n) {
     uint32_t *l1_start = l1info_.as<uint32_t>();
     uint32_t *chunk_start = l1_start + kL1 + 2;
     uint32_t *fchunk_start = chunk_start + kL1 + 2;
-    hipLaunchKernelGGL(k_l1_chunks, dim3(1), dim3(kL1), 0, st_, hist_.as<uint32_t>(), kHistL2Bits, (uint32_t)kChunkRecs,
+    const bool wide2 = l2bits >= kL2WideBits;
+    const uint32_t chunk2 = wide2 ? kChunkRecsWide : kChunkRecs;
+    hipLaunchKernelGGL(k_l1_chunks, dim3(1), dim3(kL1), 0, st_, hist_.as<uint32_t>(), kHistL2Bits, chunk2,
                        l1_start, chunk_start);
     if (pbits > kPartBitsMin)
         hipLaunchKernelGGL(k_l1_chunks, dim3(1), dim3(kL1), 0, st_, hist_.as<uint32_t>(), kHistL2Bits,
@@ -2237,19 +2243,23 @@ int Pipeline::build(const uint8_t *d_in, uint64_t n) {
     }
     // (the level-2 cursors start as copies of the partition starts, written
     // by the conversion above)
-    const unsigned grid2 = (unsigned)(N / kChunkRecs + kL1 + 1);
+    const unsigned grid2 = (unsigned)(N / chunk2 + kL1 + 1);
     phase_begin(7);
-#define SWAP_L2(OCC, MAX, B)                                                                                   \
-    hipLaunchKernelGGL((k_part2<OCC, MAX, B>), dim3(grid2), dim3(kL2Threads), 0, st_, rec1_.as<uint32_t>(), l1_start, \
+#define SWAP_L2(OCC, MAX, B, CH, TH)                                                                           \
+    hipLaunchKernelGGL((k_part2<OCC, MAX, B, CH, TH>), dim3(grid2), dim3(TH), 0, st_, rec1_.as<uint32_t>(), l1_start,   \
                        chunk_start, k_, l2bits, cursor_.as<uint32_t>(), rec_.as<uint32_t>(), d_kbase, nsamp)
     if (pbits == kPartBitsMin) {
-        if (batch_off_) SWAP_L2(true, kHistL2, true);
-        else if (ref_order_) SWAP_L2(true, kHistL2, false);
-        else SWAP_L2(false, kHistL2, false);
+        if (batch_off_) SWAP_L2(true, kHistL2, true, kChunkRecs, kL2Threads);
+        else if (ref_order_) SWAP_L2(true, kHistL2, false, kChunkRecs, kL2Threads);
+        else SWAP_L2(false, kHistL2, false, kChunkRecs, kL2Threads);
+    } else if (!wide2) {
+        if (batch_off_) SWAP_L2(true, (1 << kL2MaxBits), true, kChunkRecs, kL2Threads);
+        else if (ref_order_) SWAP_L2(true, (1 << kL2MaxBits), false, kChunkRecs, kL2Threads);
+        else SWAP_L2(false, (1 << kL2MaxBits), false, kChunkRecs, kL2Threads);
     } else {
-        if (batch_off_) SWAP_L2(true, (1 << kL2MaxBits), true);
-        else if (ref_order_) SWAP_L2(true, (1 << kL2MaxBits), false);
-        else SWAP_L2(false, (1 << kL2MaxBits), false);
+        if (batch_off_) SWAP_L2(true, (1 << kL2MaxBits), true, kChunkRecsWide, kL2ThreadsWide);
+        else if (ref_order_) SWAP_L2(true, (1 << kL2MaxBits), false, kChunkRecsWide, kL2ThreadsWide);
+        else SWAP_L2(false, (1 << kL2MaxBits), false, kChunkRecsWide, kL2ThreadsWide);
     }
 #undef SWAP_L2
     phase_end(7);

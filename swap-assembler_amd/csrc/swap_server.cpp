@@ -304,9 +304,6 @@ int swap_server_submit_fused(swap_server *s, const swap_job *jobs, size_t n, siz
         if (j.k != jobs[0].k || j.c != jobs[0].c || j.flags != jobs[0].flags) return SWAP_E_INVAL;
         if (j.flags & (SWAP_FLAG_KMER_GRAPH | SWAP_FLAG_HASH_ORDER)) return SWAP_E_INVAL;
         if (j.k < swapk::kMergeMinK || j.k > 31 || (j.k % 2) == 0) return SWAP_E_INVAL;
-        if (j.outputs & (1u << SWAP_FILE_KMER_GRAPH)) {
-            // allowed: the k-mer graph of a merge job is written from its build
-        }
     }
     // largest input first, cut into groups of at most group_bytes (and the
     // batch build's sample limit); an input above the budget is a group alone
