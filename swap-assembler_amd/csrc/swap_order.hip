@@ -93,8 +93,12 @@ struct OrdEntry {
 constexpr int kBinThreads = 1024;
 constexpr int kBinBatch = 4;  // keys a thread loads at once in the count and scatter passes
 // bins per table: with more, a workgroup's keys of a bin become runs too short
-// for whole-line stores in the scatter
-constexpr int kMaxBinBits = 12;
+// for whole-line stores in the scatter.  The bins aim at order_bin_keys keys
+// (2048 by default: 2^11 bins for the bacterial workload's 4.6M survivors);
+// 2^14 keeps the bins of ~100M survivors near the kSortCap keys an LDS
+// ordering holds (the count and scatter passes hold one 32-bit cursor per bin
+// of both tables in LDS, 128 KB: they run one workgroup per CU anyway)
+constexpr int kMaxBinBits = 14;
 
 struct BinShape {
     uint64_t P1, M1, P2, M2;  // final bucket counts of table 1 and 2 (M: see mod_p)
