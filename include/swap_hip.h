@@ -70,6 +70,9 @@ extern "C" {
                                     on-chip table use; same results, slower */
 #define SWAP_FLAG_WIDE_PARTITIONS 8u /* testing aid: at least 2^18 count partitions, the
                                         path inputs of billions of k-mers take; same results */
+#define SWAP_FLAG_COARSE_RENUMBER 32u /* testing aid: the merge numbers the survivors by coarse
+                                         first-occurrence bins only, the order inputs of more
+                                         than 2^31 k-mers get; same results */
 #define SWAP_FLAG_SINGLE_FILL 16u /* testing aid: one count partition per fill of the on-chip
                                      table, as on large inputs, also where small inputs'
                                      partitions would share fills; same results */

@@ -2463,7 +2463,7 @@ int Pipeline::build_batch(const uint8_t *d_in, const std::vector<uint64_t> &off)
     // survivors in first-occurrence order: each sample's are one run
     if (S) {
         bool renumbered = false;
-        rc = renumber_survivors(&renumbered);
+        rc = renumber_survivors(&renumbered, true);
         if (rc) return rc;
         if (!renumbered) return fail(-4, "too many input k-mers in one batch to order its survivors");
         SWAP_CHECK(hipMemcpyAsync(surv_key_.p, rn_[R_KEY].p, S * 8, hipMemcpyDeviceToDevice, st_));

@@ -253,6 +253,7 @@ int swap_create(const swap_cfg *cfg, swap_ctx **out) {
     c->pipe->set_table_spill((cfg->flags & SWAP_FLAG_TABLE_SPILL) != 0);
     c->pipe->set_wide_parts((cfg->flags & SWAP_FLAG_WIDE_PARTITIONS) != 0);
     c->pipe->set_single_fill((cfg->flags & SWAP_FLAG_SINGLE_FILL) != 0);
+    c->pipe->set_coarse_renumber((cfg->flags & SWAP_FLAG_COARSE_RENUMBER) != 0);
     // the static scratch's device address, once per device (the Pipeline made
     // this device current); a failure only disables swap_assemble_static
     (void)swapk::static_scratch_setup(cfg->device);
@@ -561,6 +562,7 @@ int swap_reconfigure(swap_ctx *c, const swap_cfg *cfg) {
     c->pipe->set_table_spill((cfg->flags & SWAP_FLAG_TABLE_SPILL) != 0);
     c->pipe->set_wide_parts((cfg->flags & SWAP_FLAG_WIDE_PARTITIONS) != 0);
     c->pipe->set_single_fill((cfg->flags & SWAP_FLAG_SINGLE_FILL) != 0);
+    c->pipe->set_coarse_renumber((cfg->flags & SWAP_FLAG_COARSE_RENUMBER) != 0);
     c->built = c->merged = c->have_graph = c->have_compacted = c->have_ce = false;
     return SWAP_OK;
 }

@@ -168,6 +168,9 @@ public:
     void set_wide_parts(bool on) { wide_parts_ = on; }
     // one partition per on-chip table fill on any input (a testing aid)
     void set_single_fill(bool on) { single_fill_ = on; }
+    // survivors renumbered by coarse first-occurrence bins only, as inputs
+    // beyond the exact ordering's range are (a testing aid)
+    void set_coarse_renumber(bool on) { coarse_renumber_ = on; }
     void set_tuning(const Tuning &t) { tune_ = clamp_tuning(t); }
     const Tuning &tuning() const { return tune_; }
     int device() const { return dev_; }
@@ -194,7 +197,8 @@ public:
     int fetch_merge(MergeDev &out);
     // on stream s (the side stream in a merge)
     int visit_priorities(uint64_t *d_prio, hipStream_t s);
-    int renumber_survivors(bool *renumbered);
+    // exact_only: the exact first-occurrence order or none (batch builds slice by it)
+    int renumber_survivors(bool *renumbered, bool exact_only = false);
 
     // Counters of the last build / merge.
     uint64_t n_reads = 0, n_occ = 0, n_distinct = 0, n_surv = 0, n_kept = 0, n_overflow = 0;
@@ -235,6 +239,7 @@ private:
     bool table_spill_ = false;
     bool wide_parts_ = false;
     bool single_fill_ = false;
+    bool coarse_renumber_ = false;
     Tuning tune_;
     int pbits_ = kPartBitsMin;  // partition bits of the last build
     // batch builds: the samples' byte offsets while build() runs for

@@ -65,13 +65,29 @@ __global__ void k_rn_sample(const uint32_t *__restrict__ fo, uint64_t S, int csh
 
 // Coarse bucket c starts a bin (head[c] = 1) where its stretch --
 // floor(sampled survivors before it (pre, an exclusive scan of the sampled
-// counts) x kRnSample / keys) -- or its block differs from c - 1's.
-__global__ void k_rn_heads(const uint64_t *__restrict__ pre, uint32_t nc, uint32_t keys,
+// counts) x kRnSample / keys) -- or its block of 2^block_bits coarse buckets
+// differs from c - 1's.
+__global__ void k_rn_heads(const uint64_t *__restrict__ pre, uint32_t nc, uint32_t keys, int block_bits,
                            uint32_t *__restrict__ head) {
     const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= nc) return;
     head[c] = c == 0 || (pre[c] * kRnSample) / keys != (pre[c - 1] * kRnSample) / keys ||
-              (c >> kRnBlockBits) != ((c - 1) >> kRnBlockBits);
+              (c >> block_bits) != ((c - 1) >> block_bits);
+}
+
+// Coarse numbering: the survivors in their bins' order as the scatter left
+// them (first occurrences ordered between bins, not inside one).
+__global__ void k_rn_unpack(const ulonglong2 *__restrict__ kv, const uint2 *__restrict__ fi, uint64_t S,
+                            uint64_t *__restrict__ key, uint64_t *__restrict__ mult, uint32_t *__restrict__ first,
+                            uint32_t *__restrict__ perm) {
+    const uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= S) return;
+    const ulonglong2 x = kv[j];
+    const uint2 f = fi[j];
+    key[j] = x.x;
+    mult[j] = x.y;
+    first[j] = f.x;
+    perm[f.y] = (uint32_t)j;
 }
 
 // bin of every coarse bucket: the heads up to it, less one (hpre: exclusive scan of head)
@@ -334,16 +350,15 @@ __global__ void __launch_bounds__(kRnThreads) k_rn_order(RnOrderArgs a, uint32_t
 // New order of the survivors: keys into rn_[R_KEY] (the survivor index is
 // built from the old order next), multiplicities and first occurrences in
 // place (their old order is dead once the survivors are scattered into bins),
-// and the new number of every old one into rn_[R_PERM]; *renumbered false (nothing
-// launched) above 2^kRnMaxBinBits bins or 2^(kRnCoarseBitsMax + 18) input
-// k-mers.
-int Pipeline::renumber_survivors(bool *renumbered) {
+// and the new number of every old one into rn_[R_PERM].  exact_only: the exact
+// first-occurrence order or nothing (*renumbered false, nothing launched,
+// above 2^kRnMaxBinBits bins or 2^(kRnCoarseBitsMax + 18) input k-mers).
+int Pipeline::renumber_survivors(bool *renumbered, bool exact_only) {
     *renumbered = false;
     const uint64_t S = n_surv;
     if (S == 0 || n_occ == 0 || !ref_order_) return 0;
     int cshift = kRnCoarseBitsMin;
     while (((n_occ - 1) >> cshift) >= kRnCoarseMax) cshift++;
-    if (cshift > kRnCoarseBitsMax) return 0;
     const uint32_t nc = (uint32_t)((n_occ - 1) >> cshift) + 1;
     const uint64_t nsamp = (S + kRnSample - 1) / kRnSample;
     RnShape sh;
@@ -351,9 +366,23 @@ int Pipeline::renumber_survivors(bool *renumbered) {
     // survivors a bin aims at: a bin of them plus one coarse bucket plus the
     // sampling error fits the LDS capacity
     const uint32_t width = 1u << cshift;
-    const uint32_t keys = kRnCap > width + kRnSlack + 1024 ? kRnCap - width - kRnSlack : 1024;
+    uint32_t keys = kRnCap > width + kRnSlack + 1024 ? kRnCap - width - kRnSlack : 1024;
+    int block_bits = kRnBlockBits;
     // bins: stretches, each cut at most once more by every block boundary
-    const uint64_t nb = nsamp * kRnSample / keys + 1 + ((nc - 1) >> kRnBlockBits);
+    uint64_t nb = nsamp * kRnSample / keys + 1 + ((nc - 1) >> kRnBlockBits);
+    // Beyond the exact ordering's range (its bitmaps and bin cursors), or
+    // with the testing aid, the survivors are numbered by bins only: bins of
+    // about S / 2^14 survivors with no block cuts, first occurrences ordered
+    // between bins and left in scatter order inside one.  A read's new k-mers
+    // still share a bin, which is what the merge's locality needs; no result
+    // depends on the numbering.  A batch build needs the exact order.
+    const bool coarse = (coarse_renumber_ && !exact_only) || cshift > kRnCoarseBitsMax || nb > (1u << kRnMaxBinBits);
+    if (coarse) {
+        if (exact_only) return 0;
+        block_bits = 31;
+        keys = (uint32_t)std::max<uint64_t>(1024, nsamp * kRnSample / ((1u << kRnMaxBinBits) - 2) + 1);
+        nb = nsamp * kRnSample / keys + 1;
+    }
     if (nb > (1u << kRnMaxBinBits)) return 0;
     sh.nb = (uint32_t)nb;
     int ncu = 0;
@@ -384,7 +413,7 @@ int Pipeline::renumber_survivors(bool *renumbered) {
     RCHECK(scan_excl<uint32_t>(chist, cpre, nc, scan_scratch_.as<uint64_t>(), cpre + nc, st_));
     uint32_t *head = rn_[R_HEAD].as<uint32_t>();
     uint64_t *hpre = rn_[R_HPRE].as<uint64_t>();
-    hipLaunchKernelGGL(k_rn_heads, dim3((nc + 255) / 256), dim3(256), 0, st_, cpre, nc, keys, head);
+    hipLaunchKernelGGL(k_rn_heads, dim3((nc + 255) / 256), dim3(256), 0, st_, cpre, nc, keys, block_bits, head);
     RCHECK(scan_excl<uint32_t>(head, hpre, nc, scan_scratch_.as<uint64_t>(), hpre + nc, st_));
     hipLaunchKernelGGL(k_rn_binmap, dim3((nc + 255) / 256), dim3(256), 0, st_, hpre, head, nc, binmap);
     hipLaunchKernelGGL(k_rn_count, dim3((unsigned)nblk), dim3(kRnThreads), 0, st_, fo, S, sh, binmap,
@@ -394,6 +423,13 @@ int Pipeline::renumber_survivors(bool *renumbered) {
     hipLaunchKernelGGL(k_rn_scatter, dim3((unsigned)nblk), dim3(kRnThreads), 0, st_, fo, surv_key_.as<uint64_t>(),
                        surv_mult_.as<uint64_t>(), S, sh, binmap, off, rn_[R_KV].as<ulonglong2>(),
                        rn_[R_FI].as<uint2>(), rn_[R_BSTART].as<uint32_t>());
+    if (coarse) {
+        hipLaunchKernelGGL(k_rn_unpack, dim3((unsigned)((S + 255) / 256)), dim3(256), 0, st_, rn_[R_KV].as<ulonglong2>(),
+                           rn_[R_FI].as<uint2>(), S, rn_[R_KEY].as<uint64_t>(), surv_mult_.as<uint64_t>(),
+                           surv_first_.as<uint32_t>(), rn_[R_PERM].as<uint32_t>());
+        *renumbered = true;
+        return 0;
+    }
     RnOrderArgs a;
     a.kv = rn_[R_KV].as<ulonglong2>();
     a.fi = rn_[R_FI].as<uint2>();

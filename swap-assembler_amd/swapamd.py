@@ -18,6 +18,7 @@ SWAP_FLAG_HASH_ORDER = 2
 SWAP_FLAG_TABLE_SPILL = 4
 SWAP_FLAG_WIDE_PARTITIONS = 8
 SWAP_FLAG_SINGLE_FILL = 16
+SWAP_FLAG_COARSE_RENUMBER = 32
 SWAP_E_INVAL = -1  # bad argument (include/swap_hip.h)
 SWAP_E_STATE = -8  # call out of order
 SWAP_FILE_KMER_GRAPH = 0
@@ -177,13 +178,14 @@ class Assembler:
     """One assembly context on one device (a thin wrapper of swap_ctx)."""
 
     def __init__(self, k, c, device=0, kmer_graph_only=False, hash_order=False, table_spill=False,
-                 wide_partitions=False, single_fill=False):
+                 wide_partitions=False, single_fill=False, coarse_renumber=False):
         self.k, self.c = k, c
         cfg = SwapCfg(k, c, device, (SWAP_FLAG_KMER_GRAPH if kmer_graph_only else 0)
                       | (SWAP_FLAG_HASH_ORDER if hash_order else 0)
                       | (SWAP_FLAG_TABLE_SPILL if table_spill else 0)
                       | (SWAP_FLAG_WIDE_PARTITIONS if wide_partitions else 0)
-                      | (SWAP_FLAG_SINGLE_FILL if single_fill else 0))
+                      | (SWAP_FLAG_SINGLE_FILL if single_fill else 0)
+                      | (SWAP_FLAG_COARSE_RENUMBER if coarse_renumber else 0))
         h = ctypes.c_void_p()
         rc = lib().swap_create(ctypes.byref(cfg), ctypes.byref(h))
         if rc:

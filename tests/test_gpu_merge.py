@@ -198,14 +198,14 @@ NP1_FILES = [("contigGraph", swapamd.SWAP_FILE_CONTIG_GRAPH), ("noce", swapamd.S
              ("jungarc", swapamd.SWAP_FILE_JUNG_ARC), ("jungmul", swapamd.SWAP_FILE_JUNG_MUL)]
 
 
-def check_single_rank_reference(name, fa):
+def check_single_rank_reference(name, fa, **flags):
     """Every file of the GPU path equals what the unmodified reference wrote as
     one rank for this input (fixture digests; exact bytes, line order and merged
     multiplicities included)."""
     case = NP1[name]
     k, c = case["k"], case["c"]
-    print("single-rank reference case %s k=%d c=%d bytes=%d" % (name, k, c, len(fa)), flush=True)
-    a = swapamd.Assembler(k, c)
+    print("single-rank reference case %s k=%d c=%d bytes=%d %s" % (name, k, c, len(fa), flags), flush=True)
+    a = swapamd.Assembler(k, c, **flags)
     a.build(fa)
     a.merge()
     bad = [what for what, which in NP1_FILES if not same_as_np1(name, what, a.file_bytes(which))]
@@ -221,6 +221,13 @@ def check_single_rank_reference(name, fa):
 @pytest.mark.parametrize("name", NP1_SMALL)
 def test_single_rank_reference(name):
     check_single_rank_reference(name, np1_input(name))
+
+
+@pytest.mark.parametrize("name", NP1_SMALL)
+def test_coarse_renumbering(name):
+    """The survivor numbering by coarse first-occurrence bins (the one inputs of
+    more than 2^31 k-mers get) changes no file."""
+    check_single_rank_reference(name, np1_input(name), coarse_renumber=True)
 
 
 @pytest.mark.parametrize("tuning", [
