@@ -68,8 +68,9 @@ extern "C" {
 #define SWAP_FLAG_TABLE_SPILL 4u /* testing aid: every count partition is recounted in the
                                     device-memory table that partitions too large for the
                                     on-chip table use; same results, slower */
-#define SWAP_FLAG_WIDE_PARTITIONS 8u /* testing aid: at least 2^18 count partitions, the
-                                        path inputs of billions of k-mers take; same results */
+#define SWAP_FLAG_WIDE_PARTITIONS 8u /* testing aid: at least 2^18 count partitions and
+                                        one visit-order binning pass per table, the
+                                        paths inputs of billions of k-mers take; same results */
 #define SWAP_FLAG_COARSE_RENUMBER 32u /* testing aid: the merge numbers the survivors by coarse
                                          first-occurrence bins only, the order inputs of more
                                          than 2^31 k-mers get; same results */

@@ -164,7 +164,8 @@ public:
     void reconfigure(int k, int c, bool reference_order);
     // every count partition through the device-memory table (a testing aid)
     void set_table_spill(bool on) { table_spill_ = on; }
-    // at least 2^18 partitions on any input (a testing aid for large inputs)
+    // at least 2^18 partitions and per-table visit-order bin passes on any
+    // input (a testing aid for large inputs)
     void set_wide_parts(bool on) { wide_parts_ = on; }
     // one partition per on-chip table fill on any input (a testing aid)
     void set_single_fill(bool on) { single_fill_ = on; }

@@ -95,10 +95,12 @@ constexpr int kBinBatch = 4;  // keys a thread loads at once in the count and sc
 // bins per table: with more, a workgroup's keys of a bin become runs too short
 // for whole-line stores in the scatter.  The bins aim at order_bin_keys keys
 // (2048 by default: 2^11 bins for the bacterial workload's 4.6M survivors);
-// 2^14 keeps the bins of ~100M survivors near the kSortCap keys an LDS
-// ordering holds (the count and scatter passes hold one 32-bit cursor per bin
-// of both tables in LDS, 128 KB: they run one workgroup per CU anyway)
-constexpr int kMaxBinBits = 14;
+// 2^15 keeps the bins of ~100M survivors within the kSortCap keys an LDS
+// ordering holds.  The count and scatter passes hold one 32-bit cursor per bin
+// in LDS (kBinCursors of them, 128 KB: they run one workgroup per CU anyway):
+// both tables in one pass while their bins fit, else one pass per table.
+constexpr int kMaxBinBits = 15;
+constexpr uint32_t kBinCursors = 1u << 15;
 
 struct BinShape {
     uint64_t P1, M1, P2, M2;  // final bucket counts of table 1 and 2 (M: see mod_p)
@@ -120,10 +122,14 @@ __device__ __forceinline__ void bins_of(uint64_t key, const BinShape &bs, uint64
     b2 = bs.nb1 + (uint32_t)(mod_p(h, bs.P2, bs.M2) >> bs.shift2);
 }
 
+// which: 0 both tables (bins 0 .. nb1 + nb2 - 1), 1 table 1 only, 2 table 2
+// only (its bins' columns after table 1's)
 __global__ void __launch_bounds__(kBinThreads) k_bin_count(const uint64_t *__restrict__ key, uint64_t n,
-                                                           uint64_t per, BinShape bs, uint32_t *__restrict__ cnt) {
-    __shared__ uint32_t hist[2u << kMaxBinBits];
-    const uint32_t nb = bs.nb1 + bs.nb2;
+                                                           uint64_t per, BinShape bs, uint32_t *__restrict__ cnt,
+                                                           int which) {
+    __shared__ uint32_t hist[kBinCursors];
+    const uint32_t nb = which == 0 ? bs.nb1 + bs.nb2 : which == 1 ? bs.nb1 : bs.nb2;
+    const uint32_t col0 = which == 2 ? bs.nb1 : 0;
     for (uint32_t i = threadIdx.x; i < nb; i += blockDim.x) hist[i] = 0;
     __syncthreads();
     const uint32_t w = xcd_range(blockIdx.x, gridDim.x);
@@ -141,31 +147,35 @@ __global__ void __launch_bounds__(kBinThreads) k_bin_count(const uint64_t *__res
             uint64_t h;
             uint32_t b1, b2;
             bins_of(k[u], bs, h, b1, b2);
-            atomicAdd(&hist[b1], 1u);
-            atomicAdd(&hist[b2], 1u);
+            if (which != 2) atomicAdd(&hist[b1], 1u);
+            if (which != 1) atomicAdd(&hist[b2 - col0], 1u);
         }
     }
     __syncthreads();
-    for (uint32_t i = threadIdx.x; i < nb; i += blockDim.x) cnt[(uint64_t)i * gridDim.x + w] = hist[i];
+    for (uint32_t i = threadIdx.x; i < nb; i += blockDim.x) cnt[(uint64_t)(col0 + i) * gridDim.x + w] = hist[i];
 }
 
 // off[bin * gridDim.x + block]: first slot of the block's keys of that bin
 // (table 1's slots 0..S-1, table 2's S..2S-1).  Workgroup 0 also writes every
-// bin's first slot and the end (2n).
+// bin's first slot and the end (2n).  which as in k_bin_count; with one pass
+// per table, table 2's runs first and leaves every key's table-2 slot in s2
+// (by survivor) for table 1's entries.
 __global__ void __launch_bounds__(kBinThreads) k_bin_scatter(const uint64_t *__restrict__ key,
                                                              const uint32_t *__restrict__ first, uint64_t n,
                                                              uint64_t per, BinShape bs,
                                                              const uint64_t *__restrict__ off,
                                                              OrdEntry *__restrict__ ent,
-                                                             uint32_t *__restrict__ bin_start) {
-    __shared__ uint32_t cur[2u << kMaxBinBits];
-    const uint32_t nb = bs.nb1 + bs.nb2;
+                                                             uint32_t *__restrict__ bin_start, int which,
+                                                             uint32_t *__restrict__ s2) {
+    __shared__ uint32_t cur[kBinCursors];
+    const uint32_t nb = which == 0 ? bs.nb1 + bs.nb2 : which == 1 ? bs.nb1 : bs.nb2;
+    const uint32_t col0 = which == 2 ? bs.nb1 : 0;
     const uint32_t w = xcd_range(blockIdx.x, gridDim.x);
     for (uint32_t i = threadIdx.x; i < nb; i += blockDim.x) {
-        cur[i] = (uint32_t)off[(uint64_t)i * gridDim.x + w];
-        if (w == 0) bin_start[i] = cur[i];
+        cur[i] = (uint32_t)off[(uint64_t)(col0 + i) * gridDim.x + w];
+        if (w == 0) bin_start[col0 + i] = cur[i];
     }
-    if (w == 0 && threadIdx.x == 0) bin_start[nb] = (uint32_t)(2 * n);
+    if (w == 0 && threadIdx.x == 0 && which != 1) bin_start[bs.nb1 + bs.nb2] = (uint32_t)(2 * n);
     __syncthreads();
     const uint64_t b = (uint64_t)w * per, e = min(n, b + per);
     for (uint64_t i0 = b + threadIdx.x; i0 < e; i0 += kBinBatch * blockDim.x) {
@@ -186,9 +196,18 @@ __global__ void __launch_bounds__(kBinThreads) k_bin_scatter(const uint64_t *__r
             uint64_t h;
             uint32_t b1, b2;
             bins_of(k[u], bs, h, b1, b2);
-            const uint32_t s1 = atomicAdd(&cur[b1], 1u), s2 = atomicAdd(&cur[b2], 1u);
-            ent[s1] = OrdEntry{h, fo[u], s2 - (uint32_t)n};
-            ent[s2] = OrdEntry{h, (uint32_t)i, 0u};
+            if (which == 0) {
+                const uint32_t s1 = atomicAdd(&cur[b1], 1u), t2 = atomicAdd(&cur[b2], 1u);
+                ent[s1] = OrdEntry{h, fo[u], t2 - (uint32_t)n};
+                ent[t2] = OrdEntry{h, (uint32_t)i, 0u};
+            } else if (which == 2) {
+                const uint32_t t2 = atomicAdd(&cur[b2 - col0], 1u);
+                ent[t2] = OrdEntry{h, (uint32_t)i, 0u};
+                s2[i] = t2 - (uint32_t)n;
+            } else {
+                const uint32_t s1 = atomicAdd(&cur[b1], 1u);
+                ent[s1] = OrdEntry{h, fo[u], s2[i]};
+            }
         }
     }
 }
@@ -494,15 +513,30 @@ int Pipeline::visit_priorities(uint64_t *d_prio, hipStream_t s) {
     // above the average, which stays below 2 x order_bin_keys by default)
     OCHECK(ob_[O_GSORT].reserve(S * 8));
     const uint64_t *key = surv_key_.as<uint64_t>();
-    hipLaunchKernelGGL(k_bin_count, dim3((unsigned)nblk), dim3(kBinThreads), 0, s, key, S, per, bs,
-                       ob_[O_BCNT].as<uint32_t>());
+    // both tables' bins in one count and scatter pass while their cursors fit
+    // LDS, else a pass per table (table 2's scatter first: table 1's entries
+    // carry table-2 slots, kept in r2 until table 1's ordering rewrites it)
+    // (the wide-partition testing aid takes the per-table passes on any input)
+    const bool split = nb > kBinCursors || wide_parts_;
+    if (split) {
+        hipLaunchKernelGGL(k_bin_count, dim3((unsigned)nblk), dim3(kBinThreads), 0, s, key, S, per, bs,
+                           ob_[O_BCNT].as<uint32_t>(), 1);
+        hipLaunchKernelGGL(k_bin_count, dim3((unsigned)nblk), dim3(kBinThreads), 0, s, key, S, per, bs,
+                           ob_[O_BCNT].as<uint32_t>(), 2);
+    } else {
+        hipLaunchKernelGGL(k_bin_count, dim3((unsigned)nblk), dim3(kBinThreads), 0, s, key, S, per, bs,
+                           ob_[O_BCNT].as<uint32_t>(), 0);
+    }
     OCHECK(scan_excl<uint32_t>(ob_[O_BCNT].as<uint32_t>(), ob_[O_BOFF].as<uint64_t>(), ncnt,
                                scan_scratch2_.as<uint64_t>(), ob_[O_BOFF].as<uint64_t>() + ncnt, s));
     // table 1's ranks are first occurrences; its epoch table's starts are the
     // first occurrences of the growth ranks, selected on the device by the build
-    hipLaunchKernelGGL(k_bin_scatter, dim3((unsigned)nblk), dim3(kBinThreads), 0, s, key,
-                       surv_first_.as<uint32_t>(), S, per, bs, ob_[O_BOFF].as<uint64_t>(), ob_[O_ENT].as<OrdEntry>(),
-                       ob_[O_BSTART].as<uint32_t>());
+    const int passes[2] = {split ? 2 : 0, 1};
+    for (int q = 0; q < (split ? 2 : 1); q++)
+        hipLaunchKernelGGL(k_bin_scatter, dim3((unsigned)nblk), dim3(kBinThreads), 0, s, key,
+                           surv_first_.as<uint32_t>(), S, per, bs, ob_[O_BOFF].as<uint64_t>(),
+                           ob_[O_ENT].as<OrdEntry>(), ob_[O_BSTART].as<uint32_t>(), passes[q],
+                           ob_[O_R2].as<uint32_t>());
     BinOrderArgs a;
     a.ent = ob_[O_ENT].as<OrdEntry>();
     a.r2 = ob_[O_R2].as<uint32_t>();
