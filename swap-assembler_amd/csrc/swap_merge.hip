@@ -916,14 +916,18 @@ __global__ void __launch_bounds__(kTreeLinkThreads) k_tree_links(TreeArgs t) {
     for (uint32_t j = threadIdx.x; j < nfar; j += kTreeLinkThreads) t.far[far_base + j] = sfar[j];
 }
 
+// A side k_tree_links left open has nothing larger between q and its tile's
+// edge, so the search starts at the edge: the hierarchy is entered at the tile
+// level at once instead of climbing out of the tile one level per load.
 __global__ void k_tree_far(TreeArgs t) {
     const uint32_t n = *t.nfar;
     for (uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; j < n; j += (uint64_t)gridDim.x * blockDim.x) {
         const uint64_t q = t.far[j];
         const uint64_t v = t.lev[q];
         const uint32_t l0 = t.nl[q], r0 = t.nr[q];
-        const uint64_t L = l0 != kNil ? l0 : nearest_left(t, q, v);
-        const uint64_t R = r0 != kNil ? r0 : nearest_right(t, q, v);
+        const uint64_t tb = q & ~(uint64_t)(kTreeTile - 1);  // q's tile: [tb, tb + kTreeTile)
+        const uint64_t L = l0 != kNil ? l0 : nearest_left(t, tb, v);
+        const uint64_t R = r0 != kNil ? r0 : nearest_right(t, tb + kTreeTile - 1, v);
         if (L >= t.M || R >= t.M) {  // reported through err; keep the position out of the evaluation
             t.par[q] = kNil;
             t.need[q] = 255;
