@@ -71,12 +71,12 @@ extern "C" {
 #define SWAP_FLAG_WIDE_PARTITIONS 8u /* testing aid: at least 2^18 count partitions and
                                         one visit-order binning pass per table, the
                                         paths inputs of billions of k-mers take; same results */
-#define SWAP_FLAG_COARSE_RENUMBER 32u /* testing aid: the merge numbers the survivors by coarse
-                                         first-occurrence bins only, the order inputs of more
-                                         than 2^31 k-mers get; same results */
 #define SWAP_FLAG_SINGLE_FILL 16u /* testing aid: one count partition per fill of the on-chip
                                      table, as on large inputs, also where small inputs'
                                      partitions would share fills; same results */
+#define SWAP_FLAG_COARSE_RENUMBER 32u /* testing aid: the merge numbers the survivors by coarse
+                                         first-occurrence bins only, the order inputs of more
+                                         than 2^31 k-mers get; same results */
 
 typedef struct swap_cfg {
     int k;          /* odd k-mer length, 9..31; swap_merge needs 19..31 (the reference CLI's range) */

@@ -52,6 +52,15 @@ def test_server_rejects_bad_slot_count():
     assert swapamd.lib().swap_server_create(0, 65, 0, ctypes.byref(h)) == -1
 
 
+def test_binding_constants_match_the_header():
+    text = open(os.path.join(REPO, "include", "swap_hip.h")).read()
+    defs = {m.group(1): int(m.group(2)) for m in re.finditer(r"^#define (SWAP_\w+) \(?(-?\d+)u?\)?", text, re.M)}
+    names = [n for n in dir(swapamd) if re.fullmatch(r"SWAP_(FLAG|FILE|E|JOB)_\w+", n)]
+    assert len(names) >= 16
+    for n in names:
+        assert defs[n] == getattr(swapamd, n), n
+
+
 def test_batch_entry_points_reject_bad_arguments():
     L = swapamd.lib()
     ptrs = (ctypes.c_void_p * 1)(None)
