@@ -218,3 +218,22 @@ def test_server_fused_submission():
     srv.close()
     for x in [d for _, _, d in inputs] + [d for _, d in extra]:
         x.close()
+
+
+@pytest.mark.parametrize("seed", [1, 2, 3])
+def test_random_batches(seed):
+    """Seeded random batches: k, c, read lengths, error rates, coverages and
+    genome sizes drawn per batch and per sample; every sample's files and
+    counts equal its own build's."""
+    import random
+    rng = random.Random(seed)
+    k = rng.choice([19, 23, 27])
+    c = rng.randint(1, 3)
+    inputs = []
+    for i in range(rng.randint(4, 9)):
+        L = rng.choice([100, 150, 250])
+        g = rng.choice([2000, 8000, 30000, 90000])
+        inputs.append(swapamd.generate_params(g, L, rng.uniform(8.0, 40.0), rng.choice([0.0, 0.005, 0.02]),
+                                              1000 * seed + i))
+    print("random batch %d: k=%d c=%d, %d samples" % (seed, k, c, len(inputs)), flush=True)
+    check(inputs, k, c)
