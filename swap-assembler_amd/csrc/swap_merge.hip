@@ -457,8 +457,10 @@ __global__ void k_rs_jump(const Walk *__restrict__ win, const uint8_t *__restric
 
 // All jump rounds in one workgroup when the rulers are few (small samples):
 // one launch instead of one per round.  Round r reads buffer r & 1 and writes
-// the other; the fences around each barrier make a round's device-memory
-// writes visible to every wave of the workgroup.  A thread holds its
+// the other; the barrier's workgroup-scope release/acquire makes a round's
+// device-memory writes visible to every wave of the workgroup (all of them
+// run on one CU; a device-scope fence would write back and invalidate the
+// L2 twice a round for nothing).  A thread holds its
 // kJumpOnePer rulers in registers and issues each hop's loads for all of them
 // together (three waves of independent loads per round instead of a chain of
 // dependent ones per ruler).
@@ -514,9 +516,7 @@ __global__ void __launch_bounds__(kJumpOneThreads) k_rs_jump_one(Walk *__restric
                 spout[v] = f[j];
             }
         }
-        __threadfence();
         __syncthreads();
-        __threadfence();
     }
 }
 
