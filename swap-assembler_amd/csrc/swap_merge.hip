@@ -829,8 +829,8 @@ __device__ __forceinline__ void tree_node(const TreeArgs &t, uint64_t q, uint64_
 // that k_tree_far finishes with the global hierarchy.
 constexpr int kTreeBits = 11;
 constexpr uint32_t kTreeTile = 1u << kTreeBits;
-constexpr int kTreeLinkThreads = 512;
-constexpr int kTreeEvalThreads = 512;
+constexpr int kTreeLinkThreads = 1024;
+constexpr int kTreeEvalThreads = 1024;
 constexpr uint32_t kTreeLevTotal = 2 * kTreeTile - 1;  // entries of levels 0..kTreeBits
 
 __device__ __forceinline__ uint32_t tree_loff(int l) { return 2 * kTreeTile - (2 * kTreeTile >> l); }
