@@ -41,7 +41,7 @@ CANDIDATES = [
     ("index_load", [50, 60, 67, 75, 85]),
     ("ruler_stride", [8, 16, 32, 64, 128, 256]),
     ("occ_groups", [1, 2]),
-    ("order_bin_keys", [1024, 2048, 4096, 8192]),
+    ("order_bin_keys", [256, 512, 1024, 2048, 4096, 8192]),
 ]
 # size classes: (name, largest input in bytes, input generator arguments, k, c,
 # objective).  Inputs of the two small classes come in batches (the many-sample
@@ -147,6 +147,13 @@ def tune_class(name, bound, gen, k, c, objective, reps, report):
                 best, best_ms = cand, ms
         # re-time the incumbent so later comparisons use a fresh figure
         best_ms, _ = measure(best)
+    # changes kept for a gain that noise made: the defaults, timed afresh, win
+    # unless the incumbent still beats them
+    if best != DEFAULTS:
+        default_ms, _ = measure(dict(DEFAULTS))
+        entry["default_retimed_ms"] = default_ms
+        if default_ms <= best_ms:
+            best, best_ms = dict(DEFAULTS), default_ms
     entry["best"] = best
     entry["best_ms"] = best_ms
     entry["gain"] = entry["default_ms"] / best_ms
